@@ -1,0 +1,185 @@
+/*
+ * oeb.h -- C-ABI of liboeb, the gfx950 (MI355X) implementation of the ten
+ * open-earth-benchmarks climate stencils.
+ *
+ * One entry point per stencil.  Each replaces the C++ free function of the same
+ * name in the reference's kernels/<name>.h (cited per entry below) and takes the
+ * same arguments in the same order.  Three things the reference passes
+ * implicitly are explicit here:
+ *   - the extents (the reference reads the globals domain_size and
+ *     domain_height, src/<name>.cpp:4-5), now an oeb_domain;
+ *   - the driver-level constants (uvbke's dt5, src/uvbke.cpp:11-12; fvtp2d's
+ *     p1/p2, src/fvtp2d.cpp:11-12), now scalar arguments;
+ *   - the stream, a hipStream_t passed as void* (NULL = the null stream).
+ *
+ * Arrays are described by oeb_storage3d_f64 / oeb_storage1d_f64.  They have
+ * the field order of the reference's Storage<double,3> / Storage<double,1>
+ * (include/util.h:24-30).  Element (i,j,k) lives at
+ *   alignedPtr[offset + strides[2]*i + strides[1]*j + strides[0]*k]
+ * (include/util.h:84-94).  sizes[] and strides[] are in reverse order:
+ * index 2 is the i (fastest) axis and index 0 is k.  Negative indices address
+ * the halo.  The data pointers are DEVICE pointers.  The descriptor structs
+ * themselves live in host memory.
+ *
+ * Conventions:
+ *   - Every call validates its descriptors before launching anything.  The
+ *     region the stencil touches, halo reach included, must lie inside each
+ *     array's allocated box.  All 3-D arrays of one call must share
+ *     strides[1] and strides[0], with strides[2] == 1.  Outputs may not overlap
+ *     inputs.  A failed check returns a non-zero status and launches nothing.
+ *     oeb_last_error() then gives a message naming the array and the reason.
+ *   - Launches are asynchronous on `stream`.  A return of OEB_OK means the
+ *     kernels were enqueued.  It says nothing about completion.
+ *   - Scratch arguments (the reference's temporaries: lap, flx, fly, T_sqr_s,
+ *     ...) are accepted for signature compatibility.  They may be NULL.  The
+ *     library never reads or writes them, because every temporary stays on
+ *     chip.  Scratch contents after a call are unspecified in the reference
+ *     too: its variants disagree on them.
+ *   - Outputs are written only on each stencil's valid region (interior
+ *     [0,nx)x[0,ny)x[0,nz), except fvtp2d's extended regions).
+ *   - Calls on distinct streams are thread-safe.
+ */
+#ifndef OEB_H
+#define OEB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define OEB_OK 0
+#define OEB_ERR_NULL 1   /* a required descriptor or data pointer is NULL */
+#define OEB_ERR_DOMAIN 2 /* nx, ny or nz is < 1, or too large for 32-bit in-plane indexing */
+#define OEB_ERR_LAYOUT 3 /* a descriptor does not cover the region the stencil touches, or strides differ */
+#define OEB_ERR_ALIAS 4  /* an output array overlaps an input array */
+#define OEB_ERR_HIP 5    /* a HIP runtime call or kernel launch failed */
+
+typedef struct oeb_domain {
+  int64_t nx; /* interior extent along i (the reference's domain_size) */
+  int64_t ny; /* interior extent along j (domain_size; the reference is square) */
+  int64_t nz; /* number of levels along k (domain_height) */
+} oeb_domain;
+
+typedef struct oeb_storage3d_f64 {
+  double *allocatedPtr;
+  double *alignedPtr;
+  int64_t offset;
+  int64_t sizes[3];   /* [k extent, j extent, i extent], halos included */
+  int64_t strides[3]; /* [k stride, j stride, i stride == 1], in elements */
+} oeb_storage3d_f64;
+
+typedef struct oeb_storage1d_f64 {
+  double *allocatedPtr;
+  double *alignedPtr;
+  int64_t offset;
+  int64_t sizes[1];
+  int64_t strides[1];
+} oeb_storage1d_f64;
+
+typedef oeb_storage3d_f64 oeb_s3;
+typedef oeb_storage1d_f64 oeb_s1;
+
+/* Library version as "major.minor.patch". */
+const char *oeb_version(void);
+/* Short, static description of a status code. */
+const char *oeb_status_string(int status);
+/* Message of the last failing call on this host thread ("" if none). */
+const char *oeb_last_error(void);
+
+/* laplace -- kernels/laplace.h:4 `laplace(const Storage3D& in, Storage3D& out)`;
+ * arithmetic of laplace_openmp, kernels/laplace.h:34-43.  Reach (1,1,0). */
+int oeb_laplace_f64(const oeb_domain *d, const oeb_s3 *in, const oeb_s3 *out, void *stream);
+
+/* hdiffsa -- kernels/hdiffsa.h:4-5 `hdiffsa(out, in, mask, crlato, crlatu, lap, flx, fly)`;
+ * arithmetic of hdiffsa_openmp, kernels/hdiffsa.h:95-123.  Reach (2,2,0).
+ * lap, flx, fly are scratch (may be NULL). */
+int oeb_hdiffsa_f64(const oeb_domain *d, const oeb_s3 *out, const oeb_s3 *in, const oeb_s3 *mask,
+                    const oeb_s1 *crlato, const oeb_s1 *crlatu, const oeb_s3 *lap, const oeb_s3 *flx,
+                    const oeb_s3 *fly, void *stream);
+
+/* hdiffsmag -- kernels/hdiffsmag.h:4-7 `hdiffsmag(uout, vout, uin, vin, mask, crlavo, crlavu, crlato,
+ * crlatu, acrlat0, T_sqr_s, S_sqr_uv, eddlat, eddlon, tau_smag, weight_smag)`;
+ * arithmetic of hdiffsmag_openmp, kernels/hdiffsmag.h:205-263.  Reach (1,1,0).
+ * T_sqr_s, S_sqr_uv are scratch (may be NULL). */
+int oeb_hdiffsmag_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
+                      const oeb_s3 *vin, const oeb_s3 *mask, const oeb_s1 *crlavo, const oeb_s1 *crlavu,
+                      const oeb_s1 *crlato, const oeb_s1 *crlatu, const oeb_s1 *acrlat0, const oeb_s3 *T_sqr_s,
+                      const oeb_s3 *S_sqr_uv, double eddlat, double eddlon, double tau_smag, double weight_smag,
+                      void *stream);
+
+/* hadvuv -- kernels/hadvuv.h:29-32 `hadvuv(uout, vout, uin, vin, acrlat0, acrlat1, tgrlatda0,
+ * tgrlatda1, uatupos, vatupos, uatvpos, vatvpos, uavg, vavg, ures, vres, eddlat, eddlon)`;
+ * arithmetic of hadvuv_openmp, kernels/hadvuv.h:178-210 (3rd-order upwind,
+ * advectionDriver kernels/hadvuv.h:4-27).  Reach (2,2,0).  The eight arrays
+ * uatupos..vres are scratch (may be NULL). */
+int oeb_hadvuv_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
+                   const oeb_s3 *vin, const oeb_s1 *acrlat0, const oeb_s1 *acrlat1, const oeb_s1 *tgrlatda0,
+                   const oeb_s1 *tgrlatda1, const oeb_s3 *uatupos, const oeb_s3 *vatupos, const oeb_s3 *uatvpos,
+                   const oeb_s3 *vatvpos, const oeb_s3 *uavg, const oeb_s3 *vavg, const oeb_s3 *ures,
+                   const oeb_s3 *vres, double eddlat, double eddlon, void *stream);
+
+/* hadvuv5th -- kernels/hadvuv5th.h:41-44, same parameter list as hadvuv;
+ * arithmetic of hadvuv5th_openmp, kernels/hadvuv5th.h:177-201 (5th-order
+ * upwind, advectionDriver kernels/hadvuv5th.h:4-39).  Reach (3,3,0). */
+int oeb_hadvuv5th_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
+                      const oeb_s3 *vin, const oeb_s1 *acrlat0, const oeb_s1 *acrlat1, const oeb_s1 *tgrlatda0,
+                      const oeb_s1 *tgrlatda1, const oeb_s3 *uatupos, const oeb_s3 *vatupos,
+                      const oeb_s3 *uatvpos, const oeb_s3 *vatvpos, const oeb_s3 *uavg, const oeb_s3 *vavg,
+                      const oeb_s3 *ures, const oeb_s3 *vres, double eddlat, double eddlon, void *stream);
+
+/* fastwavesuv -- kernels/fastwavesuv.h:4-7 `fastwavesuv(uout, vout, uin, vin, utens, vtens, wgtfac,
+ * ppuv, hhl, rho, fx, ppgk, ppgc, ppgu, ppgv, edadlat, dt)`;
+ * arithmetic of fastwavesuv_openmp, kernels/fastwavesuv.h:154-191.
+ * Reach i,j in [0,+1], k in [-1,+1]: the k-halo planes -1 and nz are read.
+ * ppgk, ppgc, ppgu, ppgv are scratch (may be NULL). */
+int oeb_fastwavesuv_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
+                        const oeb_s3 *vin, const oeb_s3 *utens, const oeb_s3 *vtens, const oeb_s3 *wgtfac,
+                        const oeb_s3 *ppuv, const oeb_s3 *hhl, const oeb_s3 *rho, const oeb_s1 *fx,
+                        const oeb_s3 *ppgk, const oeb_s3 *ppgc, const oeb_s3 *ppgu, const oeb_s3 *ppgv,
+                        double edadlat, double dt, void *stream);
+
+/* uvbke -- kernels/uvbke.h:4-5 `uvbke(ub, vb, uc, vc, cosa, rsina)`; arithmetic of
+ * uvbke_openmp, kernels/uvbke.h:61-75.  dt5 is the driver constant
+ * (src/uvbke.cpp:11-12: 0.5 * 225.0).  Reach i,j in [-1,0]. */
+int oeb_uvbke_f64(const oeb_domain *d, const oeb_s3 *ub, const oeb_s3 *vb, const oeb_s3 *uc, const oeb_s3 *vc,
+                  const oeb_s3 *cosa, const oeb_s3 *rsina, double dt5, void *stream);
+
+/* p_grad_c -- kernels/p_grad_c.h:4-6 `p_grad_c(uout, vout, uin, vin, rdxc, rdyc, delpc, gz, pkc, wk, dt2)`;
+ * arithmetic of p_grad_c_openmp, kernels/p_grad_c.h:80-101 (reads delpc(i-1)
+ * directly, so there is no uninitialised-scratch ring).  Reach i,j in [-1,0],
+ * k in [0,+1].  wk is scratch (may be NULL). */
+int oeb_p_grad_c_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
+                     const oeb_s3 *vin, const oeb_s3 *rdxc, const oeb_s3 *rdyc, const oeb_s3 *delpc,
+                     const oeb_s3 *gz, const oeb_s3 *pkc, const oeb_s3 *wk, double dt2, void *stream);
+
+/* nh_p_grad -- kernels/nh_p_grad.h:4-6 `nh_p_grad(uout, vout, uin, vin, rdx, rdy, gz, pp, pk3, wk1, wk,
+ * du, dv, dt)`; arithmetic of nh_p_grad_openmp, kernels/nh_p_grad.h:116-146.
+ * Reach i,j in [0,+1], k in [0,+1].  wk, du, dv are scratch (may be NULL). */
+int oeb_nh_p_grad_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
+                      const oeb_s3 *vin, const oeb_s3 *rdx, const oeb_s3 *rdy, const oeb_s3 *gz, const oeb_s3 *pp,
+                      const oeb_s3 *pk3, const oeb_s3 *wk1, const oeb_s3 *wk, const oeb_s3 *du, const oeb_s3 *dv,
+                      double dt, void *stream);
+
+/* fvtp2d -- kernels/fvtp2d.h:4-7 `fvtp2d(q_i, q_j, fx1, fx2, fy1, fy2, q, crx, cry, ra_x, ra_y, xfx, yfx,
+ * area, fxx, fyy, al, almq, br, b0, smt5)`; arithmetic of the DEFAULT variant,
+ * kernels/fvtp2d.h:4-177 (the reference has no OpenMP variant, and its
+ * fullfusion variant reads an unwritten array).  p1, p2 are the PPM
+ * constants, which the reference driver defines as 0.583333 and 0.083333
+ * (src/fvtp2d.cpp:11-12).  Reach (3,3,0).  Output regions (half-open):
+ *   fy2  i in [-3,nx+2), j in [0,ny+1)     q_i  i in [-3,nx+2), j in [0,ny)
+ *   fx2  i in [0,nx+1),  j in [-3,ny+2)    q_j  i in [0,nx),    j in [-3,ny+2)
+ *   fx1, fy1  [0,nx) x [0,ny)
+ * fxx, fyy, al, almq, br, b0, smt5 are scratch (may be NULL). */
+int oeb_fvtp2d_f64(const oeb_domain *d, const oeb_s3 *q_i, const oeb_s3 *q_j, const oeb_s3 *fx1,
+                   const oeb_s3 *fx2, const oeb_s3 *fy1, const oeb_s3 *fy2, const oeb_s3 *q, const oeb_s3 *crx,
+                   const oeb_s3 *cry, const oeb_s3 *ra_x, const oeb_s3 *ra_y, const oeb_s3 *xfx, const oeb_s3 *yfx,
+                   const oeb_s3 *area, const oeb_s3 *fxx, const oeb_s3 *fyy, const oeb_s3 *al, const oeb_s3 *almq,
+                   const oeb_s3 *br, const oeb_s3 *b0, const oeb_s3 *smt5, double p1, double p2, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OEB_H */

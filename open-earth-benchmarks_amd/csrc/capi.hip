@@ -1,0 +1,155 @@
+// capi.hip -- status strings, descriptor validation and error reporting
+// shared by every entry point of liboeb.
+#include <cstdio>
+#include <sstream>
+
+#include "oeb_common.hpp"
+
+namespace oeb {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int status, const std::string &msg) {
+  set_error(msg);
+  return status;
+}
+
+int check_domain(const char *kname, const oeb_domain *d) {
+  if (d == nullptr) return fail(OEB_ERR_NULL, std::string(kname) + ": domain is NULL");
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1) {
+    std::ostringstream m;
+    m << kname << ": domain extents must be >= 1 (nx=" << d->nx << " ny=" << d->ny << " nz=" << d->nz << ")";
+    return fail(OEB_ERR_DOMAIN, m.str());
+  }
+  if (d->nx > (int64_t(1) << 24) || d->ny > (int64_t(1) << 24) || d->nz > (int64_t(1) << 24)) {
+    return fail(OEB_ERR_DOMAIN, std::string(kname) + ": an extent exceeds 2^24");
+  }
+  return OEB_OK;
+}
+
+int check3(const char *kname, const char *name, const oeb_s3 *s, const Box &b, Geo &g, bool first,
+           const double **base) {
+  std::ostringstream m;
+  m << kname << ": array '" << name << "' ";
+  if (s == nullptr || s->alignedPtr == nullptr) return fail(OEB_ERR_NULL, m.str() + "is NULL");
+  const int64_t si = s->strides[2], sj = s->strides[1], sk = s->strides[0];
+  if (si != 1) return fail(OEB_ERR_LAYOUT, m.str() + "must have i stride (strides[2]) 1");
+  if (sj < 1 || sk < 1 || s->offset < 0 || s->sizes[0] < 1 || s->sizes[1] < 1 || s->sizes[2] < 1)
+    return fail(OEB_ERR_LAYOUT, m.str() + "has a non-positive size or stride, or a negative offset");
+  if (s->sizes[2] > sj || s->sizes[1] * sj > sk)
+    return fail(OEB_ERR_LAYOUT, m.str() + "has strides smaller than its sizes (rows or planes overlap)");
+  if (sk >= (int64_t(1) << 31))
+    return fail(OEB_ERR_LAYOUT, m.str() + "has a k stride >= 2^31 elements (one plane must fit 32-bit indexing)");
+  if (first) {
+    g.sj = int(sj);
+    g.sk = sk;
+  } else if (sj != g.sj || sk != g.sk) {
+    m << "has strides (k " << sk << ", j " << sj << ") that differ from the first array's (k " << g.sk << ", j "
+      << g.sj << "); all 3-D arrays of one call must share a layout";
+    return fail(OEB_ERR_LAYOUT, m.str());
+  }
+  // Position of element (0,0,0) inside the allocated box.
+  const int64_t ok = s->offset / sk, rem = s->offset % sk;
+  const int64_t oj = rem / sj, oi = rem % sj;
+  const bool inside = b.ilo >= -oi && b.ihi <= s->sizes[2] - oi && b.jlo >= -oj && b.jhi <= s->sizes[1] - oj &&
+                      b.klo >= -ok && b.khi <= s->sizes[0] - ok;
+  if (!inside) {
+    m << "does not cover the region the stencil touches: needs i [" << b.ilo << "," << b.ihi << ") j [" << b.jlo
+      << "," << b.jhi << ") k [" << b.klo << "," << b.khi << "), allocation spans i [" << -oi << ","
+      << s->sizes[2] - oi << ") j [" << -oj << "," << s->sizes[1] - oj << ") k [" << -ok << ","
+      << s->sizes[0] - ok << ")";
+    return fail(OEB_ERR_LAYOUT, m.str());
+  }
+  *base = s->alignedPtr + s->offset;
+  return OEB_OK;
+}
+
+int check1(const char *kname, const char *name, const oeb_s1 *s, int64_t lo, int64_t hi, const double **base,
+           int *stride) {
+  std::ostringstream m;
+  m << kname << ": 1-D array '" << name << "' ";
+  if (s == nullptr || s->alignedPtr == nullptr) return fail(OEB_ERR_NULL, m.str() + "is NULL");
+  const int64_t st = s->strides[0];
+  if (st < 1 || st > (int64_t(1) << 20) || s->offset < 0 || s->sizes[0] < 1)
+    return fail(OEB_ERR_LAYOUT, m.str() + "has a non-positive size or stride, or a negative offset");
+  if (s->offset + lo * st < 0 || s->offset + (hi - 1) * st > (s->sizes[0] - 1) * st) {
+    m << "does not cover indices [" << lo << "," << hi << ")";
+    return fail(OEB_ERR_LAYOUT, m.str());
+  }
+  *base = s->alignedPtr + s->offset;
+  *stride = int(st);
+  return OEB_OK;
+}
+
+void AliasSet::add(const char *nm, const double *base, const Box &b, const Geo &g, bool is_out) {
+  if (n >= kMax) return;
+  name[n] = nm;
+  lo[n] = reinterpret_cast<uintptr_t>(base + (b.klo * g.sk + b.jlo * int64_t(g.sj) + b.ilo));
+  hi[n] = reinterpret_cast<uintptr_t>(base + ((b.khi - 1) * g.sk + (b.jhi - 1) * int64_t(g.sj) + (b.ihi - 1)) + 1);
+  out[n] = is_out;
+  ++n;
+}
+
+void AliasSet::add1(const char *nm, const double *base, int64_t lo_i, int64_t hi_i, int stride) {
+  if (n >= kMax) return;
+  name[n] = nm;
+  lo[n] = reinterpret_cast<uintptr_t>(base + lo_i * stride);
+  hi[n] = reinterpret_cast<uintptr_t>(base + (hi_i - 1) * stride + 1);
+  out[n] = false;
+  ++n;
+}
+
+int AliasSet::check(const char *kname) const {
+  for (int a = 0; a < n; ++a) {
+    if (!out[a]) continue;
+    for (int b = 0; b < n; ++b) {
+      if (a == b) continue;
+      if (lo[a] < hi[b] && lo[b] < hi[a]) {
+        std::ostringstream m;
+        m << kname << ": output '" << name[a] << "' overlaps " << (out[b] ? "output" : "input") << " '" << name[b]
+          << "'";
+        return fail(OEB_ERR_ALIAS, m.str());
+      }
+    }
+  }
+  return OEB_OK;
+}
+
+int check_tiles(const char *kname, const Tiles &t, int64_t ni, int64_t nj, int nz, int tile_i, int tile_j) {
+  const int64_t n = ((ni + tile_i - 1) / tile_i) * ((nj + tile_j - 1) / tile_j) *
+                    ((int64_t(nz) + t.kchunk - 1) / t.kchunk);
+  if (n < 1 || n > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(kname) + ": domain too large");
+  return OEB_OK;
+}
+
+int launch_status(const char *kname) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(OEB_ERR_HIP, std::string(kname) + ": launch failed: " + hipGetErrorString(e));
+  return OEB_OK;
+}
+
+}  // namespace oeb
+
+extern "C" {
+
+const char *oeb_version(void) { return "0.1.0"; }
+
+const char *oeb_status_string(int status) {
+  switch (status) {
+    case OEB_OK: return "ok";
+    case OEB_ERR_NULL: return "null descriptor or pointer";
+    case OEB_ERR_DOMAIN: return "invalid domain extents";
+    case OEB_ERR_LAYOUT: return "descriptor does not cover the stencil's region, or strides differ";
+    case OEB_ERR_ALIAS: return "an output overlaps another array";
+    case OEB_ERR_HIP: return "HIP runtime error";
+  }
+  return "unknown status";
+}
+
+const char *oeb_last_error(void) { return oeb::g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,92 @@
+// hdiffsa.hip -- COSMO horizontal diffusion with flux limiter, fused.
+// Reference: kernels/hdiffsa.h:141-169 (hdiffsa_openmp).  Per level the
+// reference makes three sweeps through scratch arrays:
+//   lap (:147-149) -> flx, fly with limiters (:152-159) -> out (:164-165).
+// Here a lane evaluates, for its point, the five Laplacians it depends on:
+// centre and its four neighbours.  From them it forms the four limited fluxes
+// across its cell faces, then writes out.  Nothing round-trips through
+// memory.  Each Laplacian is evaluated with the reference expression, so a
+// value shared between neighbouring points is the same double in both.
+// Reach (2,2,0).  Algorithmic traffic: 2 inputs + 1 output = 24 B per point.
+#include "oeb_common.hpp"
+
+namespace oeb {
+namespace {
+
+constexpr int kTileI = 64, kTileJ = 4, kChunk = 8;
+
+// Laplacian at the point p points to, with that row's crlato / crlatu
+// (kernels/hdiffsa.h:147).
+__device__ __forceinline__ double lap_at(const double *__restrict__ p, int sj, double lo, double lu) {
+  const double c = p[0];
+  return p[-1] + p[1] - 2.0 * c + lo * (p[sj] - c) + lu * (p[-sj] - c);
+}
+
+__global__ void __launch_bounds__(kTileI *kTileJ)
+    hdiffsa_kernel(double *__restrict__ out, const double *__restrict__ in, const double *__restrict__ mask,
+                   const double *__restrict__ crlato, int so, const double *__restrict__ crlatu, int su, Geo g,
+                   Tiles t) {
+  const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
+  const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
+  if (i >= g.nx || j >= g.ny) return;
+  const int sj = g.sj;
+  const double lo_s = crlato[int64_t(j - 1) * so], lo_c = crlato[int64_t(j) * so], lo_n = crlato[int64_t(j + 1) * so];
+  const double lu_s = crlatu[int64_t(j - 1) * su], lu_c = crlatu[int64_t(j) * su], lu_n = crlatu[int64_t(j + 1) * su];
+  int64_t at = int64_t(c.k0) * g.sk + (i + j * sj);
+  for (int k = c.k0; k < c.k1; ++k, at += g.sk) {
+    const double *p = in + at;
+    const double lap_c = lap_at(p, sj, lo_c, lu_c);
+    const double lap_w = lap_at(p - 1, sj, lo_c, lu_c);
+    const double lap_e = lap_at(p + 1, sj, lo_c, lu_c);
+    const double lap_s = lap_at(p - sj, sj, lo_s, lu_s);
+    const double lap_n = lap_at(p + sj, sj, lo_n, lu_n);
+    const double q0 = p[0];
+    // flx(i-1,j), flx(i,j), fly(i,j-1), fly(i,j) with the limiter of :153-158.
+    double flx_w = lap_c - lap_w;
+    if (flx_w * (q0 - p[-1]) > 0) flx_w = 0;
+    double flx_e = lap_e - lap_c;
+    if (flx_e * (p[1] - q0) > 0) flx_e = 0;
+    double fly_s = lo_s * (lap_c - lap_s);
+    if (fly_s * (q0 - p[-sj]) > 0) fly_s = 0;
+    double fly_n = lo_c * (lap_n - lap_c);
+    if (fly_n * (p[sj] - q0) > 0) fly_n = 0;
+    out[at] = q0 + (flx_w - flx_e + fly_s - fly_n) * mask[at];
+  }
+}
+
+}  // namespace
+}  // namespace oeb
+
+extern "C" int oeb_hdiffsa_f64(const oeb_domain *d, const oeb_s3 *out, const oeb_s3 *in, const oeb_s3 *mask,
+                               const oeb_s1 *crlato, const oeb_s1 *crlatu, const oeb_s3 *lap, const oeb_s3 *flx,
+                               const oeb_s3 *fly, void *stream) {
+  using namespace oeb;
+  (void)lap;
+  (void)flx;
+  (void)fly;
+  const char *K = "hdiffsa";
+  OEB_TRY(check_domain(K, d));
+  const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
+  Geo g{int(nx), int(ny), int(nz), 0, 0};
+  const Box r0{0, nx, 0, ny, 0, nz};
+  const Box r_in{-2, nx + 2, -2, ny + 2, 0, nz};
+  const double *p_out, *p_in, *p_mask, *p_lo, *p_lu;
+  int s_lo = 1, s_lu = 1;
+  OEB_TRY(check3(K, "out", out, r0, g, true, &p_out));
+  OEB_TRY(check3(K, "in", in, r_in, g, false, &p_in));
+  OEB_TRY(check3(K, "mask", mask, r0, g, false, &p_mask));
+  OEB_TRY(check1(K, "crlato", crlato, -1, ny + 1, &p_lo, &s_lo));
+  OEB_TRY(check1(K, "crlatu", crlatu, -1, ny + 1, &p_lu, &s_lu));
+  AliasSet as;
+  as.add("out", p_out, r0, g, true);
+  as.add("in", p_in, r_in, g, false);
+  as.add("mask", p_mask, r0, g, false);
+  as.add1("crlato", p_lo, -1, ny + 1, s_lo);
+  as.add1("crlatu", p_lu, -1, ny + 1, s_lu);
+  OEB_TRY(as.check(K));
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kChunk);
+  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+  hipLaunchKernelGGL(hdiffsa_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
+                     const_cast<double *>(p_out), p_in, p_mask, p_lo, s_lo, p_lu, s_lu, g, t);
+  return launch_status(K);
+}

@@ -1,0 +1,51 @@
+// laplace.hip -- 5-point horizontal Laplacian.
+// Reference: kernels/laplace.h:34-43 (laplace_openmp).  Expression :39,
+// parenthesization kept: out = -4*in + ((in[i-1] + in[i+1]) + (in[j+1] + in[j-1])).
+// Algorithmic traffic: 1 input + 1 output field = 16 B per point (FP64).
+#include "oeb_common.hpp"
+
+namespace oeb {
+namespace {
+
+constexpr int kTileI = 64, kTileJ = 4, kChunk = 8;
+
+__global__ void __launch_bounds__(kTileI *kTileJ)
+    laplace_kernel(const double *__restrict__ in, double *__restrict__ out, Geo g, Tiles t) {
+  const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
+  const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
+  if (i >= g.nx || j >= g.ny) return;
+  const int sj = g.sj;
+  const int64_t at = int64_t(c.k0) * g.sk + (i + j * sj);
+  const double *p = in + at;
+  double *q = out + at;
+  for (int k = c.k0; k < c.k1; ++k) {
+    const double w = p[-1], e = p[1], n = p[sj], s = p[-sj], cc = p[0];
+    q[0] = -4.0 * cc + ((w + e) + (n + s));
+    p += g.sk;
+    q += g.sk;
+  }
+}
+
+}  // namespace
+}  // namespace oeb
+
+extern "C" int oeb_laplace_f64(const oeb_domain *d, const oeb_s3 *in, const oeb_s3 *out, void *stream) {
+  using namespace oeb;
+  const char *K = "laplace";
+  OEB_TRY(check_domain(K, d));
+  const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
+  Geo g{int(nx), int(ny), int(nz), 0, 0};
+  const Box r_in{-1, nx + 1, -1, ny + 1, 0, nz}, r_out{0, nx, 0, ny, 0, nz};
+  const double *p_in, *p_out;
+  OEB_TRY(check3(K, "in", in, r_in, g, true, &p_in));
+  OEB_TRY(check3(K, "out", out, r_out, g, false, &p_out));
+  AliasSet as;
+  as.add("in", p_in, r_in, g, false);
+  as.add("out", p_out, r_out, g, true);
+  OEB_TRY(as.check(K));
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kChunk);
+  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+  hipLaunchKernelGGL(laplace_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), p_in,
+                     const_cast<double *>(p_out), g, t);
+  return launch_status(K);
+}

@@ -1,0 +1,134 @@
+// oeb_common.hpp -- shared host and device pieces of liboeb.
+//
+// Geometry.  Every 3-D array of one call shares the j stride `sj` and the k
+// stride `sk` (in elements, i stride 1).  A kernel receives a pointer to each
+// array's element (0,0,0), so element (i,j,k) is p[i + j*sj + k*sk].  The
+// in-plane part i + j*sj is 32-bit (the host checks that a plane fits).  The
+// plane part k*sk is 64-bit.
+//
+// Work decomposition.  A workgroup owns a tile of TILE_I x TILE_J columns and
+// a chunk of levels.  Each lane owns one column (i, j) and marches up its
+// chunk.  Vertical neighbours are carried in registers from one level to the
+// next.  Horizontal neighbours come from L1/L2, or from LDS where a stencil
+// stages a tile.  Workgroups are numbered so that the 8 XCDs of the MI355X each
+// receive a contiguous run of tiles: neighbouring tiles share halo rows, and
+// each XCD has its own L2, so contiguous runs turn halo re-reads into L2 hits
+// instead of HBM or MALL traffic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/oeb.h"
+
+namespace oeb {
+
+// ------------------------------------------------------------------ geometry
+struct Geo {
+  int nx, ny, nz;
+  int sj;      // j stride, elements
+  int64_t sk;  // k stride, elements
+};
+
+// Half-open index box [ilo,ihi) x [jlo,jhi) x [klo,khi), relative to the
+// domain origin, for which a stencil reads or writes an array.
+struct Box {
+  int64_t ilo, ihi, jlo, jhi, klo, khi;
+};
+
+// Number of XCDs on MI355X (gfx950): workgroups are dispatched round-robin
+// over them.
+constexpr int kNumXcd = 8;
+
+// Maps the hardware workgroup id to a logical tile id such that XCD x (which
+// receives hardware ids x, x+8, x+16, ...) processes one contiguous range of
+// logical ids.  A bijection on [0, n) for every n.
+__device__ __forceinline__ int xcd_remap(int hw_id, int n) {
+  const int per = (n + kNumXcd - 1) / kNumXcd;
+  int tall = n % kNumXcd;
+  if (tall == 0) tall = kNumXcd;
+  const int x = hw_id % kNumXcd;
+  const int slot = hw_id / kNumXcd;
+  return (x < tall) ? x * per + slot : tall * per + (x - tall) * (per - 1) + slot;
+}
+
+// Tile layout of a column launch: logical id -> (tile_i, tile_j, chunk),
+// tile_i fastest.
+struct Tiles {
+  int ti, tj, nchunk, kchunk, n;
+};
+
+inline Tiles make_tiles(int64_t ni, int64_t nj, int nz, int tile_i, int tile_j, int kchunk) {
+  Tiles t;
+  t.ti = int((ni + tile_i - 1) / tile_i);
+  t.tj = int((nj + tile_j - 1) / tile_j);
+  t.kchunk = kchunk < 1 ? 1 : kchunk;
+  t.nchunk = (nz + t.kchunk - 1) / t.kchunk;
+  t.n = t.ti * t.tj * t.nchunk;
+  return t;
+}
+
+struct TileCoord {
+  int i0, j0, k0, k1;
+};
+
+__device__ __forceinline__ TileCoord tile_coord(const Tiles &t, int tile_i, int tile_j, int nz) {
+  const int id = xcd_remap(int(blockIdx.x), t.n);
+  const int ci = id % t.ti;
+  const int rest = id / t.ti;
+  const int cj = rest % t.tj;
+  const int ck = rest / t.tj;
+  TileCoord c;
+  c.i0 = ci * tile_i;
+  c.j0 = cj * tile_j;
+  c.k0 = ck * t.kchunk;
+  c.k1 = min(nz, c.k0 + t.kchunk);
+  return c;
+}
+
+// ------------------------------------------------------------- host checks
+// Thread-local message of the last failing call (oeb_last_error).
+void set_error(const std::string &msg);
+int fail(int status, const std::string &msg);
+
+// Validates the 3-D descriptor `s` of array `name` for kernel `kname`: the
+// box `b` must lie inside the allocation it describes and its strides must
+// match `g` (the first array checked fills g.sj / g.sk).  On success *base is
+// the device address of element (0,0,0).
+int check3(const char *kname, const char *name, const oeb_s3 *s, const Box &b, Geo &g, bool first,
+           const double **base);
+// Validates a 1-D descriptor over the index range [lo, hi).  *base is the
+// address of element 0 and *stride the element stride.
+int check1(const char *kname, const char *name, const oeb_s1 *s, int64_t lo, int64_t hi, const double **base,
+           int *stride);
+int check_domain(const char *kname, const oeb_domain *d);
+
+// Tracks byte ranges of the arrays of one call so that an output overlapping
+// any other array is rejected.
+struct AliasSet {
+  static constexpr int kMax = 32;
+  const char *name[kMax];
+  uintptr_t lo[kMax], hi[kMax];
+  bool out[kMax];
+  int n = 0;
+  void add(const char *nm, const double *base, const Box &b, const Geo &g, bool is_out);
+  void add1(const char *nm, const double *base, int64_t lo_i, int64_t hi_i, int stride);
+  int check(const char *kname) const;
+};
+
+// Converts the state of the last launch into a status.
+int launch_status(const char *kname);
+
+// Fails with OEB_ERR_DOMAIN when a launch would need more workgroups than one
+// grid dimension holds.
+int check_tiles(const char *kname, const Tiles &t, int64_t ni, int64_t nj, int nz, int tile_i, int tile_j);
+
+#define OEB_TRY(expr)              \
+  do {                             \
+    const int oeb_s_ = (expr);     \
+    if (oeb_s_ != OEB_OK) return oeb_s_; \
+  } while (0)
+
+}  // namespace oeb

@@ -1,0 +1,177 @@
+"""Summaries of rocprofv3 output databases (the SQLite ``*_results.db`` that
+rocprofv3 writes) as short markdown for profiles/.
+
+  python -m oeb.profiling stats  <results.db> --command "<cmd>" [--out file.md]
+      per-kernel dispatch count, average / min / max duration and resources
+  python -m oeb.profiling pmc    --fetch <db> --write <db> --copy-bytes N --shape 256x256x80
+      --command-fetch "<cmd>" --command-write "<cmd>" [--out-md f.md] [--out-json f.json]
+      HBM bytes per launch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
+      each scaled by what the same counter reads for the plain device copy of
+      N bytes that bench.py --copy-mib runs in the same process
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sqlite3
+import statistics
+import sys
+from collections import defaultdict
+from pathlib import Path
+
+def short_name(name: str) -> str:
+    """'void oeb::(anonymous namespace)::hadvuv_kernel<5>(double*, ...)' -> 'hadvuv_kernel<5>'."""
+    s = name.replace("(anonymous namespace)::", "")
+    if s.startswith("void "):
+        s = s[5:]
+    depth, cut = 0, len(s)
+    for pos, ch in enumerate(s):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = pos
+            break
+    s = s[:cut].strip()
+    depth, last = 0, 0
+    for pos, ch in enumerate(s):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == ":" and depth == 0 and pos + 1 < len(s) and s[pos + 1] == ":":
+            last = pos + 2
+    return s[last:]
+
+
+def _rows(db: Path, sql: str):
+    con = sqlite3.connect(str(db))
+    try:
+        return list(con.execute(sql))
+    finally:
+        con.close()
+
+
+def kernel_stats(db: Path) -> dict:
+    rows = _rows(db, "select name, duration, vgpr_count, accum_vgpr_count, sgpr_count, lds_size, scratch_size, "
+                     "grid_x, grid_y, grid_z, workgroup_x, workgroup_y, workgroup_z from kernels")
+    out = defaultdict(lambda: {"durations": []})
+    for (name, dur, vgpr, agpr, sgpr, lds, scratch, gx, gy, gz, wx, wy, wz) in rows:
+        k = out[short_name(name)]
+        k["durations"].append(float(dur) / 1e3)  # ns -> us
+        k.update(vgpr=vgpr, agpr=agpr, sgpr=sgpr, lds=lds, scratch=scratch, grid=(gx, gy, gz), wg=(wx, wy, wz))
+    return out
+
+
+def stats_markdown(db: Path, command: str) -> str:
+    st = kernel_stats(db)
+    lines = [
+        f"# rocprofv3 kernel trace: `{db.name}`",
+        "",
+        "Command (run from the repository root on the GPU box):",
+        "",
+        "```",
+        command,
+        "```",
+        "",
+        "| kernel | calls | avg us | min us | max us | VGPR | SGPR | LDS B | scratch B | grid (threads) | workgroup |",
+        "|---|---|---|---|---|---|---|---|---|---|---|",
+    ]
+    for name, k in sorted(st.items(), key=lambda kv: -statistics.mean(kv[1]["durations"])):
+        d = k["durations"]
+        lines.append(
+            f"| {name} | {len(d)} | {statistics.mean(d):.2f} | {min(d):.2f} | {max(d):.2f} | {k['vgpr']} | "
+            f"{k['sgpr']} | {k['lds']} | {k['scratch']} | {k['grid'][0]} | {k['wg'][0]}x{k['wg'][1]} |")
+    return "\n".join(lines) + "\n"
+
+
+def pmc_per_kernel(db: Path, counter: str) -> dict:
+    """Average value of `counter` per dispatch, per kernel."""
+    rows = _rows(db, "select k.name, p.value from pmc_events p join kernels k on p.event_id = k.id "
+                     f"where p.counter_name = '{counter}'")
+    if not rows:  # schema without pmc_events.event_id -> kernels.id link
+        rows = _rows(db, f"select kernel_name, value from counters_collection where counter_name = '{counter}'")
+    acc = defaultdict(list)
+    for name, val in rows:
+        acc[short_name(name)].append(float(val))
+    return {k: statistics.mean(v) for k, v in acc.items()}
+
+
+def main(argv=None) -> int:
+    p = argparse.ArgumentParser(prog="python -m oeb.profiling")
+    sub = p.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("db")
+    s.add_argument("--command", required=True)
+    s.add_argument("--out")
+    m = sub.add_parser("pmc")
+    m.add_argument("--fetch", required=True)
+    m.add_argument("--write", required=True)
+    m.add_argument("--copy-bytes", type=int, required=True)
+    m.add_argument("--copy-kernel", default="__amd_rocclr_copyBuffer")
+    m.add_argument("--shape", required=True, help="NXxNYxNZ")
+    m.add_argument("--command-fetch", required=True)
+    m.add_argument("--command-write", required=True)
+    m.add_argument("--out-md")
+    m.add_argument("--out-json")
+    a = p.parse_args(argv)
+    if a.cmd == "stats":
+        md = stats_markdown(Path(a.db), a.command)
+        if a.out:
+            Path(a.out).write_text(md)
+        print(md)
+        return 0
+    return _pmc(a)
+
+
+def _pmc(a) -> int:
+    sys.path.insert(0, str(Path(__file__).resolve().parent.parent))
+    from oeb.spec import KERNELS
+
+    fetch = pmc_per_kernel(Path(a.fetch), "FETCH_SIZE")
+    write = pmc_per_kernel(Path(a.write), "WRITE_SIZE")
+    ck = short_name(a.copy_kernel)
+    if ck not in fetch or ck not in write:
+        print(f"copy kernel {ck} not found in the counter runs", file=sys.stderr)
+        return 1
+    # FETCH_SIZE / WRITE_SIZE are KiB per dispatch; the copy reads and writes
+    # copy_bytes each, so scale = true bytes / counted bytes.
+    f_scale = a.copy_bytes / (fetch[ck] * 1024.0)
+    w_scale = a.copy_bytes / (write[ck] * 1024.0)
+    nx, ny, nz = (int(v) for v in a.shape.lower().split("x"))
+    pts = nx * ny * nz
+    res = {"shape": a.shape, "copy_bytes": a.copy_bytes, "fetch_scale": f_scale, "write_scale": w_scale,
+           "command_fetch": a.command_fetch, "command_write": a.command_write, "kernels": {}}
+    stencil_of = {}
+    for name in KERNELS:
+        stencil_of[name] = [k for k in fetch if k.startswith(name + "_") or k.startswith(name + "<")]
+    lines = [f"# HBM traffic per launch, {a.shape} FP64 (rocprofv3 FETCH_SIZE / WRITE_SIZE)", "",
+             "Commands (each from the repository root on the GPU box):", "", "```", a.command_fetch,
+             a.command_write, "```", "",
+             f"Calibration: a device copy of {a.copy_bytes} B read FETCH_SIZE {fetch[ck]:.0f} KiB and WRITE_SIZE "
+             f"{write[ck]:.0f} KiB; scale factors {f_scale:.4f} (fetch) and {w_scale:.4f} (write).", "",
+             "| stencil | kernels | fetch MB | write MB | HBM MB/launch | algorithmic MB | ratio |",
+             "|---|---|---|---|---|---|---|"]
+    for name, ks in stencil_of.items():
+        ks = [k for k in ks if k in write]
+        if not ks:
+            continue
+        fb = sum(fetch[k] for k in ks) * 1024.0 * f_scale
+        wb = sum(write[k] for k in ks) * 1024.0 * w_scale
+        alg = KERNELS[name].bytes_per_point_f64 * pts
+        res["kernels"][name] = {"dispatch_kernels": ks, "fetch_bytes": fb, "write_bytes": wb,
+                                "hbm_bytes_per_launch": fb + wb, "algorithmic_bytes": alg}
+        lines.append(f"| {name} | {', '.join(ks)} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {(fb + wb) / 1e6:.1f} | "
+                     f"{alg / 1e6:.1f} | {(fb + wb) / alg:.3f} |")
+    md = "\n".join(lines) + "\n"
+    if a.out_md:
+        Path(a.out_md).write_text(md)
+    if a.out_json:
+        Path(a.out_json).write_text(json.dumps(res, indent=1))
+    print(md)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

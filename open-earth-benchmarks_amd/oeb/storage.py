@@ -1,0 +1,95 @@
+"""Storage3D / Storage1D: arrays with the reference's descriptor, backed by a
+torch tensor on the GPU (the product path) or a numpy array on the host.
+
+Device memory comes from torch's allocator.  torch is the plumbing for
+memory and streams only: every stencil runs in liboeb.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .abi import OebStorage1D, OebStorage3D
+from .layout import Layout, Layout1D
+
+
+def desc3(ptr: int, L: Layout) -> OebStorage3D:
+    """Descriptor of a flat allocation of L.size doubles at address ptr."""
+    d = OebStorage3D()
+    d.allocatedPtr = ptr
+    d.alignedPtr = ptr
+    d.offset = L.offset
+    d.sizes[:] = L.sizes()
+    d.strides[:] = L.strides()
+    return d
+
+
+def desc1(ptr: int, L1: Layout1D) -> OebStorage1D:
+    d = OebStorage1D()
+    d.allocatedPtr = ptr
+    d.alignedPtr = ptr
+    d.offset = L1.offset
+    d.sizes[0] = L1.size
+    d.strides[0] = 1
+    return d
+
+
+def _ptr(buf) -> int:
+    if isinstance(buf, np.ndarray):
+        assert buf.dtype == np.float64 and buf.flags.c_contiguous
+        return buf.ctypes.data
+    # torch tensor
+    assert buf.dtype.is_floating_point and buf.element_size() == 8 and buf.is_contiguous()
+    return int(buf.data_ptr())
+
+
+class Storage3D:
+    """A 3-D field: flat float64 buffer plus its Layout."""
+
+    def __init__(self, layout: Layout, buf):
+        if (buf.size if isinstance(buf, np.ndarray) else buf.numel()) != layout.size:
+            raise ValueError("buffer size does not match layout")
+        self.layout = layout
+        self.buf = buf
+        self._desc = desc3(_ptr(buf), layout)
+
+    @property
+    def desc(self) -> OebStorage3D:
+        return self._desc
+
+    def ref(self):
+        return C.byref(self._desc)
+
+    @classmethod
+    def to_device(cls, layout: Layout, host: np.ndarray, device="cuda") -> "Storage3D":
+        import torch
+
+        return cls(layout, torch.from_numpy(np.ascontiguousarray(host)).to(device))
+
+    def to_host(self) -> np.ndarray:
+        if isinstance(self.buf, np.ndarray):
+            return self.buf
+        return self.buf.detach().to("cpu").numpy()
+
+
+class Storage1D:
+    def __init__(self, layout: Layout1D, buf):
+        if (buf.size if isinstance(buf, np.ndarray) else buf.numel()) != layout.size:
+            raise ValueError("buffer size does not match layout")
+        self.layout = layout
+        self.buf = buf
+        self._desc = desc1(_ptr(buf), layout)
+
+    @property
+    def desc(self) -> OebStorage1D:
+        return self._desc
+
+    def ref(self):
+        return C.byref(self._desc)
+
+    @classmethod
+    def to_device(cls, layout: Layout1D, host: np.ndarray, device="cuda") -> "Storage1D":
+        import torch
+
+        return cls(layout, torch.from_numpy(np.ascontiguousarray(host)).to(device))

@@ -1,0 +1,75 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures: outputs of the reference's own kernels
+(oracle/_ref/libref.so, compiled in place from the reference sources by
+oracle/Makefile) on seeded inputs.
+
+Each fixture  tests/golden/<kernel>_<family>_<nx>x<ny>x<nz>.npz  holds:
+  meta      JSON: kernel, family, seed, layout, reference variant, and the
+            sha256 of every input array (the seeded generator must reproduce
+            them, or the test reports that the generator changed)
+  <output>  the output on its valid region, shape (nz, j-extent, i-extent)
+
+Run:  make -C oracle ref && python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "open-earth-benchmarks_amd"))
+
+from oeb.generators import generate  # noqa: E402
+from oeb.layout import Layout  # noqa: E402
+from oeb.spec import KERNELS, ORDER  # noqa: E402
+from oracle import binding as B  # noqa: E402
+
+# (family, nx, ny, nz, align, seed)
+CASES = [("baseline", 12, 12, 5, 16, 21), ("refdriver", 20, 20, 3, 1, 0)]
+EXTRA = {"hdiffsmag": [("stress", 12, 12, 5, 16, 22)]}
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def fixture_name(kernel, family, nx, ny, nz) -> str:
+    return f"{kernel}_{family}_{nx}x{ny}x{nz}.npz"
+
+
+def cases(kernel):
+    return CASES + EXTRA.get(kernel, [])
+
+
+def main() -> int:
+    if not B.ref_available():
+        print("oracle/_ref/libref.so missing: build it with `make -C oracle ref`", file=sys.stderr)
+        return 1
+    for kernel in ORDER:
+        spec = KERNELS[kernel]
+        for family, nx, ny, nz, align, seed in cases(kernel):
+            L = Layout(nx, ny, nz, h=3, hk=1, align=align)
+            host = generate(kernel, L, family, seed)
+            ins = {a.name: sha(host[a.name]) for a in spec.args if a.kind in ("in", "in1")}
+            B.run_ref(kernel, host, L, threads=1)
+            arrays = {}
+            for o in spec.outputs:
+                i0, i1, j0, j1 = spec.region(o, nx, ny)
+                arrays[o] = np.ascontiguousarray(L.region(host[o], i0, i1, j0, j1))
+                assert np.isfinite(arrays[o]).all()
+            meta = {"kernel": kernel, "family": family, "seed": seed, "nx": nx, "ny": ny, "nz": nz, "h": 3, "hk": 1,
+                    "align": align, "reference_variant": B.ORACLE_VARIANT[kernel], "input_sha256": ins}
+            np.savez_compressed(HERE / fixture_name(kernel, family, nx, ny, nz), meta=np.array(json.dumps(meta)),
+                                **arrays)
+    print("golden fixtures written to", HERE)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,82 @@
+"""GPU parity: every stencil through the C-ABI on cuda:0 against the C oracle
+on the same seeded inputs.
+
+Bar: relative max-norm (max|gpu - oracle| / max|oracle|) <= 1e-12 on each
+output's valid region.  Both sides compile without FMA contraction, so the
+results are expected to agree bit for bit.  test_bit_exact asserts that.
+Points outside each output's region must keep their NaN fill, which shows
+that the kernel writes nothing outside its region.
+"""
+import numpy as np
+import pytest
+
+from common import bit_diffs, copy, inputs, out_region, outside_mask, rel_maxnorm
+from gpu_util import run_gpu
+from oeb.spec import KERNELS, ORDER
+from oracle import binding as B
+
+TOL = 1e-12
+
+# (nx, ny, nz, align): square, reference layout (align 1), rectangular, one
+# row, one point, and the 64x64x60 shape of the hdiffsa benchmark.
+SMALL = [(12, 12, 5, 16), (20, 20, 3, 1), (24, 17, 4, 16), (65, 3, 2, 1), (1, 1, 1, 16), (64, 64, 60, 16)]
+
+
+def _check(name, L, host, gpu, tol=TOL, require_exact=False):
+    ref = copy(host)
+    B.run_oracle(name, ref, L)
+    report = []
+    for o in KERNELS[name].outputs:
+        x = out_region(name, o, L, gpu[o])
+        y = out_region(name, o, L, ref[o])
+        assert np.isfinite(y).all(), f"{name}.{o}: oracle produced non-finite values"
+        assert np.isfinite(x).all(), f"{name}.{o}: GPU left unwritten (NaN) points in its region"
+        err = rel_maxnorm(x, y)
+        nd = bit_diffs(x, y)
+        report.append((o, err, nd))
+        assert err <= tol, f"{name}.{o}: rel max-norm {err:.3e} > {tol:.0e} ({nd} values differ)"
+        if require_exact:
+            assert nd == 0, f"{name}.{o}: {nd} values differ from the oracle (rel {err:.3e})"
+        outside = gpu[o].reshape(L.levels, L.rows, L.sj)[outside_mask(name, o, L)]
+        assert np.isnan(outside).all(), f"{name}.{o}: wrote {np.count_nonzero(~np.isnan(outside))} points outside"
+    return report
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("family", ["baseline", "refdriver", "stress"])
+@pytest.mark.parametrize("shape", SMALL, ids=lambda s: "x".join(map(str, s[:3])) + f"a{s[3]}")
+@pytest.mark.parametrize("name", ORDER)
+def test_parity_small(name, shape, family):
+    nx, ny, nz, align = shape
+    L, host = inputs(name, nx, ny, nz, family, seed=11, align=align)
+    print(f"[gpu] {name} {family} {nx}x{ny}x{nz} align={align}", flush=True)
+    gpu = run_gpu(name, copy(host), L)
+    _check(name, L, host, gpu)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ORDER)
+def test_bit_exact(name):
+    """Without contraction on either side, the GPU reproduces the oracle's
+    doubles exactly (division and sqrt are correctly rounded on both)."""
+    L, host = inputs(name, 48, 40, 6, "baseline", seed=5)
+    print(f"[gpu] bit-exact {name}", flush=True)
+    _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ORDER)
+def test_parity_256(name):
+    """The all-stencil workload shape: 256x256x80, halo 3, k-halo 1."""
+    L, host = inputs(name, 256, 256, 80, "baseline", seed=2)
+    print(f"[gpu] {name} 256x256x80", flush=True)
+    _check(name, L, host, run_gpu(name, copy(host), L))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ORDER)
+def test_parity_512(name):
+    """The 512x512x80 shape of the bandwidth target."""
+    L, host = inputs(name, 512, 512, 80, "baseline", seed=3)
+    print(f"[gpu] {name} 512x512x80", flush=True)
+    _check(name, L, host, run_gpu(name, copy(host), L))
