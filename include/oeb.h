@@ -172,11 +172,19 @@ int oeb_nh_p_grad_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vou
  *   fx2  i in [0,nx+1),  j in [-3,ny+2)    q_j  i in [0,nx),    j in [-3,ny+2)
  *   fx1, fy1  [0,nx) x [0,ny)
  * fxx, fyy, al, almq, br, b0, smt5 are scratch (may be NULL). */
+/* The reference drivers define p1 and p2 as macros before including any
+ * header (src/fvtp2d.cpp:11-12); keep them from rewriting this prototype. */
+#pragma push_macro("p1")
+#pragma push_macro("p2")
+#undef p1
+#undef p2
 int oeb_fvtp2d_f64(const oeb_domain *d, const oeb_s3 *q_i, const oeb_s3 *q_j, const oeb_s3 *fx1,
                    const oeb_s3 *fx2, const oeb_s3 *fy1, const oeb_s3 *fy2, const oeb_s3 *q, const oeb_s3 *crx,
                    const oeb_s3 *cry, const oeb_s3 *ra_x, const oeb_s3 *ra_y, const oeb_s3 *xfx, const oeb_s3 *yfx,
                    const oeb_s3 *area, const oeb_s3 *fxx, const oeb_s3 *fyy, const oeb_s3 *al, const oeb_s3 *almq,
                    const oeb_s3 *br, const oeb_s3 *b0, const oeb_s3 *smt5, double p1, double p2, void *stream);
+#pragma pop_macro("p2")
+#pragma pop_macro("p1")
 
 #ifdef __cplusplus
 }

@@ -49,8 +49,8 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
     const double gcC = gkC1 - gkC, gcE = gkE1 - gkE, gcN = gkN1 - gkN;
     const double ppgu = (ppE - ppC) + (gcE + gcC) * 0.5 * ((hC1 + hC) - (hE1 + hE)) / ((hC1 - hC) + (hE1 - hE));
     const double ppgv = (ppN - ppC) + (gcN + gcC) * 0.5 * ((hC1 + hC) - (hN1 + hN)) / ((hC1 - hC) + (hN1 - hN));
-    uout[at] = uin[at] + dt * (utens[at] - ppgu * (2.0 * fxj / (rE + rC)));
-    vout[at] = vin[at] + dt * (vtens[at] - ppgv * (2.0 * edadlat / (rN + rC)));
+    st_nt(uout + at, ld_nt(uin + at) + dt * (ld_nt(utens + at) - ppgu * (2.0 * fxj / (rE + rC))));
+    st_nt(vout + at, ld_nt(vin + at) + dt * (ld_nt(vtens + at) - ppgv * (2.0 * edadlat / (rN + rC))));
 
     ppC = ppC1;
     ppE = ppE1;

@@ -1,29 +1,37 @@
-// fvtp2d.hip -- FV3 2-D finite-volume tracer transport (PPM fluxes).
+// fvtp2d.hip -- FV3 2-D finite-volume tracer transport (PPM fluxes), fused.
 // Reference: kernels/fvtp2d.h:4-177 (the DEFAULT variant; there is no OpenMP
-// variant).  The reference builds each PPM flux from scratch arrays
-// al, almq, br, b0, smt5 (e.g. :9-37).  Here ppm_face() forms the same
-// quantities at the three faces m-1, m, m+1 that one flux value needs, with
-// the reference's expressions, and keeps them in registers.
+// variant).  The reference makes ~14 sweeps per level through scratch arrays
+// (al, almq, br, b0, smt5, fxx, fyy).  Here one workgroup takes a tile of
+// TI x TJ output points of one level at a time.  All intermediates stay in
+// LDS, and the four PPM stages and both inner updates run in one launch:
 //
-// Three launches, in stream order:
-//   A  fy2 = PPM_j(q, cry) on i [-3,nx+2) x j [0,ny+1)          (:9-37)
-//      fx2 = PPM_i(q, crx) on i [0,nx+1)  x j [-3,ny+2)         (:84-113)
-//   B  q_i = (q*area + yfx*fy2 - yfx*fy2[j+1]) / ra_y  on i [-3,nx+2) x j [0,ny)  (:39-51)
-//      q_j = (q*area + xfx*fx2 - xfx*fx2[i+1]) / ra_x  on i [0,nx) x j [-3,ny+2)  (:115-127)
-//   C  fx1 = 0.5*(PPM_i(q_i, crx) + fx2)*xfx,  fy1 = 0.5*(PPM_j(q_j, cry) + fy2)*yfx
-//      on [0,nx) x [0,ny)                                       (:53-82, :129-176)
-// fy2, fx2, q_i and q_j are outputs of the reference's signature, so each is
-// written once in any case.  B and C read them back.
-// Algorithmic traffic: 8 inputs + 6 outputs = 112 B per point (FP64).
+//   A  q on the tile plus a 3-point halo                       -> sq
+//   B  fy2 = PPM_j(q, cry) on i [i0-3, i0+TI+2) x j [j0, j0+TJ] -> sfy2   (:9-37)
+//      fx2 = PPM_i(q, crx) on i [i0, i0+TI] x j [j0-3, j0+TJ+2) -> sfx2   (:84-113)
+//   C  q_i = (q area + yfx fy2 - yfx[j+1] fy2[j+1]) / ra_y      -> sqi    (:39-51)
+//      q_j = (q area + xfx fx2 - xfx[i+1] fx2[i+1]) / ra_x      -> sqj    (:115-127)
+//   D  fx1 = 0.5 (PPM_i(q_i, crx) + fx2) xfx, fy1 = 0.5 (PPM_j(q_j, cry) + fy2) yfx
+//                                                                       (:53-82, :129-176)
+//
+// ppm_face() forms al, almq, br, b0 and smt5 at the three faces one flux needs,
+// with the reference's expressions, so every value equals the reference's
+// double.  fy2, fx2, q_i and q_j are outputs on extended regions.  A tile
+// computes them on its tile plus halo, and writes only the points it owns:
+// its own rows and columns, plus the region's extension beyond the domain when
+// the tile touches that edge.  So every point is written exactly once.
+// Algorithmic traffic: 8 inputs + 6 outputs = 112 B per interior point (FP64).
 #include "oeb_common.hpp"
 
 namespace oeb {
 namespace {
 
-constexpr int kTileI = 64, kTileJ = 4, kChunk = 8;
+constexpr int kTI = 64;   // tile width along i (one wavefront)
+constexpr int kTJ = 8;    // tile height along j
+constexpr int kNT = 256;  // threads per workgroup
+constexpr int kChunk = 4;  // levels per workgroup
 
-// PPM flux at face m between cells m-1 and m of one row or column, given
-// f(m-3) .. f(m+2) and the Courant number c at m (kernels/fvtp2d.h:12, 20-23, 28-34).
+// PPM flux at face m between cells m-1 and m along one axis, given
+// f(m-3) .. f(m+2) and the Courant number c at m (kernels/fvtp2d.h:12, 20-34).
 __device__ __forceinline__ double ppm_face(double fm3, double fm2, double fm1, double f0, double fp1, double fp2,
                                            double c, double p1, double p2) {
   const double al_m = ((p1 * (fm2 + fm1)) + (p2 * (fm3 + f0)));  // al(m-1)
@@ -34,17 +42,9 @@ __device__ __forceinline__ double ppm_face(double fm3, double fm2, double fm1, d
   const double smt_m = ((almq_m * br_m) < 0.0) ? 1.0 : 0.0;
   const double smt_0 = ((almq_0 * br_0) < 0.0) ? 1.0 : 0.0;
   const double smt = (smt_m + (smt_0 * (smt_m == 0.0 ? 1.0 : 0.0)));
-  if (c > 0.0) {
-    const double ctmp = ((1.0 - c) * (br_m - (c * b0_m)));
-    return (fm1 + (ctmp * smt));
-  }
-  const double ctmp = ((1.0 + c) * (almq_0 + (c * b0_0)));
-  return (f0 + (ctmp * smt));
-}
-
-// Flux of field f at (i,j) along the axis of stride s: face between (m-1) and m.
-__device__ __forceinline__ double ppm_at(const double *__restrict__ f, int s, double c, double p1, double p2) {
-  return ppm_face(f[-3 * s], f[-2 * s], f[-s], f[0], f[s], f[2 * s], c, p1, p2);
+  const double up = (fm1 + (((1.0 - c) * (br_m - (c * b0_m))) * smt));
+  const double dn = (f0 + (((1.0 + c) * (almq_0 + (c * b0_0))) * smt));
+  return (c > 0.0) ? up : dn;
 }
 
 struct FvIn {
@@ -54,58 +54,112 @@ struct FvOut {
   double *q_i, *q_j, *fx1, *fx2, *fy1, *fy2;
 };
 
-// Launch A and B cover the box i [-3, nx+2) x j [-3, ny+2); C covers [0,nx) x [0,ny).
-__global__ void __launch_bounds__(kTileI *kTileJ)
-    fvtp2d_flux2_kernel(FvIn a, FvOut o, double p1, double p2, Geo g, Tiles t) {
-  const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
-  const int i = c.i0 + int(threadIdx.x) - 3, j = c.j0 + int(threadIdx.y) - 3;
-  if (i >= g.nx + 2 || j >= g.ny + 2) return;
-  const bool do_fy2 = j >= 0 && j < g.ny + 1;
-  const bool do_fx2 = i >= 0 && i < g.nx + 1;
-  const int sj = g.sj;
-  int64_t at = int64_t(c.k0) * g.sk + (i + int64_t(j) * sj);
-  for (int k = c.k0; k < c.k1; ++k, at += g.sk) {
-    if (do_fy2) o.fy2[at] = ppm_at(a.q + at, sj, a.cry[at], p1, p2);
-    if (do_fx2) o.fx2[at] = ppm_at(a.q + at, 1, a.crx[at], p1, p2);
-  }
-}
+// LDS tile extents (doubles).
+constexpr int kQW = kTI + 6, kQH = kTJ + 6;    // q:   i [i0-3, i0+TI+3) x j [j0-3, j0+TJ+3)
+constexpr int kYW = kTI + 5, kYH = kTJ + 1;    // fy2: i [i0-3, i0+TI+2) x j [j0, j0+TJ+1)
+constexpr int kXW = kTI + 1, kXH = kTJ + 5;    // fx2: i [i0, i0+TI+1)   x j [j0-3, j0+TJ+2)
+constexpr int kIW = kTI + 5, kIH = kTJ;        // q_i: i [i0-3, i0+TI+2) x j [j0, j0+TJ)
+constexpr int kJW = kTI, kJH = kTJ + 5;        // q_j: i [i0, i0+TI)     x j [j0-3, j0+TJ+2)
 
-__global__ void __launch_bounds__(kTileI *kTileJ)
-    fvtp2d_qij_kernel(FvIn a, FvOut o, Geo g, Tiles t) {
-  const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
-  const int i = c.i0 + int(threadIdx.x) - 3, j = c.j0 + int(threadIdx.y) - 3;
-  if (i >= g.nx + 2 || j >= g.ny + 2) return;
-  const bool do_qi = j >= 0 && j < g.ny;
-  const bool do_qj = i >= 0 && i < g.nx;
-  const int sj = g.sj;
-  int64_t at = int64_t(c.k0) * g.sk + (i + int64_t(j) * sj);
-  for (int k = c.k0; k < c.k1; ++k, at += g.sk) {
-    const double qa = a.q[at] * a.area[at];
-    if (do_qi) {
-      const double fyy0 = (a.yfx[at] * o.fy2[at]);
-      const double fyy1 = (a.yfx[at + sj] * o.fy2[at + sj]);
-      o.q_i[at] = (((qa + fyy0) - fyy1) / a.ra_y[at]);
-    }
-    if (do_qj) {
-      const double fxx0 = (a.xfx[at] * o.fx2[at]);
-      const double fxx1 = (a.xfx[at + 1] * o.fx2[at + 1]);
-      o.q_j[at] = (((qa + fxx0) - fxx1) / a.ra_x[at]);
-    }
-  }
-}
+__global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn a, FvOut o, double p1, double p2, Geo g, Tiles t) {
+  __shared__ double sq[kQH][kQW];
+  __shared__ double sfy2[kYH][kYW];
+  __shared__ double sfx2[kXH][kXW];
+  __shared__ double sqi[kIH][kIW];
+  __shared__ double sqj[kJH][kJW];
 
-__global__ void __launch_bounds__(kTileI *kTileJ)
-    fvtp2d_flux1_kernel(FvIn a, FvOut o, double p1, double p2, Geo g, Tiles t) {
-  const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
-  const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
-  if (i >= g.nx || j >= g.ny) return;
-  const int sj = g.sj;
-  int64_t at = int64_t(c.k0) * g.sk + (i + int64_t(j) * sj);
-  for (int k = c.k0; k < c.k1; ++k, at += g.sk) {
-    const double fx1 = ppm_at(o.q_i + at, 1, a.crx[at], p1, p2);
-    const double fy1 = ppm_at(o.q_j + at, sj, a.cry[at], p1, p2);
-    o.fx1[at] = ((0.5 * (fx1 + o.fx2[at])) * a.xfx[at]);
-    o.fy1[at] = ((0.5 * (fy1 + o.fy2[at])) * a.yfx[at]);
+  const TileCoord tc = tile_coord(t, kTI, kTJ, g.nz);
+  const int i0 = tc.i0, j0 = tc.j0;
+  const int nx = g.nx, ny = g.ny, sj = g.sj;
+  const int tid = int(threadIdx.x);
+  // Ownership of extended-region points: a tile owns its own columns/rows,
+  // plus everything left of 0 (first tile) or right of its end (last tile).
+  const bool first_i = i0 == 0, last_i = i0 + kTI >= nx;
+  const bool first_j = j0 == 0, last_j = j0 + kTJ >= ny;
+  auto own_i = [&](int gi) { return (gi >= i0 || first_i) && (gi < i0 + kTI || last_i); };
+  auto own_j = [&](int gj) { return (gj >= j0 || first_j) && (gj < j0 + kTJ || last_j); };
+
+  for (int k = tc.k0; k < tc.k1; ++k) {
+    const int64_t plane = int64_t(k) * g.sk;
+    const double *q = a.q + plane;
+
+    // A: q tile with halo.  Points beyond the allocated halo are never read by
+    // a valid task below; they are filled with 0.
+    for (int e = tid; e < kQH * kQW; e += kNT) {
+      const int r = e / kQW, c = e % kQW;
+      const int gi = i0 - 3 + c, gj = j0 - 3 + r;
+      sq[r][c] = (gi < nx + 3 && gj < ny + 3) ? q[gi + gj * sj] : 0.0;
+    }
+    __syncthreads();
+
+    // B: fy2 on i [-3, nx+2) x j [0, ny+1) and fx2 on i [0, nx+1) x j [-3, ny+2).
+    for (int e = tid; e < kYH * kYW; e += kNT) {
+      const int r = e / kYW, c = e % kYW;
+      const int gi = i0 - 3 + c, gj = j0 + r;
+      if (gi < nx + 2 && gj < ny + 1) {
+        const int at = gi + gj * sj;
+        const double v = ppm_face(sq[r][c], sq[r + 1][c], sq[r + 2][c], sq[r + 3][c], sq[r + 4][c], sq[r + 5][c],
+                                  a.cry[plane + at], p1, p2);
+        sfy2[r][c] = v;
+        if (own_i(gi) && own_j(gj)) st_nt(o.fy2 + plane + at, v);
+      }
+    }
+    for (int e = tid; e < kXH * kXW; e += kNT) {
+      const int r = e / kXW, c = e % kXW;
+      const int gi = i0 + c, gj = j0 - 3 + r;
+      if (gi < nx + 1 && gj < ny + 2) {
+        const int at = gi + gj * sj;
+        const double *row = &sq[r][c];
+        const double v = ppm_face(row[0], row[1], row[2], row[3], row[4], row[5], a.crx[plane + at], p1, p2);
+        sfx2[r][c] = v;
+        if (own_i(gi) && own_j(gj)) st_nt(o.fx2 + plane + at, v);
+      }
+    }
+    __syncthreads();
+
+    // C: q_i on i [-3, nx+2) x j [0, ny), q_j on i [0, nx) x j [-3, ny+2).
+    for (int e = tid; e < kIH * kIW; e += kNT) {
+      const int r = e / kIW, c = e % kIW;
+      const int gi = i0 - 3 + c, gj = j0 + r;
+      if (gi < nx + 2 && gj < ny) {
+        const int at = gi + gj * sj;
+        const double fyy0 = (a.yfx[plane + at] * sfy2[r][c]);
+        const double fyy1 = (a.yfx[plane + at + sj] * sfy2[r + 1][c]);
+        const double v = ((((sq[r + 3][c] * a.area[plane + at]) + fyy0) - fyy1) / a.ra_y[plane + at]);
+        sqi[r][c] = v;
+        if (own_i(gi) && own_j(gj)) st_nt(o.q_i + plane + at, v);
+      }
+    }
+    for (int e = tid; e < kJH * kJW; e += kNT) {
+      const int r = e / kJW, c = e % kJW;
+      const int gi = i0 + c, gj = j0 - 3 + r;
+      if (gi < nx && gj < ny + 2) {
+        const int at = gi + gj * sj;
+        const double fxx0 = (a.xfx[plane + at] * sfx2[r][c]);
+        const double fxx1 = (a.xfx[plane + at + 1] * sfx2[r][c + 1]);
+        const double v = ((((sq[r][c + 3] * a.area[plane + at]) + fxx0) - fxx1) / a.ra_x[plane + at]);
+        sqj[r][c] = v;
+        if (own_i(gi) && own_j(gj)) st_nt(o.q_j + plane + at, v);
+      }
+    }
+    __syncthreads();
+
+    // D: fx1, fy1 on the tile's interior points.
+    for (int e = tid; e < kTJ * kTI; e += kNT) {
+      const int r = e / kTI, c = e % kTI;
+      const int gi = i0 + c, gj = j0 + r;
+      if (gi < nx && gj < ny) {
+        const int64_t at = plane + gi + gj * sj;
+        const double *ri = &sqi[r][c];
+        const double fx1 = ppm_face(ri[0], ri[1], ri[2], ri[3], ri[4], ri[5], a.crx[at], p1, p2);
+        const double fy1 = ppm_face(sqj[r][c], sqj[r + 1][c], sqj[r + 2][c], sqj[r + 3][c], sqj[r + 4][c],
+                                    sqj[r + 5][c], a.cry[at], p1, p2);
+        st_nt(o.fx1 + at, ((0.5 * (fx1 + sfx2[r + 3][c])) * a.xfx[at]));
+        st_nt(o.fy1 + at, ((0.5 * (fy1 + sfy2[r][c + 3])) * a.yfx[at]));
+      }
+    }
+    // The next level's phase A writes only sq, which phase D does not read;
+    // the barrier after phase A orders this phase D before the next phase B.
   }
 }
 
@@ -128,6 +182,8 @@ extern "C" int oeb_fvtp2d_f64(const oeb_domain *d, const oeb_s3 *q_i, const oeb_
   const Box r_fy2{-3, nx + 2, 0, ny + 1, 0, nz}, r_fx2{0, nx + 1, -3, ny + 2, 0, nz};
   const Box r_qi{-3, nx + 2, 0, ny, 0, nz}, r_qj{0, nx, -3, ny + 2, 0, nz};
   const Box r_q{-3, nx + 3, -3, ny + 3, 0, nz}, r_area{-3, nx + 2, -3, ny + 2, 0, nz};
+  // yfx is read at (i, j+1) for q_i rows, xfx at (i+1, j) for q_j columns: both
+  // stay inside the fy2 / fx2 regions.
   FvIn a;
   FvOut o;
   const double *pq_i, *pq_j, *pfx1, *pfx2, *pfy1, *pfy2;
@@ -168,14 +224,8 @@ extern "C" int oeb_fvtp2d_f64(const oeb_domain *d, const oeb_s3 *q_i, const oeb_
   o.fy1 = const_cast<double *>(pfy1);
   o.fy2 = const_cast<double *>(pfy2);
 
-  const hipStream_t s = hipStream_t(stream);
-  const Tiles tb = make_tiles(nx + 5, ny + 5, int(nz), kTileI, kTileJ, kChunk);
-  OEB_TRY(check_tiles(K, tb, nx + 5, ny + 5, int(nz), kTileI, kTileJ));
-  const Tiles ti = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kChunk);
-  hipLaunchKernelGGL(fvtp2d_flux2_kernel, dim3(tb.n), dim3(kTileI, kTileJ), 0, s, a, o, p1, p2, g, tb);
-  OEB_TRY(launch_status(K));
-  hipLaunchKernelGGL(fvtp2d_qij_kernel, dim3(tb.n), dim3(kTileI, kTileJ), 0, s, a, o, g, tb);
-  OEB_TRY(launch_status(K));
-  hipLaunchKernelGGL(fvtp2d_flux1_kernel, dim3(ti.n), dim3(kTileI, kTileJ), 0, s, a, o, p1, p2, g, ti);
+  const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJ, kChunk);
+  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, kTJ));
+  hipLaunchKernelGGL(fvtp2d_kernel, dim3(t.n), dim3(kNT), 0, hipStream_t(stream), a, o, p1, p2, g, t);
   return launch_status(K);
 }

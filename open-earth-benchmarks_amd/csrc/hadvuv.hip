@@ -68,12 +68,12 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
     const double uatupos = (1.0 / 3.0) * (u[-1] + u[0] + u[1]);
     const double vatupos = 0.25 * (v[1] + v[1 - sj] + v[0] + v[-sj]);
     const double ures = upwind<ORDER>(u, sj, a0 * uatupos, kEarthRadiusRecip * vatupos, eddlat, eddlon);
-    uout[at] = ures + tg0 * u[0] * vatupos;
+    st_nt(uout + at, ures + tg0 * u[0] * vatupos);
     // v component: kernels/hadvuv.h:197-206.
     const double uatvpos = 0.25 * (u[-1] + u[0] + u[sj] + u[sj - 1]);
     const double vatvpos = 1.0 / 3.0 * (v[-sj] + v[0] + v[sj]);
     const double vres = upwind<ORDER>(v, sj, a1 * uatvpos, kEarthRadiusRecip * vatvpos, eddlat, eddlon);
-    vout[at] = vres - tg1 * uatvpos * uatvpos;
+    st_nt(vout + at, vres - tg1 * uatvpos * uatvpos);
   }
 }
 

@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
     if (fly_s * (q0 - p[-sj]) > 0) fly_s = 0;
     double fly_n = lo_c * (lap_n - lap_c);
     if (fly_n * (p[sj] - q0) > 0) fly_n = 0;
-    out[at] = q0 + (flx_w - flx_e + fly_s - fly_n) * mask[at];
+    st_nt(out + at, q0 + (flx_w - flx_e + fly_s - fly_n) * ld_nt(mask + at));
   }
 }
 

@@ -49,19 +49,19 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
     const double s_jm = (u[0] - u[-sj]) * eddlat * kEarthRadiusRecip + (v[1 - sj] - v[-sj]) * ac_s * eddlon;
     const double tt0 = t0 * t0, tt_ip = t_ip * t_ip, tt_jp = t_jp * t_jp;
     const double ss0 = s0 * s0, ss_im = s_im * s_im, ss_jm = s_jm * s_jm;
-    const double hdweight = weight_smag * mask[at];
+    const double hdweight = weight_smag * ld_nt(mask + at);
 
     double smag_u = tau_smag * sqrt(0.5 * (tt_ip + tt0) + 0.5 * (ss_jm + ss0)) - hdweight;
     smag_u = cxx_min(0.5, cxx_max(0.0, smag_u));
     const double u0 = u[0];
     const double lapu = u[1] + u[-1] - 2.0 * u0 + lo * (u[sj] - u0) + lu * (u[-sj] - u0);
-    uout[at] = u0 + smag_u * lapu;
+    st_nt(uout + at, u0 + smag_u * lapu);
 
     double smag_v = tau_smag * sqrt(0.5 * (tt_jp + tt0) + 0.5 * (ss_im + ss0)) - hdweight;
     smag_v = cxx_min(0.5, cxx_max(0.0, smag_v));
     const double v0 = v[0];
     const double lapv = v[1] + v[-1] - 2.0 * v0 + vo * (v[sj] - v0) + vu * (v[-sj] - v0);
-    vout[at] = v0 + smag_v * lapv;
+    st_nt(vout + at, v0 + smag_v * lapv);
   }
 }
 

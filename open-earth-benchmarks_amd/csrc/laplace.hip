@@ -20,7 +20,7 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
   double *q = out + at;
   for (int k = c.k0; k < c.k1; ++k) {
     const double w = p[-1], e = p[1], n = p[sj], s = p[-sj], cc = p[0];
-    q[0] = -4.0 * cc + ((w + e) + (n + s));
+    st_nt(q, -4.0 * cc + ((w + e) + (n + s)));
     p += g.sk;
     q += g.sk;
   }

@@ -36,13 +36,13 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
     const double wk0 = (pkC1 - pkC), wkE = (pkE1 - pkE), wkN = (pkN1 - pkN);
 
     const double du = ((dt / (wk0 + wkE)) * (((gzC1 - gzE) * (pkE1 - pkC)) + ((gzC - gzE1) * (pkC1 - pkE))));
-    uout[at] = (((uin[at] + du) +
-                 ((dt / (w1C + w1E)) * (((gzC1 - gzE) * (ppE1 - ppC)) + ((gzC - gzE1) * (ppC1 - ppE))))) *
-                rdx[at]);
+    st_nt(uout + at, (((ld_nt(uin + at) + du) +
+                       ((dt / (w1C + w1E)) * (((gzC1 - gzE) * (ppE1 - ppC)) + ((gzC - gzE1) * (ppC1 - ppE))))) *
+                      ld_nt(rdx + at)));
     const double dv = ((dt / (wk0 + wkN)) * (((gzC1 - gzN) * (pkN1 - pkC)) + ((gzC - gzN1) * (pkC1 - pkN))));
-    vout[at] = (((vin[at] + dv) +
-                 ((dt / (w1C + w1N)) * (((gzC1 - gzN) * (ppN1 - ppC)) + ((gzC - gzN1) * (ppC1 - ppN))))) *
-                rdy[at]);
+    st_nt(vout + at, (((ld_nt(vin + at) + dv) +
+                       ((dt / (w1C + w1N)) * (((gzC1 - gzN) * (ppN1 - ppC)) + ((gzC - gzN1) * (ppC1 - ppN))))) *
+                      ld_nt(rdy + at)));
 
     gzC = gzC1;
     gzE = gzE1;

@@ -10,10 +10,18 @@
 // a chunk of levels.  Each lane owns one column (i, j) and marches up its
 // chunk.  Vertical neighbours are carried in registers from one level to the
 // next.  Horizontal neighbours come from L1/L2, or from LDS where a stencil
-// stages a tile.  Workgroups are numbered so that the 8 XCDs of the MI355X each
-// receive a contiguous run of tiles: neighbouring tiles share halo rows, and
-// each XCD has its own L2, so contiguous runs turn halo re-reads into L2 hits
-// instead of HBM or MALL traffic.
+// stages a tile.  Tiles are numbered i fastest, then j, then k.  The hardware
+// deals consecutive workgroup ids round-robin to the 8 XCDs, so neighbouring
+// tiles run at the same time on different XCDs.  Their shared halo lines are
+// then served by the MALL, which all XCDs share.  Remapping ids so that each
+// XCD owns a contiguous run measured 3-10% slower on every stencil shape tried.
+//
+// Memory access.  Outputs are written once and never read back, so they are
+// stored non-temporally (st_nt).  Inputs read only at the centre point are
+// loaded non-temporally (ld_nt).  Inputs whose neighbours are read keep the
+// normal cache policy, so the neighbour reads hit L1/L2.  Non-temporal
+// streaming avoids the burst of dirty-line write-back at kernel end.  It
+// measured 10-25% faster than normal stores for plain copies of 84-336 MB.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -38,22 +46,6 @@ struct Box {
   int64_t ilo, ihi, jlo, jhi, klo, khi;
 };
 
-// Number of XCDs on MI355X (gfx950): workgroups are dispatched round-robin
-// over them.
-constexpr int kNumXcd = 8;
-
-// Maps the hardware workgroup id to a logical tile id such that XCD x (which
-// receives hardware ids x, x+8, x+16, ...) processes one contiguous range of
-// logical ids.  A bijection on [0, n) for every n.
-__device__ __forceinline__ int xcd_remap(int hw_id, int n) {
-  const int per = (n + kNumXcd - 1) / kNumXcd;
-  int tall = n % kNumXcd;
-  if (tall == 0) tall = kNumXcd;
-  const int x = hw_id % kNumXcd;
-  const int slot = hw_id / kNumXcd;
-  return (x < tall) ? x * per + slot : tall * per + (x - tall) * (per - 1) + slot;
-}
-
 // Tile layout of a column launch: logical id -> (tile_i, tile_j, chunk),
 // tile_i fastest.
 struct Tiles {
@@ -70,12 +62,16 @@ inline Tiles make_tiles(int64_t ni, int64_t nj, int nz, int tile_i, int tile_j, 
   return t;
 }
 
+// Non-temporal (streaming) store and load of one double.
+__device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ double ld_nt(const double *p) { return __builtin_nontemporal_load(p); }
+
 struct TileCoord {
   int i0, j0, k0, k1;
 };
 
 __device__ __forceinline__ TileCoord tile_coord(const Tiles &t, int tile_i, int tile_j, int nz) {
-  const int id = xcd_remap(int(blockIdx.x), t.n);
+  const int id = int(blockIdx.x);
   const int ci = id % t.ti;
   const int rest = id / t.ti;
   const int cj = rest % t.tj;

@@ -32,10 +32,10 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
     const double gzC1 = gz[up], gzW1 = gz[up - 1], gzS1 = gz[up - sj];
     const double pkC1 = pkc[up], pkW1 = pkc[up - 1], pkS1 = pkc[up - sj];
     const double wk0 = delpc[at], wkW = delpc[at - 1], wkS = delpc[at - sj];
-    uout[at] = (uin[at] + (((dt2 * rdxc[at]) / (wkW + wk0)) *
-                           (((gzW1 - gzC) * (pkC1 - pkW)) + ((gzW - gzC1) * (pkW1 - pkC)))));
-    vout[at] = (vin[at] + (((dt2 * rdyc[at]) / (wkS + wk0)) *
-                           (((gzS1 - gzC) * (pkC1 - pkS)) + ((gzS - gzC1) * (pkS1 - pkC)))));
+    st_nt(uout + at, (ld_nt(uin + at) + (((dt2 * ld_nt(rdxc + at)) / (wkW + wk0)) *
+                                         (((gzW1 - gzC) * (pkC1 - pkW)) + ((gzW - gzC1) * (pkW1 - pkC))))));
+    st_nt(vout + at, (ld_nt(vin + at) + (((dt2 * ld_nt(rdyc + at)) / (wkS + wk0)) *
+                                         (((gzS1 - gzC) * (pkC1 - pkS)) + ((gzS - gzC1) * (pkS1 - pkC))))));
     gzC = gzC1;
     gzW = gzW1;
     gzS = gzS1;

@@ -23,9 +23,9 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
   for (int k = c.k0; k < c.k1; ++k, at += g.sk) {
     const double usum = uc[at - sj] + uc[at];
     const double vsum = vc[at - 1] + vc[at];
-    const double ca = cosa[at], rs = rsina[at];
-    ub[at] = ((dt5 * (usum - (vsum * ca))) * rs);
-    vb[at] = ((dt5 * (vsum - (usum * ca))) * rs);
+    const double ca = ld_nt(cosa + at), rs = ld_nt(rsina + at);
+    st_nt(ub + at, ((dt5 * (usum - (vsum * ca))) * rs));
+    st_nt(vb + at, ((dt5 * (vsum - (usum * ca))) * rs));
   }
 }
 

@@ -87,15 +87,23 @@ def stats_markdown(db: Path, command: str) -> str:
 
 
 def pmc_per_kernel(db: Path, counter: str) -> dict:
-    """Average value of `counter` per dispatch, per kernel."""
-    rows = _rows(db, "select k.name, p.value from pmc_events p join kernels k on p.event_id = k.id "
-                     f"where p.counter_name = '{counter}'")
-    if not rows:  # schema without pmc_events.event_id -> kernels.id link
-        rows = _rows(db, f"select kernel_name, value from counters_collection where counter_name = '{counter}'")
+    """Average value of `counter` per dispatch, per (kernel, grid size)."""
+    rows = _rows(db, "select kernel_name, grid_size, value from counters_collection "
+                     f"where counter_name = '{counter}'")
     acc = defaultdict(list)
-    for name, val in rows:
-        acc[short_name(name)].append(float(val))
+    for name, grid, val in rows:
+        acc[(short_name(name), int(grid))].append(float(val))
     return {k: statistics.mean(v) for k, v in acc.items()}
+
+
+def stencil_of(kernel: str):
+    """Which stencil a device kernel belongs to (None for other kernels)."""
+    if kernel.startswith("hadvuv_kernel"):
+        return "hadvuv5th" if "<5" in kernel else "hadvuv"
+    for name in ("laplace", "hdiffsa", "hdiffsmag", "fastwavesuv", "uvbke", "p_grad_c", "nh_p_grad", "fvtp2d"):
+        if kernel.startswith(name + "_"):
+            return name
+    return None
 
 
 def main(argv=None) -> int:
@@ -132,38 +140,44 @@ def _pmc(a) -> int:
     fetch = pmc_per_kernel(Path(a.fetch), "FETCH_SIZE")
     write = pmc_per_kernel(Path(a.write), "WRITE_SIZE")
     ck = short_name(a.copy_kernel)
-    if ck not in fetch or ck not in write:
+    # The calibration copy is the copy-kernel dispatch group that moved the most.
+    cands = [k for k in fetch if k[0] == ck and k in write]
+    if not cands:
         print(f"copy kernel {ck} not found in the counter runs", file=sys.stderr)
         return 1
+    key = max(cands, key=lambda k: write[k])
     # FETCH_SIZE / WRITE_SIZE are KiB per dispatch; the copy reads and writes
     # copy_bytes each, so scale = true bytes / counted bytes.
-    f_scale = a.copy_bytes / (fetch[ck] * 1024.0)
-    w_scale = a.copy_bytes / (write[ck] * 1024.0)
+    f_scale = a.copy_bytes / (fetch[key] * 1024.0)
+    w_scale = a.copy_bytes / (write[key] * 1024.0)
     nx, ny, nz = (int(v) for v in a.shape.lower().split("x"))
     pts = nx * ny * nz
     res = {"shape": a.shape, "copy_bytes": a.copy_bytes, "fetch_scale": f_scale, "write_scale": w_scale,
            "command_fetch": a.command_fetch, "command_write": a.command_write, "kernels": {}}
-    stencil_of = {}
-    for name in KERNELS:
-        stencil_of[name] = [k for k in fetch if k.startswith(name + "_") or k.startswith(name + "<")]
     lines = [f"# HBM traffic per launch, {a.shape} FP64 (rocprofv3 FETCH_SIZE / WRITE_SIZE)", "",
              "Commands (each from the repository root on the GPU box):", "", "```", a.command_fetch,
              a.command_write, "```", "",
-             f"Calibration: a device copy of {a.copy_bytes} B read FETCH_SIZE {fetch[ck]:.0f} KiB and WRITE_SIZE "
-             f"{write[ck]:.0f} KiB; scale factors {f_scale:.4f} (fetch) and {w_scale:.4f} (write).", "",
-             "| stencil | kernels | fetch MB | write MB | HBM MB/launch | algorithmic MB | ratio |",
+             f"Calibration: a device copy of {a.copy_bytes} B read FETCH_SIZE {fetch[key]:.0f} KiB and WRITE_SIZE "
+             f"{write[key]:.0f} KiB per dispatch; scale factors {f_scale:.4f} (fetch) and {w_scale:.4f} (write) "
+             "convert the counters of every kernel below to bytes.", "",
+             "| stencil | device kernels | fetch MB | write MB | HBM MB/launch | algorithmic MB | ratio |",
              "|---|---|---|---|---|---|---|"]
-    for name, ks in stencil_of.items():
-        ks = [k for k in ks if k in write]
+    groups = defaultdict(list)
+    for k in fetch:
+        st = stencil_of(k[0])
+        if st is not None and k in write:
+            groups[st].append(k)
+    for name in KERNELS:
+        ks = groups.get(name)
         if not ks:
             continue
         fb = sum(fetch[k] for k in ks) * 1024.0 * f_scale
         wb = sum(write[k] for k in ks) * 1024.0 * w_scale
         alg = KERNELS[name].bytes_per_point_f64 * pts
-        res["kernels"][name] = {"dispatch_kernels": ks, "fetch_bytes": fb, "write_bytes": wb,
+        res["kernels"][name] = {"dispatch_kernels": sorted({k[0] for k in ks}), "fetch_bytes": fb, "write_bytes": wb,
                                 "hbm_bytes_per_launch": fb + wb, "algorithmic_bytes": alg}
-        lines.append(f"| {name} | {', '.join(ks)} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | {(fb + wb) / 1e6:.1f} | "
-                     f"{alg / 1e6:.1f} | {(fb + wb) / alg:.3f} |")
+        lines.append(f"| {name} | {', '.join(sorted({k[0] for k in ks}))} | {fb / 1e6:.1f} | {wb / 1e6:.1f} | "
+                     f"{(fb + wb) / 1e6:.1f} | {alg / 1e6:.1f} | {(fb + wb) / alg:.3f} |")
     md = "\n".join(lines) + "\n"
     if a.out_md:
         Path(a.out_md).write_text(md)
