@@ -1,6 +1,8 @@
 // capi.hip -- status strings, descriptor validation and error reporting
 // shared by every entry point of liboeb.
+#include <cctype>
 #include <cstdio>
+#include <cstdlib>
 #include <sstream>
 
 #include "oeb_common.hpp"
@@ -124,6 +126,16 @@ int check_tiles(const char *kname, const Tiles &t, int64_t ni, int64_t nj, int n
                     ((int64_t(nz) + t.kchunk - 1) / t.kchunk);
   if (n < 1 || n > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(kname) + ": domain too large");
   return OEB_OK;
+}
+
+int kchunk_for(const char *kname, int dflt) {
+  std::string var = "OEB_KCHUNK_";
+  for (const char *c = kname; *c; ++c) var += char(std::toupper(static_cast<unsigned char>(*c)));
+  const char *v = std::getenv(var.c_str());
+  if (v == nullptr) return dflt;
+  char *end = nullptr;
+  const long n = std::strtol(v, &end, 10);
+  return (end != v && *end == '\0' && n >= 1 && n <= 4096) ? int(n) : dflt;
 }
 
 int launch_status(const char *kname) {

@@ -12,55 +12,125 @@ namespace oeb {
 namespace {
 
 constexpr int kTileI = 64, kTileJ = 4, kChunk = 16;
+constexpr int kPairI = 2 * kTileI;
 
 __device__ __forceinline__ double ppgk_of(double wgt, double pp_k, double pp_km1) {
   return wgt * pp_k + (1.0 - wgt) * pp_km1;
 }
 
+// Level-k and level-(k+1) values of one quantity at C = (i,j), E = (i+1,j),
+// N = (i,j+1).
+struct CEN {
+  double C, E, N, C1, E1, N1;
+};
+
+// One point (kernels/fastwavesuv.h:312-337).  pp: ppuv at level k (C, E, N);
+// gk: ppgk at k and k+1; h: hhl at k and k+1; r*: rho at level k.
+__device__ __forceinline__ void fw_point(double ppC, double ppE, double ppN, const CEN &gk, const CEN &h, double rC,
+                                         double rE, double rN, double uin, double vin, double utens, double vtens,
+                                         double fxj, double edadlat, double dt, double &uo, double &vo) {
+  const double gcC = gk.C1 - gk.C, gcE = gk.E1 - gk.E, gcN = gk.N1 - gk.N;
+  const double ppgu =
+      (ppE - ppC) + (gcE + gcC) * 0.5 * ((h.C1 + h.C) - (h.E1 + h.E)) / ((h.C1 - h.C) + (h.E1 - h.E));
+  const double ppgv =
+      (ppN - ppC) + (gcN + gcC) * 0.5 * ((h.C1 + h.C) - (h.N1 + h.N)) / ((h.C1 - h.C) + (h.N1 - h.N));
+  uo = uin + dt * (utens - ppgu * (2.0 * fxj / (rE + rC)));
+  vo = vin + dt * (vtens - ppgv * (2.0 * edadlat / (rN + rC)));
+}
+
+struct FwArgs {
+  double *uout, *vout;
+  const double *uin, *vin, *utens, *vtens, *wgtfac, *ppuv, *hhl, *rho, *fx;
+  int fx_stride;
+};
+
 __global__ void __launch_bounds__(kTileI *kTileJ)
-    fastwavesuv_kernel(double *__restrict__ uout, double *__restrict__ vout, const double *__restrict__ uin,
-                       const double *__restrict__ vin, const double *__restrict__ utens,
-                       const double *__restrict__ vtens, const double *__restrict__ wgtfac,
-                       const double *__restrict__ ppuv, const double *__restrict__ hhl,
-                       const double *__restrict__ rho, const double *__restrict__ fx, int fx_stride, double edadlat,
-                       double dt, Geo g, Tiles t) {
+    fastwavesuv_kernel(FwArgs a, double edadlat, double dt, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
   const int64_t sk = g.sk;
-  const double fxj = fx[int64_t(j) * fx_stride];
+  const double fxj = a.fx[int64_t(j) * a.fx_stride];
   int64_t at = int64_t(c.k0) * sk + (i + j * sj);
   const int64_t dn = at - sk;
-  double ppC = ppuv[at], ppE = ppuv[at + 1], ppN = ppuv[at + sj];
-  double gkC = ppgk_of(wgtfac[at], ppC, ppuv[dn]);
-  double gkE = ppgk_of(wgtfac[at + 1], ppE, ppuv[dn + 1]);
-  double gkN = ppgk_of(wgtfac[at + sj], ppN, ppuv[dn + sj]);
-  double hC = hhl[at], hE = hhl[at + 1], hN = hhl[at + sj];
+  double ppC = a.ppuv[at], ppE = a.ppuv[at + 1], ppN = a.ppuv[at + sj];
+  CEN gk, h;
+  gk.C = ppgk_of(a.wgtfac[at], ppC, a.ppuv[dn]);
+  gk.E = ppgk_of(a.wgtfac[at + 1], ppE, a.ppuv[dn + 1]);
+  gk.N = ppgk_of(a.wgtfac[at + sj], ppN, a.ppuv[dn + sj]);
+  h.C = a.hhl[at], h.E = a.hhl[at + 1], h.N = a.hhl[at + sj];
   for (int k = c.k0; k < c.k1; ++k, at += sk) {
     const int64_t up = at + sk;
-    const double ppC1 = ppuv[up], ppE1 = ppuv[up + 1], ppN1 = ppuv[up + sj];
-    const double gkC1 = ppgk_of(wgtfac[up], ppC1, ppC);
-    const double gkE1 = ppgk_of(wgtfac[up + 1], ppE1, ppE);
-    const double gkN1 = ppgk_of(wgtfac[up + sj], ppN1, ppN);
-    const double hC1 = hhl[up], hE1 = hhl[up + 1], hN1 = hhl[up + sj];
-    const double rC = rho[at], rE = rho[at + 1], rN = rho[at + sj];
+    const double ppC1 = a.ppuv[up], ppE1 = a.ppuv[up + 1], ppN1 = a.ppuv[up + sj];
+    gk.C1 = ppgk_of(a.wgtfac[up], ppC1, ppC);
+    gk.E1 = ppgk_of(a.wgtfac[up + 1], ppE1, ppE);
+    gk.N1 = ppgk_of(a.wgtfac[up + sj], ppN1, ppN);
+    h.C1 = a.hhl[up], h.E1 = a.hhl[up + 1], h.N1 = a.hhl[up + sj];
+    double uo, vo;
+    fw_point(ppC, ppE, ppN, gk, h, a.rho[at], a.rho[at + 1], a.rho[at + sj], ld_nt(a.uin + at), ld_nt(a.vin + at),
+             ld_nt(a.utens + at), ld_nt(a.vtens + at), fxj, edadlat, dt, uo, vo);
+    st_nt(a.uout + at, uo);
+    st_nt(a.vout + at, vo);
+    ppC = ppC1, ppE = ppE1, ppN = ppN1;
+    gk.C = gk.C1, gk.E = gk.E1, gk.N = gk.N1;
+    h.C = h.C1, h.E = h.E1, h.N = h.N1;
+  }
+}
 
-    const double gcC = gkC1 - gkC, gcE = gkE1 - gkE, gcN = gkN1 - gkN;
-    const double ppgu = (ppE - ppC) + (gcE + gcC) * 0.5 * ((hC1 + hC) - (hE1 + hE)) / ((hC1 - hC) + (hE1 - hE));
-    const double ppgv = (ppN - ppC) + (gcN + gcC) * 0.5 * ((hC1 + hC) - (hN1 + hN)) / ((hC1 - hC) + (hN1 - hN));
-    st_nt(uout + at, ld_nt(uin + at) + dt * (ld_nt(utens + at) - ppgu * (2.0 * fxj / (rE + rC))));
-    st_nt(vout + at, ld_nt(vin + at) + dt * (ld_nt(vtens + at) - ppgv * (2.0 * edadlat / (rN + rC))));
-
-    ppC = ppC1;
-    ppE = ppE1;
-    ppN = ppN1;
-    gkC = gkC1;
-    gkE = gkE1;
-    gkN = gkN1;
-    hC = hC1;
-    hE = hE1;
-    hN = hN1;
+// Two adjacent columns (i, i+1) per lane, i even, 16-byte accesses.  The east
+// neighbour of column i is column i+1 of the pair; that of column i+1 is a
+// single load at i+2.  Suffix P: the pair at (i, i+1); Q: the pair at row j+1;
+// X: column i+2 at row j.
+__global__ void __launch_bounds__(kTileI *kTileJ)
+    fastwavesuv_pair_kernel(FwArgs a, double edadlat, double dt, Geo g, Tiles t) {
+  const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
+  const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
+  if (i >= g.nx || j >= g.ny) return;
+  const int sj = g.sj;
+  const int64_t sk = g.sk;
+  const double fxj = a.fx[int64_t(j) * a.fx_stride];
+  int64_t at = int64_t(c.k0) * sk + (i + j * sj);
+  const int64_t dn = at - sk;
+  d2 ppP = ld2(a.ppuv + at), ppQ = ld2(a.ppuv + at + sj);
+  double ppX = a.ppuv[at + 2];
+  d2 gkP, gkQ;
+  double gkX;
+  {
+    const d2 wP = ld2(a.wgtfac + at), wQ = ld2(a.wgtfac + at + sj);
+    const d2 mP = ld2(a.ppuv + dn), mQ = ld2(a.ppuv + dn + sj);
+    gkP = d2{ppgk_of(wP.x, ppP.x, mP.x), ppgk_of(wP.y, ppP.y, mP.y)};
+    gkQ = d2{ppgk_of(wQ.x, ppQ.x, mQ.x), ppgk_of(wQ.y, ppQ.y, mQ.y)};
+    gkX = ppgk_of(a.wgtfac[at + 2], ppX, a.ppuv[dn + 2]);
+  }
+  d2 hP = ld2(a.hhl + at), hQ = ld2(a.hhl + at + sj);
+  double hX = a.hhl[at + 2];
+  for (int k = c.k0; k < c.k1; ++k, at += sk) {
+    const int64_t up = at + sk;
+    const d2 ppP1 = ld2(a.ppuv + up), ppQ1 = ld2(a.ppuv + up + sj);
+    const double ppX1 = a.ppuv[up + 2];
+    const d2 wP = ld2(a.wgtfac + up), wQ = ld2(a.wgtfac + up + sj);
+    const double wX = a.wgtfac[up + 2];
+    const d2 gkP1{ppgk_of(wP.x, ppP1.x, ppP.x), ppgk_of(wP.y, ppP1.y, ppP.y)};
+    const d2 gkQ1{ppgk_of(wQ.x, ppQ1.x, ppQ.x), ppgk_of(wQ.y, ppQ1.y, ppQ.y)};
+    const double gkX1 = ppgk_of(wX, ppX1, ppX);
+    const d2 hP1 = ld2(a.hhl + up), hQ1 = ld2(a.hhl + up + sj);
+    const double hX1 = a.hhl[up + 2];
+    const d2 rP = ld2(a.rho + at), rQ = ld2(a.rho + at + sj);
+    const double rX = a.rho[at + 2];
+    const d2 ui = ld2_nt(a.uin + at), vi = ld2_nt(a.vin + at);
+    const d2 ut = ld2_nt(a.utens + at), vt = ld2_nt(a.vtens + at);
+    double u0, v0, u1, v1;
+    fw_point(ppP.x, ppP.y, ppQ.x, CEN{gkP.x, gkP.y, gkQ.x, gkP1.x, gkP1.y, gkQ1.x},
+             CEN{hP.x, hP.y, hQ.x, hP1.x, hP1.y, hQ1.x}, rP.x, rP.y, rQ.x, ui.x, vi.x, ut.x, vt.x, fxj, edadlat, dt,
+             u0, v0);
+    fw_point(ppP.y, ppX, ppQ.y, CEN{gkP.y, gkX, gkQ.y, gkP1.y, gkX1, gkQ1.y}, CEN{hP.y, hX, hQ.y, hP1.y, hX1, hQ1.y},
+             rP.y, rX, rQ.y, ui.y, vi.y, ut.y, vt.y, fxj, edadlat, dt, u1, v1);
+    st2_nt(a.uout + at, u0, u1);
+    st2_nt(a.vout + at, v0, v1);
+    ppP = ppP1, ppQ = ppQ1, ppX = ppX1;
+    gkP = gkP1, gkQ = gkQ1, gkX = gkX1;
+    hP = hP1, hQ = hQ1, hX = hX1;
   }
 }
 
@@ -84,36 +154,47 @@ extern "C" int oeb_fastwavesuv_f64(const oeb_domain *d, const oeb_s3 *uout, cons
   const Box r0{0, nx, 0, ny, 0, nz};
   const Box r_h{0, nx + 1, 0, ny + 1, 0, nz};
   const Box r_w{0, nx + 1, 0, ny + 1, 0, nz + 1}, r_p{0, nx + 1, 0, ny + 1, -1, nz + 1};
-  const double *p_uo, *p_vo, *p_ui, *p_vi, *p_ut, *p_vt, *p_wg, *p_pp, *p_hh, *p_rh, *p_fx;
-  int fx_stride = 1;
+  const double *p_uo, *p_vo;
+  FwArgs a;
+  a.fx_stride = 1;
   OEB_TRY(check3(K, "uout", uout, r0, g, true, &p_uo));
   OEB_TRY(check3(K, "vout", vout, r0, g, false, &p_vo));
-  OEB_TRY(check3(K, "uin", uin, r0, g, false, &p_ui));
-  OEB_TRY(check3(K, "vin", vin, r0, g, false, &p_vi));
-  OEB_TRY(check3(K, "utens", utens, r0, g, false, &p_ut));
-  OEB_TRY(check3(K, "vtens", vtens, r0, g, false, &p_vt));
-  OEB_TRY(check3(K, "wgtfac", wgtfac, r_w, g, false, &p_wg));
-  OEB_TRY(check3(K, "ppuv", ppuv, r_p, g, false, &p_pp));
-  OEB_TRY(check3(K, "hhl", hhl, r_w, g, false, &p_hh));
-  OEB_TRY(check3(K, "rho", rho, r_h, g, false, &p_rh));
-  OEB_TRY(check1(K, "fx", fx, 0, ny, &p_fx, &fx_stride));
+  OEB_TRY(check3(K, "uin", uin, r0, g, false, &a.uin));
+  OEB_TRY(check3(K, "vin", vin, r0, g, false, &a.vin));
+  OEB_TRY(check3(K, "utens", utens, r0, g, false, &a.utens));
+  OEB_TRY(check3(K, "vtens", vtens, r0, g, false, &a.vtens));
+  OEB_TRY(check3(K, "wgtfac", wgtfac, r_w, g, false, &a.wgtfac));
+  OEB_TRY(check3(K, "ppuv", ppuv, r_p, g, false, &a.ppuv));
+  OEB_TRY(check3(K, "hhl", hhl, r_w, g, false, &a.hhl));
+  OEB_TRY(check3(K, "rho", rho, r_h, g, false, &a.rho));
+  OEB_TRY(check1(K, "fx", fx, 0, ny, &a.fx, &a.fx_stride));
   AliasSet as;
   as.add("uout", p_uo, r0, g, true);
   as.add("vout", p_vo, r0, g, true);
-  as.add("uin", p_ui, r0, g, false);
-  as.add("vin", p_vi, r0, g, false);
-  as.add("utens", p_ut, r0, g, false);
-  as.add("vtens", p_vt, r0, g, false);
-  as.add("wgtfac", p_wg, r_w, g, false);
-  as.add("ppuv", p_pp, r_p, g, false);
-  as.add("hhl", p_hh, r_w, g, false);
-  as.add("rho", p_rh, r_h, g, false);
-  as.add1("fx", p_fx, 0, ny, fx_stride);
+  as.add("uin", a.uin, r0, g, false);
+  as.add("vin", a.vin, r0, g, false);
+  as.add("utens", a.utens, r0, g, false);
+  as.add("vtens", a.vtens, r0, g, false);
+  as.add("wgtfac", a.wgtfac, r_w, g, false);
+  as.add("ppuv", a.ppuv, r_p, g, false);
+  as.add("hhl", a.hhl, r_w, g, false);
+  as.add("rho", a.rho, r_h, g, false);
+  as.add1("fx", a.fx, 0, ny, a.fx_stride);
   OEB_TRY(as.check(K));
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kChunk);
-  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-  hipLaunchKernelGGL(fastwavesuv_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
-                     const_cast<double *>(p_uo), const_cast<double *>(p_vo), p_ui, p_vi, p_ut, p_vt, p_wg, p_pp, p_hh,
-                     p_rh, p_fx, fx_stride, edadlat, dt, g, t);
+  a.uout = const_cast<double *>(p_uo);
+  a.vout = const_cast<double *>(p_vo);
+  const int kc = kchunk_for(K, kChunk);
+  if (nx % 2 == 0 &&
+      pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.utens, a.vtens, a.wgtfac, a.ppuv, a.hhl, a.rho})) {
+    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+    hipLaunchKernelGGL(fastwavesuv_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, edadlat,
+                       dt, g, t);
+  } else {
+    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+    hipLaunchKernelGGL(fastwavesuv_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, edadlat, dt,
+                       g, t);
+  }
   return launch_status(K);
 }

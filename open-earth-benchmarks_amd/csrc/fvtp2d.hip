@@ -26,9 +26,10 @@ namespace oeb {
 namespace {
 
 constexpr int kTI = 64;   // tile width along i (one wavefront)
-constexpr int kTJ = 8;    // tile height along j
 constexpr int kNT = 256;  // threads per workgroup
-constexpr int kChunk = 4;  // levels per workgroup
+// Tile height along j and levels per workgroup of the launch.
+constexpr int kTJdefault = 8;
+constexpr int kChunk = 4;
 
 // PPM flux at face m between cells m-1 and m along one axis, given
 // f(m-3) .. f(m+2) and the Courant number c at m (kernels/fvtp2d.h:12, 20-34).
@@ -54,14 +55,14 @@ struct FvOut {
   double *q_i, *q_j, *fx1, *fx2, *fy1, *fy2;
 };
 
-// LDS tile extents (doubles).
-constexpr int kQW = kTI + 6, kQH = kTJ + 6;    // q:   i [i0-3, i0+TI+3) x j [j0-3, j0+TJ+3)
-constexpr int kYW = kTI + 5, kYH = kTJ + 1;    // fy2: i [i0-3, i0+TI+2) x j [j0, j0+TJ+1)
-constexpr int kXW = kTI + 1, kXH = kTJ + 5;    // fx2: i [i0, i0+TI+1)   x j [j0-3, j0+TJ+2)
-constexpr int kIW = kTI + 5, kIH = kTJ;        // q_i: i [i0-3, i0+TI+2) x j [j0, j0+TJ)
-constexpr int kJW = kTI, kJH = kTJ + 5;        // q_j: i [i0, i0+TI)     x j [j0-3, j0+TJ+2)
-
+template <int kTJ>
 __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn a, FvOut o, double p1, double p2, Geo g, Tiles t) {
+  // LDS tile extents (doubles).
+  constexpr int kQW = kTI + 6, kQH = kTJ + 6;  // q:   i [i0-3, i0+TI+3) x j [j0-3, j0+TJ+3)
+  constexpr int kYW = kTI + 5, kYH = kTJ + 1;  // fy2: i [i0-3, i0+TI+2) x j [j0, j0+TJ+1)
+  constexpr int kXW = kTI + 1, kXH = kTJ + 5;  // fx2: i [i0, i0+TI+1)   x j [j0-3, j0+TJ+2)
+  constexpr int kIW = kTI + 5, kIH = kTJ;      // q_i: i [i0-3, i0+TI+2) x j [j0, j0+TJ)
+  constexpr int kJW = kTI, kJH = kTJ + 5;      // q_j: i [i0, i0+TI)     x j [j0-3, j0+TJ+2)
   __shared__ double sq[kQH][kQW];
   __shared__ double sfy2[kYH][kYW];
   __shared__ double sfx2[kXH][kXW];
@@ -224,8 +225,8 @@ extern "C" int oeb_fvtp2d_f64(const oeb_domain *d, const oeb_s3 *q_i, const oeb_
   o.fy1 = const_cast<double *>(pfy1);
   o.fy2 = const_cast<double *>(pfy2);
 
-  const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJ, kChunk);
-  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, kTJ));
-  hipLaunchKernelGGL(fvtp2d_kernel, dim3(t.n), dim3(kNT), 0, hipStream_t(stream), a, o, p1, p2, g, t);
+  const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJdefault, kchunk_for(K, kChunk));
+  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, kTJdefault));
+  hipLaunchKernelGGL(fvtp2d_kernel<kTJdefault>, dim3(t.n), dim3(kNT), 0, hipStream_t(stream), a, o, p1, p2, g, t);
   return launch_status(K);
 }

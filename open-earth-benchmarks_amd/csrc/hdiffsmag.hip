@@ -106,7 +106,7 @@ extern "C" int oeb_hdiffsmag_f64(const oeb_domain *d, const oeb_s3 *uout, const 
   as.add1("crlatu", p_tu1, 0, ny, s_tu);
   as.add1("acrlat0", p_ac, -1, ny + 1, s_ac);
   OEB_TRY(as.check(K));
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kChunk);
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kchunk_for(K, kChunk));
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
   hipLaunchKernelGGL(hdiffsmag_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
                      const_cast<double *>(p_uo), const_cast<double *>(p_vo), p_ui, p_vi, p_m, p_vo1, s_vo, p_vu1, s_vu,

@@ -11,48 +11,100 @@ namespace oeb {
 namespace {
 
 constexpr int kTileI = 64, kTileJ = 4, kChunk = 16;
+constexpr int kPairI = 2 * kTileI;
 
-__global__ void __launch_bounds__(kTileI *kTileJ)
-    nh_p_grad_kernel(double *__restrict__ uout, double *__restrict__ vout, const double *__restrict__ uin,
-                     const double *__restrict__ vin, const double *__restrict__ rdx, const double *__restrict__ rdy,
-                     const double *__restrict__ gz, const double *__restrict__ pp, const double *__restrict__ pk3,
-                     const double *__restrict__ wk1, double dt, Geo g, Tiles t) {
+// Level-k and level-(k+1) values of one field at the centre C = (i,j), the
+// east point E = (i+1,j) and the north point N = (i,j+1).
+struct CEN {
+  double C, E, N, C1, E1, N1;
+};
+
+// One point (kernels/nh_p_grad.h:123-142).
+__device__ __forceinline__ void nh_point(const CEN &gz, const CEN &pk, const CEN &pp, double w1C, double w1E,
+                                         double w1N, double uin, double vin, double rdx, double rdy, double dt,
+                                         double &uo, double &vo) {
+  const double wk0 = (pk.C1 - pk.C), wkE = (pk.E1 - pk.E), wkN = (pk.N1 - pk.N);
+  const double du = ((dt / (wk0 + wkE)) * (((gz.C1 - gz.E) * (pk.E1 - pk.C)) + ((gz.C - gz.E1) * (pk.C1 - pk.E))));
+  uo = (((uin + du) + ((dt / (w1C + w1E)) * (((gz.C1 - gz.E) * (pp.E1 - pp.C)) + ((gz.C - gz.E1) * (pp.C1 - pp.E))))) *
+        rdx);
+  const double dv = ((dt / (wk0 + wkN)) * (((gz.C1 - gz.N) * (pk.N1 - pk.C)) + ((gz.C - gz.N1) * (pk.C1 - pk.N))));
+  vo = (((vin + dv) + ((dt / (w1C + w1N)) * (((gz.C1 - gz.N) * (pp.N1 - pp.C)) + ((gz.C - gz.N1) * (pp.C1 - pp.N))))) *
+        rdy);
+}
+
+struct NhArgs {
+  double *uout, *vout;
+  const double *uin, *vin, *rdx, *rdy, *gz, *pp, *pk3, *wk1;
+};
+
+__global__ void __launch_bounds__(kTileI *kTileJ) nh_p_grad_kernel(NhArgs a, double dt, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
   const int64_t sk = g.sk;
   int64_t at = int64_t(c.k0) * sk + (i + j * sj);
-  // Level k at the centre C, east E = (i+1) and north N = (j+1).
-  double gzC = gz[at], gzE = gz[at + 1], gzN = gz[at + sj];
-  double pkC = pk3[at], pkE = pk3[at + 1], pkN = pk3[at + sj];
-  double ppC = pp[at], ppE = pp[at + 1], ppN = pp[at + sj];
+  CEN gz, pk, pp;
+  gz.C = a.gz[at], gz.E = a.gz[at + 1], gz.N = a.gz[at + sj];
+  pk.C = a.pk3[at], pk.E = a.pk3[at + 1], pk.N = a.pk3[at + sj];
+  pp.C = a.pp[at], pp.E = a.pp[at + 1], pp.N = a.pp[at + sj];
   for (int k = c.k0; k < c.k1; ++k, at += sk) {
     const int64_t up = at + sk;
-    const double gzC1 = gz[up], gzE1 = gz[up + 1], gzN1 = gz[up + sj];
-    const double pkC1 = pk3[up], pkE1 = pk3[up + 1], pkN1 = pk3[up + sj];
-    const double ppC1 = pp[up], ppE1 = pp[up + 1], ppN1 = pp[up + sj];
-    const double w1C = wk1[at], w1E = wk1[at + 1], w1N = wk1[at + sj];
-    const double wk0 = (pkC1 - pkC), wkE = (pkE1 - pkE), wkN = (pkN1 - pkN);
+    gz.C1 = a.gz[up], gz.E1 = a.gz[up + 1], gz.N1 = a.gz[up + sj];
+    pk.C1 = a.pk3[up], pk.E1 = a.pk3[up + 1], pk.N1 = a.pk3[up + sj];
+    pp.C1 = a.pp[up], pp.E1 = a.pp[up + 1], pp.N1 = a.pp[up + sj];
+    double uo, vo;
+    nh_point(gz, pk, pp, a.wk1[at], a.wk1[at + 1], a.wk1[at + sj], ld_nt(a.uin + at), ld_nt(a.vin + at),
+             ld_nt(a.rdx + at), ld_nt(a.rdy + at), dt, uo, vo);
+    st_nt(a.uout + at, uo);
+    st_nt(a.vout + at, vo);
+    gz.C = gz.C1, gz.E = gz.E1, gz.N = gz.N1;
+    pk.C = pk.C1, pk.E = pk.E1, pk.N = pk.N1;
+    pp.C = pp.C1, pp.E = pp.E1, pp.N = pp.N1;
+  }
+}
 
-    const double du = ((dt / (wk0 + wkE)) * (((gzC1 - gzE) * (pkE1 - pkC)) + ((gzC - gzE1) * (pkC1 - pkE))));
-    st_nt(uout + at, (((ld_nt(uin + at) + du) +
-                       ((dt / (w1C + w1E)) * (((gzC1 - gzE) * (ppE1 - ppC)) + ((gzC - gzE1) * (ppC1 - ppE))))) *
-                      ld_nt(rdx + at)));
-    const double dv = ((dt / (wk0 + wkN)) * (((gzC1 - gzN) * (pkN1 - pkC)) + ((gzC - gzN1) * (pkC1 - pkN))));
-    st_nt(vout + at, (((ld_nt(vin + at) + dv) +
-                       ((dt / (w1C + w1N)) * (((gzC1 - gzN) * (ppN1 - ppC)) + ((gzC - gzN1) * (ppC1 - ppN))))) *
-                      ld_nt(rdy + at)));
+// Two adjacent columns (i, i+1) per lane, i even, 16-byte accesses.  The east
+// neighbour of column i is column i+1 of the pair; that of column i+1 is a
+// single load at i+2.
+struct PairCol {
+  d2 C, N, C1, N1;   // pair at (i, i+1), rows j and j+1, levels k and k+1
+  double E, E1;      // column i+2, row j, levels k and k+1
+  __device__ CEN lo() const { return CEN{C.x, C.y, N.x, C1.x, C1.y, N1.x}; }
+  __device__ CEN hi() const { return CEN{C.y, E, N.y, C1.y, E1, N1.y}; }
+  __device__ void load_up(const double *f, int64_t up, int sj) {
+    C1 = ld2(f + up), N1 = ld2(f + up + sj), E1 = f[up + 2];
+  }
+  __device__ void roll() { C = C1, N = N1, E = E1; }
+};
 
-    gzC = gzC1;
-    gzE = gzE1;
-    gzN = gzN1;
-    pkC = pkC1;
-    pkE = pkE1;
-    pkN = pkN1;
-    ppC = ppC1;
-    ppE = ppE1;
-    ppN = ppN1;
+__global__ void __launch_bounds__(kTileI *kTileJ) nh_p_grad_pair_kernel(NhArgs a, double dt, Geo g, Tiles t) {
+  const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
+  const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
+  if (i >= g.nx || j >= g.ny) return;
+  const int sj = g.sj;
+  const int64_t sk = g.sk;
+  int64_t at = int64_t(c.k0) * sk + (i + j * sj);
+  PairCol gz, pk, pp;
+  gz.C = ld2(a.gz + at), gz.N = ld2(a.gz + at + sj), gz.E = a.gz[at + 2];
+  pk.C = ld2(a.pk3 + at), pk.N = ld2(a.pk3 + at + sj), pk.E = a.pk3[at + 2];
+  pp.C = ld2(a.pp + at), pp.N = ld2(a.pp + at + sj), pp.E = a.pp[at + 2];
+  for (int k = c.k0; k < c.k1; ++k, at += sk) {
+    const int64_t up = at + sk;
+    gz.load_up(a.gz, up, sj);
+    pk.load_up(a.pk3, up, sj);
+    pp.load_up(a.pp, up, sj);
+    const d2 w1C = ld2(a.wk1 + at), w1N = ld2(a.wk1 + at + sj);
+    const double w1E = a.wk1[at + 2];
+    const d2 ui = ld2_nt(a.uin + at), vi = ld2_nt(a.vin + at), rx = ld2_nt(a.rdx + at), ry = ld2_nt(a.rdy + at);
+    double u0, v0, u1, v1;
+    nh_point(gz.lo(), pk.lo(), pp.lo(), w1C.x, w1C.y, w1N.x, ui.x, vi.x, rx.x, ry.x, dt, u0, v0);
+    nh_point(gz.hi(), pk.hi(), pp.hi(), w1C.y, w1E, w1N.y, ui.y, vi.y, rx.y, ry.y, dt, u1, v1);
+    st2_nt(a.uout + at, u0, u1);
+    st2_nt(a.vout + at, v0, v1);
+    gz.roll();
+    pk.roll();
+    pp.roll();
   }
 }
 
@@ -73,33 +125,41 @@ extern "C" int oeb_nh_p_grad_f64(const oeb_domain *d, const oeb_s3 *uout, const 
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz};
   const Box r_h{0, nx + 1, 0, ny + 1, 0, nz}, r_k{0, nx + 1, 0, ny + 1, 0, nz + 1};
-  const double *p_uo, *p_vo, *p_ui, *p_vi, *p_rx, *p_ry, *p_gz, *p_pp, *p_pk, *p_w1;
+  const double *p_uo, *p_vo;
+  NhArgs a;
   OEB_TRY(check3(K, "uout", uout, r0, g, true, &p_uo));
   OEB_TRY(check3(K, "vout", vout, r0, g, false, &p_vo));
-  OEB_TRY(check3(K, "uin", uin, r0, g, false, &p_ui));
-  OEB_TRY(check3(K, "vin", vin, r0, g, false, &p_vi));
-  OEB_TRY(check3(K, "rdx", rdx, r0, g, false, &p_rx));
-  OEB_TRY(check3(K, "rdy", rdy, r0, g, false, &p_ry));
-  OEB_TRY(check3(K, "gz", gz, r_k, g, false, &p_gz));
-  OEB_TRY(check3(K, "pp", pp, r_k, g, false, &p_pp));
-  OEB_TRY(check3(K, "pk3", pk3, r_k, g, false, &p_pk));
-  OEB_TRY(check3(K, "wk1", wk1, r_h, g, false, &p_w1));
+  OEB_TRY(check3(K, "uin", uin, r0, g, false, &a.uin));
+  OEB_TRY(check3(K, "vin", vin, r0, g, false, &a.vin));
+  OEB_TRY(check3(K, "rdx", rdx, r0, g, false, &a.rdx));
+  OEB_TRY(check3(K, "rdy", rdy, r0, g, false, &a.rdy));
+  OEB_TRY(check3(K, "gz", gz, r_k, g, false, &a.gz));
+  OEB_TRY(check3(K, "pp", pp, r_k, g, false, &a.pp));
+  OEB_TRY(check3(K, "pk3", pk3, r_k, g, false, &a.pk3));
+  OEB_TRY(check3(K, "wk1", wk1, r_h, g, false, &a.wk1));
   AliasSet as;
   as.add("uout", p_uo, r0, g, true);
   as.add("vout", p_vo, r0, g, true);
-  as.add("uin", p_ui, r0, g, false);
-  as.add("vin", p_vi, r0, g, false);
-  as.add("rdx", p_rx, r0, g, false);
-  as.add("rdy", p_ry, r0, g, false);
-  as.add("gz", p_gz, r_k, g, false);
-  as.add("pp", p_pp, r_k, g, false);
-  as.add("pk3", p_pk, r_k, g, false);
-  as.add("wk1", p_w1, r_h, g, false);
+  as.add("uin", a.uin, r0, g, false);
+  as.add("vin", a.vin, r0, g, false);
+  as.add("rdx", a.rdx, r0, g, false);
+  as.add("rdy", a.rdy, r0, g, false);
+  as.add("gz", a.gz, r_k, g, false);
+  as.add("pp", a.pp, r_k, g, false);
+  as.add("pk3", a.pk3, r_k, g, false);
+  as.add("wk1", a.wk1, r_h, g, false);
   OEB_TRY(as.check(K));
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kChunk);
-  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-  hipLaunchKernelGGL(nh_p_grad_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
-                     const_cast<double *>(p_uo), const_cast<double *>(p_vo), p_ui, p_vi, p_rx, p_ry, p_gz, p_pp, p_pk,
-                     p_w1, dt, g, t);
+  a.uout = const_cast<double *>(p_uo);
+  a.vout = const_cast<double *>(p_vo);
+  const int kc = kchunk_for(K, kChunk);
+  if (nx % 2 == 0 && pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdx, a.rdy, a.gz, a.pp, a.pk3, a.wk1})) {
+    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+    hipLaunchKernelGGL(nh_p_grad_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt, g, t);
+  } else {
+    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+    hipLaunchKernelGGL(nh_p_grad_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt, g, t);
+  }
   return launch_status(K);
 }

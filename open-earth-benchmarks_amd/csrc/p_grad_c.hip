@@ -12,36 +12,78 @@ namespace oeb {
 namespace {
 
 constexpr int kTileI = 64, kTileJ = 4, kChunk = 16;
+constexpr int kPairI = 2 * kTileI;
 
-__global__ void __launch_bounds__(kTileI *kTileJ)
-    p_grad_c_kernel(double *__restrict__ uout, double *__restrict__ vout, const double *__restrict__ uin,
-                    const double *__restrict__ vin, const double *__restrict__ rdxc, const double *__restrict__ rdyc,
-                    const double *__restrict__ delpc, const double *__restrict__ gz, const double *__restrict__ pkc,
-                    double dt2, Geo g, Tiles t) {
+// One point (kernels/p_grad_c.h:87-99).  C = (i,j), W = (i-1,j), S = (i,j-1);
+// suffix 1 = level k+1.
+__device__ __forceinline__ void pgc_point(double gzC, double gzW, double gzS, double gzC1, double gzW1, double gzS1,
+                                          double pkC, double pkW, double pkS, double pkC1, double pkW1, double pkS1,
+                                          double wk0, double wkW, double wkS, double uin, double vin, double rdxc,
+                                          double rdyc, double dt2, double &uo, double &vo) {
+  uo = (uin + (((dt2 * rdxc) / (wkW + wk0)) * (((gzW1 - gzC) * (pkC1 - pkW)) + ((gzW - gzC1) * (pkW1 - pkC)))));
+  vo = (vin + (((dt2 * rdyc) / (wkS + wk0)) * (((gzS1 - gzC) * (pkC1 - pkS)) + ((gzS - gzC1) * (pkS1 - pkC)))));
+}
+
+struct PgcArgs {
+  double *uout, *vout;
+  const double *uin, *vin, *rdxc, *rdyc, *delpc, *gz, *pkc;
+};
+
+__global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_kernel(PgcArgs a, double dt2, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
   const int64_t sk = g.sk;
   int64_t at = int64_t(c.k0) * sk + (i + j * sj);
-  // Level k at the centre C, west W = (i-1) and south S = (j-1).
-  double gzC = gz[at], gzW = gz[at - 1], gzS = gz[at - sj];
-  double pkC = pkc[at], pkW = pkc[at - 1], pkS = pkc[at - sj];
+  double gzC = a.gz[at], gzW = a.gz[at - 1], gzS = a.gz[at - sj];
+  double pkC = a.pkc[at], pkW = a.pkc[at - 1], pkS = a.pkc[at - sj];
   for (int k = c.k0; k < c.k1; ++k, at += sk) {
     const int64_t up = at + sk;
-    const double gzC1 = gz[up], gzW1 = gz[up - 1], gzS1 = gz[up - sj];
-    const double pkC1 = pkc[up], pkW1 = pkc[up - 1], pkS1 = pkc[up - sj];
-    const double wk0 = delpc[at], wkW = delpc[at - 1], wkS = delpc[at - sj];
-    st_nt(uout + at, (ld_nt(uin + at) + (((dt2 * ld_nt(rdxc + at)) / (wkW + wk0)) *
-                                         (((gzW1 - gzC) * (pkC1 - pkW)) + ((gzW - gzC1) * (pkW1 - pkC))))));
-    st_nt(vout + at, (ld_nt(vin + at) + (((dt2 * ld_nt(rdyc + at)) / (wkS + wk0)) *
-                                         (((gzS1 - gzC) * (pkC1 - pkS)) + ((gzS - gzC1) * (pkS1 - pkC))))));
-    gzC = gzC1;
-    gzW = gzW1;
-    gzS = gzS1;
-    pkC = pkC1;
-    pkW = pkW1;
-    pkS = pkS1;
+    const double gzC1 = a.gz[up], gzW1 = a.gz[up - 1], gzS1 = a.gz[up - sj];
+    const double pkC1 = a.pkc[up], pkW1 = a.pkc[up - 1], pkS1 = a.pkc[up - sj];
+    double uo, vo;
+    pgc_point(gzC, gzW, gzS, gzC1, gzW1, gzS1, pkC, pkW, pkS, pkC1, pkW1, pkS1, a.delpc[at], a.delpc[at - 1],
+              a.delpc[at - sj], ld_nt(a.uin + at), ld_nt(a.vin + at), ld_nt(a.rdxc + at), ld_nt(a.rdyc + at), dt2,
+              uo, vo);
+    st_nt(a.uout + at, uo);
+    st_nt(a.vout + at, vo);
+    gzC = gzC1, gzW = gzW1, gzS = gzS1;
+    pkC = pkC1, pkW = pkW1, pkS = pkS1;
+  }
+}
+
+// Two adjacent columns (i, i+1) per lane, i even, 16-byte accesses.  For
+// column i+1 the west neighbour is column i of the pair.
+__global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_pair_kernel(PgcArgs a, double dt2, Geo g, Tiles t) {
+  const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
+  const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
+  if (i >= g.nx || j >= g.ny) return;
+  const int sj = g.sj;
+  const int64_t sk = g.sk;
+  int64_t at = int64_t(c.k0) * sk + (i + j * sj);
+  d2 gzC = ld2(a.gz + at), gzS = ld2(a.gz + at - sj);
+  double gzW = a.gz[at - 1];
+  d2 pkC = ld2(a.pkc + at), pkS = ld2(a.pkc + at - sj);
+  double pkW = a.pkc[at - 1];
+  for (int k = c.k0; k < c.k1; ++k, at += sk) {
+    const int64_t up = at + sk;
+    const d2 gzC1 = ld2(a.gz + up), gzS1 = ld2(a.gz + up - sj);
+    const double gzW1 = a.gz[up - 1];
+    const d2 pkC1 = ld2(a.pkc + up), pkS1 = ld2(a.pkc + up - sj);
+    const double pkW1 = a.pkc[up - 1];
+    const d2 dpC = ld2(a.delpc + at), dpS = ld2(a.delpc + at - sj);
+    const double dpW = a.delpc[at - 1];
+    const d2 ui = ld2_nt(a.uin + at), vi = ld2_nt(a.vin + at), rx = ld2_nt(a.rdxc + at), ry = ld2_nt(a.rdyc + at);
+    double u0, v0, u1, v1;
+    pgc_point(gzC.x, gzW, gzS.x, gzC1.x, gzW1, gzS1.x, pkC.x, pkW, pkS.x, pkC1.x, pkW1, pkS1.x, dpC.x, dpW, dpS.x,
+              ui.x, vi.x, rx.x, ry.x, dt2, u0, v0);
+    pgc_point(gzC.y, gzC.x, gzS.y, gzC1.y, gzC1.x, gzS1.y, pkC.y, pkC.x, pkS.y, pkC1.y, pkC1.x, pkS1.y, dpC.y, dpC.x,
+              dpS.y, ui.y, vi.y, rx.y, ry.y, dt2, u1, v1);
+    st2_nt(a.uout + at, u0, u1);
+    st2_nt(a.vout + at, v0, v1);
+    gzC = gzC1, gzS = gzS1, gzW = gzW1;
+    pkC = pkC1, pkS = pkS1, pkW = pkW1;
   }
 }
 
@@ -59,31 +101,39 @@ extern "C" int oeb_p_grad_c_f64(const oeb_domain *d, const oeb_s3 *uout, const o
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz};
   const Box r_dp{-1, nx, -1, ny, 0, nz}, r_k{-1, nx, -1, ny, 0, nz + 1};
-  const double *p_uo, *p_vo, *p_ui, *p_vi, *p_rx, *p_ry, *p_dp, *p_gz, *p_pk;
+  const double *p_uo, *p_vo;
+  PgcArgs a;
   OEB_TRY(check3(K, "uout", uout, r0, g, true, &p_uo));
   OEB_TRY(check3(K, "vout", vout, r0, g, false, &p_vo));
-  OEB_TRY(check3(K, "uin", uin, r0, g, false, &p_ui));
-  OEB_TRY(check3(K, "vin", vin, r0, g, false, &p_vi));
-  OEB_TRY(check3(K, "rdxc", rdxc, r0, g, false, &p_rx));
-  OEB_TRY(check3(K, "rdyc", rdyc, r0, g, false, &p_ry));
-  OEB_TRY(check3(K, "delpc", delpc, r_dp, g, false, &p_dp));
-  OEB_TRY(check3(K, "gz", gz, r_k, g, false, &p_gz));
-  OEB_TRY(check3(K, "pkc", pkc, r_k, g, false, &p_pk));
+  OEB_TRY(check3(K, "uin", uin, r0, g, false, &a.uin));
+  OEB_TRY(check3(K, "vin", vin, r0, g, false, &a.vin));
+  OEB_TRY(check3(K, "rdxc", rdxc, r0, g, false, &a.rdxc));
+  OEB_TRY(check3(K, "rdyc", rdyc, r0, g, false, &a.rdyc));
+  OEB_TRY(check3(K, "delpc", delpc, r_dp, g, false, &a.delpc));
+  OEB_TRY(check3(K, "gz", gz, r_k, g, false, &a.gz));
+  OEB_TRY(check3(K, "pkc", pkc, r_k, g, false, &a.pkc));
   AliasSet as;
   as.add("uout", p_uo, r0, g, true);
   as.add("vout", p_vo, r0, g, true);
-  as.add("uin", p_ui, r0, g, false);
-  as.add("vin", p_vi, r0, g, false);
-  as.add("rdxc", p_rx, r0, g, false);
-  as.add("rdyc", p_ry, r0, g, false);
-  as.add("delpc", p_dp, r_dp, g, false);
-  as.add("gz", p_gz, r_k, g, false);
-  as.add("pkc", p_pk, r_k, g, false);
+  as.add("uin", a.uin, r0, g, false);
+  as.add("vin", a.vin, r0, g, false);
+  as.add("rdxc", a.rdxc, r0, g, false);
+  as.add("rdyc", a.rdyc, r0, g, false);
+  as.add("delpc", a.delpc, r_dp, g, false);
+  as.add("gz", a.gz, r_k, g, false);
+  as.add("pkc", a.pkc, r_k, g, false);
   OEB_TRY(as.check(K));
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kChunk);
-  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-  hipLaunchKernelGGL(p_grad_c_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
-                     const_cast<double *>(p_uo), const_cast<double *>(p_vo), p_ui, p_vi, p_rx, p_ry, p_dp, p_gz, p_pk,
-                     dt2, g, t);
+  a.uout = const_cast<double *>(p_uo);
+  a.vout = const_cast<double *>(p_vo);
+  const int kc = kchunk_for(K, kChunk);
+  if (nx % 2 == 0 && pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdxc, a.rdyc, a.delpc, a.gz, a.pkc})) {
+    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+    hipLaunchKernelGGL(p_grad_c_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g, t);
+  } else {
+    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+    hipLaunchKernelGGL(p_grad_c_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g, t);
+  }
   return launch_status(K);
 }

@@ -9,7 +9,7 @@
 namespace oeb {
 namespace {
 
-constexpr int kTileI = 64, kTileJ = 4, kChunk = 8;
+constexpr int kTileI = 64, kTileJ = 4, kChunk = 2;
 
 __global__ void __launch_bounds__(kTileI *kTileJ)
     uvbke_kernel(double *__restrict__ ub, double *__restrict__ vb, const double *__restrict__ uc,
@@ -26,6 +26,30 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
     const double ca = ld_nt(cosa + at), rs = ld_nt(rsina + at);
     st_nt(ub + at, ((dt5 * (usum - (vsum * ca))) * rs));
     st_nt(vb + at, ((dt5 * (vsum - (usum * ca))) * rs));
+  }
+}
+
+// Two adjacent columns (i, i+1) per lane, i even: 16-byte accesses, the
+// arithmetic of uvbke_kernel.
+constexpr int kPairI = 2 * kTileI;
+
+__global__ void __launch_bounds__(kTileI *kTileJ)
+    uvbke_pair_kernel(double *__restrict__ ub, double *__restrict__ vb, const double *__restrict__ uc,
+                      const double *__restrict__ vc, const double *__restrict__ cosa,
+                      const double *__restrict__ rsina, double dt5, Geo g, Tiles t) {
+  const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
+  const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
+  if (i >= g.nx || j >= g.ny) return;
+  const int sj = g.sj;
+  int64_t at = int64_t(c.k0) * g.sk + (i + j * sj);
+  for (int k = c.k0; k < c.k1; ++k, at += g.sk) {
+    const d2 us = ld2(uc + at - sj), u0 = ld2(uc + at), v0 = ld2(vc + at);
+    const double vw = vc[at - 1];
+    const d2 ca = ld2_nt(cosa + at), rs = ld2_nt(rsina + at);
+    const double usum0 = us.x + u0.x, usum1 = us.y + u0.y;
+    const double vsum0 = vw + v0.x, vsum1 = v0.x + v0.y;
+    st2_nt(ub + at, ((dt5 * (usum0 - (vsum0 * ca.x))) * rs.x), ((dt5 * (usum1 - (vsum1 * ca.y))) * rs.y));
+    st2_nt(vb + at, ((dt5 * (vsum0 - (usum0 * ca.x))) * rs.x), ((dt5 * (vsum1 - (usum1 * ca.y))) * rs.y));
   }
 }
 
@@ -56,9 +80,17 @@ extern "C" int oeb_uvbke_f64(const oeb_domain *d, const oeb_s3 *ub, const oeb_s3
   as.add("cosa", p_ca, r0, g, false);
   as.add("rsina", p_rs, r0, g, false);
   OEB_TRY(as.check(K));
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kChunk);
-  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-  hipLaunchKernelGGL(uvbke_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
-                     const_cast<double *>(p_ub), const_cast<double *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5, g, t);
+  const int kc = kchunk_for(K, kChunk);
+  if (nx % 2 == 0 && pairs_ok(g, {p_ub, p_vb, p_uc, p_vc, p_ca, p_rs})) {
+    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+    hipLaunchKernelGGL(uvbke_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
+                       const_cast<double *>(p_ub), const_cast<double *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5, g, t);
+  } else {
+    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+    hipLaunchKernelGGL(uvbke_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
+                       const_cast<double *>(p_ub), const_cast<double *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5, g, t);
+  }
   return launch_status(K);
 }
