@@ -11,11 +11,20 @@ of each stencil (inputs read once, outputs written once, FP64).  Between two
 launches of one stencil the other nine move ~3 GB, so nothing it reads is
 still in the 256 MB MALL when it runs again.
 
+fp32: the same step in the FP32 form (oeb_<name>_f32: FP32 storage and
+arithmetic, FP64 islands in fastwavesuv, p_grad_c and nh_p_grad), on the same
+inputs rounded to float32.  Reported per stencil: time, speed-up over FP64,
+bytes per point of the FP32 form, and the relative max / RMS error of each
+output against the FP64 path's output (which equals the FP64 oracle's bit for
+bit), with the fraction of points whose error exceeds 1e-5 of the output's
+max-norm.
+
 cpu_baseline: the reference's own kernels (OpenMP variants, serial DEFAULT for
 fvtp2d, which has no OpenMP variant), compiled by oracle/Makefile, timed on
 this host over a bounded sample of the same workload.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--nz 80]
+                  [--precision both|f64|f32]
 """
 from __future__ import annotations
 
@@ -46,6 +55,8 @@ def parse_args(argv=None):
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--cpu-budget-s", type=float, default=20.0, help="wall-clock budget of the CPU baseline")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--precision", choices=("both", "f64", "f32"), default="both",
+                   help="forms to time: the FP64 line plus its fp32 block (both), or one form alone")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="also time a plain device copy of this many MiB (HBM counter calibration)")
     return p.parse_args(argv)
@@ -105,6 +116,54 @@ def cpu_baseline(names, hosts, L, budget_s: float):
     }
 
 
+def time_steps(names, dev, run, steps: int, warmup: int):
+    """Runs `warmup` untimed steps, then `steps` timed ones.  Returns the mean
+    step time (ms, from events around the whole timed region) and each
+    stencil's launch durations (ms, from events around each launch, on the
+    stream the launches go to)."""
+    import torch
+
+    def step(ev=None):
+        for name in names:
+            if ev is not None:
+                ev[name][0].record()
+            run(name, dev[name])
+            if ev is not None:
+                ev[name][1].record()
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [{nm: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for nm in names}
+              for _ in range(steps)]
+    begin, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    begin.record()
+    for s in range(steps):
+        step(events[s])
+    end.record()
+    torch.cuda.synchronize()
+    durs = {nm: [events[s][nm][0].elapsed_time(events[s][nm][1]) for s in range(steps)] for nm in names}
+    return begin.elapsed_time(end) / steps, durs
+
+
+def per_stencil(names, durs, pts, bytes_per_pt):
+    out = {}
+    for name in names:
+        avg_ms = sum(durs[name]) / len(durs[name])
+        bpp = bytes_per_pt[name]
+        gbs = bpp * pts / (avg_ms * 1e-3) / 1e9
+        out[name] = {
+            "us": round(avg_ms * 1e3, 3),
+            "us_median": round(statistics.median(durs[name]) * 1e3, 3),
+            "gpts_s": round(pts / (avg_ms * 1e-3) / 1e9, 3),
+            "bytes_per_pt": bpp,
+            "gbs": round(gbs, 1),
+            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+        }
+    return out
+
+
 def main(argv=None):
     args = parse_args(argv)
     if args.gpus != 1:
@@ -116,7 +175,7 @@ def main(argv=None):
     import torch
 
     from oeb import stencils
-    from oeb.generators import generate
+    from oeb.generators import cast, generate
     from oeb.layout import Layout, Layout1D
     from oeb.spec import KERNELS, ORDER
     from oeb.storage import Storage1D, Storage3D
@@ -130,60 +189,34 @@ def main(argv=None):
     L = Layout(n, n, args.nz, h=3, hk=1, align=16)
     L1 = Layout1D(n, 3)
     pts = n * n * args.nz
+    forms = ["f64", "f32"] if args.precision == "both" else [args.precision]
 
-    # Inputs: seeded host arrays, copied once to HBM; every stencil has its own.
+    # Inputs: seeded host arrays, copied once to HBM; every stencil has its own
+    # arrays in each form.
     t0 = time.perf_counter()
-    hosts, dev = {}, {}
+    hosts = {}
+    dev = {p: {} for p in forms}
     for name in names:
         h = generate(name, L, "baseline", args.seed)
         hosts[name] = h
-        d = {}
-        for a in KERNELS[name].args:
-            if a.kind == "scratch":
-                continue
-            d[a.name] = (Storage1D if a.kind == "in1" else Storage3D).to_device(L1 if a.kind == "in1" else L,
-                                                                               h[a.name])
-        dev[name] = d
+        for prec in forms:
+            hp = cast(name, h, prec)
+            d = {}
+            for a in KERNELS[name].args:
+                if a.kind == "scratch":
+                    continue
+                cls, lay = (Storage1D, L1) if a.kind == "in1" else (Storage3D, L)
+                d[a.name] = cls.to_device(lay, hp[a.name])
+            dev[prec][name] = d
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
 
-    def step(ev=None):
-        for name in names:
-            if ev is not None:
-                ev[name][0].record()
-            stencils.run(name, dev[name], domain=(n, n, args.nz))
-            if ev is not None:
-                ev[name][1].record()
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-
-    events = [{nm: (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for nm in names}
-              for _ in range(args.steps)]
-    begin, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    begin.record()
-    for s in range(args.steps):
-        step(events[s])
-    end.record()
-    torch.cuda.synchronize()
-    total_ms = begin.elapsed_time(end)
-
-    per = {}
-    for name in names:
-        durs = [events[s][name][0].elapsed_time(events[s][name][1]) for s in range(args.steps)]
-        avg_ms = sum(durs) / len(durs)
-        bpp = KERNELS[name].bytes_per_point_f64
-        gbs = bpp * pts / (avg_ms * 1e-3) / 1e9
-        per[name] = {
-            "us": round(avg_ms * 1e3, 3),
-            "us_median": round(statistics.median(durs) * 1e3, 3),
-            "gpts_s": round(pts / (avg_ms * 1e-3) / 1e9, 3),
-            "bytes_per_pt": bpp,
-            "gbs": round(gbs, 1),
-            "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-        }
+    run = lambda name, fields: stencils.run(name, fields, domain=(n, n, args.nz))  # noqa: E731
+    timed = {}
+    for prec in forms:
+        step_ms, durs = time_steps(names, dev[prec], run, args.steps, args.warmup)
+        bpp = {nm: KERNELS[nm].bytes_per_point(prec) for nm in names}
+        timed[prec] = (step_ms, per_stencil(names, durs, pts, bpp))
 
     copy_line = None
     if args.copy_mib > 0:
@@ -202,11 +235,13 @@ def main(argv=None):
         copy_line = {"bytes_read": nbytes, "bytes_written": nbytes, "us": round(ms * 1e3, 2),
                      "gbs": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
 
+    main_prec = forms[0]
+    step_ms, per = timed[main_prec]
     dom = max(names, key=lambda nm: per[nm]["us"])
-    bpp = KERNELS[dom].bytes_per_point_f64
+    bpp = KERNELS[dom].bytes_per_point(main_prec)
     achieved = bpp * pts / (per[dom]["us"] * 1e-6) / 1e9
-    step_ms = total_ms / args.steps
     value = len(names) * pts / (step_ms * 1e-3) / 1e9
+    form_text = "plain FP64" if main_prec == "f64" else "FP32 with FP64 islands"
 
     result = {
         "metric": METRIC,
@@ -219,14 +254,14 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": main_prec,
         "data": "synthetic",
         "config": {
-            "workload": f"{len(names)} stencils, plain FP64, forward, {n}x{n}x{args.nz} interior, halo 3, k-halo 1",
+            "workload": f"{len(names)} stencils, {form_text}, forward, {n}x{n}x{args.nz} interior, halo 3, k-halo 1",
             "stencils": names,
             "nx": n, "ny": n, "nz": args.nz,
             "points_per_stencil": pts,
-            "layout": f"i-fastest, row pitch {L.sj} (interior rows 128 B aligned)",
+            "layout": f"i-fastest, row pitch {L.sj} (interior rows 64 B aligned in FP32, 128 B in FP64)",
         },
         "roofline": {
             "kernel": dom,
@@ -235,12 +270,38 @@ def main(argv=None):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": _traffic(n, args.nz, dom),
+            "traffic": _traffic(n, args.nz, dom) if main_prec == "f64" else None,
             "algorithmic_bytes_per_launch": bpp * pts,
         },
         "per_stencil": per,
         "setup_s": round(t_setup, 2),
     }
+
+    if main_prec == "f64" and "f32" in forms:
+        # FP32 outputs of the last timed step against the FP64 outputs of the same inputs.
+        step32, per32 = timed["f32"]
+        for name in names:
+            errs = {}
+            for o in KERNELS[name].outputs:
+                i0, i1, j0, j1 = KERNELS[name].region(o, n, n)
+                x = L.region(dev["f32"][name][o].buf, i0, i1, j0, j1).double()
+                y = L.region(dev["f64"][name][o].buf, i0, i1, j0, j1)
+                diff = (x - y).abs()
+                ymax = y.abs().max()
+                errs[o] = {"rel_max": float(diff.max() / ymax),
+                           "rel_rms": float(diff.pow(2).mean().sqrt() / y.pow(2).mean().sqrt()),
+                           "frac_above_1e-5": float((diff > 1e-5 * ymax).double().mean())}
+            per32[name]["speedup"] = round(per[name]["us"] / per32[name]["us"], 3)
+            per32[name]["error_vs_f64"] = {o: {k: float(f"{v:.3e}") for k, v in e.items()} for o, e in errs.items()}
+        value32 = len(names) * pts / (step32 * 1e-3) / 1e9
+        result["fp32"] = {
+            "value": round(value32, 3),
+            "unit": "Gpts/s",
+            "ms_per_step": round(step32, 4),
+            "speedup_vs_f64": round(value32 / value, 3),
+            "error_reference": "FP64 outputs of the same step (equal to the FP64 oracle's bit for bit)",
+            "per_stencil": per32,
+        }
     if copy_line:
         result["device_copy"] = copy_line
     if not args.no_cpu_baseline:

@@ -55,6 +55,7 @@ extern "C" {
 #define OEB_ERR_LAYOUT 3 /* a descriptor does not cover the region the stencil touches, or strides differ */
 #define OEB_ERR_ALIAS 4  /* an output array overlaps an input array */
 #define OEB_ERR_HIP 5    /* a HIP runtime call or kernel launch failed */
+#define OEB_ERR_ARG 6    /* a non-array argument is out of its range */
 
 typedef struct oeb_domain {
   int64_t nx; /* interior extent along i (the reference's domain_size) */
@@ -78,8 +79,28 @@ typedef struct oeb_storage1d_f64 {
   int64_t strides[1];
 } oeb_storage1d_f64;
 
+/* FP32 descriptors: the same fields with float data pointers (the reference's
+ * Storage<float,3> / Storage<float,1> when ElementType is float). */
+typedef struct oeb_storage3d_f32 {
+  float *allocatedPtr;
+  float *alignedPtr;
+  int64_t offset;
+  int64_t sizes[3];
+  int64_t strides[3];
+} oeb_storage3d_f32;
+
+typedef struct oeb_storage1d_f32 {
+  float *allocatedPtr;
+  float *alignedPtr;
+  int64_t offset;
+  int64_t sizes[1];
+  int64_t strides[1];
+} oeb_storage1d_f32;
+
 typedef oeb_storage3d_f64 oeb_s3;
 typedef oeb_storage1d_f64 oeb_s1;
+typedef oeb_storage3d_f32 oeb_s3f;
+typedef oeb_storage1d_f32 oeb_s1f;
 
 /* Library version as "major.minor.patch". */
 const char *oeb_version(void);
@@ -183,8 +204,112 @@ int oeb_fvtp2d_f64(const oeb_domain *d, const oeb_s3 *q_i, const oeb_s3 *q_j, co
                    const oeb_s3 *cry, const oeb_s3 *ra_x, const oeb_s3 *ra_y, const oeb_s3 *xfx, const oeb_s3 *yfx,
                    const oeb_s3 *area, const oeb_s3 *fxx, const oeb_s3 *fyy, const oeb_s3 *al, const oeb_s3 *almq,
                    const oeb_s3 *br, const oeb_s3 *b0, const oeb_s3 *smt5, double p1, double p2, void *stream);
+
+/* ------------------------------------------------------------------- FP32
+ * oeb_<name>_f32 runs the same stencil with FP32 storage and arithmetic: the
+ * reference compiled with `typedef float ElementType` (src/<name>.cpp:9).
+ * Arguments come in the same order as in the FP64 entry point.  Arrays are
+ * oeb_storage3d_f32 / oeb_storage1d_f32, and scalars are float, the
+ * reference's `const ElementType`.  uvbke's dt5 stays double: the reference
+ * declares it `const double` whatever ElementType is (src/uvbke.cpp:11-12).
+ * As in the reference's float build, laplace's -4.0 literal and uvbke's dt5
+ * are double, so those two products are formed in double and rounded once
+ * on the store (kernels/laplace.h:39, kernels/uvbke.h:67-71).
+ *
+ * laplace, hdiffsa, hdiffsmag, hadvuv, hadvuv5th and uvbke compute exactly
+ * what the reference's float build computes.  Three stencils keep FP64
+ * islands, because FP32 storage of pressure (~1e5 Pa) and geopotential
+ * (~2e5 m^2/s^2) loses their small differences to cancellation:
+ *   p_grad_c     gz and pkc are FP64 arrays (oeb_storage3d_f64).  The
+ *                bracket (dgz*dpk + dgz'*dpk') is formed in FP64 and
+ *                rounded once to float.
+ *   nh_p_grad    gz, pp and pk3 are FP64 arrays.  wk = pk3(k+1) - pk3(k),
+ *                the pressure-thickness sums, du/dv and the pp bracket are
+ *                formed in FP64.  du/dv and the bracket are rounded once to
+ *                float.
+ *   fastwavesuv  ppuv and hhl are FP64 arrays.  ppgk, ppgc and ppgu/ppgv
+ *                are formed in FP64.  ppgu/ppgv are rounded once to float.
+ * All 3-D arrays of one call, FP32 and FP64 alike, share strides counted
+ * in elements.  fvtp2d forms its flux divergences q*area + fyy - fyy(j+1)
+ * and q*area + fxx - fxx(i+1) with error-free transformations (the product
+ * by fma, the sums by TwoSum) and adds the collected rounding errors once
+ * before the division.  The numerator then carries about one float rounding
+ * instead of three.  Its PPM stages are those of the float build. */
+int oeb_laplace_f32(const oeb_domain *d, const oeb_s3f *in, const oeb_s3f *out, void *stream);
+int oeb_hdiffsa_f32(const oeb_domain *d, const oeb_s3f *out, const oeb_s3f *in, const oeb_s3f *mask,
+                    const oeb_s1f *crlato, const oeb_s1f *crlatu, const oeb_s3f *lap, const oeb_s3f *flx,
+                    const oeb_s3f *fly, void *stream);
+int oeb_hdiffsmag_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
+                      const oeb_s3f *vin, const oeb_s3f *mask, const oeb_s1f *crlavo, const oeb_s1f *crlavu,
+                      const oeb_s1f *crlato, const oeb_s1f *crlatu, const oeb_s1f *acrlat0,
+                      const oeb_s3f *T_sqr_s, const oeb_s3f *S_sqr_uv, float eddlat, float eddlon, float tau_smag,
+                      float weight_smag, void *stream);
+int oeb_hadvuv_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
+                   const oeb_s3f *vin, const oeb_s1f *acrlat0, const oeb_s1f *acrlat1, const oeb_s1f *tgrlatda0,
+                   const oeb_s1f *tgrlatda1, const oeb_s3f *uatupos, const oeb_s3f *vatupos,
+                   const oeb_s3f *uatvpos, const oeb_s3f *vatvpos, const oeb_s3f *uavg, const oeb_s3f *vavg,
+                   const oeb_s3f *ures, const oeb_s3f *vres, float eddlat, float eddlon, void *stream);
+int oeb_hadvuv5th_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
+                      const oeb_s3f *vin, const oeb_s1f *acrlat0, const oeb_s1f *acrlat1,
+                      const oeb_s1f *tgrlatda0, const oeb_s1f *tgrlatda1, const oeb_s3f *uatupos,
+                      const oeb_s3f *vatupos, const oeb_s3f *uatvpos, const oeb_s3f *vatvpos, const oeb_s3f *uavg,
+                      const oeb_s3f *vavg, const oeb_s3f *ures, const oeb_s3f *vres, float eddlat, float eddlon,
+                      void *stream);
+int oeb_fastwavesuv_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
+                        const oeb_s3f *vin, const oeb_s3f *utens, const oeb_s3f *vtens, const oeb_s3f *wgtfac,
+                        const oeb_s3 *ppuv, const oeb_s3 *hhl, const oeb_s3f *rho, const oeb_s1f *fx,
+                        const oeb_s3f *ppgk, const oeb_s3f *ppgc, const oeb_s3f *ppgu, const oeb_s3f *ppgv,
+                        float edadlat, float dt, void *stream);
+int oeb_uvbke_f32(const oeb_domain *d, const oeb_s3f *ub, const oeb_s3f *vb, const oeb_s3f *uc,
+                  const oeb_s3f *vc, const oeb_s3f *cosa, const oeb_s3f *rsina, double dt5, void *stream);
+int oeb_p_grad_c_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
+                     const oeb_s3f *vin, const oeb_s3f *rdxc, const oeb_s3f *rdyc, const oeb_s3f *delpc,
+                     const oeb_s3 *gz, const oeb_s3 *pkc, const oeb_s3f *wk, float dt2, void *stream);
+int oeb_nh_p_grad_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
+                      const oeb_s3f *vin, const oeb_s3f *rdx, const oeb_s3f *rdy, const oeb_s3 *gz,
+                      const oeb_s3 *pp, const oeb_s3 *pk3, const oeb_s3f *wk1, const oeb_s3f *wk,
+                      const oeb_s3f *du, const oeb_s3f *dv, float dt, void *stream);
+int oeb_fvtp2d_f32(const oeb_domain *d, const oeb_s3f *q_i, const oeb_s3f *q_j, const oeb_s3f *fx1,
+                   const oeb_s3f *fx2, const oeb_s3f *fy1, const oeb_s3f *fy2, const oeb_s3f *q,
+                   const oeb_s3f *crx, const oeb_s3f *cry, const oeb_s3f *ra_x, const oeb_s3f *ra_y,
+                   const oeb_s3f *xfx, const oeb_s3f *yfx, const oeb_s3f *area, const oeb_s3f *fxx,
+                   const oeb_s3f *fyy, const oeb_s3f *al, const oeb_s3f *almq, const oeb_s3f *br,
+                   const oeb_s3f *b0, const oeb_s3f *smt5, float p1, float p2, void *stream);
 #pragma pop_macro("p2")
 #pragma pop_macro("p1")
+
+/* ------------------------------------------------------------ halo update
+ * No counterpart in the reference: its kernels write only their output
+ * regions and only the drivers' fillMath (include/util.h:155-163) writes
+ * halos.  A chain of stencils, whose outputs feed the next stencil's inputs,
+ * refills the halo the next stencil reads with these.  Two launches on
+ * `stream`:
+ *   - the i/j ring [-halo_ij, nx+halo_ij) x [-halo_ij, ny+halo_ij) minus the
+ *     interior, on levels [0, nz): OEB_HALO_PERIODIC copies the interior
+ *     point at the wrapped index (i mod nx, j mod ny), OEB_HALO_ZERO_GRADIENT
+ *     the one at the clamped index;
+ *   - the k-halo planes [-halo_k, 0) and [nz, nz+halo_k) over the whole i/j
+ *     box: copies of planes 0 and nz-1, in both modes.
+ * halo_ij and halo_k lie in [0, 16]; the descriptor must cover the box. */
+#define OEB_HALO_PERIODIC 0
+#define OEB_HALO_ZERO_GRADIENT 1
+int oeb_halo_fill_f64(const oeb_domain *d, const oeb_s3 *field, int halo_ij, int halo_k, int mode, void *stream);
+int oeb_halo_fill_f32(const oeb_domain *d, const oeb_s3f *field, int halo_ij, int halo_k, int mode, void *stream);
+
+/* ----------------------------------------------------------------- graphs
+ * No counterpart in the reference.  Records every launch the oeb_* entry
+ * points enqueue on `stream` between oeb_graph_begin and oeb_graph_end as a
+ * HIP graph (stream capture, thread-local mode), and replays it with one
+ * oeb_graph_launch.  The descriptors are validated when the calls are
+ * recorded; a replay runs the same kernels on the same device addresses.
+ * `stream` must be a created stream, not the null stream.  The entry points
+ * enqueue kernels only (no memset, copy or allocation), so a recorded step
+ * replays exactly. */
+typedef struct oeb_graph oeb_graph;
+int oeb_graph_begin(void *stream);
+int oeb_graph_end(void *stream, oeb_graph **graph);
+int oeb_graph_launch(const oeb_graph *graph, void *stream);
+int oeb_graph_destroy(oeb_graph *graph);
 
 #ifdef __cplusplus
 }

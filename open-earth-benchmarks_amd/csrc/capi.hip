@@ -33,8 +33,11 @@ int check_domain(const char *kname, const oeb_domain *d) {
   return OEB_OK;
 }
 
-int check3(const char *kname, const char *name, const oeb_s3 *s, const Box &b, Geo &g, bool first,
-           const double **base) {
+namespace {
+
+template <class S, class T>
+int check3_impl(const char *kname, const char *name, const S *s, const Box &b, Geo &g, bool first,
+                const T **base) {
   std::ostringstream m;
   m << kname << ": array '" << name << "' ";
   if (s == nullptr || s->alignedPtr == nullptr) return fail(OEB_ERR_NULL, m.str() + "is NULL");
@@ -70,8 +73,9 @@ int check3(const char *kname, const char *name, const oeb_s3 *s, const Box &b, G
   return OEB_OK;
 }
 
-int check1(const char *kname, const char *name, const oeb_s1 *s, int64_t lo, int64_t hi, const double **base,
-           int *stride) {
+template <class S, class T>
+int check1_impl(const char *kname, const char *name, const S *s, int64_t lo, int64_t hi, const T **base,
+                int *stride) {
   std::ostringstream m;
   m << kname << ": 1-D array '" << name << "' ";
   if (s == nullptr || s->alignedPtr == nullptr) return fail(OEB_ERR_NULL, m.str() + "is NULL");
@@ -87,21 +91,31 @@ int check1(const char *kname, const char *name, const oeb_s1 *s, int64_t lo, int
   return OEB_OK;
 }
 
-void AliasSet::add(const char *nm, const double *base, const Box &b, const Geo &g, bool is_out) {
-  if (n >= kMax) return;
-  name[n] = nm;
-  lo[n] = reinterpret_cast<uintptr_t>(base + (b.klo * g.sk + b.jlo * int64_t(g.sj) + b.ilo));
-  hi[n] = reinterpret_cast<uintptr_t>(base + ((b.khi - 1) * g.sk + (b.jhi - 1) * int64_t(g.sj) + (b.ihi - 1)) + 1);
-  out[n] = is_out;
-  ++n;
+}  // namespace
+
+int check3(const char *kname, const char *name, const oeb_s3 *s, const Box &b, Geo &g, bool first,
+           const double **base) {
+  return check3_impl(kname, name, s, b, g, first, base);
+}
+int check3(const char *kname, const char *name, const oeb_s3f *s, const Box &b, Geo &g, bool first,
+           const float **base) {
+  return check3_impl(kname, name, s, b, g, first, base);
+}
+int check1(const char *kname, const char *name, const oeb_s1 *s, int64_t lo, int64_t hi, const double **base,
+           int *stride) {
+  return check1_impl(kname, name, s, lo, hi, base, stride);
+}
+int check1(const char *kname, const char *name, const oeb_s1f *s, int64_t lo, int64_t hi, const float **base,
+           int *stride) {
+  return check1_impl(kname, name, s, lo, hi, base, stride);
 }
 
-void AliasSet::add1(const char *nm, const double *base, int64_t lo_i, int64_t hi_i, int stride) {
+void AliasSet::add_range(const char *nm, uintptr_t lo_b, uintptr_t hi_b, bool is_out) {
   if (n >= kMax) return;
   name[n] = nm;
-  lo[n] = reinterpret_cast<uintptr_t>(base + lo_i * stride);
-  hi[n] = reinterpret_cast<uintptr_t>(base + (hi_i - 1) * stride + 1);
-  out[n] = false;
+  lo[n] = lo_b;
+  hi[n] = hi_b;
+  out[n] = is_out;
   ++n;
 }
 
@@ -128,15 +142,23 @@ int check_tiles(const char *kname, const Tiles &t, int64_t ni, int64_t nj, int n
   return OEB_OK;
 }
 
-int kchunk_for(const char *kname, int dflt) {
-  std::string var = "OEB_KCHUNK_";
+namespace {
+// Value of the environment variable <prefix><KNAME> (kernel name in upper
+// case) when it holds an integer in [lo, hi], else `dflt`.
+int env_tunable(const char *prefix, const char *kname, int dflt, long lo, long hi) {
+  std::string var = prefix;
   for (const char *c = kname; *c; ++c) var += char(std::toupper(static_cast<unsigned char>(*c)));
   const char *v = std::getenv(var.c_str());
   if (v == nullptr) return dflt;
   char *end = nullptr;
   const long n = std::strtol(v, &end, 10);
-  return (end != v && *end == '\0' && n >= 1 && n <= 4096) ? int(n) : dflt;
+  return (end != v && *end == '\0' && n >= lo && n <= hi) ? int(n) : dflt;
 }
+}  // namespace
+
+int kchunk_for(const char *kname, int dflt) { return env_tunable("OEB_KCHUNK_", kname, dflt, 1, 4096); }
+
+bool pairs_for(const char *kname, bool dflt) { return env_tunable("OEB_PAIRS_", kname, dflt ? 1 : 0, 0, 1) != 0; }
 
 int launch_status(const char *kname) {
   const hipError_t e = hipGetLastError();
@@ -158,6 +180,7 @@ const char *oeb_status_string(int status) {
     case OEB_ERR_LAYOUT: return "descriptor does not cover the stencil's region, or strides differ";
     case OEB_ERR_ALIAS: return "an output overlaps another array";
     case OEB_ERR_HIP: return "HIP runtime error";
+    case OEB_ERR_ARG: return "an argument is out of its range";
   }
   return "unknown status";
 }

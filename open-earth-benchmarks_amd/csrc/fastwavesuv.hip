@@ -5,17 +5,23 @@
 // level k+1 to level k of the next step, and so are ppuv and hhl, so each plane
 // of wgtfac, ppuv and hhl is loaded once per chunk.
 // Reach: i, j in [0, +1]; k in [-1, +1] (the k-halo planes -1 and nz are read).
-// Algorithmic traffic: 8 inputs + 2 outputs = 80 B per point (FP64).
+// FP32 form: ppuv and hhl stay FP64 arrays (pressure ~1e5 Pa and heights lose
+// their small differences to cancellation in FP32), and ppgk, ppgc and
+// ppgu/ppgv are formed in FP64; ppgu/ppgv are rounded once to float.
+// Algorithmic traffic: 8 inputs + 2 outputs = 80 B per point (FP64); 48 B in
+// the FP32 form (ppuv and hhl 8 B, the rest 4 B).
 #include "oeb_common.hpp"
 
 namespace oeb {
 namespace {
 
-constexpr int kTileI = 64, kTileJ = 4, kChunk = 16;
+constexpr int kTileI = 64, kTileJ = 4, kChunk = 8;
 constexpr int kPairI = 2 * kTileI;
 
-__device__ __forceinline__ double ppgk_of(double wgt, double pp_k, double pp_km1) {
-  return wgt * pp_k + (1.0 - wgt) * pp_km1;
+template <class T>
+__device__ __forceinline__ double ppgk_of(T wgt, double pp_k, double pp_km1) {
+  const double w = wgt;
+  return w * pp_k + (1.0 - w) * pp_km1;
 }
 
 // Level-k and level-(k+1) values of one quantity at C = (i,j), E = (i+1,j),
@@ -26,32 +32,35 @@ struct CEN {
 
 // One point (kernels/fastwavesuv.h:312-337).  pp: ppuv at level k (C, E, N);
 // gk: ppgk at k and k+1; h: hhl at k and k+1; r*: rho at level k.
-__device__ __forceinline__ void fw_point(double ppC, double ppE, double ppN, const CEN &gk, const CEN &h, double rC,
-                                         double rE, double rN, double uin, double vin, double utens, double vtens,
-                                         double fxj, double edadlat, double dt, double &uo, double &vo) {
+template <class T>
+__device__ __forceinline__ void fw_point(double ppC, double ppE, double ppN, const CEN &gk, const CEN &h, T rC, T rE,
+                                         T rN, T uin, T vin, T utens, T vtens, T fxj, T edadlat, T dt, T &uo, T &vo) {
   const double gcC = gk.C1 - gk.C, gcE = gk.E1 - gk.E, gcN = gk.N1 - gk.N;
   const double ppgu =
       (ppE - ppC) + (gcE + gcC) * 0.5 * ((h.C1 + h.C) - (h.E1 + h.E)) / ((h.C1 - h.C) + (h.E1 - h.E));
   const double ppgv =
       (ppN - ppC) + (gcN + gcC) * 0.5 * ((h.C1 + h.C) - (h.N1 + h.N)) / ((h.C1 - h.C) + (h.N1 - h.N));
-  uo = uin + dt * (utens - ppgu * (2.0 * fxj / (rE + rC)));
-  vo = vin + dt * (vtens - ppgv * (2.0 * edadlat / (rN + rC)));
+  uo = uin + dt * (utens - T(ppgu) * (T(2.0) * fxj / (rE + rC)));
+  vo = vin + dt * (vtens - T(ppgv) * (T(2.0) * edadlat / (rN + rC)));
 }
 
+template <class T>
 struct FwArgs {
-  double *uout, *vout;
-  const double *uin, *vin, *utens, *vtens, *wgtfac, *ppuv, *hhl, *rho, *fx;
+  T *uout, *vout;
+  const T *uin, *vin, *utens, *vtens, *wgtfac, *rho, *fx;
+  const double *ppuv, *hhl;
   int fx_stride;
 };
 
+template <class T>
 __global__ void __launch_bounds__(kTileI *kTileJ)
-    fastwavesuv_kernel(FwArgs a, double edadlat, double dt, Geo g, Tiles t) {
+    fastwavesuv_kernel(FwArgs<T> a, T edadlat, T dt, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
   const int64_t sk = g.sk;
-  const double fxj = a.fx[int64_t(j) * a.fx_stride];
+  const T fxj = a.fx[int64_t(j) * a.fx_stride];
   int64_t at = int64_t(c.k0) * sk + (i + j * sj);
   const int64_t dn = at - sk;
   double ppC = a.ppuv[at], ppE = a.ppuv[at + 1], ppN = a.ppuv[at + sj];
@@ -67,7 +76,7 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
     gk.E1 = ppgk_of(a.wgtfac[up + 1], ppE1, ppE);
     gk.N1 = ppgk_of(a.wgtfac[up + sj], ppN1, ppN);
     h.C1 = a.hhl[up], h.E1 = a.hhl[up + 1], h.N1 = a.hhl[up + sj];
-    double uo, vo;
+    T uo, vo;
     fw_point(ppC, ppE, ppN, gk, h, a.rho[at], a.rho[at + 1], a.rho[at + sj], ld_nt(a.uin + at), ld_nt(a.vin + at),
              ld_nt(a.utens + at), ld_nt(a.vtens + at), fxj, edadlat, dt, uo, vo);
     st_nt(a.uout + at, uo);
@@ -83,7 +92,7 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
 // single load at i+2.  Suffix P: the pair at (i, i+1); Q: the pair at row j+1;
 // X: column i+2 at row j.
 __global__ void __launch_bounds__(kTileI *kTileJ)
-    fastwavesuv_pair_kernel(FwArgs a, double edadlat, double dt, Geo g, Tiles t) {
+    fastwavesuv_pair_kernel(FwArgs<double> a, double edadlat, double dt, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
   const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
@@ -134,19 +143,10 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
   }
 }
 
-}  // namespace
-}  // namespace oeb
-
-extern "C" int oeb_fastwavesuv_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
-                                   const oeb_s3 *vin, const oeb_s3 *utens, const oeb_s3 *vtens, const oeb_s3 *wgtfac,
-                                   const oeb_s3 *ppuv, const oeb_s3 *hhl, const oeb_s3 *rho, const oeb_s1 *fx,
-                                   const oeb_s3 *ppgk, const oeb_s3 *ppgc, const oeb_s3 *ppgu, const oeb_s3 *ppgv,
-                                   double edadlat, double dt, void *stream) {
-  using namespace oeb;
-  (void)ppgk;
-  (void)ppgc;
-  (void)ppgu;
-  (void)ppgv;
+template <class T, class S3, class S1>
+int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *uin, const S3 *vin,
+                    const S3 *utens, const S3 *vtens, const S3 *wgtfac, const oeb_s3 *ppuv, const oeb_s3 *hhl,
+                    const S3 *rho, const S1 *fx, T edadlat, T dt, void *stream) {
   const char *K = "fastwavesuv";
   OEB_TRY(check_domain(K, d));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
@@ -154,8 +154,8 @@ extern "C" int oeb_fastwavesuv_f64(const oeb_domain *d, const oeb_s3 *uout, cons
   const Box r0{0, nx, 0, ny, 0, nz};
   const Box r_h{0, nx + 1, 0, ny + 1, 0, nz};
   const Box r_w{0, nx + 1, 0, ny + 1, 0, nz + 1}, r_p{0, nx + 1, 0, ny + 1, -1, nz + 1};
-  const double *p_uo, *p_vo;
-  FwArgs a;
+  const T *p_uo, *p_vo;
+  FwArgs<T> a;
   a.fx_stride = 1;
   OEB_TRY(check3(K, "uout", uout, r0, g, true, &p_uo));
   OEB_TRY(check3(K, "vout", vout, r0, g, false, &p_vo));
@@ -181,20 +181,46 @@ extern "C" int oeb_fastwavesuv_f64(const oeb_domain *d, const oeb_s3 *uout, cons
   as.add("rho", a.rho, r_h, g, false);
   as.add1("fx", a.fx, 0, ny, a.fx_stride);
   OEB_TRY(as.check(K));
-  a.uout = const_cast<double *>(p_uo);
-  a.vout = const_cast<double *>(p_vo);
+  a.uout = const_cast<T *>(p_uo);
+  a.vout = const_cast<T *>(p_vo);
   const int kc = kchunk_for(K, kChunk);
-  if (nx % 2 == 0 &&
-      pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.utens, a.vtens, a.wgtfac, a.ppuv, a.hhl, a.rho})) {
-    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-    hipLaunchKernelGGL(fastwavesuv_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, edadlat,
-                       dt, g, t);
-  } else {
-    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-    hipLaunchKernelGGL(fastwavesuv_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, edadlat, dt,
-                       g, t);
+  if constexpr (sizeof(T) == 8) {
+    if (pairs_for(K, true) && nx % 2 == 0 &&
+        pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.utens, a.vtens, a.wgtfac, a.ppuv, a.hhl, a.rho})) {
+      const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
+      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+      hipLaunchKernelGGL(fastwavesuv_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a,
+                         edadlat, dt, g, t);
+      return launch_status(K);
+    }
   }
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
+  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+  hipLaunchKernelGGL(fastwavesuv_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, edadlat, dt,
+                     g, t);
   return launch_status(K);
+}
+
+}  // namespace
+}  // namespace oeb
+
+// ppgk, ppgc, ppgu and ppgv are the reference's scratch arrays; nothing is stored in them.
+extern "C" int oeb_fastwavesuv_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
+                                   const oeb_s3 *vin, const oeb_s3 *utens, const oeb_s3 *vtens, const oeb_s3 *wgtfac,
+                                   const oeb_s3 *ppuv, const oeb_s3 *hhl, const oeb_s3 *rho, const oeb_s1 *fx,
+                                   const oeb_s3 *ppgk, const oeb_s3 *ppgc, const oeb_s3 *ppgu, const oeb_s3 *ppgv,
+                                   double edadlat, double dt, void *stream) {
+  (void)ppgk, (void)ppgc, (void)ppgu, (void)ppgv;
+  return oeb::fastwavesuv_run<double>(d, uout, vout, uin, vin, utens, vtens, wgtfac, ppuv, hhl, rho, fx, edadlat, dt,
+                                      stream);
+}
+
+extern "C" int oeb_fastwavesuv_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout,
+                                   const oeb_s3f *uin, const oeb_s3f *vin, const oeb_s3f *utens,
+                                   const oeb_s3f *vtens, const oeb_s3f *wgtfac, const oeb_s3 *ppuv, const oeb_s3 *hhl,
+                                   const oeb_s3f *rho, const oeb_s1f *fx, const oeb_s3f *ppgk, const oeb_s3f *ppgc,
+                                   const oeb_s3f *ppgu, const oeb_s3f *ppgv, float edadlat, float dt, void *stream) {
+  (void)ppgk, (void)ppgc, (void)ppgu, (void)ppgv;
+  return oeb::fastwavesuv_run<float>(d, uout, vout, uin, vin, utens, vtens, wgtfac, ppuv, hhl, rho, fx, edadlat, dt,
+                                     stream);
 }

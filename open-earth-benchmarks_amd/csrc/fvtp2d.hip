@@ -15,11 +15,14 @@
 //
 // ppm_face() forms al, almq, br, b0 and smt5 at the three faces one flux needs,
 // with the reference's expressions, so every value equals the reference's
-// double.  fy2, fx2, q_i and q_j are outputs on extended regions.  A tile
+// value.  fy2, fx2, q_i and q_j are outputs on extended regions.  A tile
 // computes them on its tile plus halo, and writes only the points it owns:
 // its own rows and columns, plus the region's extension beyond the domain when
 // the tile touches that edge.  So every point is written exactly once.
-// Algorithmic traffic: 8 inputs + 6 outputs = 112 B per interior point (FP64).
+// FP32 form: the PPM stages are those of the reference's float build; the flux
+// divergences of q_i and q_j are compensated (flux_div below).
+// Algorithmic traffic: 8 inputs + 6 outputs = 112 B per interior point (FP64),
+// 56 B (FP32).
 #include "oeb_common.hpp"
 
 namespace oeb {
@@ -29,45 +32,72 @@ constexpr int kTI = 64;   // tile width along i (one wavefront)
 constexpr int kNT = 256;  // threads per workgroup
 // Tile height along j and levels per workgroup of the launch.
 constexpr int kTJdefault = 8;
-constexpr int kChunk = 4;
+constexpr int kChunk = 2;
 
 // PPM flux at face m between cells m-1 and m along one axis, given
 // f(m-3) .. f(m+2) and the Courant number c at m (kernels/fvtp2d.h:12, 20-34).
-__device__ __forceinline__ double ppm_face(double fm3, double fm2, double fm1, double f0, double fp1, double fp2,
-                                           double c, double p1, double p2) {
-  const double al_m = ((p1 * (fm2 + fm1)) + (p2 * (fm3 + f0)));  // al(m-1)
-  const double al_0 = ((p1 * (fm1 + f0)) + (p2 * (fm2 + fp1)));  // al(m)
-  const double al_p = ((p1 * (f0 + fp1)) + (p2 * (fm1 + fp2)));  // al(m+1)
-  const double almq_m = (al_m - fm1), br_m = (al_0 - fm1), b0_m = (almq_m + br_m);
-  const double almq_0 = (al_0 - f0), br_0 = (al_p - f0), b0_0 = (almq_0 + br_0);
-  const double smt_m = ((almq_m * br_m) < 0.0) ? 1.0 : 0.0;
-  const double smt_0 = ((almq_0 * br_0) < 0.0) ? 1.0 : 0.0;
-  const double smt = (smt_m + (smt_0 * (smt_m == 0.0 ? 1.0 : 0.0)));
-  const double up = (fm1 + (((1.0 - c) * (br_m - (c * b0_m))) * smt));
-  const double dn = (f0 + (((1.0 + c) * (almq_0 + (c * b0_0))) * smt));
-  return (c > 0.0) ? up : dn;
+template <class T>
+__device__ __forceinline__ T ppm_face(T fm3, T fm2, T fm1, T f0, T fp1, T fp2, T c, T p1, T p2) {
+  const T al_m = ((p1 * (fm2 + fm1)) + (p2 * (fm3 + f0)));  // al(m-1)
+  const T al_0 = ((p1 * (fm1 + f0)) + (p2 * (fm2 + fp1)));  // al(m)
+  const T al_p = ((p1 * (f0 + fp1)) + (p2 * (fm1 + fp2)));  // al(m+1)
+  const T almq_m = (al_m - fm1), br_m = (al_0 - fm1), b0_m = (almq_m + br_m);
+  const T almq_0 = (al_0 - f0), br_0 = (al_p - f0), b0_0 = (almq_0 + br_0);
+  const T smt_m = ((almq_m * br_m) < T(0)) ? T(1) : T(0);
+  const T smt_0 = ((almq_0 * br_0) < T(0)) ? T(1) : T(0);
+  const T smt = (smt_m + (smt_0 * (smt_m == T(0) ? T(1) : T(0))));
+  const T up = (fm1 + (((T(1) - c) * (br_m - (c * b0_m))) * smt));
+  const T dn = (f0 + (((T(1) + c) * (almq_0 + (c * b0_0))) * smt));
+  return (c > T(0)) ? up : dn;
 }
 
+// Flux divergence ((q*area + xa*ya) - xb*yb) / ra: q_i with fyy = yfx*fy2 at
+// j and j+1 (kernels/fvtp2d.h:39-51), q_j with fxx = xfx*fx2 at i and i+1
+// (:115-127).  FP64 keeps the reference's expression.
+__device__ __forceinline__ double flux_div(double q, double area, double xa, double ya, double xb, double yb,
+                                           double ra) {
+  const double fa = (xa * ya), fb = (xb * yb);
+  return ((((q * area) + fa) - fb) / ra);
+}
+
+// FP32: the numerator as a compensated dot product (Ogita, Rump and Oishi,
+// "Accurate sum and dot product", SIAM J. Sci. Comput. 26(6), 2005, Dot2).
+// Each product's rounding error comes exactly from an fma (TwoProduct), each
+// sum's from Knuth's TwoSum.  The errors are added once to the rounded
+// numerator before the division.  The numerator is then as accurate as if it
+// were formed in twice the working precision and rounded once.
+__device__ __forceinline__ float flux_div(float q, float area, float xa, float ya, float xb, float yb, float ra) {
+  const float p0 = q * area, e0 = fmaf(q, area, -p0);
+  const float p1 = xa * ya, e1 = fmaf(xa, ya, -p1);
+  const float p2 = xb * yb, e2 = fmaf(xb, yb, -p2);
+  const float s1 = p0 + p1, z1 = s1 - p0, t1 = (p0 - (s1 - z1)) + (p1 - z1);
+  const float s2 = s1 - p2, z2 = s2 - s1, t2 = (s1 - (s2 - z2)) + (-p2 - z2);
+  const float err = ((e0 + e1) - e2) + (t1 + t2);
+  return (s2 + err) / ra;
+}
+
+template <class T>
 struct FvIn {
-  const double *q, *crx, *cry, *ra_x, *ra_y, *xfx, *yfx, *area;
+  const T *q, *crx, *cry, *ra_x, *ra_y, *xfx, *yfx, *area;
 };
+template <class T>
 struct FvOut {
-  double *q_i, *q_j, *fx1, *fx2, *fy1, *fy2;
+  T *q_i, *q_j, *fx1, *fx2, *fy1, *fy2;
 };
 
-template <int kTJ>
-__global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn a, FvOut o, double p1, double p2, Geo g, Tiles t) {
-  // LDS tile extents (doubles).
+template <int kTJ, class T>
+__global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn<T> a, FvOut<T> o, T p1, T p2, Geo g, Tiles t) {
+  // LDS tile extents (elements).
   constexpr int kQW = kTI + 6, kQH = kTJ + 6;  // q:   i [i0-3, i0+TI+3) x j [j0-3, j0+TJ+3)
   constexpr int kYW = kTI + 5, kYH = kTJ + 1;  // fy2: i [i0-3, i0+TI+2) x j [j0, j0+TJ+1)
   constexpr int kXW = kTI + 1, kXH = kTJ + 5;  // fx2: i [i0, i0+TI+1)   x j [j0-3, j0+TJ+2)
   constexpr int kIW = kTI + 5, kIH = kTJ;      // q_i: i [i0-3, i0+TI+2) x j [j0, j0+TJ)
   constexpr int kJW = kTI, kJH = kTJ + 5;      // q_j: i [i0, i0+TI)     x j [j0-3, j0+TJ+2)
-  __shared__ double sq[kQH][kQW];
-  __shared__ double sfy2[kYH][kYW];
-  __shared__ double sfx2[kXH][kXW];
-  __shared__ double sqi[kIH][kIW];
-  __shared__ double sqj[kJH][kJW];
+  __shared__ T sq[kQH][kQW];
+  __shared__ T sfy2[kYH][kYW];
+  __shared__ T sfx2[kXH][kXW];
+  __shared__ T sqi[kIH][kIW];
+  __shared__ T sqj[kJH][kJW];
 
   const TileCoord tc = tile_coord(t, kTI, kTJ, g.nz);
   const int i0 = tc.i0, j0 = tc.j0;
@@ -82,14 +112,14 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn a, FvOut o, double p1,
 
   for (int k = tc.k0; k < tc.k1; ++k) {
     const int64_t plane = int64_t(k) * g.sk;
-    const double *q = a.q + plane;
+    const T *q = a.q + plane;
 
     // A: q tile with halo.  Points beyond the allocated halo are never read by
     // a valid task below; they are filled with 0.
     for (int e = tid; e < kQH * kQW; e += kNT) {
       const int r = e / kQW, c = e % kQW;
       const int gi = i0 - 3 + c, gj = j0 - 3 + r;
-      sq[r][c] = (gi < nx + 3 && gj < ny + 3) ? q[gi + gj * sj] : 0.0;
+      sq[r][c] = (gi < nx + 3 && gj < ny + 3) ? q[gi + gj * sj] : T(0);
     }
     __syncthreads();
 
@@ -99,8 +129,8 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn a, FvOut o, double p1,
       const int gi = i0 - 3 + c, gj = j0 + r;
       if (gi < nx + 2 && gj < ny + 1) {
         const int at = gi + gj * sj;
-        const double v = ppm_face(sq[r][c], sq[r + 1][c], sq[r + 2][c], sq[r + 3][c], sq[r + 4][c], sq[r + 5][c],
-                                  a.cry[plane + at], p1, p2);
+        const T v = ppm_face(sq[r][c], sq[r + 1][c], sq[r + 2][c], sq[r + 3][c], sq[r + 4][c], sq[r + 5][c],
+                             a.cry[plane + at], p1, p2);
         sfy2[r][c] = v;
         if (own_i(gi) && own_j(gj)) st_nt(o.fy2 + plane + at, v);
       }
@@ -110,8 +140,8 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn a, FvOut o, double p1,
       const int gi = i0 + c, gj = j0 - 3 + r;
       if (gi < nx + 1 && gj < ny + 2) {
         const int at = gi + gj * sj;
-        const double *row = &sq[r][c];
-        const double v = ppm_face(row[0], row[1], row[2], row[3], row[4], row[5], a.crx[plane + at], p1, p2);
+        const T *row = &sq[r][c];
+        const T v = ppm_face(row[0], row[1], row[2], row[3], row[4], row[5], a.crx[plane + at], p1, p2);
         sfx2[r][c] = v;
         if (own_i(gi) && own_j(gj)) st_nt(o.fx2 + plane + at, v);
       }
@@ -124,9 +154,8 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn a, FvOut o, double p1,
       const int gi = i0 - 3 + c, gj = j0 + r;
       if (gi < nx + 2 && gj < ny) {
         const int at = gi + gj * sj;
-        const double fyy0 = (a.yfx[plane + at] * sfy2[r][c]);
-        const double fyy1 = (a.yfx[plane + at + sj] * sfy2[r + 1][c]);
-        const double v = ((((sq[r + 3][c] * a.area[plane + at]) + fyy0) - fyy1) / a.ra_y[plane + at]);
+        const T v = flux_div(sq[r + 3][c], a.area[plane + at], a.yfx[plane + at], sfy2[r][c], a.yfx[plane + at + sj],
+                             sfy2[r + 1][c], a.ra_y[plane + at]);
         sqi[r][c] = v;
         if (own_i(gi) && own_j(gj)) st_nt(o.q_i + plane + at, v);
       }
@@ -136,9 +165,8 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn a, FvOut o, double p1,
       const int gi = i0 + c, gj = j0 - 3 + r;
       if (gi < nx && gj < ny + 2) {
         const int at = gi + gj * sj;
-        const double fxx0 = (a.xfx[plane + at] * sfx2[r][c]);
-        const double fxx1 = (a.xfx[plane + at + 1] * sfx2[r][c + 1]);
-        const double v = ((((sq[r][c + 3] * a.area[plane + at]) + fxx0) - fxx1) / a.ra_x[plane + at]);
+        const T v = flux_div(sq[r][c + 3], a.area[plane + at], a.xfx[plane + at], sfx2[r][c], a.xfx[plane + at + 1],
+                             sfx2[r][c + 1], a.ra_x[plane + at]);
         sqj[r][c] = v;
         if (own_i(gi) && own_j(gj)) st_nt(o.q_j + plane + at, v);
       }
@@ -151,12 +179,12 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn a, FvOut o, double p1,
       const int gi = i0 + c, gj = j0 + r;
       if (gi < nx && gj < ny) {
         const int64_t at = plane + gi + gj * sj;
-        const double *ri = &sqi[r][c];
-        const double fx1 = ppm_face(ri[0], ri[1], ri[2], ri[3], ri[4], ri[5], a.crx[at], p1, p2);
-        const double fy1 = ppm_face(sqj[r][c], sqj[r + 1][c], sqj[r + 2][c], sqj[r + 3][c], sqj[r + 4][c],
-                                    sqj[r + 5][c], a.cry[at], p1, p2);
-        st_nt(o.fx1 + at, ((0.5 * (fx1 + sfx2[r + 3][c])) * a.xfx[at]));
-        st_nt(o.fy1 + at, ((0.5 * (fy1 + sfy2[r][c + 3])) * a.yfx[at]));
+        const T *ri = &sqi[r][c];
+        const T fx1 = ppm_face(ri[0], ri[1], ri[2], ri[3], ri[4], ri[5], a.crx[at], p1, p2);
+        const T fy1 = ppm_face(sqj[r][c], sqj[r + 1][c], sqj[r + 2][c], sqj[r + 3][c], sqj[r + 4][c], sqj[r + 5][c],
+                               a.cry[at], p1, p2);
+        st_nt(o.fx1 + at, ((T(0.5) * (fx1 + sfx2[r + 3][c])) * a.xfx[at]));
+        st_nt(o.fy1 + at, ((T(0.5) * (fy1 + sfy2[r][c + 3])) * a.yfx[at]));
       }
     }
     // The next level's phase A writes only sq, which phase D does not read;
@@ -164,17 +192,10 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn a, FvOut o, double p1,
   }
 }
 
-}  // namespace
-}  // namespace oeb
-
-extern "C" int oeb_fvtp2d_f64(const oeb_domain *d, const oeb_s3 *q_i, const oeb_s3 *q_j, const oeb_s3 *fx1,
-                              const oeb_s3 *fx2, const oeb_s3 *fy1, const oeb_s3 *fy2, const oeb_s3 *q,
-                              const oeb_s3 *crx, const oeb_s3 *cry, const oeb_s3 *ra_x, const oeb_s3 *ra_y,
-                              const oeb_s3 *xfx, const oeb_s3 *yfx, const oeb_s3 *area, const oeb_s3 *fxx,
-                              const oeb_s3 *fyy, const oeb_s3 *al, const oeb_s3 *almq, const oeb_s3 *br,
-                              const oeb_s3 *b0, const oeb_s3 *smt5, double p1, double p2, void *stream) {
-  using namespace oeb;
-  (void)fxx, (void)fyy, (void)al, (void)almq, (void)br, (void)b0, (void)smt5;
+template <class T, class S>
+int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, const S *fx2, const S *fy1,
+               const S *fy2, const S *q, const S *crx, const S *cry, const S *ra_x, const S *ra_y, const S *xfx,
+               const S *yfx, const S *area, T p1, T p2, void *stream) {
   const char *K = "fvtp2d";
   OEB_TRY(check_domain(K, d));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
@@ -185,9 +206,9 @@ extern "C" int oeb_fvtp2d_f64(const oeb_domain *d, const oeb_s3 *q_i, const oeb_
   const Box r_q{-3, nx + 3, -3, ny + 3, 0, nz}, r_area{-3, nx + 2, -3, ny + 2, 0, nz};
   // yfx is read at (i, j+1) for q_i rows, xfx at (i+1, j) for q_j columns: both
   // stay inside the fy2 / fx2 regions.
-  FvIn a;
-  FvOut o;
-  const double *pq_i, *pq_j, *pfx1, *pfx2, *pfy1, *pfy2;
+  FvIn<T> a;
+  FvOut<T> o;
+  const T *pq_i, *pq_j, *pfx1, *pfx2, *pfy1, *pfy2;
   OEB_TRY(check3(K, "q_i", q_i, r_qi, g, true, &pq_i));
   OEB_TRY(check3(K, "q_j", q_j, r_qj, g, false, &pq_j));
   OEB_TRY(check3(K, "fx1", fx1, r0, g, false, &pfx1));
@@ -218,15 +239,43 @@ extern "C" int oeb_fvtp2d_f64(const oeb_domain *d, const oeb_s3 *q_i, const oeb_
   as.add("yfx", a.yfx, r_fy2, g, false);
   as.add("area", a.area, r_area, g, false);
   OEB_TRY(as.check(K));
-  o.q_i = const_cast<double *>(pq_i);
-  o.q_j = const_cast<double *>(pq_j);
-  o.fx1 = const_cast<double *>(pfx1);
-  o.fx2 = const_cast<double *>(pfx2);
-  o.fy1 = const_cast<double *>(pfy1);
-  o.fy2 = const_cast<double *>(pfy2);
+  o.q_i = const_cast<T *>(pq_i);
+  o.q_j = const_cast<T *>(pq_j);
+  o.fx1 = const_cast<T *>(pfx1);
+  o.fx2 = const_cast<T *>(pfx2);
+  o.fy1 = const_cast<T *>(pfy1);
+  o.fy2 = const_cast<T *>(pfy2);
 
   const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJdefault, kchunk_for(K, kChunk));
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, kTJdefault));
-  hipLaunchKernelGGL(fvtp2d_kernel<kTJdefault>, dim3(t.n), dim3(kNT), 0, hipStream_t(stream), a, o, p1, p2, g, t);
+  hipLaunchKernelGGL((fvtp2d_kernel<kTJdefault, T>), dim3(t.n), dim3(kNT), 0, hipStream_t(stream), a, o, p1, p2, g,
+                     t);
   return launch_status(K);
+}
+
+}  // namespace
+}  // namespace oeb
+
+// fxx, fyy, al, almq, br, b0 and smt5 are the reference's scratch arrays;
+// nothing is stored in them.
+extern "C" int oeb_fvtp2d_f64(const oeb_domain *d, const oeb_s3 *q_i, const oeb_s3 *q_j, const oeb_s3 *fx1,
+                              const oeb_s3 *fx2, const oeb_s3 *fy1, const oeb_s3 *fy2, const oeb_s3 *q,
+                              const oeb_s3 *crx, const oeb_s3 *cry, const oeb_s3 *ra_x, const oeb_s3 *ra_y,
+                              const oeb_s3 *xfx, const oeb_s3 *yfx, const oeb_s3 *area, const oeb_s3 *fxx,
+                              const oeb_s3 *fyy, const oeb_s3 *al, const oeb_s3 *almq, const oeb_s3 *br,
+                              const oeb_s3 *b0, const oeb_s3 *smt5, double p1, double p2, void *stream) {
+  (void)fxx, (void)fyy, (void)al, (void)almq, (void)br, (void)b0, (void)smt5;
+  return oeb::fvtp2d_run<double>(d, q_i, q_j, fx1, fx2, fy1, fy2, q, crx, cry, ra_x, ra_y, xfx, yfx, area, p1, p2,
+                                 stream);
+}
+
+extern "C" int oeb_fvtp2d_f32(const oeb_domain *d, const oeb_s3f *q_i, const oeb_s3f *q_j, const oeb_s3f *fx1,
+                              const oeb_s3f *fx2, const oeb_s3f *fy1, const oeb_s3f *fy2, const oeb_s3f *q,
+                              const oeb_s3f *crx, const oeb_s3f *cry, const oeb_s3f *ra_x, const oeb_s3f *ra_y,
+                              const oeb_s3f *xfx, const oeb_s3f *yfx, const oeb_s3f *area, const oeb_s3f *fxx,
+                              const oeb_s3f *fyy, const oeb_s3f *al, const oeb_s3f *almq, const oeb_s3f *br,
+                              const oeb_s3f *b0, const oeb_s3f *smt5, float p1, float p2, void *stream) {
+  (void)fxx, (void)fyy, (void)al, (void)almq, (void)br, (void)b0, (void)smt5;
+  return oeb::fvtp2d_run<float>(d, q_i, q_j, fx1, fx2, fy1, fy2, q, crx, cry, ra_x, ra_y, xfx, yfx, area, p1, p2,
+                                stream);
 }

@@ -7,37 +7,41 @@
 // The reference has both helpers under one name; here they are upwind3 and
 // upwind5.  The branch on the sign of the averaged velocity (strict > 0, zero
 // takes the second form) is kept.
-// Algorithmic traffic: 2 inputs + 2 outputs = 32 B per point (FP64).
+// Algorithmic traffic: 2 inputs + 2 outputs = 32 B per point (FP64), 16 B (FP32).
 #include "oeb_common.hpp"
 
 namespace oeb {
 namespace {
 
-constexpr int kChunk = 1;
-constexpr double kEarthRadiusRecip = 1.0 / 6371.229e3;
+constexpr int kChunk = 4;
+
+// EARTH_RADIUS_RECIP, include/util.h:17-18, formed in the element type.
+template <class T>
+constexpr T kEarthRadiusRecip = T(1.0) / T(6371.229e3);
 
 // 3rd-order upwind along the axis of stride s (kernels/hadvuv.h:10-24).
-__device__ __forceinline__ double upwind3_axis(const double *__restrict__ f, int s, double a) {
+template <class T>
+__device__ __forceinline__ T upwind3_axis(const T *f, int s, T a) {
   if (a > 0) {
-    return a * (-1.0 / 6.0 * f[-2 * s] + f[-s] + -0.5 * f[0] + -1.0 / 3.0 * f[s]);
+    return a * (T(-1.0) / T(6.0) * f[-2 * s] + f[-s] + T(-0.5) * f[0] + T(-1.0) / T(3.0) * f[s]);
   }
-  return -a * (-1.0 / 3.0 * f[-s] + -0.5 * f[0] + f[s] + -1.0 / 6.0 * f[2 * s]);
+  return -a * (T(-1.0) / T(3.0) * f[-s] + T(-0.5) * f[0] + f[s] + T(-1.0) / T(6.0) * f[2 * s]);
 }
 
 // 5th-order upwind along the axis of stride s (kernels/hadvuv5th.h:10-36).
-__device__ __forceinline__ double upwind5_axis(const double *__restrict__ f, int s, double a) {
+template <class T>
+__device__ __forceinline__ T upwind5_axis(const T *f, int s, T a) {
   if (a > 0) {
-    return a * (1.0 / 30.0 * f[-3 * s] + -1.0 / 4.0 * f[-2 * s] + f[-s] + -1.0 / 3.0 * f[0] + -1.0 / 2.0 * f[s] +
-                1.0 / 20.0 * f[2 * s]);
+    return a * (T(1.0) / T(30.0) * f[-3 * s] + T(-1.0) / T(4.0) * f[-2 * s] + f[-s] + T(-1.0) / T(3.0) * f[0] +
+                T(-1.0) / T(2.0) * f[s] + T(1.0) / T(20.0) * f[2 * s]);
   }
-  return -a * (1.0 / 20.0 * f[-2 * s] + -1.0 / 2.0 * f[-s] + -1.0 / 3.0 * f[0] + f[s] + -1.0 / 4.0 * f[2 * s] +
-               1.0 / 30.0 * f[3 * s]);
+  return -a * (T(1.0) / T(20.0) * f[-2 * s] + T(-1.0) / T(2.0) * f[-s] + T(-1.0) / T(3.0) * f[0] + f[s] +
+               T(-1.0) / T(4.0) * f[2 * s] + T(1.0) / T(30.0) * f[3 * s]);
 }
 
-template <int ORDER>
-__device__ __forceinline__ double upwind(const double *__restrict__ f, int sj, double ua, double va, double eddlat,
-                                         double eddlon) {
-  double rx, ry;
+template <int ORDER, class T>
+__device__ __forceinline__ T upwind(const T *f, int sj, T ua, T va, T eddlat, T eddlon) {
+  T rx, ry;
   if (ORDER == 3) {
     rx = upwind3_axis(f, 1, ua);
     ry = upwind3_axis(f, sj, va);
@@ -48,69 +52,120 @@ __device__ __forceinline__ double upwind(const double *__restrict__ f, int sj, d
   return eddlat * rx + eddlon * ry;
 }
 
-// One level of a 64 x 16 tile per workgroup.  u and v are staged in LDS with
-// the stencil's halo (R = 2 or 3), so each value is read from memory once per
-// tile.  The 14 neighbour reads of each field per point then come from LDS:
-// read through L1 they made this stencil TA-bound, and the LDS version
-// measured 1.67x faster at 256x256x80 and 512x512x80.  256 lanes; each lane
-// computes 4 rows of one column.
-constexpr int kTI = 64, kTJ = 16, kNT = 256;
+// Tiles of 64 x 16 columns, 256 lanes; each lane computes 4 rows of one
+// column.  u and v are staged in LDS with the stencil's halo (R = 2 or 3), so
+// each value is read from memory once per tile and the 14 neighbour reads of
+// each field per point come from LDS.  Read through L1, those reads made the
+// stencil TA-bound; LDS measured 1.67x faster.  The workgroup walks a chunk
+// of levels with two LDS buffers: while it computes level k from one, the
+// loads of level k+1 are already in flight into registers, and are written
+// to the other buffer after the compute.  The loads' latency then overlaps the
+// arithmetic.  Against one buffer per level this measured 1.2-1.25x faster at
+// 256x256x80 and 512x512x80 (5th order).
+constexpr int kTI = 64, kTJ = 16, kNT = 256, kRows = kTJ / (kNT / kTI);
 
-template <int ORDER>
+template <int ORDER, class T>
 __global__ void __launch_bounds__(kNT)
-    hadvuv_kernel(double *__restrict__ uout, double *__restrict__ vout, const double *__restrict__ uin,
-                  const double *__restrict__ vin, const double *__restrict__ acrlat0, int s0,
-                  const double *__restrict__ acrlat1, int s1, const double *__restrict__ tgrlatda0, int s2,
-                  const double *__restrict__ tgrlatda1, int s3, double eddlat, double eddlon, Geo g, Tiles t) {
+    hadvuv_kernel(T *__restrict__ uout, T *__restrict__ vout, const T *__restrict__ uin, const T *__restrict__ vin,
+                  const T *__restrict__ acrlat0, int s0, const T *__restrict__ acrlat1, int s1,
+                  const T *__restrict__ tgrlatda0, int s2, const T *__restrict__ tgrlatda1, int s3, T eddlat,
+                  T eddlon, Geo g, Tiles t) {
   constexpr int R = ORDER == 3 ? 2 : 3;
-  constexpr int W = kTI + 2 * R;
-  __shared__ double su[kTJ + 2 * R][W];
-  __shared__ double sv[kTJ + 2 * R][W];
+  constexpr int W = kTI + 2 * R, H = kTJ + 2 * R, kPer = (W * H + kNT - 1) / kNT;
+  constexpr T rr = kEarthRadiusRecip<T>;
+  __shared__ T su[2][H][W];
+  __shared__ T sv[2][H][W];
   const TileCoord c = tile_coord(t, kTI, kTJ, g.nz);
   const int tid = int(threadIdx.x) + kTI * int(threadIdx.y);
   const int sj = g.sj;
-  for (int k = c.k0; k < c.k1; ++k) {
-    const int64_t plane = int64_t(k) * g.sk;
-    stage_tile<kTI, kTJ, R, kNT>(su, uin + plane, c.i0, c.j0, sj, g.nx + R, g.ny + R, tid);
-    stage_tile<kTI, kTJ, R, kNT>(sv, vin + plane, c.i0, c.j0, sj, g.nx + R, g.ny + R, tid);
-    __syncthreads();
-    const int i = c.i0 + int(threadIdx.x);
+  const int i = c.i0 + int(threadIdx.x);
+
+  // Metric coefficients of this lane's rows (the clamp keeps rows past ny,
+  // which compute nothing, on a valid index).
+  T a0[kRows], a1[kRows], tg0[kRows], tg1[kRows];
 #pragma unroll
-    for (int t4 = 0; t4 < kTJ / 4; ++t4) {
-      const int lj = int(threadIdx.y) * (kTJ / 4) + t4;
-      const int j = c.j0 + lj;
-      if (i < g.nx && j < g.ny) {
-        const double *u = &su[lj + R][threadIdx.x + R];
-        const double *v = &sv[lj + R][threadIdx.x + R];
-        const double a0 = acrlat0[int64_t(j) * s0], a1 = acrlat1[int64_t(j) * s1];
-        const double tg0 = tgrlatda0[int64_t(j) * s2], tg1 = tgrlatda1[int64_t(j) * s3];
-        const int64_t at = plane + i + int64_t(j) * sj;
-        // u component: kernels/hadvuv.h:186-195.
-        const double uatupos = (1.0 / 3.0) * (u[-1] + u[0] + u[1]);
-        const double vatupos = 0.25 * (v[1] + v[1 - W] + v[0] + v[-W]);
-        const double ures = upwind<ORDER>(u, W, a0 * uatupos, kEarthRadiusRecip * vatupos, eddlat, eddlon);
-        st_nt(uout + at, ures + tg0 * u[0] * vatupos);
-        // v component: kernels/hadvuv.h:197-206.
-        const double uatvpos = 0.25 * (u[-1] + u[0] + u[W] + u[W - 1]);
-        const double vatvpos = 1.0 / 3.0 * (v[-W] + v[0] + v[W]);
-        const double vres = upwind<ORDER>(v, W, a1 * uatvpos, kEarthRadiusRecip * vatvpos, eddlat, eddlon);
-        st_nt(vout + at, vres - tg1 * uatvpos * uatvpos);
+  for (int r = 0; r < kRows; ++r) {
+    const int64_t j = min(c.j0 + int(threadIdx.y) * kRows + r, g.ny - 1);
+    a0[r] = acrlat0[j * s0], a1[r] = acrlat1[j * s1];
+    tg0[r] = tgrlatda0[j * s2], tg1[r] = tgrlatda1[j * s3];
+  }
+  // This lane's share of the staged tile: element e = tid + q*kNT of the
+  // H x W tile, at in-plane offset off[q].  Elements past the field's valid
+  // box (i >= nx + R or j >= ny + R) are set to 0 and never used.
+  int off[kPer];
+  bool ok[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int e = tid + q * kNT, r = e / W, col = e % W;
+    const int gi = c.i0 - R + col, gj = c.j0 - R + r;
+    ok[q] = e < W * H && gi < g.nx + R && gj < g.ny + R;
+    off[q] = ok[q] ? gi + gj * sj : 0;
+  }
+  T ru[kPer], rv[kPer];
+  auto load = [&](int64_t plane) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      ru[q] = ok[q] ? uin[plane + off[q]] : T(0);
+      rv[q] = ok[q] ? vin[plane + off[q]] : T(0);
+    }
+  };
+  auto store = [&](int b) {
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = tid + q * kNT;
+      if (e < W * H) {
+        su[b][e / W][e % W] = ru[q];
+        sv[b][e / W][e % W] = rv[q];
       }
     }
-    if (k + 1 < c.k1) __syncthreads();
+  };
+
+  load(int64_t(c.k0) * g.sk);
+  store(0);
+  __syncthreads();
+  int cur = 0;
+  for (int k = c.k0; k < c.k1; ++k) {
+    const int64_t plane = int64_t(k) * g.sk;
+    const bool more = k + 1 < c.k1;
+    if (more) load(plane + g.sk);
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+      const int lj = int(threadIdx.y) * kRows + r;
+      const int j = c.j0 + lj;
+      if (i < g.nx && j < g.ny) {
+        const T *u = &su[cur][lj + R][threadIdx.x + R];
+        const T *v = &sv[cur][lj + R][threadIdx.x + R];
+        const int64_t at = plane + i + int64_t(j) * sj;
+        // u component: kernels/hadvuv.h:186-195.
+        const T uatupos = (T(1.0) / T(3.0)) * (u[-1] + u[0] + u[1]);
+        const T vatupos = T(0.25) * (v[1] + v[1 - W] + v[0] + v[-W]);
+        const T ures = upwind<ORDER>(u, W, a0[r] * uatupos, rr * vatupos, eddlat, eddlon);
+        st_nt(uout + at, ures + tg0[r] * u[0] * vatupos);
+        // v component: kernels/hadvuv.h:197-206.
+        const T uatvpos = T(0.25) * (u[-1] + u[0] + u[W] + u[W - 1]);
+        const T vatvpos = T(1.0) / T(3.0) * (v[-W] + v[0] + v[W]);
+        const T vres = upwind<ORDER>(v, W, a1[r] * uatvpos, rr * vatvpos, eddlat, eddlon);
+        st_nt(vout + at, vres - tg1[r] * uatvpos * uatvpos);
+      }
+    }
+    // Buffer cur^1 was last read before the previous barrier, so it can be
+    // overwritten now; the barrier below publishes it for the next level.
+    if (more) store(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
   }
 }
 
-template <int ORDER>
-int hadv_entry(const char *K, const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
-               const oeb_s3 *vin, const oeb_s1 *acrlat0, const oeb_s1 *acrlat1, const oeb_s1 *tgrlatda0,
-               const oeb_s1 *tgrlatda1, double eddlat, double eddlon, void *stream) {
+template <int ORDER, class T, class S3, class S1>
+int hadv_run(const char *K, const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *uin, const S3 *vin,
+             const S1 *acrlat0, const S1 *acrlat1, const S1 *tgrlatda0, const S1 *tgrlatda1, T eddlat, T eddlon,
+             void *stream) {
   OEB_TRY(check_domain(K, d));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz, h = ORDER == 3 ? 2 : 3;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz};
   const Box r_in{-h, nx + h, -h, ny + h, 0, nz};
-  const double *p_uo, *p_vo, *p_ui, *p_vi, *p_a0, *p_a1, *p_t0, *p_t1;
+  const T *p_uo, *p_vo, *p_ui, *p_vi, *p_a0, *p_a1, *p_t0, *p_t1;
   int s0 = 1, s1 = 1, s2 = 1, s3 = 1;
   OEB_TRY(check3(K, "uout", uout, r0, g, true, &p_uo));
   OEB_TRY(check3(K, "vout", vout, r0, g, false, &p_vo));
@@ -132,15 +187,16 @@ int hadv_entry(const char *K, const oeb_domain *d, const oeb_s3 *uout, const oeb
   OEB_TRY(as.check(K));
   const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJ, kchunk_for(K, kChunk));
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, kTJ));
-  hipLaunchKernelGGL(hadvuv_kernel<ORDER>, dim3(t.n), dim3(kTI, kNT / kTI), 0, hipStream_t(stream),
-                     const_cast<double *>(p_uo), const_cast<double *>(p_vo), p_ui, p_vi, p_a0, s0, p_a1, s1, p_t0, s2,
-                     p_t1, s3, eddlat, eddlon, g, t);
+  hipLaunchKernelGGL((hadvuv_kernel<ORDER, T>), dim3(t.n), dim3(kTI, kNT / kTI), 0, hipStream_t(stream),
+                     const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_a0, s0, p_a1, s1, p_t0, s2, p_t1,
+                     s3, eddlat, eddlon, g, t);
   return launch_status(K);
 }
 
 }  // namespace
 }  // namespace oeb
 
+// uatupos .. vres are the reference's scratch arrays; nothing is stored in them.
 extern "C" int oeb_hadvuv_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
                               const oeb_s3 *vin, const oeb_s1 *acrlat0, const oeb_s1 *acrlat1,
                               const oeb_s1 *tgrlatda0, const oeb_s1 *tgrlatda1, const oeb_s3 *uatupos,
@@ -148,8 +204,8 @@ extern "C" int oeb_hadvuv_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb
                               const oeb_s3 *vavg, const oeb_s3 *ures, const oeb_s3 *vres, double eddlat,
                               double eddlon, void *stream) {
   (void)uatupos, (void)vatupos, (void)uatvpos, (void)vatvpos, (void)uavg, (void)vavg, (void)ures, (void)vres;
-  return oeb::hadv_entry<3>("hadvuv", d, uout, vout, uin, vin, acrlat0, acrlat1, tgrlatda0, tgrlatda1, eddlat,
-                            eddlon, stream);
+  return oeb::hadv_run<3>("hadvuv", d, uout, vout, uin, vin, acrlat0, acrlat1, tgrlatda0, tgrlatda1, eddlat, eddlon,
+                          stream);
 }
 
 extern "C" int oeb_hadvuv5th_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
@@ -159,6 +215,29 @@ extern "C" int oeb_hadvuv5th_f64(const oeb_domain *d, const oeb_s3 *uout, const 
                                  const oeb_s3 *uavg, const oeb_s3 *vavg, const oeb_s3 *ures, const oeb_s3 *vres,
                                  double eddlat, double eddlon, void *stream) {
   (void)uatupos, (void)vatupos, (void)uatvpos, (void)vatvpos, (void)uavg, (void)vavg, (void)ures, (void)vres;
-  return oeb::hadv_entry<5>("hadvuv5th", d, uout, vout, uin, vin, acrlat0, acrlat1, tgrlatda0, tgrlatda1, eddlat,
-                            eddlon, stream);
+  return oeb::hadv_run<5>("hadvuv5th", d, uout, vout, uin, vin, acrlat0, acrlat1, tgrlatda0, tgrlatda1, eddlat,
+                          eddlon, stream);
+}
+
+extern "C" int oeb_hadvuv_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
+                              const oeb_s3f *vin, const oeb_s1f *acrlat0, const oeb_s1f *acrlat1,
+                              const oeb_s1f *tgrlatda0, const oeb_s1f *tgrlatda1, const oeb_s3f *uatupos,
+                              const oeb_s3f *vatupos, const oeb_s3f *uatvpos, const oeb_s3f *vatvpos,
+                              const oeb_s3f *uavg, const oeb_s3f *vavg, const oeb_s3f *ures, const oeb_s3f *vres,
+                              float eddlat, float eddlon, void *stream) {
+  (void)uatupos, (void)vatupos, (void)uatvpos, (void)vatvpos, (void)uavg, (void)vavg, (void)ures, (void)vres;
+  return oeb::hadv_run<3>("hadvuv", d, uout, vout, uin, vin, acrlat0, acrlat1, tgrlatda0, tgrlatda1, eddlat, eddlon,
+                          stream);
+}
+
+extern "C" int oeb_hadvuv5th_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout,
+                                 const oeb_s3f *uin, const oeb_s3f *vin, const oeb_s1f *acrlat0,
+                                 const oeb_s1f *acrlat1, const oeb_s1f *tgrlatda0, const oeb_s1f *tgrlatda1,
+                                 const oeb_s3f *uatupos, const oeb_s3f *vatupos, const oeb_s3f *uatvpos,
+                                 const oeb_s3f *vatvpos, const oeb_s3f *uavg, const oeb_s3f *vavg,
+                                 const oeb_s3f *ures, const oeb_s3f *vres, float eddlat, float eddlon,
+                                 void *stream) {
+  (void)uatupos, (void)vatupos, (void)uatvpos, (void)vatvpos, (void)uavg, (void)vavg, (void)ures, (void)vres;
+  return oeb::hadv_run<5>("hadvuv5th", d, uout, vout, uin, vin, acrlat0, acrlat1, tgrlatda0, tgrlatda1, eddlat,
+                          eddlon, stream);
 }

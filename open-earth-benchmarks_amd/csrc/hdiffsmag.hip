@@ -6,83 +6,85 @@
 // (:239-244, :254-259).  Both clamps are the C++ library's std::min and
 // std::max, written as the same comparisons.
 // Reach (1,1,0) plus the diagonals (i+1,j-1) and (i-1,j+1).
-// Algorithmic traffic: 3 inputs + 2 outputs = 40 B per point (FP64).
+// Algorithmic traffic: 3 inputs + 2 outputs = 40 B per point (FP64), 20 B (FP32).
 #include "oeb_common.hpp"
 
 namespace oeb {
 namespace {
 
-constexpr int kTileI = 64, kTileJ = 4, kChunk = 8;
-// EARTH_RADIUS_RECIP, include/util.h:17-18.
-constexpr double kEarthRadiusRecip = 1.0 / 6371.229e3;
+constexpr int kTileI = 64, kTileJ = 4, kChunk = 32;
+// EARTH_RADIUS_RECIP, include/util.h:17-18: both operands are ElementType,
+// so the quotient is formed in the element type.
+template <class T>
+constexpr T kEarthRadiusRecip = T(1.0) / T(6371.229e3);
 
 // std::max(a, b) / std::min(a, b) exactly as the C++ library defines them.
-__device__ __forceinline__ double cxx_max(double a, double b) { return (a < b) ? b : a; }
-__device__ __forceinline__ double cxx_min(double a, double b) { return (b < a) ? b : a; }
+template <class T>
+__device__ __forceinline__ T cxx_max(T a, T b) {
+  return (a < b) ? b : a;
+}
+template <class T>
+__device__ __forceinline__ T cxx_min(T a, T b) {
+  return (b < a) ? b : a;
+}
 
+template <class T>
 __global__ void __launch_bounds__(kTileI *kTileJ)
-    hdiffsmag_kernel(double *__restrict__ uout, double *__restrict__ vout, const double *__restrict__ uin,
-                     const double *__restrict__ vin, const double *__restrict__ mask,
-                     const double *__restrict__ crlavo, int s_vo, const double *__restrict__ crlavu, int s_vu,
-                     const double *__restrict__ crlato, int s_to, const double *__restrict__ crlatu, int s_tu,
-                     const double *__restrict__ acrlat0, int s_ac, double eddlat, double eddlon, double tau_smag,
-                     double weight_smag, Geo g, Tiles t) {
+    hdiffsmag_kernel(T *__restrict__ uout, T *__restrict__ vout, const T *__restrict__ uin,
+                     const T *__restrict__ vin, const T *__restrict__ mask, const T *__restrict__ crlavo, int s_vo,
+                     const T *__restrict__ crlavu, int s_vu, const T *__restrict__ crlato, int s_to,
+                     const T *__restrict__ crlatu, int s_tu, const T *__restrict__ acrlat0, int s_ac, T eddlat,
+                     T eddlon, T tau_smag, T weight_smag, Geo g, Tiles t) {
+  constexpr T rr = kEarthRadiusRecip<T>;
   const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
-  const double ac_s = acrlat0[int64_t(j - 1) * s_ac], ac_c = acrlat0[int64_t(j) * s_ac],
-               ac_n = acrlat0[int64_t(j + 1) * s_ac];
-  const double lo = crlato[int64_t(j) * s_to], lu = crlatu[int64_t(j) * s_tu];
-  const double vo = crlavo[int64_t(j) * s_vo], vu = crlavu[int64_t(j) * s_vu];
+  const T ac_s = acrlat0[int64_t(j - 1) * s_ac], ac_c = acrlat0[int64_t(j) * s_ac],
+          ac_n = acrlat0[int64_t(j + 1) * s_ac];
+  const T lo = crlato[int64_t(j) * s_to], lu = crlatu[int64_t(j) * s_tu];
+  const T vo = crlavo[int64_t(j) * s_vo], vu = crlavu[int64_t(j) * s_vu];
   int64_t at = int64_t(c.k0) * g.sk + (i + j * sj);
   for (int k = c.k0; k < c.k1; ++k, at += g.sk) {
-    const double *u = uin + at;
-    const double *v = vin + at;
+    const T *u = uin + at;
+    const T *v = vin + at;
     // Tension at (p,q) = (v(p,q-1) - v(p,q)) * eddlat * R^-1 - (u(p-1,q) - u(p,q)) * acrlat0(q) * eddlon.
-    const double t0 = (v[-sj] - v[0]) * eddlat * kEarthRadiusRecip - (u[-1] - u[0]) * ac_c * eddlon;
-    const double t_ip = (v[1 - sj] - v[1]) * eddlat * kEarthRadiusRecip - (u[0] - u[1]) * ac_c * eddlon;
-    const double t_jp = (v[0] - v[sj]) * eddlat * kEarthRadiusRecip - (u[sj - 1] - u[sj]) * ac_n * eddlon;
+    const T t0 = (v[-sj] - v[0]) * eddlat * rr - (u[-1] - u[0]) * ac_c * eddlon;
+    const T t_ip = (v[1 - sj] - v[1]) * eddlat * rr - (u[0] - u[1]) * ac_c * eddlon;
+    const T t_jp = (v[0] - v[sj]) * eddlat * rr - (u[sj - 1] - u[sj]) * ac_n * eddlon;
     // Shear at (p,q) = (u(p,q+1) - u(p,q)) * eddlat * R^-1 + (v(p+1,q) - v(p,q)) * acrlat0(q) * eddlon.
-    const double s0 = (u[sj] - u[0]) * eddlat * kEarthRadiusRecip + (v[1] - v[0]) * ac_c * eddlon;
-    const double s_im = (u[sj - 1] - u[-1]) * eddlat * kEarthRadiusRecip + (v[0] - v[-1]) * ac_c * eddlon;
-    const double s_jm = (u[0] - u[-sj]) * eddlat * kEarthRadiusRecip + (v[1 - sj] - v[-sj]) * ac_s * eddlon;
-    const double tt0 = t0 * t0, tt_ip = t_ip * t_ip, tt_jp = t_jp * t_jp;
-    const double ss0 = s0 * s0, ss_im = s_im * s_im, ss_jm = s_jm * s_jm;
-    const double hdweight = weight_smag * ld_nt(mask + at);
+    const T s0 = (u[sj] - u[0]) * eddlat * rr + (v[1] - v[0]) * ac_c * eddlon;
+    const T s_im = (u[sj - 1] - u[-1]) * eddlat * rr + (v[0] - v[-1]) * ac_c * eddlon;
+    const T s_jm = (u[0] - u[-sj]) * eddlat * rr + (v[1 - sj] - v[-sj]) * ac_s * eddlon;
+    const T tt0 = t0 * t0, tt_ip = t_ip * t_ip, tt_jp = t_jp * t_jp;
+    const T ss0 = s0 * s0, ss_im = s_im * s_im, ss_jm = s_jm * s_jm;
+    const T hdweight = weight_smag * ld_nt(mask + at);
 
-    double smag_u = tau_smag * sqrt(0.5 * (tt_ip + tt0) + 0.5 * (ss_jm + ss0)) - hdweight;
-    smag_u = cxx_min(0.5, cxx_max(0.0, smag_u));
-    const double u0 = u[0];
-    const double lapu = u[1] + u[-1] - 2.0 * u0 + lo * (u[sj] - u0) + lu * (u[-sj] - u0);
+    T smag_u = tau_smag * sqrt(T(0.5) * (tt_ip + tt0) + T(0.5) * (ss_jm + ss0)) - hdweight;
+    smag_u = cxx_min(T(0.5), cxx_max(T(0.0), smag_u));
+    const T u0 = u[0];
+    const T lapu = u[1] + u[-1] - T(2.0) * u0 + lo * (u[sj] - u0) + lu * (u[-sj] - u0);
     st_nt(uout + at, u0 + smag_u * lapu);
 
-    double smag_v = tau_smag * sqrt(0.5 * (tt_jp + tt0) + 0.5 * (ss_im + ss0)) - hdweight;
-    smag_v = cxx_min(0.5, cxx_max(0.0, smag_v));
-    const double v0 = v[0];
-    const double lapv = v[1] + v[-1] - 2.0 * v0 + vo * (v[sj] - v0) + vu * (v[-sj] - v0);
+    T smag_v = tau_smag * sqrt(T(0.5) * (tt_jp + tt0) + T(0.5) * (ss_im + ss0)) - hdweight;
+    smag_v = cxx_min(T(0.5), cxx_max(T(0.0), smag_v));
+    const T v0 = v[0];
+    const T lapv = v[1] + v[-1] - T(2.0) * v0 + vo * (v[sj] - v0) + vu * (v[-sj] - v0);
     st_nt(vout + at, v0 + smag_v * lapv);
   }
 }
 
-}  // namespace
-}  // namespace oeb
-
-extern "C" int oeb_hdiffsmag_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
-                                 const oeb_s3 *vin, const oeb_s3 *mask, const oeb_s1 *crlavo, const oeb_s1 *crlavu,
-                                 const oeb_s1 *crlato, const oeb_s1 *crlatu, const oeb_s1 *acrlat0,
-                                 const oeb_s3 *T_sqr_s, const oeb_s3 *S_sqr_uv, double eddlat, double eddlon,
-                                 double tau_smag, double weight_smag, void *stream) {
-  using namespace oeb;
-  (void)T_sqr_s;
-  (void)S_sqr_uv;
+template <class T, class S3, class S1>
+int hdiffsmag_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *uin, const S3 *vin, const S3 *mask,
+                  const S1 *crlavo, const S1 *crlavu, const S1 *crlato, const S1 *crlatu, const S1 *acrlat0,
+                  T eddlat, T eddlon, T tau_smag, T weight_smag, void *stream) {
   const char *K = "hdiffsmag";
   OEB_TRY(check_domain(K, d));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz};
   const Box r_uv{-1, nx + 1, -1, ny + 1, 0, nz};
-  const double *p_uo, *p_vo, *p_ui, *p_vi, *p_m, *p_vo1, *p_vu1, *p_to1, *p_tu1, *p_ac;
+  const T *p_uo, *p_vo, *p_ui, *p_vi, *p_m, *p_vo1, *p_vu1, *p_to1, *p_tu1, *p_ac;
   int s_vo = 1, s_vu = 1, s_to = 1, s_tu = 1, s_ac = 1;
   OEB_TRY(check3(K, "uout", uout, r0, g, true, &p_uo));
   OEB_TRY(check3(K, "vout", vout, r0, g, false, &p_vo));
@@ -108,8 +110,35 @@ extern "C" int oeb_hdiffsmag_f64(const oeb_domain *d, const oeb_s3 *uout, const 
   OEB_TRY(as.check(K));
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kchunk_for(K, kChunk));
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-  hipLaunchKernelGGL(hdiffsmag_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
-                     const_cast<double *>(p_uo), const_cast<double *>(p_vo), p_ui, p_vi, p_m, p_vo1, s_vo, p_vu1, s_vu,
-                     p_to1, s_to, p_tu1, s_tu, p_ac, s_ac, eddlat, eddlon, tau_smag, weight_smag, g, t);
+  hipLaunchKernelGGL(hdiffsmag_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
+                     const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_m, p_vo1, s_vo, p_vu1, s_vu, p_to1,
+                     s_to, p_tu1, s_tu, p_ac, s_ac, eddlat, eddlon, tau_smag, weight_smag, g, t);
   return launch_status(K);
+}
+
+}  // namespace
+}  // namespace oeb
+
+// T_sqr_s and S_sqr_uv are the reference's scratch arrays; nothing is stored in them.
+extern "C" int oeb_hdiffsmag_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
+                                 const oeb_s3 *vin, const oeb_s3 *mask, const oeb_s1 *crlavo, const oeb_s1 *crlavu,
+                                 const oeb_s1 *crlato, const oeb_s1 *crlatu, const oeb_s1 *acrlat0,
+                                 const oeb_s3 *T_sqr_s, const oeb_s3 *S_sqr_uv, double eddlat, double eddlon,
+                                 double tau_smag, double weight_smag, void *stream) {
+  (void)T_sqr_s;
+  (void)S_sqr_uv;
+  return oeb::hdiffsmag_run<double>(d, uout, vout, uin, vin, mask, crlavo, crlavu, crlato, crlatu, acrlat0, eddlat,
+                                    eddlon, tau_smag, weight_smag, stream);
+}
+
+extern "C" int oeb_hdiffsmag_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout,
+                                 const oeb_s3f *uin, const oeb_s3f *vin, const oeb_s3f *mask,
+                                 const oeb_s1f *crlavo, const oeb_s1f *crlavu, const oeb_s1f *crlato,
+                                 const oeb_s1f *crlatu, const oeb_s1f *acrlat0, const oeb_s3f *T_sqr_s,
+                                 const oeb_s3f *S_sqr_uv, float eddlat, float eddlon, float tau_smag,
+                                 float weight_smag, void *stream) {
+  (void)T_sqr_s;
+  (void)S_sqr_uv;
+  return oeb::hdiffsmag_run<float>(d, uout, vout, uin, vin, mask, crlavo, crlavu, crlato, crlatu, acrlat0, eddlat,
+                                   eddlon, tau_smag, weight_smag, stream);
 }

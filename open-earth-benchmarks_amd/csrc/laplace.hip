@@ -1,34 +1,41 @@
 // laplace.hip -- 5-point horizontal Laplacian.
 // Reference: kernels/laplace.h:34-43 (laplace_openmp).  Expression :39,
 // parenthesization kept: out = -4*in + ((in[i-1] + in[i+1]) + (in[j+1] + in[j-1])).
-// Algorithmic traffic: 1 input + 1 output field = 16 B per point (FP64).
+// -4.0 is a double literal there, so with float storage the product and the
+// final sum are formed in double and rounded once on the store.
+// Algorithmic traffic: 1 input + 1 output field = 16 B per point (FP64), 8 B (FP32).
 #include "oeb_common.hpp"
 
 namespace oeb {
 namespace {
 
-constexpr int kTileI = 64, kTileJ = 4, kChunk = 2;
+constexpr int kTileI = 64, kTileJ = 4, kChunk = 1;
 
+template <class T>
+__device__ __forceinline__ T lap_point(T c, T w, T e, T n, T s) {
+  return T(-4.0 * c + ((w + e) + (n + s)));
+}
+
+template <class T>
 __global__ void __launch_bounds__(kTileI *kTileJ)
-    laplace_kernel(const double *__restrict__ in, double *__restrict__ out, Geo g, Tiles t) {
+    laplace_kernel(const T *__restrict__ in, T *__restrict__ out, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
   const int64_t at = int64_t(c.k0) * g.sk + (i + j * sj);
-  const double *p = in + at;
-  double *q = out + at;
+  const T *p = in + at;
+  T *q = out + at;
   for (int k = c.k0; k < c.k1; ++k) {
-    const double w = p[-1], e = p[1], n = p[sj], s = p[-sj], cc = p[0];
-    st_nt(q, -4.0 * cc + ((w + e) + (n + s)));
+    st_nt(q, lap_point<T>(p[0], p[-1], p[1], p[sj], p[-sj]));
     p += g.sk;
     q += g.sk;
   }
 }
 
-// Two adjacent columns (i, i+1) per lane, i even: 16-byte loads and stores.
-// Same arithmetic as laplace_kernel: for the pair, in[i-1] and in[i+2] are
-// single loads and the inner neighbours come from the pair itself.
+// Two adjacent columns (i, i+1) per lane, i even: 16-byte loads and stores
+// (FP64 only).  Same arithmetic as laplace_kernel: for the pair, in[i-1] and
+// in[i+2] are single loads and the inner neighbours come from the pair.
 constexpr int kPairI = 2 * kTileI;
 
 __global__ void __launch_bounds__(kTileI *kTileJ)
@@ -43,23 +50,20 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
   for (int k = c.k0; k < c.k1; ++k) {
     const d2 cc = ld2(p), n = ld2(p + sj), s = ld2(p - sj);
     const double w = p[-1], e = p[2];
-    st2_nt(q, -4.0 * cc.x + ((w + cc.y) + (n.x + s.x)), -4.0 * cc.y + ((cc.x + e) + (n.y + s.y)));
+    st2_nt(q, lap_point(cc.x, w, cc.y, n.x, s.x), lap_point(cc.y, cc.x, e, n.y, s.y));
     p += g.sk;
     q += g.sk;
   }
 }
 
-}  // namespace
-}  // namespace oeb
-
-extern "C" int oeb_laplace_f64(const oeb_domain *d, const oeb_s3 *in, const oeb_s3 *out, void *stream) {
-  using namespace oeb;
+template <class T, class S>
+int laplace_run(const oeb_domain *d, const S *in, const S *out, void *stream) {
   const char *K = "laplace";
   OEB_TRY(check_domain(K, d));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r_in{-1, nx + 1, -1, ny + 1, 0, nz}, r_out{0, nx, 0, ny, 0, nz};
-  const double *p_in, *p_out;
+  const T *p_in, *p_out;
   OEB_TRY(check3(K, "in", in, r_in, g, true, &p_in));
   OEB_TRY(check3(K, "out", out, r_out, g, false, &p_out));
   AliasSet as;
@@ -67,16 +71,29 @@ extern "C" int oeb_laplace_f64(const oeb_domain *d, const oeb_s3 *in, const oeb_
   as.add("out", p_out, r_out, g, true);
   OEB_TRY(as.check(K));
   const int kc = kchunk_for(K, kChunk);
-  if (nx % 2 == 0 && pairs_ok(g, {p_in, p_out})) {
-    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-    hipLaunchKernelGGL(laplace_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), p_in,
-                       const_cast<double *>(p_out), g, t);
-  } else {
-    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-    hipLaunchKernelGGL(laplace_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), p_in,
-                       const_cast<double *>(p_out), g, t);
+  if constexpr (sizeof(T) == 8) {
+    if (pairs_for(K, true) && nx % 2 == 0 && pairs_ok(g, {p_in, p_out})) {
+      const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
+      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+      hipLaunchKernelGGL(laplace_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), p_in,
+                         const_cast<T *>(p_out), g, t);
+      return launch_status(K);
+    }
   }
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
+  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+  hipLaunchKernelGGL(laplace_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), p_in,
+                     const_cast<T *>(p_out), g, t);
   return launch_status(K);
+}
+
+}  // namespace
+}  // namespace oeb
+
+extern "C" int oeb_laplace_f64(const oeb_domain *d, const oeb_s3 *in, const oeb_s3 *out, void *stream) {
+  return oeb::laplace_run<double>(d, in, out, stream);
+}
+
+extern "C" int oeb_laplace_f32(const oeb_domain *d, const oeb_s3f *in, const oeb_s3f *out, void *stream) {
+  return oeb::laplace_run<float>(d, in, out, stream);
 }

@@ -4,13 +4,17 @@
 // and north points.  Reach: i, j in [0, +1], k in [0, +1].  Each lane marches
 // up its column and carries the level k+1 values of gz, pk3 and pp over to the
 // next step.
-// Algorithmic traffic: 8 inputs + 2 outputs = 80 B per point (FP64).
+// FP32 form: gz, pp and pk3 stay FP64 arrays.  wk, the pressure-thickness
+// sums, du/dv and the pp bracket are formed in FP64; du/dv and the bracket are
+// rounded once to float, and the rest is float.
+// Algorithmic traffic: 8 inputs + 2 outputs = 80 B per point (FP64); 52 B in
+// the FP32 form (gz, pp and pk3 8 B, the rest 4 B).
 #include "oeb_common.hpp"
 
 namespace oeb {
 namespace {
 
-constexpr int kTileI = 64, kTileJ = 4, kChunk = 16;
+constexpr int kTileI = 64, kTileJ = 4, kChunk = 8;
 constexpr int kPairI = 2 * kTileI;
 
 // Level-k and level-(k+1) values of one field at the centre C = (i,j), the
@@ -20,24 +24,28 @@ struct CEN {
 };
 
 // One point (kernels/nh_p_grad.h:123-142).
-__device__ __forceinline__ void nh_point(const CEN &gz, const CEN &pk, const CEN &pp, double w1C, double w1E,
-                                         double w1N, double uin, double vin, double rdx, double rdy, double dt,
-                                         double &uo, double &vo) {
+template <class T>
+__device__ __forceinline__ void nh_point(const CEN &gz, const CEN &pk, const CEN &pp, T w1C, T w1E, T w1N, T uin,
+                                         T vin, T rdx, T rdy, T dt, T &uo, T &vo) {
+  const double dtd = dt;
   const double wk0 = (pk.C1 - pk.C), wkE = (pk.E1 - pk.E), wkN = (pk.N1 - pk.N);
-  const double du = ((dt / (wk0 + wkE)) * (((gz.C1 - gz.E) * (pk.E1 - pk.C)) + ((gz.C - gz.E1) * (pk.C1 - pk.E))));
-  uo = (((uin + du) + ((dt / (w1C + w1E)) * (((gz.C1 - gz.E) * (pp.E1 - pp.C)) + ((gz.C - gz.E1) * (pp.C1 - pp.E))))) *
-        rdx);
-  const double dv = ((dt / (wk0 + wkN)) * (((gz.C1 - gz.N) * (pk.N1 - pk.C)) + ((gz.C - gz.N1) * (pk.C1 - pk.N))));
-  vo = (((vin + dv) + ((dt / (w1C + w1N)) * (((gz.C1 - gz.N) * (pp.N1 - pp.C)) + ((gz.C - gz.N1) * (pp.C1 - pp.N))))) *
-        rdy);
+  const double du = ((dtd / (wk0 + wkE)) * (((gz.C1 - gz.E) * (pk.E1 - pk.C)) + ((gz.C - gz.E1) * (pk.C1 - pk.E))));
+  const double bu = ((gz.C1 - gz.E) * (pp.E1 - pp.C)) + ((gz.C - gz.E1) * (pp.C1 - pp.E));
+  uo = (((uin + T(du)) + ((dt / (w1C + w1E)) * T(bu))) * rdx);
+  const double dv = ((dtd / (wk0 + wkN)) * (((gz.C1 - gz.N) * (pk.N1 - pk.C)) + ((gz.C - gz.N1) * (pk.C1 - pk.N))));
+  const double bv = ((gz.C1 - gz.N) * (pp.N1 - pp.C)) + ((gz.C - gz.N1) * (pp.C1 - pp.N));
+  vo = (((vin + T(dv)) + ((dt / (w1C + w1N)) * T(bv))) * rdy);
 }
 
+template <class T>
 struct NhArgs {
-  double *uout, *vout;
-  const double *uin, *vin, *rdx, *rdy, *gz, *pp, *pk3, *wk1;
+  T *uout, *vout;
+  const T *uin, *vin, *rdx, *rdy, *wk1;
+  const double *gz, *pp, *pk3;
 };
 
-__global__ void __launch_bounds__(kTileI *kTileJ) nh_p_grad_kernel(NhArgs a, double dt, Geo g, Tiles t) {
+template <class T>
+__global__ void __launch_bounds__(kTileI *kTileJ) nh_p_grad_kernel(NhArgs<T> a, T dt, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
@@ -53,7 +61,7 @@ __global__ void __launch_bounds__(kTileI *kTileJ) nh_p_grad_kernel(NhArgs a, dou
     gz.C1 = a.gz[up], gz.E1 = a.gz[up + 1], gz.N1 = a.gz[up + sj];
     pk.C1 = a.pk3[up], pk.E1 = a.pk3[up + 1], pk.N1 = a.pk3[up + sj];
     pp.C1 = a.pp[up], pp.E1 = a.pp[up + 1], pp.N1 = a.pp[up + sj];
-    double uo, vo;
+    T uo, vo;
     nh_point(gz, pk, pp, a.wk1[at], a.wk1[at + 1], a.wk1[at + sj], ld_nt(a.uin + at), ld_nt(a.vin + at),
              ld_nt(a.rdx + at), ld_nt(a.rdy + at), dt, uo, vo);
     st_nt(a.uout + at, uo);
@@ -78,7 +86,8 @@ struct PairCol {
   __device__ void roll() { C = C1, N = N1, E = E1; }
 };
 
-__global__ void __launch_bounds__(kTileI *kTileJ) nh_p_grad_pair_kernel(NhArgs a, double dt, Geo g, Tiles t) {
+__global__ void __launch_bounds__(kTileI *kTileJ)
+    nh_p_grad_pair_kernel(NhArgs<double> a, double dt, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
   const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
@@ -108,25 +117,18 @@ __global__ void __launch_bounds__(kTileI *kTileJ) nh_p_grad_pair_kernel(NhArgs a
   }
 }
 
-}  // namespace
-}  // namespace oeb
-
-extern "C" int oeb_nh_p_grad_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
-                                 const oeb_s3 *vin, const oeb_s3 *rdx, const oeb_s3 *rdy, const oeb_s3 *gz,
-                                 const oeb_s3 *pp, const oeb_s3 *pk3, const oeb_s3 *wk1, const oeb_s3 *wk,
-                                 const oeb_s3 *du, const oeb_s3 *dv, double dt, void *stream) {
-  using namespace oeb;
-  (void)wk;
-  (void)du;
-  (void)dv;
+template <class T, class S3>
+int nh_p_grad_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *uin, const S3 *vin, const S3 *rdx,
+                  const S3 *rdy, const oeb_s3 *gz, const oeb_s3 *pp, const oeb_s3 *pk3, const S3 *wk1, T dt,
+                  void *stream) {
   const char *K = "nh_p_grad";
   OEB_TRY(check_domain(K, d));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz};
   const Box r_h{0, nx + 1, 0, ny + 1, 0, nz}, r_k{0, nx + 1, 0, ny + 1, 0, nz + 1};
-  const double *p_uo, *p_vo;
-  NhArgs a;
+  const T *p_uo, *p_vo;
+  NhArgs<T> a;
   OEB_TRY(check3(K, "uout", uout, r0, g, true, &p_uo));
   OEB_TRY(check3(K, "vout", vout, r0, g, false, &p_vo));
   OEB_TRY(check3(K, "uin", uin, r0, g, false, &a.uin));
@@ -149,17 +151,41 @@ extern "C" int oeb_nh_p_grad_f64(const oeb_domain *d, const oeb_s3 *uout, const 
   as.add("pk3", a.pk3, r_k, g, false);
   as.add("wk1", a.wk1, r_h, g, false);
   OEB_TRY(as.check(K));
-  a.uout = const_cast<double *>(p_uo);
-  a.vout = const_cast<double *>(p_vo);
+  a.uout = const_cast<T *>(p_uo);
+  a.vout = const_cast<T *>(p_vo);
   const int kc = kchunk_for(K, kChunk);
-  if (nx % 2 == 0 && pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdx, a.rdy, a.gz, a.pp, a.pk3, a.wk1})) {
-    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-    hipLaunchKernelGGL(nh_p_grad_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt, g, t);
-  } else {
-    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-    hipLaunchKernelGGL(nh_p_grad_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt, g, t);
+  if constexpr (sizeof(T) == 8) {
+    if (pairs_for(K, false) && nx % 2 == 0 &&
+        pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdx, a.rdy, a.gz, a.pp, a.pk3, a.wk1})) {
+      const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
+      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+      hipLaunchKernelGGL(nh_p_grad_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt, g,
+                         t);
+      return launch_status(K);
+    }
   }
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
+  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+  hipLaunchKernelGGL(nh_p_grad_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt, g, t);
   return launch_status(K);
+}
+
+}  // namespace
+}  // namespace oeb
+
+// wk, du and dv are the reference's scratch arrays; nothing is stored in them.
+extern "C" int oeb_nh_p_grad_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
+                                 const oeb_s3 *vin, const oeb_s3 *rdx, const oeb_s3 *rdy, const oeb_s3 *gz,
+                                 const oeb_s3 *pp, const oeb_s3 *pk3, const oeb_s3 *wk1, const oeb_s3 *wk,
+                                 const oeb_s3 *du, const oeb_s3 *dv, double dt, void *stream) {
+  (void)wk, (void)du, (void)dv;
+  return oeb::nh_p_grad_run<double>(d, uout, vout, uin, vin, rdx, rdy, gz, pp, pk3, wk1, dt, stream);
+}
+
+extern "C" int oeb_nh_p_grad_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
+                                 const oeb_s3f *vin, const oeb_s3f *rdx, const oeb_s3f *rdy, const oeb_s3 *gz,
+                                 const oeb_s3 *pp, const oeb_s3 *pk3, const oeb_s3f *wk1, const oeb_s3f *wk,
+                                 const oeb_s3f *du, const oeb_s3f *dv, float dt, void *stream) {
+  (void)wk, (void)du, (void)dv;
+  return oeb::nh_p_grad_run<float>(d, uout, vout, uin, vin, rdx, rdy, gz, pp, pk3, wk1, dt, stream);
 }

@@ -63,9 +63,15 @@ inline Tiles make_tiles(int64_t ni, int64_t nj, int nz, int tile_i, int tile_j, 
   return t;
 }
 
-// Non-temporal (streaming) store and load of one double.
-__device__ __forceinline__ void st_nt(double *p, double v) { __builtin_nontemporal_store(v, p); }
-__device__ __forceinline__ double ld_nt(const double *p) { return __builtin_nontemporal_load(p); }
+// Non-temporal (streaming) store and load of one element (float or double).
+template <class T>
+__device__ __forceinline__ void st_nt(T *p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+template <class T>
+__device__ __forceinline__ T ld_nt(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
 
 // Pairs of adjacent doubles along i, moved with one 16-byte access.  Used by
 // the paired kernels, which the host selects only when every 3-D array's
@@ -91,9 +97,9 @@ __device__ __forceinline__ void st2_nt(double *p, double x, double y) {
 }
 
 // True when the paired (16-byte) kernels may be used for these arrays.
-inline bool pairs_ok(const Geo &g, std::initializer_list<const double *> bases) {
+inline bool pairs_ok(const Geo &g, std::initializer_list<const void *> bases) {
   if ((g.sj & 1) || (g.sk & 1)) return false;
-  for (const double *p : bases)
+  for (const void *p : bases)
     if (reinterpret_cast<uintptr_t>(p) & 15) return false;
   return true;
 }
@@ -103,14 +109,14 @@ inline bool pairs_ok(const Geo &g, std::initializer_list<const double *> bases) 
 // lanes read consecutive doubles.  Points with i >= ilim or j >= jlim lie
 // outside the field's valid box and are set to 0; the caller never uses them.
 // `f` points at element (0,0) of the level.
-template <int TI, int TJ, int R, int NT>
-__device__ __forceinline__ void stage_tile(double (*s)[TI + 2 * R], const double *__restrict__ f, int i0, int j0,
-                                           int sj, int ilim, int jlim, int tid) {
+template <int TI, int TJ, int R, int NT, class T>
+__device__ __forceinline__ void stage_tile(T (*s)[TI + 2 * R], const T *__restrict__ f, int i0, int j0, int sj,
+                                           int ilim, int jlim, int tid) {
   constexpr int W = TI + 2 * R, H = TJ + 2 * R;
   for (int e = tid; e < W * H; e += NT) {
     const int r = e / W, c = e % W;
     const int gi = i0 - R + c, gj = j0 - R + r;
-    s[r][c] = (gi < ilim && gj < jlim) ? f[gi + gj * sj] : 0.0;
+    s[r][c] = (gi < ilim && gj < jlim) ? f[gi + gj * sj] : T(0);
   }
 }
 
@@ -141,11 +147,17 @@ int fail(int status, const std::string &msg);
 // box `b` must lie inside the allocation it describes and its strides must
 // match `g` (the first array checked fills g.sj / g.sk).  On success *base is
 // the device address of element (0,0,0).
+// Strides are counted in elements, so FP32 and FP64 arrays of one call share
+// them.
 int check3(const char *kname, const char *name, const oeb_s3 *s, const Box &b, Geo &g, bool first,
            const double **base);
+int check3(const char *kname, const char *name, const oeb_s3f *s, const Box &b, Geo &g, bool first,
+           const float **base);
 // Validates a 1-D descriptor over the index range [lo, hi).  *base is the
 // address of element 0 and *stride the element stride.
 int check1(const char *kname, const char *name, const oeb_s1 *s, int64_t lo, int64_t hi, const double **base,
+           int *stride);
+int check1(const char *kname, const char *name, const oeb_s1f *s, int64_t lo, int64_t hi, const float **base,
            int *stride);
 int check_domain(const char *kname, const oeb_domain *d);
 
@@ -157,8 +169,18 @@ struct AliasSet {
   uintptr_t lo[kMax], hi[kMax];
   bool out[kMax];
   int n = 0;
-  void add(const char *nm, const double *base, const Box &b, const Geo &g, bool is_out);
-  void add1(const char *nm, const double *base, int64_t lo_i, int64_t hi_i, int stride);
+  template <class T>
+  void add(const char *nm, const T *base, const Box &b, const Geo &g, bool is_out) {
+    add_range(nm, reinterpret_cast<uintptr_t>(base + (b.klo * g.sk + b.jlo * int64_t(g.sj) + b.ilo)),
+              reinterpret_cast<uintptr_t>(base + ((b.khi - 1) * g.sk + (b.jhi - 1) * int64_t(g.sj) + (b.ihi - 1)) + 1),
+              is_out);
+  }
+  template <class T>
+  void add1(const char *nm, const T *base, int64_t lo_i, int64_t hi_i, int stride) {
+    add_range(nm, reinterpret_cast<uintptr_t>(base + lo_i * stride),
+              reinterpret_cast<uintptr_t>(base + (hi_i - 1) * stride + 1), false);
+  }
+  void add_range(const char *nm, uintptr_t lo_b, uintptr_t hi_b, bool is_out);
   int check(const char *kname) const;
 };
 
@@ -173,6 +195,10 @@ int check_tiles(const char *kname, const Tiles &t, int64_t ni, int64_t nj, int n
 // variable OEB_KCHUNK_<KNAME> (upper case, e.g. OEB_KCHUNK_P_GRAD_C) holds an
 // integer in [1, 4096].  This lets tuning sweeps run without a rebuild.
 int kchunk_for(const char *kname, int dflt);
+
+// Whether kernel `kname` may take its paired (16-byte) path: `dflt`, unless
+// OEB_PAIRS_<KNAME> holds 0 or 1.  The paired path still needs pairs_ok().
+bool pairs_for(const char *kname, bool dflt);
 
 #define OEB_TRY(expr)              \
   do {                             \

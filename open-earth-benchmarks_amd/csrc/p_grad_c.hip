@@ -5,31 +5,39 @@
 // Reach: i, j in [-1, 0], k in [0, +1].  Each lane marches up its column and
 // carries the gz / pkc values of level k+1 over as level k of the next step,
 // so every plane of gz and pkc is loaded once per chunk.
-// Algorithmic traffic: 7 inputs + 2 outputs = 72 B per point (FP64).
+// FP32 form: gz and pkc stay FP64 arrays, and the bracket
+// (dgz*dpk + dgz'*dpk') is formed in FP64 and rounded once to float; the rest
+// is float.  Algorithmic traffic: 7 inputs + 2 outputs = 72 B per point
+// (FP64); 44 B in the FP32 form (gz and pkc 8 B, the rest 4 B).
 #include "oeb_common.hpp"
 
 namespace oeb {
 namespace {
 
-constexpr int kTileI = 64, kTileJ = 4, kChunk = 16;
+constexpr int kTileI = 64, kTileJ = 4, kChunk = 4;
 constexpr int kPairI = 2 * kTileI;
 
 // One point (kernels/p_grad_c.h:87-99).  C = (i,j), W = (i-1,j), S = (i,j-1);
 // suffix 1 = level k+1.
+template <class T>
 __device__ __forceinline__ void pgc_point(double gzC, double gzW, double gzS, double gzC1, double gzW1, double gzS1,
                                           double pkC, double pkW, double pkS, double pkC1, double pkW1, double pkS1,
-                                          double wk0, double wkW, double wkS, double uin, double vin, double rdxc,
-                                          double rdyc, double dt2, double &uo, double &vo) {
-  uo = (uin + (((dt2 * rdxc) / (wkW + wk0)) * (((gzW1 - gzC) * (pkC1 - pkW)) + ((gzW - gzC1) * (pkW1 - pkC)))));
-  vo = (vin + (((dt2 * rdyc) / (wkS + wk0)) * (((gzS1 - gzC) * (pkC1 - pkS)) + ((gzS - gzC1) * (pkS1 - pkC)))));
+                                          T wk0, T wkW, T wkS, T uin, T vin, T rdxc, T rdyc, T dt2, T &uo, T &vo) {
+  const double bu = ((gzW1 - gzC) * (pkC1 - pkW)) + ((gzW - gzC1) * (pkW1 - pkC));
+  const double bv = ((gzS1 - gzC) * (pkC1 - pkS)) + ((gzS - gzC1) * (pkS1 - pkC));
+  uo = (uin + (((dt2 * rdxc) / (wkW + wk0)) * T(bu)));
+  vo = (vin + (((dt2 * rdyc) / (wkS + wk0)) * T(bv)));
 }
 
+template <class T>
 struct PgcArgs {
-  double *uout, *vout;
-  const double *uin, *vin, *rdxc, *rdyc, *delpc, *gz, *pkc;
+  T *uout, *vout;
+  const T *uin, *vin, *rdxc, *rdyc, *delpc;
+  const double *gz, *pkc;
 };
 
-__global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_kernel(PgcArgs a, double dt2, Geo g, Tiles t) {
+template <class T>
+__global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_kernel(PgcArgs<T> a, T dt2, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
@@ -42,7 +50,7 @@ __global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_kernel(PgcArgs a, dou
     const int64_t up = at + sk;
     const double gzC1 = a.gz[up], gzW1 = a.gz[up - 1], gzS1 = a.gz[up - sj];
     const double pkC1 = a.pkc[up], pkW1 = a.pkc[up - 1], pkS1 = a.pkc[up - sj];
-    double uo, vo;
+    T uo, vo;
     pgc_point(gzC, gzW, gzS, gzC1, gzW1, gzS1, pkC, pkW, pkS, pkC1, pkW1, pkS1, a.delpc[at], a.delpc[at - 1],
               a.delpc[at - sj], ld_nt(a.uin + at), ld_nt(a.vin + at), ld_nt(a.rdxc + at), ld_nt(a.rdyc + at), dt2,
               uo, vo);
@@ -55,7 +63,8 @@ __global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_kernel(PgcArgs a, dou
 
 // Two adjacent columns (i, i+1) per lane, i even, 16-byte accesses.  For
 // column i+1 the west neighbour is column i of the pair.
-__global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_pair_kernel(PgcArgs a, double dt2, Geo g, Tiles t) {
+__global__ void __launch_bounds__(kTileI *kTileJ)
+    p_grad_c_pair_kernel(PgcArgs<double> a, double dt2, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
   const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
@@ -87,22 +96,17 @@ __global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_pair_kernel(PgcArgs a
   }
 }
 
-}  // namespace
-}  // namespace oeb
-
-extern "C" int oeb_p_grad_c_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
-                                const oeb_s3 *vin, const oeb_s3 *rdxc, const oeb_s3 *rdyc, const oeb_s3 *delpc,
-                                const oeb_s3 *gz, const oeb_s3 *pkc, const oeb_s3 *wk, double dt2, void *stream) {
-  using namespace oeb;
-  (void)wk;
+template <class T, class S3>
+int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *uin, const S3 *vin, const S3 *rdxc,
+                 const S3 *rdyc, const S3 *delpc, const oeb_s3 *gz, const oeb_s3 *pkc, T dt2, void *stream) {
   const char *K = "p_grad_c";
   OEB_TRY(check_domain(K, d));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz};
   const Box r_dp{-1, nx, -1, ny, 0, nz}, r_k{-1, nx, -1, ny, 0, nz + 1};
-  const double *p_uo, *p_vo;
-  PgcArgs a;
+  const T *p_uo, *p_vo;
+  PgcArgs<T> a;
   OEB_TRY(check3(K, "uout", uout, r0, g, true, &p_uo));
   OEB_TRY(check3(K, "vout", vout, r0, g, false, &p_vo));
   OEB_TRY(check3(K, "uin", uin, r0, g, false, &a.uin));
@@ -123,17 +127,39 @@ extern "C" int oeb_p_grad_c_f64(const oeb_domain *d, const oeb_s3 *uout, const o
   as.add("gz", a.gz, r_k, g, false);
   as.add("pkc", a.pkc, r_k, g, false);
   OEB_TRY(as.check(K));
-  a.uout = const_cast<double *>(p_uo);
-  a.vout = const_cast<double *>(p_vo);
+  a.uout = const_cast<T *>(p_uo);
+  a.vout = const_cast<T *>(p_vo);
   const int kc = kchunk_for(K, kChunk);
-  if (nx % 2 == 0 && pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdxc, a.rdyc, a.delpc, a.gz, a.pkc})) {
-    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-    hipLaunchKernelGGL(p_grad_c_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g, t);
-  } else {
-    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-    hipLaunchKernelGGL(p_grad_c_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g, t);
+  if constexpr (sizeof(T) == 8) {
+    if (pairs_for(K, false) && nx % 2 == 0 &&
+        pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdxc, a.rdyc, a.delpc, a.gz, a.pkc})) {
+      const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
+      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+      hipLaunchKernelGGL(p_grad_c_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g,
+                         t);
+      return launch_status(K);
+    }
   }
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
+  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+  hipLaunchKernelGGL(p_grad_c_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g, t);
   return launch_status(K);
+}
+
+}  // namespace
+}  // namespace oeb
+
+// wk is the reference's scratch array; nothing is stored in it.
+extern "C" int oeb_p_grad_c_f64(const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
+                                const oeb_s3 *vin, const oeb_s3 *rdxc, const oeb_s3 *rdyc, const oeb_s3 *delpc,
+                                const oeb_s3 *gz, const oeb_s3 *pkc, const oeb_s3 *wk, double dt2, void *stream) {
+  (void)wk;
+  return oeb::p_grad_c_run<double>(d, uout, vout, uin, vin, rdxc, rdyc, delpc, gz, pkc, dt2, stream);
+}
+
+extern "C" int oeb_p_grad_c_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
+                                const oeb_s3f *vin, const oeb_s3f *rdxc, const oeb_s3f *rdyc, const oeb_s3f *delpc,
+                                const oeb_s3 *gz, const oeb_s3 *pkc, const oeb_s3f *wk, float dt2, void *stream) {
+  (void)wk;
+  return oeb::p_grad_c_run<float>(d, uout, vout, uin, vin, rdxc, rdyc, delpc, gz, pkc, dt2, stream);
 }

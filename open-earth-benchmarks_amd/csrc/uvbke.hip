@@ -2,30 +2,32 @@
 // Reference: kernels/uvbke.h:61-75 (uvbke_openmp), expressions :67-71:
 //   ub = ((dt5 * ((uc[j-1] + uc) - ((vc[i-1] + vc) * cosa))) * rsina)
 //   vb = ((dt5 * ((vc[i-1] + vc) - ((uc[j-1] + uc) * cosa))) * rsina)
-// dt5 is the driver's constant 0.5 * 225 (src/uvbke.cpp:11-12), an argument here.
-// Algorithmic traffic: 4 inputs + 2 outputs = 48 B per point (FP64).
+// dt5 is the driver's constant 0.5 * 225 (src/uvbke.cpp:11-12), an argument
+// here.  It is a double whatever the element type, so with float storage the
+// two outer products are formed in double and rounded once on the store.
+// Algorithmic traffic: 4 inputs + 2 outputs = 48 B per point (FP64), 24 B (FP32).
 #include "oeb_common.hpp"
 
 namespace oeb {
 namespace {
 
-constexpr int kTileI = 64, kTileJ = 4, kChunk = 2;
+constexpr int kTileI = 64, kTileJ = 4, kChunk = 1;
 
+template <class T>
 __global__ void __launch_bounds__(kTileI *kTileJ)
-    uvbke_kernel(double *__restrict__ ub, double *__restrict__ vb, const double *__restrict__ uc,
-                 const double *__restrict__ vc, const double *__restrict__ cosa, const double *__restrict__ rsina,
-                 double dt5, Geo g, Tiles t) {
+    uvbke_kernel(T *__restrict__ ub, T *__restrict__ vb, const T *__restrict__ uc, const T *__restrict__ vc,
+                 const T *__restrict__ cosa, const T *__restrict__ rsina, double dt5, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
   int64_t at = int64_t(c.k0) * g.sk + (i + j * sj);
   for (int k = c.k0; k < c.k1; ++k, at += g.sk) {
-    const double usum = uc[at - sj] + uc[at];
-    const double vsum = vc[at - 1] + vc[at];
-    const double ca = ld_nt(cosa + at), rs = ld_nt(rsina + at);
-    st_nt(ub + at, ((dt5 * (usum - (vsum * ca))) * rs));
-    st_nt(vb + at, ((dt5 * (vsum - (usum * ca))) * rs));
+    const T usum = uc[at - sj] + uc[at];
+    const T vsum = vc[at - 1] + vc[at];
+    const T ca = ld_nt(cosa + at), rs = ld_nt(rsina + at);
+    st_nt(ub + at, T((dt5 * (usum - (vsum * ca))) * rs));
+    st_nt(vb + at, T((dt5 * (vsum - (usum * ca))) * rs));
   }
 }
 
@@ -53,19 +55,16 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
   }
 }
 
-}  // namespace
-}  // namespace oeb
-
-extern "C" int oeb_uvbke_f64(const oeb_domain *d, const oeb_s3 *ub, const oeb_s3 *vb, const oeb_s3 *uc,
-                             const oeb_s3 *vc, const oeb_s3 *cosa, const oeb_s3 *rsina, double dt5, void *stream) {
-  using namespace oeb;
+template <class T, class S>
+int uvbke_run(const oeb_domain *d, const S *ub, const S *vb, const S *uc, const S *vc, const S *cosa,
+              const S *rsina, double dt5, void *stream) {
   const char *K = "uvbke";
   OEB_TRY(check_domain(K, d));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz};
   const Box r_uc{0, nx, -1, ny, 0, nz}, r_vc{-1, nx, 0, ny, 0, nz};
-  const double *p_ub, *p_vb, *p_uc, *p_vc, *p_ca, *p_rs;
+  const T *p_ub, *p_vb, *p_uc, *p_vc, *p_ca, *p_rs;
   OEB_TRY(check3(K, "ub", ub, r0, g, true, &p_ub));
   OEB_TRY(check3(K, "vb", vb, r0, g, false, &p_vb));
   OEB_TRY(check3(K, "uc", uc, r_uc, g, false, &p_uc));
@@ -81,16 +80,32 @@ extern "C" int oeb_uvbke_f64(const oeb_domain *d, const oeb_s3 *ub, const oeb_s3
   as.add("rsina", p_rs, r0, g, false);
   OEB_TRY(as.check(K));
   const int kc = kchunk_for(K, kChunk);
-  if (nx % 2 == 0 && pairs_ok(g, {p_ub, p_vb, p_uc, p_vc, p_ca, p_rs})) {
-    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-    hipLaunchKernelGGL(uvbke_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
-                       const_cast<double *>(p_ub), const_cast<double *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5, g, t);
-  } else {
-    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-    hipLaunchKernelGGL(uvbke_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
-                       const_cast<double *>(p_ub), const_cast<double *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5, g, t);
+  if constexpr (sizeof(T) == 8) {
+    if (pairs_for(K, true) && nx % 2 == 0 && pairs_ok(g, {p_ub, p_vb, p_uc, p_vc, p_ca, p_rs})) {
+      const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
+      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+      hipLaunchKernelGGL(uvbke_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
+                         const_cast<T *>(p_ub), const_cast<T *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5, g, t);
+      return launch_status(K);
+    }
   }
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
+  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+  hipLaunchKernelGGL(uvbke_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
+                     const_cast<T *>(p_ub), const_cast<T *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5, g, t);
   return launch_status(K);
+}
+
+}  // namespace
+}  // namespace oeb
+
+extern "C" int oeb_uvbke_f64(const oeb_domain *d, const oeb_s3 *ub, const oeb_s3 *vb, const oeb_s3 *uc,
+                             const oeb_s3 *vc, const oeb_s3 *cosa, const oeb_s3 *rsina, double dt5, void *stream) {
+  return oeb::uvbke_run<double>(d, ub, vb, uc, vc, cosa, rsina, dt5, stream);
+}
+
+extern "C" int oeb_uvbke_f32(const oeb_domain *d, const oeb_s3f *ub, const oeb_s3f *vb, const oeb_s3f *uc,
+                             const oeb_s3f *vc, const oeb_s3f *cosa, const oeb_s3f *rsina, double dt5,
+                             void *stream) {
+  return oeb::uvbke_run<float>(d, ub, vb, uc, vc, cosa, rsina, dt5, stream);
 }

@@ -19,6 +19,10 @@ OEB_ERR_DOMAIN = 2
 OEB_ERR_LAYOUT = 3
 OEB_ERR_ALIAS = 4
 OEB_ERR_HIP = 5
+OEB_ERR_ARG = 6
+
+OEB_HALO_PERIODIC = 0
+OEB_HALO_ZERO_GRADIENT = 1
 
 
 class OebDomain(C.Structure):
@@ -28,8 +32,10 @@ class OebDomain(C.Structure):
 
 
 class OebStorage3D(C.Structure):
-    """``oeb_storage3d_f64``: field order of the reference's Storage<double,3>
-    (include/util.h:24-30); sizes/strides reversed (index 2 = i)."""
+    """``oeb_storage3d_f64`` / ``oeb_storage3d_f32``: field order of the
+    reference's Storage<ElementType,3> (include/util.h:24-30); sizes/strides
+    reversed (index 2 = i).  The two C structs differ only in the pointee type
+    of their data pointers, so one ctypes struct serves both."""
 
     _fields_ = [
         ("allocatedPtr", C.c_void_p),
@@ -41,7 +47,8 @@ class OebStorage3D(C.Structure):
 
 
 class OebStorage1D(C.Structure):
-    """``oeb_storage1d_f64`` (Storage<double,1>, include/util.h:24-30)."""
+    """``oeb_storage1d_f64`` / ``oeb_storage1d_f32`` (Storage<ElementType,1>,
+    include/util.h:24-30)."""
 
     _fields_ = [
         ("allocatedPtr", C.c_void_p),
@@ -57,7 +64,9 @@ P1 = C.POINTER(OebStorage1D)
 PD = C.POINTER(OebDomain)
 
 # Argument kinds of the entry points, in include/oeb.h order after the domain.
-#   '3' -> const oeb_s3*, '1' -> const oeb_s1*, 'd' -> double
+#   '3' -> 3-D array of the entry point's element type, '1' -> 1-D array of it,
+#   'D' -> FP64 3-D array in an FP32 entry point (an FP64 island),
+#   'd' -> double, 'f' -> float
 SIGNATURES = {
     "laplace": "33",
     "hdiffsa": "333113" "33",
@@ -71,7 +80,35 @@ SIGNATURES = {
     "fvtp2d": "33333333333333" "3333333" "dd",
 }
 
-EXPORTS = ["oeb_version", "oeb_status_string", "oeb_last_error"] + [f"oeb_{k}_f64" for k in SIGNATURES]
+# oeb_<name>_f32: FP32 arrays and float scalars, except the FP64 island arrays
+# of fastwavesuv (ppuv, hhl), p_grad_c (gz, pkc) and nh_p_grad (gz, pp, pk3)
+# and uvbke's double dt5.
+SIGNATURES_F32 = {
+    "laplace": "33",
+    "hdiffsa": "333113" "33",
+    "hdiffsmag": "33333" "11111" "33" "ffff",
+    "hadvuv": "3333" "1111" "33333333" "ff",
+    "hadvuv5th": "3333" "1111" "33333333" "ff",
+    "fastwavesuv": "3333333" "DD" "3" "1" "3333" "ff",
+    "uvbke": "333333" "d",
+    "p_grad_c": "3333333" "DD" "3" "f",
+    "nh_p_grad": "333333" "DDD" "3333" "f",
+    "fvtp2d": "33333333333333" "3333333" "ff",
+}
+assert SIGNATURES_F32.keys() == SIGNATURES.keys()
+assert all(len(SIGNATURES[k]) == len(SIGNATURES_F32[k]) for k in SIGNATURES)
+
+PRECISIONS = ("f64", "f32")
+
+
+def signature(name: str, precision: str = "f64") -> str:
+    return {"f64": SIGNATURES, "f32": SIGNATURES_F32}[precision][name]
+
+
+EXPORTS = ["oeb_version", "oeb_status_string", "oeb_last_error"] + [
+    f"oeb_{k}_{p}" for p in PRECISIONS for k in SIGNATURES
+] + ["oeb_halo_fill_f64", "oeb_halo_fill_f32", "oeb_graph_begin", "oeb_graph_end", "oeb_graph_launch",
+     "oeb_graph_destroy"]
 
 
 class OebLibraryError(RuntimeError):
@@ -91,7 +128,7 @@ _lib = None
 
 
 def _ctype(kind: str):
-    return {"3": P3, "1": P1, "d": C.c_double}[kind]
+    return {"3": P3, "D": P3, "1": P1, "d": C.c_double, "f": C.c_float}[kind]
 
 
 def load(path: str | os.PathLike | None = None) -> C.CDLL:
@@ -114,10 +151,23 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.oeb_status_string.argtypes = [C.c_int]
     lib.oeb_last_error.restype = C.c_char_p
     lib.oeb_last_error.argtypes = []
-    for name, sig in SIGNATURES.items():
-        fn = getattr(lib, f"oeb_{name}_f64")
+    for prec in PRECISIONS:
+        for name in SIGNATURES:
+            fn = getattr(lib, f"oeb_{name}_{prec}")
+            fn.restype = C.c_int
+            fn.argtypes = [PD] + [_ctype(ch) for ch in signature(name, prec)] + [C.c_void_p]
+    for prec in PRECISIONS:
+        fn = getattr(lib, f"oeb_halo_fill_{prec}")
         fn.restype = C.c_int
-        fn.argtypes = [PD] + [_ctype(ch) for ch in sig] + [C.c_void_p]
+        fn.argtypes = [PD, P3, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    lib.oeb_graph_begin.restype = C.c_int
+    lib.oeb_graph_begin.argtypes = [C.c_void_p]
+    lib.oeb_graph_end.restype = C.c_int
+    lib.oeb_graph_end.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+    lib.oeb_graph_launch.restype = C.c_int
+    lib.oeb_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
+    lib.oeb_graph_destroy.restype = C.c_int
+    lib.oeb_graph_destroy.argtypes = [C.c_void_p]
     if path is None:
         _lib = lib
     return lib

@@ -236,3 +236,16 @@ def generate(kernel: str, L: Layout, family: str = "baseline", seed: int = 0) ->
         else:
             out[arg.name] = make1(recipes[arg.name], L1, L.ny, L.h, rng)
     return out
+
+
+def cast(kernel: str, host: dict, precision: str, islands: bool = True) -> dict[str, np.ndarray]:
+    """Copies of `host` in the element types of the given form: unchanged
+    float64 for "f64"; for "f32" every array rounded to float32 except the
+    FP64 islands of the FP32 form (spec.ISLANDS), unless islands=False (the
+    reference's float build, where every array is float32)."""
+    spec = KERNELS[kernel]
+    out = {}
+    for name, arr in host.items():
+        keep = precision == "f64" or (islands and spec.arg_precision(name, precision) == "f64")
+        out[name] = arr.astype(np.float64 if keep else np.float32)
+    return out

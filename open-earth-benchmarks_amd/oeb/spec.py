@@ -53,9 +53,35 @@ class Kernel:
         each 3-D output written once, 8 B each (1-D arrays and halos excluded)."""
         return 8 * (len(self.inputs3) + len(self.outputs))
 
+    @property
+    def islands(self) -> frozenset[str]:
+        """3-D inputs that stay FP64 in the FP32 form (oeb_<name>_f32)."""
+        return ISLANDS.get(self.name, frozenset())
+
+    def bytes_per_point(self, precision: str = "f64") -> int:
+        """Algorithmic bytes per interior point in the given form: FP32 arrays
+        count 4 B, FP64 (and FP64 island) arrays 8 B."""
+        if precision == "f64":
+            return self.bytes_per_point_f64
+        n3 = self.inputs3 + self.outputs
+        return sum(8 if a in self.islands else 4 for a in n3)
+
+    def arg_precision(self, arg: str, precision: str) -> str:
+        """Element type of array `arg` in the given form."""
+        return "f64" if precision == "f64" or arg in self.islands else "f32"
+
     def region(self, out: str, nx: int, ny: int) -> tuple[int, int, int, int]:
         i0, i1, j0, j1 = self.regions.get(out, (0, 0, 0, 0))
         return (i0, nx + i1, j0, ny + j1)
+
+
+# FP64 islands of the FP32 form: pressure and geopotential (and, in
+# fastwavesuv, the half-level heights) whose small differences cancel.
+ISLANDS = {
+    "fastwavesuv": frozenset({"ppuv", "hhl"}),
+    "p_grad_c": frozenset({"gz", "pkc"}),
+    "nh_p_grad": frozenset({"gz", "pp", "pk3"}),
+}
 
 
 def _a(spec: str) -> tuple[Arg, ...]:

@@ -48,20 +48,32 @@ def _domain(domain) -> abi.OebDomain:
     return abi.OebDomain(nx, ny, nz)
 
 
-def call(name: str, arrays: list, scalars: list, domain=None, stream=None) -> None:
-    """Calls oeb_<name>_f64 with arrays in reference order (None -> NULL)."""
+def call(name: str, arrays: list, scalars: list, domain=None, stream=None, precision=None) -> None:
+    """Calls oeb_<name>_<precision> with arrays in reference order (None ->
+    NULL).  The precision is that of the first array (an output) unless
+    given.  In the FP32 form the FP64 island arrays of fastwavesuv, p_grad_c
+    and nh_p_grad must be float64, every other array float32; the element
+    types are checked here, because the C-ABI cannot see them."""
     lib = abi.load()
     dom = _domain(domain)
-    sig = abi.SIGNATURES[name]
+    if precision is None:
+        precision = arrays[0].precision if arrays and arrays[0] is not None else "f64"
+    sig = abi.signature(name, precision)
     assert len(arrays) + len(scalars) == len(sig), name
     cargs = []
-    for a in arrays:
+    spec_args = KERNELS[name].args
+    for a, kind, spec_arg in zip(arrays, sig, spec_args):
+        if a is not None:
+            want = "f64" if (precision == "f64" or kind == "D") else "f32"
+            if a.precision != want:
+                raise TypeError(f"{name}_{precision}: array '{spec_arg.name}' must be {want}, got {a.precision}")
         cargs.append(None if a is None else a.ref())
-    cargs += [C.c_double(float(s)) for s in scalars]
-    abi.check(getattr(lib, f"oeb_{name}_f64")(C.byref(dom), *cargs, _stream(stream)))
+    for s_, kind in zip(scalars, sig[len(arrays):]):
+        cargs.append(C.c_double(float(s_)) if kind == "d" else C.c_float(float(s_)))
+    abi.check(getattr(lib, f"oeb_{name}_{precision}")(C.byref(dom), *cargs, _stream(stream)))
 
 
-def run(name: str, fields: dict, scalars: dict | None = None, domain=None, stream=None) -> None:
+def run(name: str, fields: dict, scalars: dict | None = None, domain=None, stream=None, precision=None) -> None:
     """Calls stencil `name` with arrays looked up by reference argument name
     (missing scratch -> NULL) and scalars defaulting to the driver values."""
     spec = KERNELS[name]
@@ -69,7 +81,7 @@ def run(name: str, fields: dict, scalars: dict | None = None, domain=None, strea
     sc = dict(spec.scalars)
     if scalars:
         sc.update(scalars)
-    call(name, arrays, [sc[n] for n, _ in spec.scalars], domain, stream)
+    call(name, arrays, [sc[n] for n, _ in spec.scalars], domain, stream, precision)
 
 
 # ------------------------------------------------- reference signatures

@@ -15,7 +15,7 @@ from .layout import Layout, Layout1D
 
 
 def desc3(ptr: int, L: Layout) -> OebStorage3D:
-    """Descriptor of a flat allocation of L.size doubles at address ptr."""
+    """Descriptor of a flat allocation of L.size elements at address ptr."""
     d = OebStorage3D()
     d.allocatedPtr = ptr
     d.alignedPtr = ptr
@@ -37,21 +37,30 @@ def desc1(ptr: int, L1: Layout1D) -> OebStorage1D:
 
 def _ptr(buf) -> int:
     if isinstance(buf, np.ndarray):
-        assert buf.dtype == np.float64 and buf.flags.c_contiguous
+        if buf.dtype not in (np.float64, np.float32) or not buf.flags.c_contiguous:
+            raise TypeError("a storage buffer must be a contiguous float64 or float32 array")
         return buf.ctypes.data
     # torch tensor
-    assert buf.dtype.is_floating_point and buf.element_size() == 8 and buf.is_contiguous()
+    if not (buf.dtype.is_floating_point and buf.element_size() in (4, 8) and buf.is_contiguous()):
+        raise TypeError("a storage buffer must be a contiguous float64 or float32 tensor")
     return int(buf.data_ptr())
 
 
+def _precision(buf) -> str:
+    size = buf.itemsize if isinstance(buf, np.ndarray) else buf.element_size()
+    return "f64" if size == 8 else "f32"
+
+
 class Storage3D:
-    """A 3-D field: flat float64 buffer plus its Layout."""
+    """A 3-D field: flat float64 or float32 buffer plus its Layout.
+    ``precision`` is "f64" or "f32", the element type of the buffer."""
 
     def __init__(self, layout: Layout, buf):
         if (buf.size if isinstance(buf, np.ndarray) else buf.numel()) != layout.size:
             raise ValueError("buffer size does not match layout")
         self.layout = layout
         self.buf = buf
+        self.precision = _precision(buf)
         self._desc = desc3(_ptr(buf), layout)
 
     @property
@@ -79,6 +88,7 @@ class Storage1D:
             raise ValueError("buffer size does not match layout")
         self.layout = layout
         self.buf = buf
+        self.precision = _precision(buf)
         self._desc = desc1(_ptr(buf), layout)
 
     @property
@@ -93,3 +103,8 @@ class Storage1D:
         import torch
 
         return cls(layout, torch.from_numpy(np.ascontiguousarray(host)).to(device))
+
+    def to_host(self) -> np.ndarray:
+        if isinstance(self.buf, np.ndarray):
+            return self.buf
+        return self.buf.detach().to("cpu").numpy()

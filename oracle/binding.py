@@ -1,7 +1,11 @@
 """ctypes binding of the oracle (liboracle.so) and of the compiled reference
-(_ref/libref.so).  TEST INFRASTRUCTURE ONLY.
+(_ref/libref.so, and _ref/libref_f32.so with ElementType float).
+TEST INFRASTRUCTURE ONLY.
 
-Both take host numpy arrays with the descriptors of include/oeb.h.
+Both take host numpy arrays with the descriptors of include/oeb.h.  With
+precision "f32" the arrays are float32, except the FP64 islands of the
+oracle's FP32 form (oeb.spec.ISLANDS), which stay float64 for the oracle.
+The reference's float build has no islands: every array is float32.
 """
 from __future__ import annotations
 
@@ -24,6 +28,7 @@ from oeb.storage import desc1, desc3  # noqa: E402
 
 ORACLE_SO = HERE / "liboracle.so"
 REF_SO = HERE / "_ref" / "libref.so"
+REF_F32_SO = HERE / "_ref" / "libref_f32.so"
 REF_TREE = Path("/root/reference")
 
 # Variant numbers of the reference (include/util.h:10-13).
@@ -34,7 +39,7 @@ ORACLE_VARIANT = {k: OPENMP for k in KERNELS}
 ORACLE_VARIANT["fvtp2d"] = DEFAULT
 
 _oracle = None
-_ref = None
+_ref = {}
 
 
 def build(ref: bool | None = None) -> None:
@@ -46,7 +51,7 @@ def build(ref: bool | None = None) -> None:
         subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
 
 
-def _argtypes(name: str, with_variant: bool, scalars: int) -> list:
+def _argtypes(name: str, with_variant: bool, scalar_kinds: str) -> list:
     sig = abi.SIGNATURES[name]
     spec = KERNELS[name]
     arr = []
@@ -54,7 +59,12 @@ def _argtypes(name: str, with_variant: bool, scalars: int) -> list:
         if a.kind == "scratch":
             continue
         arr.append(abi.P3 if ch == "3" else abi.P1)
-    return ([C.c_int] if with_variant else []) + [abi.PD] + arr + [C.c_double] * scalars
+    sc = [C.c_float if ch == "f" else C.c_double for ch in scalar_kinds]
+    return ([C.c_int] if with_variant else []) + [abi.PD] + arr + sc
+
+
+def _oracle_name(name: str, precision: str) -> str:
+    return f"oracle_{name}" if precision == "f64" else f"oracle_{name}_f32"
 
 
 def oracle_lib() -> C.CDLL:
@@ -64,9 +74,11 @@ def oracle_lib() -> C.CDLL:
             build(ref=False)
         lib = C.CDLL(str(ORACLE_SO))
         for name, spec in KERNELS.items():
-            fn = getattr(lib, f"oracle_{name}")
-            fn.argtypes = _argtypes(name, False, len(spec.scalars))
-            fn.restype = C.c_int if name == "fvtp2d" else None
+            for prec in abi.PRECISIONS:
+                fn = getattr(lib, _oracle_name(name, prec))
+                n_arrays = len(spec.args)
+                fn.argtypes = _argtypes(name, False, abi.signature(name, prec)[n_arrays:])
+                fn.restype = C.c_int if name == "fvtp2d" else None
         _oracle = lib
     return _oracle
 
@@ -76,31 +88,37 @@ def oracle_lib() -> C.CDLL:
 _REF_FIXED = {"uvbke": ("dt5",), "fvtp2d": ("p1", "p2")}
 
 
-def ref_available() -> bool:
-    return REF_SO.exists()
+def ref_available(precision: str = "f64") -> bool:
+    return (REF_SO if precision == "f64" else REF_F32_SO).exists()
 
 
-def ref_lib() -> C.CDLL:
-    global _ref
-    if _ref is None:
-        if not REF_SO.exists():
-            raise FileNotFoundError(f"{REF_SO} not built (needs the reference tree at build time)")
-        lib = C.CDLL(str(REF_SO))
+def ref_lib(precision: str = "f64") -> C.CDLL:
+    """The compiled reference: ElementType double ("f64") or float ("f32").
+    Its scalar arguments are double in both (converted to ElementType at the
+    kernel call, as the reference's `const ElementType` parameters do)."""
+    if precision not in _ref:
+        so = REF_SO if precision == "f64" else REF_F32_SO
+        if not so.exists():
+            raise FileNotFoundError(f"{so} not built (needs the reference tree at build time)")
+        lib = C.CDLL(str(so))
         for name, spec in KERNELS.items():
             fn = getattr(lib, f"ref_{name}")
             n_sc = len([s for s, _ in spec.scalars if s not in _REF_FIXED.get(name, ())])
-            fn.argtypes = _argtypes(name, True, n_sc)
+            fn.argtypes = _argtypes(name, True, "d" * n_sc)
             fn.restype = C.c_int
         lib.ref_set_threads.argtypes = [C.c_int]
         lib.ref_max_threads.restype = C.c_int
         lib.ref_uvbke_dt5.restype = C.c_double
         lib.ref_fvtp2d_p1.restype = C.c_double
         lib.ref_fvtp2d_p2.restype = C.c_double
-        _ref = lib
-    return _ref
+        _ref[precision] = lib
+    return _ref[precision]
 
 
-def _descs(name: str, host: dict, L: Layout):
+def _descs(name: str, host: dict, L: Layout, precision: str = "f64", islands: bool = True):
+    """Descriptors of the non-scratch arrays.  Element types are checked:
+    float64 everywhere for "f64"; for "f32" float32, except the FP64 islands
+    when `islands` (the oracle's FP32 form)."""
     spec = KERNELS[name]
     L1 = Layout1D(L.ny, L.h)
     keep = []
@@ -108,7 +126,10 @@ def _descs(name: str, host: dict, L: Layout):
         if a.kind == "scratch":
             continue
         buf = host[a.name]
-        assert buf.dtype == np.float64 and buf.flags.c_contiguous
+        want = spec.arg_precision(a.name, precision) if islands else precision
+        dt = np.float64 if want == "f64" else np.float32
+        if buf.dtype != dt or not buf.flags.c_contiguous:
+            raise TypeError(f"{name}: array '{a.name}' must be a contiguous {np.dtype(dt).name} array")
         d = desc1(buf.ctypes.data, L1) if a.kind == "in1" else desc3(buf.ctypes.data, L)
         keep.append(d)
     return keep
@@ -121,24 +142,26 @@ def _scalar_values(name: str, scalars: dict | None, skip=()) -> list[float]:
     return [float(sc[n]) for n, _ in KERNELS[name].scalars if n not in skip]
 
 
-def run_oracle(name: str, host: dict, L: Layout, scalars: dict | None = None) -> None:
-    """Runs the C oracle in place on the host arrays."""
-    ds = _descs(name, host, L)
+def run_oracle(name: str, host: dict, L: Layout, scalars: dict | None = None, precision: str = "f64") -> None:
+    """Runs the C oracle (FP64, or the FP32 form) in place on the host arrays."""
+    ds = _descs(name, host, L, precision)
     dom = abi.OebDomain(L.nx, L.ny, L.nz)
-    rc = getattr(oracle_lib(), f"oracle_{name}")(C.byref(dom), *[C.byref(d) for d in ds],
-                                                 *_scalar_values(name, scalars))
+    fname = _oracle_name(name, precision)
+    rc = getattr(oracle_lib(), fname)(C.byref(dom), *[C.byref(d) for d in ds], *_scalar_values(name, scalars))
     if rc:
-        raise RuntimeError(f"oracle_{name} failed ({rc})")
+        raise RuntimeError(f"{fname} failed ({rc})")
 
 
 def run_ref(name: str, host: dict, L: Layout, variant: int | None = None, scalars: dict | None = None,
-            threads: int | None = None) -> None:
-    """Runs the reference's own kernel (variant as the reference's ALGO) in place."""
-    lib = ref_lib()
+            threads: int | None = None, precision: str = "f64") -> None:
+    """Runs the reference's own kernel (variant as the reference's ALGO) in
+    place.  precision "f32" runs its ElementType = float build, where every
+    array is float32."""
+    lib = ref_lib(precision)
     if threads is not None:
         lib.ref_set_threads(int(threads))
     v = ORACLE_VARIANT[name] if variant is None else variant
-    ds = _descs(name, host, L)
+    ds = _descs(name, host, L, precision, islands=False)
     dom = abi.OebDomain(L.nx, L.ny, L.nz)
     skip = _REF_FIXED.get(name, ())
     rc = getattr(lib, f"ref_{name}")(v, C.byref(dom), *[C.byref(d) for d in ds], *_scalar_values(name, scalars, skip))

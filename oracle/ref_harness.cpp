@@ -4,7 +4,7 @@
 // TEST INFRASTRUCTURE ONLY.  It pins the oracle and serves as the CPU
 // baseline.  oracle/Makefile compiles it against the reference headers in
 // place (-I<reference>/include -I<reference>/kernels) into
-// oracle/_ref/libref.so.  No reference source is copied into this
+// oracle/_ref/libref.so (and libref_f32.so).  No reference source is copied into this
 // repository: this file #includes the headers where they lie.
 //
 // The reference drivers (src/<name>.cpp:1-12) define the globals
@@ -30,7 +30,19 @@ int64_t domain_height = 60;
 int64_t halo_width = 3;
 int64_t count = 1;
 
+// Built twice: ElementType double (_ref/libref.so) and, with -DOEB_REF_FLOAT,
+// ElementType float (_ref/libref_f32.so), the two element types the
+// reference drivers can select (src/<name>.cpp:9).  RS3 / RS1 are the
+// descriptors of include/oeb.h for that element type.
+#ifdef OEB_REF_FLOAT
+typedef float ElementType;
+typedef oeb_s3f RS3;
+typedef oeb_s1f RS1;
+#else
 typedef double ElementType;
+typedef oeb_s3 RS3;
+typedef oeb_s1 RS1;
+#endif
 
 #define p1 ElementType(0.583333)
 #define p2 ElementType(0.083333)
@@ -75,7 +87,7 @@ namespace {
 // Variant numbers, as the ALGO values of include/util.h:10-13.
 enum { V_DEFAULT = 0, V_FULL_FUSION = 1, V_PARTIAL_FUSION = 2, V_OPENMP = 3 };
 
-Storage3D to_ref(const oeb_s3 *s) {
+Storage3D to_ref(const RS3 *s) {
   Storage3D r;
   r.allocatedPtr = s->allocatedPtr;
   r.alignedPtr = s->alignedPtr;
@@ -87,7 +99,7 @@ Storage3D to_ref(const oeb_s3 *s) {
   return r;
 }
 
-Storage1D to_ref(const oeb_s1 *s) {
+Storage1D to_ref(const RS1 *s) {
   Storage1D r;
   r.allocatedPtr = s->allocatedPtr;
   r.alignedPtr = s->alignedPtr;
@@ -100,10 +112,10 @@ Storage1D to_ref(const oeb_s1 *s) {
 // Zero-filled scratch with the layout of `like`; released by ~Scratch.
 struct Scratch {
   Storage3D s;
-  explicit Scratch(const oeb_s3 *like) {
+  explicit Scratch(const RS3 *like) {
     s = to_ref(like);
     const size_t n = size_t(like->sizes[0] * like->strides[0]) + 64;
-    s.allocatedPtr = static_cast<double *>(calloc(n, sizeof(double)));
+    s.allocatedPtr = static_cast<ElementType *>(calloc(n, sizeof(ElementType)));
     s.alignedPtr = s.allocatedPtr;
   }
   ~Scratch() { free(s.allocatedPtr); }
@@ -129,7 +141,7 @@ int ref_max_threads(void) { return omp_get_max_threads(); }
 // Each ref_<name> returns 0 on success, -1 for a non-square domain and -2 for
 // a variant the reference does not have.
 
-int ref_laplace(int variant, const oeb_domain *d, const oeb_s3 *in, const oeb_s3 *out) {
+int ref_laplace(int variant, const oeb_domain *d, const RS3 *in, const RS3 *out) {
   if (!set_domain(d)) return -1;
   Storage3D i = to_ref(in), o = to_ref(out);
   switch (variant) {
@@ -141,8 +153,8 @@ int ref_laplace(int variant, const oeb_domain *d, const oeb_s3 *in, const oeb_s3
   return -2;
 }
 
-int ref_hdiffsa(int variant, const oeb_domain *d, const oeb_s3 *out, const oeb_s3 *in, const oeb_s3 *mask,
-                const oeb_s1 *crlato, const oeb_s1 *crlatu) {
+int ref_hdiffsa(int variant, const oeb_domain *d, const RS3 *out, const RS3 *in, const RS3 *mask,
+                const RS1 *crlato, const RS1 *crlatu) {
   if (!set_domain(d)) return -1;
   Scratch lap(in), flx(in), fly(in);
   Storage3D o = to_ref(out), i = to_ref(in), m = to_ref(mask);
@@ -156,9 +168,9 @@ int ref_hdiffsa(int variant, const oeb_domain *d, const oeb_s3 *out, const oeb_s
   return -2;
 }
 
-int ref_hdiffsmag(int variant, const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
-                  const oeb_s3 *vin, const oeb_s3 *mask, const oeb_s1 *crlavo, const oeb_s1 *crlavu,
-                  const oeb_s1 *crlato, const oeb_s1 *crlatu, const oeb_s1 *acrlat0, double eddlat, double eddlon,
+int ref_hdiffsmag(int variant, const oeb_domain *d, const RS3 *uout, const RS3 *vout, const RS3 *uin,
+                  const RS3 *vin, const RS3 *mask, const RS1 *crlavo, const RS1 *crlavu,
+                  const RS1 *crlato, const RS1 *crlatu, const RS1 *acrlat0, double eddlat, double eddlon,
                   double tau_smag, double weight_smag) {
   if (!set_domain(d)) return -1;
   Scratch t(uin), s(uin);
@@ -188,9 +200,9 @@ int ref_hdiffsmag(int variant, const oeb_domain *d, const oeb_s3 *uout, const oe
   Scratch s0(like), s1(like), s2(like), s3(like), s4(like), s5(like), s6(like), s7(like)
 #define HADV_ARGS uo, vo, ui, vi, a0, a1, t0, t1, s0.s, s1.s, s2.s, s3.s, s4.s, s5.s, s6.s, s7.s, eddlat, eddlon
 
-int ref_hadvuv(int variant, const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
-               const oeb_s3 *vin, const oeb_s1 *acrlat0, const oeb_s1 *acrlat1, const oeb_s1 *tgrlatda0,
-               const oeb_s1 *tgrlatda1, double eddlat, double eddlon) {
+int ref_hadvuv(int variant, const oeb_domain *d, const RS3 *uout, const RS3 *vout, const RS3 *uin,
+               const RS3 *vin, const RS1 *acrlat0, const RS1 *acrlat1, const RS1 *tgrlatda0,
+               const RS1 *tgrlatda1, double eddlat, double eddlon) {
   if (!set_domain(d)) return -1;
   HADV_SCRATCH(uin);
   Storage3D uo = to_ref(uout), vo = to_ref(vout), ui = to_ref(uin), vi = to_ref(vin);
@@ -204,9 +216,9 @@ int ref_hadvuv(int variant, const oeb_domain *d, const oeb_s3 *uout, const oeb_s
   return -2;
 }
 
-int ref_hadvuv5th(int variant, const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
-                  const oeb_s3 *vin, const oeb_s1 *acrlat0, const oeb_s1 *acrlat1, const oeb_s1 *tgrlatda0,
-                  const oeb_s1 *tgrlatda1, double eddlat, double eddlon) {
+int ref_hadvuv5th(int variant, const oeb_domain *d, const RS3 *uout, const RS3 *vout, const RS3 *uin,
+                  const RS3 *vin, const RS1 *acrlat0, const RS1 *acrlat1, const RS1 *tgrlatda0,
+                  const RS1 *tgrlatda1, double eddlat, double eddlon) {
   if (!set_domain(d)) return -1;
   HADV_SCRATCH(uin);
   Storage3D uo = to_ref(uout), vo = to_ref(vout), ui = to_ref(uin), vi = to_ref(vin);
@@ -220,9 +232,9 @@ int ref_hadvuv5th(int variant, const oeb_domain *d, const oeb_s3 *uout, const oe
   return -2;
 }
 
-int ref_fastwavesuv(int variant, const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
-                    const oeb_s3 *vin, const oeb_s3 *utens, const oeb_s3 *vtens, const oeb_s3 *wgtfac,
-                    const oeb_s3 *ppuv, const oeb_s3 *hhl, const oeb_s3 *rho, const oeb_s1 *fx, double edadlat,
+int ref_fastwavesuv(int variant, const oeb_domain *d, const RS3 *uout, const RS3 *vout, const RS3 *uin,
+                    const RS3 *vin, const RS3 *utens, const RS3 *vtens, const RS3 *wgtfac,
+                    const RS3 *ppuv, const RS3 *hhl, const RS3 *rho, const RS1 *fx, double edadlat,
                     double dtv) {
   if (!set_domain(d)) return -1;
   Scratch a(uin), b(uin), c(uin), e(uin);
@@ -250,8 +262,8 @@ int ref_fastwavesuv(int variant, const oeb_domain *d, const oeb_s3 *uout, const 
 
 // dt5 is the driver's constant (src/uvbke.cpp:11-12); the reference kernel
 // reads the global, so this entry has no dt5 argument.
-int ref_uvbke(int variant, const oeb_domain *d, const oeb_s3 *ub, const oeb_s3 *vb, const oeb_s3 *uc,
-              const oeb_s3 *vc, const oeb_s3 *cosa, const oeb_s3 *rsina) {
+int ref_uvbke(int variant, const oeb_domain *d, const RS3 *ub, const RS3 *vb, const RS3 *uc,
+              const RS3 *vc, const RS3 *cosa, const RS3 *rsina) {
   if (!set_domain(d)) return -1;
   Storage3D a = to_ref(ub), b = to_ref(vb), c = to_ref(uc), e = to_ref(vc), f = to_ref(cosa), g = to_ref(rsina);
   switch (variant) {
@@ -264,9 +276,9 @@ int ref_uvbke(int variant, const oeb_domain *d, const oeb_s3 *ub, const oeb_s3 *
 }
 double ref_uvbke_dt5(void) { return dt5; }
 
-int ref_p_grad_c(int variant, const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
-                 const oeb_s3 *vin, const oeb_s3 *rdxc, const oeb_s3 *rdyc, const oeb_s3 *delpc, const oeb_s3 *gz,
-                 const oeb_s3 *pkc, double dt2) {
+int ref_p_grad_c(int variant, const oeb_domain *d, const RS3 *uout, const RS3 *vout, const RS3 *uin,
+                 const RS3 *vin, const RS3 *rdxc, const RS3 *rdyc, const RS3 *delpc, const RS3 *gz,
+                 const RS3 *pkc, double dt2) {
   if (!set_domain(d)) return -1;
   Scratch wk(uin);
   Storage3D uo = to_ref(uout), vo = to_ref(vout), ui = to_ref(uin), vi = to_ref(vin), x = to_ref(rdxc),
@@ -280,9 +292,9 @@ int ref_p_grad_c(int variant, const oeb_domain *d, const oeb_s3 *uout, const oeb
   return -2;
 }
 
-int ref_nh_p_grad(int variant, const oeb_domain *d, const oeb_s3 *uout, const oeb_s3 *vout, const oeb_s3 *uin,
-                  const oeb_s3 *vin, const oeb_s3 *rdx, const oeb_s3 *rdy, const oeb_s3 *gz, const oeb_s3 *pp,
-                  const oeb_s3 *pk3, const oeb_s3 *wk1, double dtv) {
+int ref_nh_p_grad(int variant, const oeb_domain *d, const RS3 *uout, const RS3 *vout, const RS3 *uin,
+                  const RS3 *vin, const RS3 *rdx, const RS3 *rdy, const RS3 *gz, const RS3 *pp,
+                  const RS3 *pk3, const RS3 *wk1, double dtv) {
   if (!set_domain(d)) return -1;
   Scratch wk(uin), du(uin), dv(uin);
   Storage3D uo = to_ref(uout), vo = to_ref(vout), ui = to_ref(uin), vi = to_ref(vin), x = to_ref(rdx),
@@ -304,10 +316,10 @@ int ref_nh_p_grad(int variant, const oeb_domain *d, const oeb_s3 *uout, const oe
 
 // Only DEFAULT (0).  fvtp2d_fullfusion reads the never-written br array
 // (kernels/fvtp2d.h:195-196), so it is not exposed.
-int ref_fvtp2d(int variant, const oeb_domain *d, const oeb_s3 *q_i, const oeb_s3 *q_j, const oeb_s3 *fx1,
-               const oeb_s3 *fx2, const oeb_s3 *fy1, const oeb_s3 *fy2, const oeb_s3 *q, const oeb_s3 *crx,
-               const oeb_s3 *cry, const oeb_s3 *ra_x, const oeb_s3 *ra_y, const oeb_s3 *xfx, const oeb_s3 *yfx,
-               const oeb_s3 *area) {
+int ref_fvtp2d(int variant, const oeb_domain *d, const RS3 *q_i, const RS3 *q_j, const RS3 *fx1,
+               const RS3 *fx2, const RS3 *fy1, const RS3 *fy2, const RS3 *q, const RS3 *crx,
+               const RS3 *cry, const RS3 *ra_x, const RS3 *ra_y, const RS3 *xfx, const RS3 *yfx,
+               const RS3 *area) {
   if (!set_domain(d)) return -1;
   if (variant != V_DEFAULT) return -2;
   Scratch fxx(q), fyy(q), al(q), almq(q), br(q), b0(q), smt5(q);

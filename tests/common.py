@@ -33,6 +33,8 @@ def outside_mask(name: str, out: str, L: Layout) -> np.ndarray:
 
 def rel_maxnorm(x: np.ndarray, ref: np.ndarray) -> float:
     """max|x - ref| / max|ref| (0 when both are identically zero)."""
+    x = np.asarray(x, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
     den = float(np.max(np.abs(ref))) if ref.size else 0.0
     num = float(np.max(np.abs(x - ref))) if ref.size else 0.0
     if den == 0.0:
@@ -41,4 +43,16 @@ def rel_maxnorm(x: np.ndarray, ref: np.ndarray) -> float:
 
 
 def bit_diffs(x: np.ndarray, ref: np.ndarray) -> int:
-    return int(np.count_nonzero(np.ascontiguousarray(x).view(np.int64) != np.ascontiguousarray(ref).view(np.int64)))
+    """Number of elements whose bit patterns differ (same dtype required)."""
+    assert x.dtype == ref.dtype, (x.dtype, ref.dtype)
+    it = np.int64 if x.dtype == np.float64 else np.int32
+    return int(np.count_nonzero(np.ascontiguousarray(x).view(it) != np.ascontiguousarray(ref).view(it)))
+
+
+def rel_rms(x: np.ndarray, ref: np.ndarray) -> float:
+    """RMS(x - ref) / RMS(ref)."""
+    x = np.asarray(x, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    den = float(np.sqrt(np.mean(ref * ref))) if ref.size else 0.0
+    num = float(np.sqrt(np.mean((x - ref) ** 2))) if ref.size else 0.0
+    return num if den == 0.0 else num / den

@@ -9,6 +9,13 @@ Each fixture  tests/golden/<kernel>_<family>_<nx>x<ny>x<nz>.npz  holds:
             them, or the test reports that the generator changed)
   <output>  the output on its valid region, shape (nz, j-extent, i-extent)
 
+FP32 fixtures  <kernel>_<family>_<nx>x<ny>x<nz>_f32.npz  hold the outputs of
+the reference's ElementType = float build (oracle/_ref/libref_f32.so) on the
+same inputs rounded to float32, for the outputs the oracle's FP32 form
+reproduces exactly (F32_EXACT): every output of the stencils without FP64
+islands, and fvtp2d's fx2 / fy2, which do not depend on its compensated
+divergence.
+
 Run:  make -C oracle ref && python tests/golden/make_golden.py
 """
 from __future__ import annotations
@@ -25,7 +32,7 @@ ROOT = HERE.parent.parent
 sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "open-earth-benchmarks_amd"))
 
-from oeb.generators import generate  # noqa: E402
+from oeb.generators import cast, generate  # noqa: E402
 from oeb.layout import Layout  # noqa: E402
 from oeb.spec import KERNELS, ORDER  # noqa: E402
 from oracle import binding as B  # noqa: E402
@@ -33,6 +40,12 @@ from oracle import binding as B  # noqa: E402
 # (family, nx, ny, nz, align, seed)
 CASES = [("baseline", 12, 12, 5, 16, 21), ("refdriver", 20, 20, 3, 1, 0)]
 EXTRA = {"hdiffsmag": [("stress", 12, 12, 5, 16, 22)]}
+
+
+# Outputs the FP32 oracle shares bit for bit with the reference's float build.
+F32_EXACT = {k: tuple(KERNELS[k].outputs) for k in ("laplace", "hdiffsa", "hdiffsmag", "hadvuv", "hadvuv5th", "uvbke")}
+F32_EXACT["fvtp2d"] = ("fx2", "fy2")
+F32_CASES = [("baseline", 12, 12, 5, 16, 21)]
 
 
 def sha(a: np.ndarray) -> str:
@@ -45,6 +58,10 @@ def fixture_name(kernel, family, nx, ny, nz) -> str:
 
 def cases(kernel):
     return CASES + EXTRA.get(kernel, [])
+
+
+def fixture_name_f32(kernel, family, nx, ny, nz) -> str:
+    return f"{kernel}_{family}_{nx}x{ny}x{nz}_f32.npz"
 
 
 def main() -> int:
@@ -67,6 +84,26 @@ def main() -> int:
                     "align": align, "reference_variant": B.ORACLE_VARIANT[kernel], "input_sha256": ins}
             np.savez_compressed(HERE / fixture_name(kernel, family, nx, ny, nz), meta=np.array(json.dumps(meta)),
                                 **arrays)
+    if not B.ref_available("f32"):
+        print("oracle/_ref/libref_f32.so missing: build it with `make -C oracle ref`", file=sys.stderr)
+        return 1
+    for kernel, outs in F32_EXACT.items():
+        spec = KERNELS[kernel]
+        for family, nx, ny, nz, align, seed in F32_CASES:
+            L = Layout(nx, ny, nz, h=3, hk=1, align=align)
+            host = cast(kernel, generate(kernel, L, family, seed), "f32", islands=False)
+            ins = {a.name: sha(host[a.name]) for a in spec.args if a.kind in ("in", "in1")}
+            B.run_ref(kernel, host, L, threads=1, precision="f32")
+            arrays = {}
+            for o in outs:
+                i0, i1, j0, j1 = spec.region(o, nx, ny)
+                arrays[o] = np.ascontiguousarray(L.region(host[o], i0, i1, j0, j1))
+                assert np.isfinite(arrays[o]).all()
+            meta = {"kernel": kernel, "family": family, "seed": seed, "nx": nx, "ny": ny, "nz": nz, "h": 3, "hk": 1,
+                    "align": align, "reference_variant": B.ORACLE_VARIANT[kernel], "element_type": "float",
+                    "input_sha256": ins}
+            np.savez_compressed(HERE / fixture_name_f32(kernel, family, nx, ny, nz),
+                                meta=np.array(json.dumps(meta)), **arrays)
     print("golden fixtures written to", HERE)
     return 0
 

@@ -80,3 +80,21 @@ def test_parity_512(name):
     L, host = inputs(name, 512, 512, 80, "baseline", seed=3)
     print(f"[gpu] {name} 512x512x80", flush=True)
     _check(name, L, host, run_gpu(name, copy(host), L))
+
+
+PAIRED = ["laplace", "uvbke", "fastwavesuv", "p_grad_c", "nh_p_grad"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kchunk", [1, 3, 80])
+@pytest.mark.parametrize("pairs", [0, 1])
+@pytest.mark.parametrize("name", PAIRED)
+def test_path_switches(name, pairs, kchunk, monkeypatch):
+    """Both the scalar and the paired (16-byte) path, forced through
+    OEB_PAIRS_<K>, at k-chunks that divide nz, do not divide it, and exceed
+    it, agree bit for bit with the oracle."""
+    monkeypatch.setenv(f"OEB_PAIRS_{name.upper()}", str(pairs))
+    monkeypatch.setenv(f"OEB_KCHUNK_{name.upper()}", str(kchunk))
+    L, host = inputs(name, 64, 36, 7, "baseline", seed=9)
+    print(f"[gpu] {name} pairs={pairs} kchunk={kchunk}", flush=True)
+    _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
