@@ -19,6 +19,18 @@ output against the FP64 path's output (which equals the FP64 oracle's bit for
 bit), with the fraction of points whose error exceeds 1e-5 of the output's
 max-norm.
 
+chain: the forward chain hadvuv5th -> hdiffsmag -> fastwavesuv -> uvbke with
+periodic halo refills (oeb.chain), 10 steps at the same size, enqueued
+eagerly and replayed as one recorded HIP graph; ms per step for each, per
+form.
+
+ensemble: 48 members, nx and ny drawn from {96,...,512} and nz from
+{60, 80, 137}, FP32 with FP64 islands, 10 steps of the same chain: one launch
+per member per op (plain), against one persistent launch per op for all
+members (oeb_batch), eager and replayed from a HIP graph, first with the
+built-in k-chunks, then with the autotuner's choice per (stencil, member
+shape).  Gpts/s counts 4 stencils x all member points per step.
+
 cpu_baseline: the reference's own kernels (OpenMP variants, serial DEFAULT for
 fvtp2d, which has no OpenMP variant), compiled by oracle/Makefile, timed on
 this host over a bounded sample of the same workload.
@@ -57,6 +69,9 @@ def parse_args(argv=None):
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--precision", choices=("both", "f64", "f32"), default="both",
                    help="forms to time: the FP64 line plus its fp32 block (both), or one form alone")
+    p.add_argument("--no-chain", action="store_true", help="skip the chain block (eager vs HIP-graph steps)")
+    p.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble block")
+    p.add_argument("--members", type=int, default=48, help="members of the ensemble block")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="also time a plain device copy of this many MiB (HBM counter calibration)")
     return p.parse_args(argv)
@@ -162,6 +177,135 @@ def per_stencil(names, durs, pts, bytes_per_pt):
             "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
         }
     return out
+
+
+def chain_block(L, forms, seed, steps: int = 10, reps: int = 5):
+    """ms per step of the forward chain, eager and as a replayed graph."""
+    import torch
+
+    from oeb.chain import Chain, forward_host_fields, forward_program, to_device
+
+    out = {"stencils": ["hadvuv5th", "hdiffsmag", "fastwavesuv", "uvbke"], "halo_fills_per_step": 6,
+           "steps_per_measurement": steps}
+    side = torch.cuda.Stream()
+    for prec in forms:
+        ch = Chain(forward_program(), to_device(forward_host_fields(L, prec, seed), L), (L.nx, L.ny, L.nz))
+        s = side.cuda_stream
+        for _ in range(2):
+            ch.step(s)
+        side.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        eager = []
+        for _ in range(reps):
+            e0.record(side)
+            for _ in range(steps):
+                ch.step(s)
+            e1.record(side)
+            side.synchronize()
+            eager.append(e0.elapsed_time(e1) / steps)
+        g = ch.capture(s, steps)
+        g.launch(s)
+        side.synchronize()
+        graph = []
+        for _ in range(reps):
+            e0.record(side)
+            g.launch(s)
+            e1.record(side)
+            side.synchronize()
+            graph.append(e0.elapsed_time(e1) / steps)
+        g.close()
+        u = ch.fields["u"].buf
+        out[prec] = {"eager_ms_per_step": round(statistics.median(eager), 4),
+                     "graph_ms_per_step": round(statistics.median(graph), 4),
+                     "state_finite": bool(torch.isfinite(L.interior(u)).all())}
+    return out
+
+
+def ensemble_block(n_members: int, seed: int, steps: int = 10, reps: int = 3):
+    """ms per step of the forward chain over an uneven ensemble: per-member
+    launches against batched launches (eager and from a graph)."""
+    import torch
+
+    from oeb.ensemble import draw_shapes, forward_ensemble
+    from oeb.tune import Tuner
+    from oeb.tune import clear as tune_clear
+
+    shapes = draw_shapes(n_members, seed)
+    t0 = time.perf_counter()
+    ens = forward_ensemble(shapes, "f32", seed)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    side = torch.cuda.Stream()
+    s = side.cuda_stream
+    pts = sum(nx * ny * nz for nx, ny, nz in shapes)
+
+    def timed(fn):
+        fn()
+        side.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        out = []
+        for _ in range(reps):
+            e0.record(side)
+            fn()
+            e1.record(side)
+            side.synchronize()
+            out.append(e0.elapsed_time(e1) / steps)
+        return statistics.median(out)
+
+    def plain():
+        for _ in range(steps):
+            ens.step_plain(s)
+
+    def batched_ms(batch):
+        graph = ens.capture(batch, s, steps)
+        ms = timed(lambda: graph.launch(s))
+        graph.close()
+        return ms
+
+    plain_ms = timed(plain)
+    batch = ens.record()
+    records, launches = batch.info()
+
+    def batched():
+        for _ in range(steps):
+            batch.launch(s)
+
+    batch_ms = timed(batched)
+    untuned_graph_ms = batched_ms(batch)
+    batch.close()
+
+    # Autotuned k-chunks per (stencil, member shape), then the batch again.
+    t1 = time.perf_counter()
+    tuner = Tuner(path=Path(os.environ.get("TMPDIR", "/tmp")) / "oeb_bench_tuning.json")
+    tuner.cache = {}
+    choices = ens.tune(tuner, "f32", s)
+    tune_s = time.perf_counter() - t1
+    batch = ens.record()
+    graph_ms = batched_ms(batch)
+    batch.close()
+    tune_clear()
+    finite = all(bool(torch.isfinite(m.fields["u"].layout.interior(m.fields["u"].buf)).all()) for m in ens.members)
+    gpts = lambda ms: round(4 * pts / (ms * 1e-3) / 1e9, 3)  # noqa: E731
+    return {
+        "members": n_members,
+        "shapes": [list(x) for x in shapes],
+        "member_points": pts,
+        "precision": "f32 (FP64 islands)",
+        "steps_per_measurement": steps,
+        "launches_per_step": {"plain": records, "batched": launches},
+        "plain_ms_per_step": round(plain_ms, 3),
+        "batched_ms_per_step": round(batch_ms, 3),
+        "batched_graph_untuned_ms_per_step": round(untuned_graph_ms, 3),
+        "batched_graph_ms_per_step": round(graph_ms, 3),
+        "autotune": {"choices": len(choices), "seconds": round(tune_s, 2),
+                     "kchunk_histogram": {str(k): sum(1 for v in choices.values() if v["kchunk"] == k)
+                                          for k in sorted({v["kchunk"] for v in choices.values()})}},
+        "plain_gpts_s": gpts(plain_ms),
+        "batched_graph_gpts_s": gpts(graph_ms),
+        "speedup_graph_vs_plain": round(plain_ms / graph_ms, 3),
+        "setup_s": round(setup, 2),
+        "state_finite": finite,
+    }
 
 
 def main(argv=None):
@@ -302,6 +446,10 @@ def main(argv=None):
             "error_reference": "FP64 outputs of the same step (equal to the FP64 oracle's bit for bit)",
             "per_stencil": per32,
         }
+    if not args.no_chain:
+        result["chain"] = chain_block(L, forms, args.seed)
+    if not args.no_ensemble and args.members > 0:
+        result["ensemble"] = ensemble_block(args.members, args.seed)
     if copy_line:
         result["device_copy"] = copy_line
     if not args.no_cpu_baseline:

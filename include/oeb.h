@@ -311,6 +311,46 @@ int oeb_graph_end(void *stream, oeb_graph **graph);
 int oeb_graph_launch(const oeb_graph *graph, void *stream);
 int oeb_graph_destroy(oeb_graph *graph);
 
+/* ---------------------------------------------------------------- tuning
+ * No counterpart in the reference.  Each entry point picks the levels per
+ * workgroup (k-chunk) and, where it has one, its 16-byte vector path from,
+ * in order: the environment (OEB_KCHUNK_<KERNEL>, OEB_PAIRS_<KERNEL>), this
+ * process-wide table, a built-in default.  The table is keyed by kernel name
+ * ("hadvuv5th", "laplace", ...), element size (8 or 4 bytes) and the
+ * domain's (nx, ny, nz); the project's autotuner (oeb.tune) fills it.
+ * kchunk 0 and vector -1 mean "not set"; setting both removes the entry.
+ * Thread-safe. */
+int oeb_tune_set(const char *kernel, int element_bytes, const oeb_domain *d, int kchunk, int vector);
+int oeb_tune_get(const char *kernel, int element_bytes, const oeb_domain *d, int *kchunk, int *vector);
+int oeb_tune_clear(void);
+
+/* --------------------------------------------------------------- batches
+ * No counterpart in the reference (its drivers run one field at a time).
+ * Runs one stencil over many members of an ensemble in ONE launch.
+ *   oeb_batch_record(b)  from now on, on this host thread, the batchable
+ *                        entry points (hadvuv, hadvuv5th, hdiffsmag,
+ *                        fastwavesuv, uvbke and oeb_halo_fill, both
+ *                        precisions) validate their arguments as usual and
+ *                        record their launch into b instead of launching;
+ *                        the stream argument is ignored.  Other entry points
+ *                        still launch at once.
+ *   oeb_batch_stop(b)    ends recording and uploads the member tables (the
+ *                        only allocation: device memory, freed by destroy).
+ *   oeb_batch_launch     enqueues one persistent launch per run of
+ *                        consecutive records of one kernel, on `stream`.
+ *                        Kernels only: it can be recorded in an oeb_graph.
+ * Record member after member within each stencil (all members' halo fills,
+ * then all members' first stencil, ...): records of one run execute
+ * concurrently, runs in order.  oeb_batch_info reports the number of records
+ * and of launches. */
+typedef struct oeb_batch oeb_batch;
+int oeb_batch_create(oeb_batch **batch);
+int oeb_batch_record(oeb_batch *batch);
+int oeb_batch_stop(oeb_batch *batch);
+int oeb_batch_launch(const oeb_batch *batch, void *stream);
+int oeb_batch_info(const oeb_batch *batch, int *records, int *launches);
+int oeb_batch_destroy(oeb_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

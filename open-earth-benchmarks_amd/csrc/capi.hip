@@ -3,7 +3,10 @@
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <mutex>
 #include <sstream>
+#include <unordered_map>
 
 #include "oeb_common.hpp"
 
@@ -156,9 +159,55 @@ int env_tunable(const char *prefix, const char *kname, int dflt, long lo, long h
 }
 }  // namespace
 
-int kchunk_for(const char *kname, int dflt) { return env_tunable("OEB_KCHUNK_", kname, dflt, 1, 4096); }
+namespace {
+// The tuning table: (kernel, element size, nx, ny, nz) -> (k-chunk, vector path).
+struct TuneKey {
+  std::string kernel;
+  size_t esize;
+  int64_t nx, ny, nz;
+  bool operator==(const TuneKey &o) const {
+    return kernel == o.kernel && esize == o.esize && nx == o.nx && ny == o.ny && nz == o.nz;
+  }
+};
+struct TuneHash {
+  size_t operator()(const TuneKey &k) const {
+    size_t h = std::hash<std::string>()(k.kernel);
+    for (int64_t v : {int64_t(k.esize), k.nx, k.ny, k.nz}) h = h * 1000003u ^ std::hash<int64_t>()(v);
+    return h;
+  }
+};
+struct TuneVal {
+  int kchunk;  // 0: not set
+  int vec;     // -1: not set
+};
+std::mutex g_tune_mu;
+std::unordered_map<TuneKey, TuneVal, TuneHash> g_tune;
 
-bool pairs_for(const char *kname, bool dflt) { return env_tunable("OEB_PAIRS_", kname, dflt ? 1 : 0, 0, 1) != 0; }
+bool tune_lookup(const char *kname, size_t esize, const oeb_domain *d, TuneVal *v) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  if (g_tune.empty()) return false;
+  const auto it = g_tune.find(TuneKey{kname, esize, d->nx, d->ny, d->nz});
+  if (it == g_tune.end()) return false;
+  *v = it->second;
+  return true;
+}
+}  // namespace
+
+int kchunk_for(const char *kname, size_t esize, const oeb_domain *d, int dflt) {
+  const int env = env_tunable("OEB_KCHUNK_", kname, 0, 1, 4096);
+  if (env > 0) return env;
+  TuneVal v;
+  if (tune_lookup(kname, esize, d, &v) && v.kchunk > 0) return v.kchunk;
+  return dflt;
+}
+
+bool pairs_for(const char *kname, size_t esize, const oeb_domain *d, bool dflt) {
+  const int env = env_tunable("OEB_PAIRS_", kname, -1, 0, 1);
+  if (env >= 0) return env != 0;
+  TuneVal v;
+  if (tune_lookup(kname, esize, d, &v) && v.vec >= 0) return v.vec != 0;
+  return dflt;
+}
 
 int launch_status(const char *kname) {
   const hipError_t e = hipGetLastError();
@@ -186,5 +235,38 @@ const char *oeb_status_string(int status) {
 }
 
 const char *oeb_last_error(void) { return oeb::g_last_error.c_str(); }
+
+int oeb_tune_set(const char *kernel, int element_bytes, const oeb_domain *d, int kchunk, int vector) {
+  using namespace oeb;
+  if (kernel == nullptr || d == nullptr) return fail(OEB_ERR_NULL, "tune_set: kernel or domain is NULL");
+  if (element_bytes != 4 && element_bytes != 8) return fail(OEB_ERR_ARG, "tune_set: element_bytes must be 4 or 8");
+  if (kchunk < 0 || kchunk > 4096 || vector < -1 || vector > 1)
+    return fail(OEB_ERR_ARG, "tune_set: kchunk must lie in [0, 4096] and vector in [-1, 1]");
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  const TuneKey key{kernel, size_t(element_bytes), d->nx, d->ny, d->nz};
+  if (kchunk == 0 && vector == -1) {
+    g_tune.erase(key);
+  } else {
+    g_tune[key] = TuneVal{kchunk, vector};
+  }
+  return OEB_OK;
+}
+
+int oeb_tune_get(const char *kernel, int element_bytes, const oeb_domain *d, int *kchunk, int *vector) {
+  using namespace oeb;
+  if (kernel == nullptr || d == nullptr || kchunk == nullptr || vector == nullptr)
+    return fail(OEB_ERR_NULL, "tune_get: a pointer argument is NULL");
+  TuneVal v{0, -1};
+  tune_lookup(kernel, size_t(element_bytes), d, &v);
+  *kchunk = v.kchunk;
+  *vector = v.vec;
+  return OEB_OK;
+}
+
+int oeb_tune_clear(void) {
+  std::lock_guard<std::mutex> lock(oeb::g_tune_mu);
+  oeb::g_tune.clear();
+  return OEB_OK;
+}
 
 }  // extern "C"

@@ -50,12 +50,14 @@ struct FwArgs {
   const T *uin, *vin, *utens, *vtens, *wgtfac, *rho, *fx;
   const double *ppuv, *hhl;
   int fx_stride;
+  T edadlat, dt;
 };
 
+// Logical block `id` of a launch: one 64 x 4 tile of columns over a k-chunk.
 template <class T>
-__global__ void __launch_bounds__(kTileI *kTileJ)
-    fastwavesuv_kernel(FwArgs<T> a, T edadlat, T dt, Geo g, Tiles t) {
-  const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
+__device__ __forceinline__ void fastwavesuv_body(const FwArgs<T> &a, const Geo &g, const Tiles &t, int id) {
+  const T edadlat = a.edadlat, dt = a.dt;
+  const TileCoord c = tile_coord_of(t, id, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
@@ -87,12 +89,25 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
   }
 }
 
+template <class T>
+__global__ void __launch_bounds__(kTileI *kTileJ) fastwavesuv_kernel(FwArgs<T> a, Geo g, Tiles t) {
+  fastwavesuv_body(a, g, t, int(blockIdx.x));
+}
+
+template <class T>
+hipError_t fastwavesuv_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
+  return launch_batch<FwArgs<T>, fastwavesuv_body<T>>(tab, n, total, s, dim3(kTileI, kTileJ));
+}
+template <class T>
+const BatchKind kFastwavesuvBatch{"fastwavesuv", fastwavesuv_batch<T>};
+
 // Two adjacent columns (i, i+1) per lane, i even, 16-byte accesses.  The east
 // neighbour of column i is column i+1 of the pair; that of column i+1 is a
 // single load at i+2.  Suffix P: the pair at (i, i+1); Q: the pair at row j+1;
 // X: column i+2 at row j.
 __global__ void __launch_bounds__(kTileI *kTileJ)
-    fastwavesuv_pair_kernel(FwArgs<double> a, double edadlat, double dt, Geo g, Tiles t) {
+    fastwavesuv_pair_kernel(FwArgs<double> a, Geo g, Tiles t) {
+  const double edadlat = a.edadlat, dt = a.dt;
   const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
   const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
@@ -183,21 +198,27 @@ int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S
   OEB_TRY(as.check(K));
   a.uout = const_cast<T *>(p_uo);
   a.vout = const_cast<T *>(p_vo);
-  const int kc = kchunk_for(K, kChunk);
+  a.edadlat = edadlat;
+  a.dt = dt;
+  const int kc = kchunk_for(K, sizeof(T), d, kChunk);
+  {
+    // Recorded into a batch: the scalar body (same results as the paired one).
+    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+    if (batch_record(kFastwavesuvBatch<T>, a, g, t, t.n)) return OEB_OK;
+  }
   if constexpr (sizeof(T) == 8) {
-    if (pairs_for(K, true) && nx % 2 == 0 &&
+    if (pairs_for(K, sizeof(T), d, true) && nx % 2 == 0 &&
         pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.utens, a.vtens, a.wgtfac, a.ppuv, a.hhl, a.rho})) {
       const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-      hipLaunchKernelGGL(fastwavesuv_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a,
-                         edadlat, dt, g, t);
+      hipLaunchKernelGGL(fastwavesuv_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
       return launch_status(K);
     }
   }
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-  hipLaunchKernelGGL(fastwavesuv_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, edadlat, dt,
-                     g, t);
+  hipLaunchKernelGGL(fastwavesuv_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }
 

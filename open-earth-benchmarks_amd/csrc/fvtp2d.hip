@@ -246,7 +246,7 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   o.fy1 = const_cast<T *>(pfy1);
   o.fy2 = const_cast<T *>(pfy2);
 
-  const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJdefault, kchunk_for(K, kChunk));
+  const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJdefault, kchunk_for(K, sizeof(T), d, kChunk));
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, kTJdefault));
   hipLaunchKernelGGL((fvtp2d_kernel<kTJdefault, T>), dim3(t.n), dim3(kNT), 0, hipStream_t(stream), a, o, p1, p2, g,
                      t);

@@ -64,18 +64,29 @@ __device__ __forceinline__ T upwind(const T *f, int sj, T ua, T va, T eddlat, T 
 // 256x256x80 and 512x512x80 (5th order).
 constexpr int kTI = 64, kTJ = 16, kNT = 256, kRows = kTJ / (kNT / kTI);
 
+template <class T>
+struct HadvArgs {
+  T *uout, *vout;
+  const T *uin, *vin, *acrlat0, *acrlat1, *tgrlatda0, *tgrlatda1;
+  int s0, s1, s2, s3;
+  T eddlat, eddlon;
+};
+
+// Logical block `id` of a launch.  Ends with a barrier, so a persistent
+// workgroup may run it again at once (batched launches).
 template <int ORDER, class T>
-__global__ void __launch_bounds__(kNT)
-    hadvuv_kernel(T *__restrict__ uout, T *__restrict__ vout, const T *__restrict__ uin, const T *__restrict__ vin,
-                  const T *__restrict__ acrlat0, int s0, const T *__restrict__ acrlat1, int s1,
-                  const T *__restrict__ tgrlatda0, int s2, const T *__restrict__ tgrlatda1, int s3, T eddlat,
-                  T eddlon, Geo g, Tiles t) {
+__device__ __forceinline__ void hadvuv_body(const HadvArgs<T> &a, const Geo &g, const Tiles &t, int id) {
   constexpr int R = ORDER == 3 ? 2 : 3;
   constexpr int W = kTI + 2 * R, H = kTJ + 2 * R, kPer = (W * H + kNT - 1) / kNT;
   constexpr T rr = kEarthRadiusRecip<T>;
   __shared__ T su[2][H][W];
   __shared__ T sv[2][H][W];
-  const TileCoord c = tile_coord(t, kTI, kTJ, g.nz);
+  T *__restrict__ uout = a.uout;
+  T *__restrict__ vout = a.vout;
+  const T *__restrict__ uin = a.uin;
+  const T *__restrict__ vin = a.vin;
+  const T eddlat = a.eddlat, eddlon = a.eddlon;
+  const TileCoord c = tile_coord_of(t, id, kTI, kTJ, g.nz);
   const int tid = int(threadIdx.x) + kTI * int(threadIdx.y);
   const int sj = g.sj;
   const int i = c.i0 + int(threadIdx.x);
@@ -86,8 +97,8 @@ __global__ void __launch_bounds__(kNT)
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
     const int64_t j = min(c.j0 + int(threadIdx.y) * kRows + r, g.ny - 1);
-    a0[r] = acrlat0[j * s0], a1[r] = acrlat1[j * s1];
-    tg0[r] = tgrlatda0[j * s2], tg1[r] = tgrlatda1[j * s3];
+    a0[r] = a.acrlat0[j * a.s0], a1[r] = a.acrlat1[j * a.s1];
+    tg0[r] = a.tgrlatda0[j * a.s2], tg1[r] = a.tgrlatda1[j * a.s3];
   }
   // This lane's share of the staged tile: element e = tid + q*kNT of the
   // H x W tile, at in-plane offset off[q].  Elements past the field's valid
@@ -156,6 +167,18 @@ __global__ void __launch_bounds__(kNT)
   }
 }
 
+template <int ORDER, class T>
+__global__ void __launch_bounds__(kNT) hadvuv_kernel(HadvArgs<T> a, Geo g, Tiles t) {
+  hadvuv_body<ORDER>(a, g, t, int(blockIdx.x));
+}
+
+template <int ORDER, class T>
+hipError_t hadvuv_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
+  return launch_batch<HadvArgs<T>, hadvuv_body<ORDER, T>>(tab, n, total, s, dim3(kTI, kNT / kTI));
+}
+template <int ORDER, class T>
+const BatchKind kHadvuvBatch{ORDER == 3 ? "hadvuv" : "hadvuv5th", hadvuv_batch<ORDER, T>};
+
 template <int ORDER, class T, class S3, class S1>
 int hadv_run(const char *K, const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *uin, const S3 *vin,
              const S1 *acrlat0, const S1 *acrlat1, const S1 *tgrlatda0, const S1 *tgrlatda1, T eddlat, T eddlon,
@@ -185,11 +208,12 @@ int hadv_run(const char *K, const oeb_domain *d, const S3 *uout, const S3 *vout,
   as.add1("tgrlatda0", p_t0, 0, ny, s2);
   as.add1("tgrlatda1", p_t1, 0, ny, s3);
   OEB_TRY(as.check(K));
-  const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJ, kchunk_for(K, kChunk));
+  const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJ, kchunk_for(K, sizeof(T), d, kChunk));
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, kTJ));
-  hipLaunchKernelGGL((hadvuv_kernel<ORDER, T>), dim3(t.n), dim3(kTI, kNT / kTI), 0, hipStream_t(stream),
-                     const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_a0, s0, p_a1, s1, p_t0, s2, p_t1,
-                     s3, eddlat, eddlon, g, t);
+  const HadvArgs<T> a{const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_a0, p_a1, p_t0, p_t1,
+                      s0, s1, s2, s3, eddlat, eddlon};
+  if (batch_record(kHadvuvBatch<ORDER, T>, a, g, t, t.n)) return OEB_OK;
+  hipLaunchKernelGGL((hadvuv_kernel<ORDER, T>), dim3(t.n), dim3(kTI, kNT / kTI), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }
 

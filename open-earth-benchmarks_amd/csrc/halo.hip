@@ -46,40 +46,68 @@ __device__ __forceinline__ void ring_point(int64_t e, int nx, int ny, int h, int
 }
 
 template <class T>
-__global__ void __launch_bounds__(kThreads)
-    halo_ring_kernel(T *__restrict__ f, int nx, int ny, int nz, int h, int sj, int64_t sk, int periodic,
-                     int64_t ring) {
-  const int64_t total = ring * nz;
-  for (int64_t e = int64_t(blockIdx.x) * kThreads + threadIdx.x; e < total; e += int64_t(gridDim.x) * kThreads) {
-    const int k = int(e / ring);
-    int i, j;
-    ring_point(e % ring, nx, ny, h, i, j);
-    const int si = periodic ? wrap(i, nx) : clampi(i, nx);
-    const int sjj = periodic ? wrap(j, ny) : clampi(j, ny);
-    const int64_t plane = int64_t(k) * sk;
-    f[plane + i + int64_t(j) * sj] = f[plane + si + int64_t(sjj) * sj];
-  }
+struct RingArgs {
+  T *f;
+  int nx, ny, nz, h, sj, periodic;
+  int64_t sk, ring;
+};
+
+template <class T>
+struct KArgs {
+  T *f;
+  int nx, ny, nz, h, hk, sj;
+  int64_t sk;
+};
+
+// Logical block `id`: ring elements [id*256, id*256 + 256) of all levels.
+template <class T>
+__device__ __forceinline__ void halo_ring_body(const RingArgs<T> &a, const Geo &, const Tiles &, int id) {
+  const int64_t e = int64_t(id) * kThreads + threadIdx.x;
+  if (e >= a.ring * a.nz) return;
+  const int k = int(e / a.ring);
+  int i, j;
+  ring_point(e % a.ring, a.nx, a.ny, a.h, i, j);
+  const int si = a.periodic ? wrap(i, a.nx) : clampi(i, a.nx);
+  const int sjj = a.periodic ? wrap(j, a.ny) : clampi(j, a.ny);
+  const int64_t plane = int64_t(k) * a.sk;
+  a.f[plane + i + int64_t(j) * a.sj] = a.f[plane + si + int64_t(sjj) * a.sj];
+}
+
+// Logical block `id`: elements [id*256, id*256 + 256) of the 2*hk k-halo
+// planes (each the whole i/j box).
+template <class T>
+__device__ __forceinline__ void halo_k_body(const KArgs<T> &a, const Geo &, const Tiles &, int id) {
+  const int64_t w = a.nx + 2 * a.h, box = w * (a.ny + 2 * a.h);
+  const int64_t e = int64_t(id) * kThreads + threadIdx.x;
+  if (e >= box * 2 * a.hk) return;
+  const int64_t p = e % box;
+  const int q = int(e / box);  // 0 .. 2hk-1: planes -hk..-1, then nz..nz+hk-1
+  const int k = q < a.hk ? q - a.hk : a.nz + (q - a.hk);
+  const int ks = q < a.hk ? 0 : a.nz - 1;
+  const int i = -a.h + int(p % w), j = -a.h + int(p / w);
+  const int64_t at = i + int64_t(j) * a.sj;
+  a.f[int64_t(k) * a.sk + at] = a.f[int64_t(ks) * a.sk + at];
+}
+
+template <class A, void (*Body)(const A &, const Geo &, const Tiles &, int)>
+__global__ void __launch_bounds__(kThreads) halo_kernel(A a, int blocks) {
+  for (int b = int(blockIdx.x); b < blocks; b += int(gridDim.x)) Body(a, Geo{}, Tiles{}, b);
 }
 
 template <class T>
-__global__ void __launch_bounds__(kThreads)
-    halo_k_kernel(T *__restrict__ f, int nx, int ny, int nz, int h, int hk, int sj, int64_t sk) {
-  const int64_t w = nx + 2 * h, box = w * (ny + 2 * h), total = box * 2 * hk;
-  for (int64_t e = int64_t(blockIdx.x) * kThreads + threadIdx.x; e < total; e += int64_t(gridDim.x) * kThreads) {
-    const int64_t p = e % box;
-    const int q = int(e / box);  // 0 .. 2hk-1: planes -hk..-1, then nz..nz+hk-1
-    const int k = q < hk ? q - hk : nz + (q - hk);
-    const int ks = q < hk ? 0 : nz - 1;
-    const int i = -h + int(p % w), j = -h + int(p / w);
-    const int64_t at = i + int64_t(j) * sj;
-    f[int64_t(k) * sk + at] = f[int64_t(ks) * sk + at];
-  }
+hipError_t halo_ring_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
+  return launch_batch<RingArgs<T>, halo_ring_body<T>>(tab, n, total, s, dim3(kThreads));
 }
+template <class T>
+hipError_t halo_k_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
+  return launch_batch<KArgs<T>, halo_k_body<T>>(tab, n, total, s, dim3(kThreads));
+}
+template <class T>
+const BatchKind kHaloRingBatch{"halo_fill (ring)", halo_ring_batch<T>};
+template <class T>
+const BatchKind kHaloKBatch{"halo_fill (k planes)", halo_k_batch<T>};
 
-int grid_for(int64_t total) {
-  const int64_t blocks = (total + kThreads - 1) / kThreads;
-  return int(blocks < 8192 ? (blocks < 1 ? 1 : blocks) : 8192);
-}
+int grid_for(int64_t blocks) { return int(blocks < 8192 ? (blocks < 1 ? 1 : blocks) : 8192); }
 
 template <class T, class S>
 int halo_run(const oeb_domain *d, const S *field, int halo_ij, int halo_k, int mode, void *stream) {
@@ -97,15 +125,24 @@ int halo_run(const oeb_domain *d, const S *field, int halo_ij, int halo_k, int m
   T *f = const_cast<T *>(p);
   if (h > 0) {
     const int64_t ring = 2 * h * (nx + 2 * h) + ny * 2 * h;
-    hipLaunchKernelGGL(halo_ring_kernel<T>, dim3(grid_for(ring * nz)), dim3(kThreads), 0, hipStream_t(stream), f,
-                       int(nx), int(ny), int(nz), int(h), g.sj, g.sk, mode == OEB_HALO_PERIODIC ? 1 : 0, ring);
-    OEB_TRY(launch_status(K));
+    const int64_t blocks = (ring * nz + kThreads - 1) / kThreads;
+    if (blocks > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, "halo_fill: domain too large");
+    const RingArgs<T> a{f, int(nx), int(ny), int(nz), int(h), g.sj, mode == OEB_HALO_PERIODIC ? 1 : 0, g.sk, ring};
+    if (!batch_record(kHaloRingBatch<T>, a, g, Tiles{}, int(blocks))) {
+      hipLaunchKernelGGL((halo_kernel<RingArgs<T>, halo_ring_body<T>>), dim3(grid_for(blocks)), dim3(kThreads), 0,
+                         hipStream_t(stream), a, int(blocks));
+      OEB_TRY(launch_status(K));
+    }
   }
   if (hk > 0) {
-    const int64_t total = (nx + 2 * h) * (ny + 2 * h) * 2 * hk;
-    hipLaunchKernelGGL(halo_k_kernel<T>, dim3(grid_for(total)), dim3(kThreads), 0, hipStream_t(stream), f, int(nx),
-                       int(ny), int(nz), int(h), int(hk), g.sj, g.sk);
-    OEB_TRY(launch_status(K));
+    const int64_t blocks = ((nx + 2 * h) * (ny + 2 * h) * 2 * hk + kThreads - 1) / kThreads;
+    if (blocks > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, "halo_fill: domain too large");
+    const KArgs<T> a{f, int(nx), int(ny), int(nz), int(h), int(hk), g.sj, g.sk};
+    if (!batch_record(kHaloKBatch<T>, a, g, Tiles{}, int(blocks))) {
+      hipLaunchKernelGGL((halo_kernel<KArgs<T>, halo_k_body<T>>), dim3(grid_for(blocks)), dim3(kThreads), 0,
+                         hipStream_t(stream), a, int(blocks));
+      OEB_TRY(launch_status(K));
+    }
   }
   return OEB_OK;
 }

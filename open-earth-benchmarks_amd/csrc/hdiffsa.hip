@@ -79,7 +79,7 @@ int hdiffsa_run(const oeb_domain *d, const S3 *out, const S3 *in, const S3 *mask
   as.add1("crlato", p_lo, -1, ny + 1, s_lo);
   as.add1("crlatu", p_lu, -1, ny + 1, s_lu);
   OEB_TRY(as.check(K));
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kchunk_for(K, kChunk));
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kchunk_for(K, sizeof(T), d, kChunk));
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
   hipLaunchKernelGGL(hdiffsa_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
                      const_cast<T *>(p_out), p_in, p_mask, p_lo, s_lo, p_lu, s_lu, g, t);

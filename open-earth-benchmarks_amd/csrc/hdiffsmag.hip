@@ -29,21 +29,31 @@ __device__ __forceinline__ T cxx_min(T a, T b) {
 }
 
 template <class T>
-__global__ void __launch_bounds__(kTileI *kTileJ)
-    hdiffsmag_kernel(T *__restrict__ uout, T *__restrict__ vout, const T *__restrict__ uin,
-                     const T *__restrict__ vin, const T *__restrict__ mask, const T *__restrict__ crlavo, int s_vo,
-                     const T *__restrict__ crlavu, int s_vu, const T *__restrict__ crlato, int s_to,
-                     const T *__restrict__ crlatu, int s_tu, const T *__restrict__ acrlat0, int s_ac, T eddlat,
-                     T eddlon, T tau_smag, T weight_smag, Geo g, Tiles t) {
+struct HsArgs {
+  T *uout, *vout;
+  const T *uin, *vin, *mask, *crlavo, *crlavu, *crlato, *crlatu, *acrlat0;
+  int s_vo, s_vu, s_to, s_tu, s_ac;
+  T eddlat, eddlon, tau_smag, weight_smag;
+};
+
+// Logical block `id` of a launch: one 64 x 4 tile of columns over a k-chunk.
+template <class T>
+__device__ __forceinline__ void hdiffsmag_body(const HsArgs<T> &a, const Geo &g, const Tiles &t, int id) {
   constexpr T rr = kEarthRadiusRecip<T>;
-  const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
+  const T eddlat = a.eddlat, eddlon = a.eddlon, tau_smag = a.tau_smag, weight_smag = a.weight_smag;
+  const T *__restrict__ uin = a.uin;
+  const T *__restrict__ vin = a.vin;
+  const T *__restrict__ mask = a.mask;
+  T *__restrict__ uout = a.uout;
+  T *__restrict__ vout = a.vout;
+  const TileCoord c = tile_coord_of(t, id, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
-  const T ac_s = acrlat0[int64_t(j - 1) * s_ac], ac_c = acrlat0[int64_t(j) * s_ac],
-          ac_n = acrlat0[int64_t(j + 1) * s_ac];
-  const T lo = crlato[int64_t(j) * s_to], lu = crlatu[int64_t(j) * s_tu];
-  const T vo = crlavo[int64_t(j) * s_vo], vu = crlavu[int64_t(j) * s_vu];
+  const T ac_s = a.acrlat0[int64_t(j - 1) * a.s_ac], ac_c = a.acrlat0[int64_t(j) * a.s_ac],
+          ac_n = a.acrlat0[int64_t(j + 1) * a.s_ac];
+  const T lo = a.crlato[int64_t(j) * a.s_to], lu = a.crlatu[int64_t(j) * a.s_tu];
+  const T vo = a.crlavo[int64_t(j) * a.s_vo], vu = a.crlavu[int64_t(j) * a.s_vu];
   int64_t at = int64_t(c.k0) * g.sk + (i + j * sj);
   for (int k = c.k0; k < c.k1; ++k, at += g.sk) {
     const T *u = uin + at;
@@ -73,6 +83,18 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
     st_nt(vout + at, v0 + smag_v * lapv);
   }
 }
+
+template <class T>
+__global__ void __launch_bounds__(kTileI *kTileJ) hdiffsmag_kernel(HsArgs<T> a, Geo g, Tiles t) {
+  hdiffsmag_body(a, g, t, int(blockIdx.x));
+}
+
+template <class T>
+hipError_t hdiffsmag_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
+  return launch_batch<HsArgs<T>, hdiffsmag_body<T>>(tab, n, total, s, dim3(kTileI, kTileJ));
+}
+template <class T>
+const BatchKind kHdiffsmagBatch{"hdiffsmag", hdiffsmag_batch<T>};
 
 template <class T, class S3, class S1>
 int hdiffsmag_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *uin, const S3 *vin, const S3 *mask,
@@ -108,11 +130,12 @@ int hdiffsmag_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 
   as.add1("crlatu", p_tu1, 0, ny, s_tu);
   as.add1("acrlat0", p_ac, -1, ny + 1, s_ac);
   OEB_TRY(as.check(K));
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kchunk_for(K, kChunk));
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kchunk_for(K, sizeof(T), d, kChunk));
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-  hipLaunchKernelGGL(hdiffsmag_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
-                     const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_m, p_vo1, s_vo, p_vu1, s_vu, p_to1,
-                     s_to, p_tu1, s_tu, p_ac, s_ac, eddlat, eddlon, tau_smag, weight_smag, g, t);
+  const HsArgs<T> a{const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_m, p_vo1, p_vu1, p_to1, p_tu1, p_ac,
+                    s_vo, s_vu, s_to, s_tu, s_ac, eddlat, eddlon, tau_smag, weight_smag};
+  if (batch_record(kHdiffsmagBatch<T>, a, g, t, t.n)) return OEB_OK;
+  hipLaunchKernelGGL(hdiffsmag_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }
 

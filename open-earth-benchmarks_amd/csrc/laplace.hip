@@ -33,24 +33,34 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
   }
 }
 
-// Two adjacent columns (i, i+1) per lane, i even: 16-byte loads and stores
-// (FP64 only).  Same arithmetic as laplace_kernel: for the pair, in[i-1] and
-// in[i+2] are single loads and the inner neighbours come from the pair.
-constexpr int kPairI = 2 * kTileI;
+// W adjacent columns (i .. i+W-1) per lane, i a multiple of W: 16-byte loads
+// and stores (W = 2 doubles or 4 floats).  Same arithmetic as laplace_kernel:
+// in[i-1] and in[i+W] are single loads and the inner neighbours come from the
+// vector itself.  At 256x256x80 and 512x512x80 the FP64 form measured 9-11%
+// faster this way.
+template <class T>
+constexpr int kVecW = 16 / int(sizeof(T));
 
+template <class T>
 __global__ void __launch_bounds__(kTileI *kTileJ)
-    laplace_pair_kernel(const double *__restrict__ in, double *__restrict__ out, Geo g, Tiles t) {
-  const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
-  const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
+    laplace_vec_kernel(const T *__restrict__ in, T *__restrict__ out, Geo g, Tiles t) {
+  constexpr int W = kVecW<T>;
+  const TileCoord c = tile_coord(t, W * kTileI, kTileJ, g.nz);
+  const int i = c.i0 + W * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
   const int64_t at = int64_t(c.k0) * g.sk + (i + j * sj);
-  const double *p = in + at;
-  double *q = out + at;
+  const T *p = in + at;
+  T *q = out + at;
   for (int k = c.k0; k < c.k1; ++k) {
-    const d2 cc = ld2(p), n = ld2(p + sj), s = ld2(p - sj);
-    const double w = p[-1], e = p[2];
-    st2_nt(q, lap_point(cc.x, w, cc.y, n.x, s.x), lap_point(cc.y, cc.x, e, n.y, s.y));
+    const auto cc = ldv<T, W>(p), n = ldv<T, W>(p + sj), s = ldv<T, W>(p - sj);
+    const T w = p[-1], e = p[W];
+    typename Vec<T, W>::type r;
+#pragma unroll
+    for (int x = 0; x < W; ++x) {
+      r[x] = lap_point<T>(cc[x], x == 0 ? w : cc[x - 1], x == W - 1 ? e : cc[x + 1], n[x], s[x]);
+    }
+    stv_nt<T, W>(q, r);
     p += g.sk;
     q += g.sk;
   }
@@ -70,15 +80,14 @@ int laplace_run(const oeb_domain *d, const S *in, const S *out, void *stream) {
   as.add("in", p_in, r_in, g, false);
   as.add("out", p_out, r_out, g, true);
   OEB_TRY(as.check(K));
-  const int kc = kchunk_for(K, kChunk);
-  if constexpr (sizeof(T) == 8) {
-    if (pairs_for(K, true) && nx % 2 == 0 && pairs_ok(g, {p_in, p_out})) {
-      const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
-      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-      hipLaunchKernelGGL(laplace_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), p_in,
-                         const_cast<T *>(p_out), g, t);
-      return launch_status(K);
-    }
+  const int kc = kchunk_for(K, sizeof(T), d, kChunk);
+  constexpr int W = kVecW<T>;
+  if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_in, p_out})) {
+    const Tiles t = make_tiles(nx, ny, int(nz), W * kTileI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), W * kTileI, kTileJ));
+    hipLaunchKernelGGL(laplace_vec_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), p_in,
+                       const_cast<T *>(p_out), g, t);
+    return launch_status(K);
   }
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));

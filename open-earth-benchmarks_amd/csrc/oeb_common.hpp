@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <initializer_list>
 #include <string>
+#include <type_traits>
 
 #include "../../include/oeb.h"
 
@@ -104,6 +105,36 @@ inline bool pairs_ok(const Geo &g, std::initializer_list<const void *> bases) {
   return true;
 }
 
+// W consecutive elements along i, moved with one W*sizeof(T)-byte access
+// (16 bytes for W = 2 doubles or 4 floats).  The host selects the vector
+// kernels only when vec_ok() holds, so p + (i multiple of W) + j*sj + k*sk is
+// always aligned.
+template <class T, int W>
+struct Vec {
+  typedef T type __attribute__((ext_vector_type(W)));
+};
+template <class T, int W>
+__device__ __forceinline__ typename Vec<T, W>::type ldv(const T *p) {
+  return *reinterpret_cast<const typename Vec<T, W>::type *>(p);
+}
+template <class T, int W>
+__device__ __forceinline__ typename Vec<T, W>::type ldv_nt(const T *p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const typename Vec<T, W>::type *>(p));
+}
+template <class T, int W>
+__device__ __forceinline__ void stv_nt(T *p, typename Vec<T, W>::type v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<typename Vec<T, W>::type *>(p));
+}
+
+// True when the W-wide vector kernels may be used: nx, both strides and every
+// base (element (0,0,0)) aligned to W elements of `esize` bytes.
+inline bool vec_ok(const Geo &g, int w, size_t esize, std::initializer_list<const void *> bases) {
+  if (g.nx % w || g.sj % w || g.sk % w) return false;
+  for (const void *p : bases)
+    if (reinterpret_cast<uintptr_t>(p) % (size_t(w) * esize)) return false;
+  return true;
+}
+
 // Stages one level of a field into LDS: s[r][c] = f(i0 - R + c, j0 - R + r)
 // for the (TI + 2R) x (TJ + 2R) tile, read row by row so that consecutive
 // lanes read consecutive doubles.  Points with i >= ilim or j >= jlim lie
@@ -124,8 +155,8 @@ struct TileCoord {
   int i0, j0, k0, k1;
 };
 
-__device__ __forceinline__ TileCoord tile_coord(const Tiles &t, int tile_i, int tile_j, int nz) {
-  const int id = int(blockIdx.x);
+// Tile coordinates of logical block `id` of a launch (tile_i fastest).
+__device__ __forceinline__ TileCoord tile_coord_of(const Tiles &t, int id, int tile_i, int tile_j, int nz) {
   const int ci = id % t.ti;
   const int rest = id / t.ti;
   const int cj = rest % t.tj;
@@ -136,6 +167,79 @@ __device__ __forceinline__ TileCoord tile_coord(const Tiles &t, int tile_i, int 
   c.k0 = ck * t.kchunk;
   c.k1 = min(nz, c.k0 + t.kchunk);
   return c;
+}
+
+__device__ __forceinline__ TileCoord tile_coord(const Tiles &t, int tile_i, int tile_j, int nz) {
+  return tile_coord_of(t, int(blockIdx.x), tile_i, tile_j, nz);
+}
+
+// ------------------------------------------------------------------ batches
+// A batch (oeb_batch_*) records the launches the entry points would make,
+// for many members of an ensemble, and replays each run of consecutive
+// records of one kernel as ONE launch.  That launch is persistent: a grid
+// of at most kBatchGrid workgroups walks the run's flattened block list
+// (member 0's blocks, then member 1's, ...) with stride gridDim.x.  The list
+// is the tile queue; each workgroup takes every gridDim.x-th tile, so no
+// atomic counter has to be reset between replays.  Each record keeps the
+// member's kernel arguments (a trivially copyable struct), geometry and tiles.
+constexpr int kBatchArgBytes = 224;
+constexpr int kBatchGrid = 2048;
+
+struct alignas(16) BatchMember {
+  Geo g;
+  Tiles t;
+  int first;   // first flattened block of this member in its run
+  int blocks;  // blocks of this member
+  alignas(16) unsigned char args[kBatchArgBytes];
+};
+
+// Launches one run: `table` (device memory) holds n members, `total` blocks.
+typedef hipError_t (*BatchLauncher)(const BatchMember *table, int n, int total, hipStream_t stream);
+
+struct BatchKind {
+  const char *name;
+  BatchLauncher launch;
+};
+
+// When this host thread is recording a batch, appends the launch and returns
+// true; the caller then launches nothing.
+bool batch_record(const BatchKind &kind, const void *args, size_t bytes, const Geo &g, const Tiles &t, int blocks);
+
+template <class A>
+bool batch_record(const BatchKind &kind, const A &args, const Geo &g, const Tiles &t, int blocks) {
+  static_assert(sizeof(A) <= kBatchArgBytes, "batch arguments too large");
+  static_assert(std::is_trivially_copyable<A>::value, "batch arguments must be trivially copyable");
+  return batch_record(kind, &args, sizeof(A), g, t, blocks);
+}
+
+// The member whose blocks include flattened block b (members sorted by first).
+__device__ __forceinline__ int batch_find(const BatchMember *tab, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tab[mid].first <= b) {
+      lo = mid;
+    } else {
+      hi = mid - 1;
+    }
+  }
+  return lo;
+}
+
+// The persistent kernel of a run: Body(args, geo, tiles, logical block id).
+template <class A, void (*Body)(const A &, const Geo &, const Tiles &, int)>
+__global__ void __launch_bounds__(256) batch_kernel(const BatchMember *__restrict__ tab, int n, int total) {
+  for (int b = int(blockIdx.x); b < total; b += int(gridDim.x)) {
+    const int m = batch_find(tab, n, b);
+    Body(*reinterpret_cast<const A *>(tab[m].args), tab[m].g, tab[m].t, b - tab[m].first);
+  }
+}
+
+template <class A, void (*Body)(const A &, const Geo &, const Tiles &, int)>
+hipError_t launch_batch(const BatchMember *table, int n, int total, hipStream_t stream, dim3 block) {
+  const int grid = total < kBatchGrid ? total : kBatchGrid;
+  hipLaunchKernelGGL((batch_kernel<A, Body>), dim3(grid), block, 0, stream, table, n, total);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------- host checks
@@ -191,14 +295,17 @@ int launch_status(const char *kname);
 // grid dimension holds.
 int check_tiles(const char *kname, const Tiles &t, int64_t ni, int64_t nj, int nz, int tile_i, int tile_j);
 
-// Levels per workgroup for kernel `kname`: `dflt`, unless the environment
-// variable OEB_KCHUNK_<KNAME> (upper case, e.g. OEB_KCHUNK_P_GRAD_C) holds an
-// integer in [1, 4096].  This lets tuning sweeps run without a rebuild.
-int kchunk_for(const char *kname, int dflt);
+// Levels per workgroup for kernel `kname` on element size `esize` and domain
+// `d`: the environment variable OEB_KCHUNK_<KNAME> (upper case, e.g.
+// OEB_KCHUNK_P_GRAD_C) when it holds an integer in [1, 4096]; else the
+// tuning table's entry for (kname, esize, nx, ny, nz) (oeb_tune_set); else
+// `dflt`.
+int kchunk_for(const char *kname, size_t esize, const oeb_domain *d, int dflt);
 
-// Whether kernel `kname` may take its paired (16-byte) path: `dflt`, unless
-// OEB_PAIRS_<KNAME> holds 0 or 1.  The paired path still needs pairs_ok().
-bool pairs_for(const char *kname, bool dflt);
+// Whether kernel `kname` may take its vector (16-byte) path: OEB_PAIRS_<KNAME>
+// when it holds 0 or 1; else the tuning table; else `dflt`.  The vector path
+// still needs its alignment conditions (pairs_ok / vec_ok).
+bool pairs_for(const char *kname, size_t esize, const oeb_domain *d, bool dflt);
 
 #define OEB_TRY(expr)              \
   do {                             \

@@ -129,9 +129,9 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
   OEB_TRY(as.check(K));
   a.uout = const_cast<T *>(p_uo);
   a.vout = const_cast<T *>(p_vo);
-  const int kc = kchunk_for(K, kChunk);
+  const int kc = kchunk_for(K, sizeof(T), d, kChunk);
   if constexpr (sizeof(T) == 8) {
-    if (pairs_for(K, false) && nx % 2 == 0 &&
+    if (pairs_for(K, sizeof(T), d, false) && nx % 2 == 0 &&
         pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdxc, a.rdyc, a.delpc, a.gz, a.pkc})) {
       const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));

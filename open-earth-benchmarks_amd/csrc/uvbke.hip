@@ -14,44 +14,73 @@ namespace {
 constexpr int kTileI = 64, kTileJ = 4, kChunk = 1;
 
 template <class T>
-__global__ void __launch_bounds__(kTileI *kTileJ)
-    uvbke_kernel(T *__restrict__ ub, T *__restrict__ vb, const T *__restrict__ uc, const T *__restrict__ vc,
-                 const T *__restrict__ cosa, const T *__restrict__ rsina, double dt5, Geo g, Tiles t) {
-  const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
+struct UvArgs {
+  T *ub, *vb;
+  const T *uc, *vc, *cosa, *rsina;
+  double dt5;
+};
+
+// Logical block `id` of a launch: one 64 x 4 tile of columns over a k-chunk.
+template <class T>
+__device__ __forceinline__ void uvbke_body(const UvArgs<T> &a, const Geo &g, const Tiles &t, int id) {
+  const TileCoord c = tile_coord_of(t, id, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
+  const double dt5 = a.dt5;
   int64_t at = int64_t(c.k0) * g.sk + (i + j * sj);
   for (int k = c.k0; k < c.k1; ++k, at += g.sk) {
-    const T usum = uc[at - sj] + uc[at];
-    const T vsum = vc[at - 1] + vc[at];
-    const T ca = ld_nt(cosa + at), rs = ld_nt(rsina + at);
-    st_nt(ub + at, T((dt5 * (usum - (vsum * ca))) * rs));
-    st_nt(vb + at, T((dt5 * (vsum - (usum * ca))) * rs));
+    const T usum = a.uc[at - sj] + a.uc[at];
+    const T vsum = a.vc[at - 1] + a.vc[at];
+    const T ca = ld_nt(a.cosa + at), rs = ld_nt(a.rsina + at);
+    st_nt(a.ub + at, T((dt5 * (usum - (vsum * ca))) * rs));
+    st_nt(a.vb + at, T((dt5 * (vsum - (usum * ca))) * rs));
   }
 }
 
-// Two adjacent columns (i, i+1) per lane, i even: 16-byte accesses, the
-// arithmetic of uvbke_kernel.
-constexpr int kPairI = 2 * kTileI;
+template <class T>
+__global__ void __launch_bounds__(kTileI *kTileJ) uvbke_kernel(UvArgs<T> a, Geo g, Tiles t) {
+  uvbke_body(a, g, t, int(blockIdx.x));
+}
 
+template <class T>
+hipError_t uvbke_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
+  return launch_batch<UvArgs<T>, uvbke_body<T>>(tab, n, total, s, dim3(kTileI, kTileJ));
+}
+template <class T>
+const BatchKind kUvbkeBatch{"uvbke", uvbke_batch<T>};
+
+// W adjacent columns (i .. i+W-1) per lane, i a multiple of W: 16-byte
+// accesses (W = 2 doubles or 4 floats), the arithmetic of uvbke_kernel.  The
+// west neighbour vc[i-1] of the first column is a single load; the others come
+// from the vector.
+template <class T>
+constexpr int kVecW = 16 / int(sizeof(T));
+
+template <class T>
 __global__ void __launch_bounds__(kTileI *kTileJ)
-    uvbke_pair_kernel(double *__restrict__ ub, double *__restrict__ vb, const double *__restrict__ uc,
-                      const double *__restrict__ vc, const double *__restrict__ cosa,
-                      const double *__restrict__ rsina, double dt5, Geo g, Tiles t) {
-  const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
-  const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
+    uvbke_vec_kernel(T *__restrict__ ub, T *__restrict__ vb, const T *__restrict__ uc, const T *__restrict__ vc,
+                     const T *__restrict__ cosa, const T *__restrict__ rsina, double dt5, Geo g, Tiles t) {
+  constexpr int W = kVecW<T>;
+  const TileCoord c = tile_coord(t, W * kTileI, kTileJ, g.nz);
+  const int i = c.i0 + W * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
   int64_t at = int64_t(c.k0) * g.sk + (i + j * sj);
   for (int k = c.k0; k < c.k1; ++k, at += g.sk) {
-    const d2 us = ld2(uc + at - sj), u0 = ld2(uc + at), v0 = ld2(vc + at);
-    const double vw = vc[at - 1];
-    const d2 ca = ld2_nt(cosa + at), rs = ld2_nt(rsina + at);
-    const double usum0 = us.x + u0.x, usum1 = us.y + u0.y;
-    const double vsum0 = vw + v0.x, vsum1 = v0.x + v0.y;
-    st2_nt(ub + at, ((dt5 * (usum0 - (vsum0 * ca.x))) * rs.x), ((dt5 * (usum1 - (vsum1 * ca.y))) * rs.y));
-    st2_nt(vb + at, ((dt5 * (vsum0 - (usum0 * ca.x))) * rs.x), ((dt5 * (vsum1 - (usum1 * ca.y))) * rs.y));
+    const auto us = ldv<T, W>(uc + at - sj), u0 = ldv<T, W>(uc + at), v0 = ldv<T, W>(vc + at);
+    const T vw = vc[at - 1];
+    const auto ca = ldv_nt<T, W>(cosa + at), rs = ldv_nt<T, W>(rsina + at);
+    typename Vec<T, W>::type ro, rv;
+#pragma unroll
+    for (int x = 0; x < W; ++x) {
+      const T usum = us[x] + u0[x];
+      const T vsum = (x == 0 ? vw : v0[x - 1]) + v0[x];
+      ro[x] = T((dt5 * (usum - (vsum * ca[x]))) * rs[x]);
+      rv[x] = T((dt5 * (vsum - (usum * ca[x]))) * rs[x]);
+    }
+    stv_nt<T, W>(ub + at, ro);
+    stv_nt<T, W>(vb + at, rv);
   }
 }
 
@@ -79,20 +108,25 @@ int uvbke_run(const oeb_domain *d, const S *ub, const S *vb, const S *uc, const 
   as.add("cosa", p_ca, r0, g, false);
   as.add("rsina", p_rs, r0, g, false);
   OEB_TRY(as.check(K));
-  const int kc = kchunk_for(K, kChunk);
-  if constexpr (sizeof(T) == 8) {
-    if (pairs_for(K, true) && nx % 2 == 0 && pairs_ok(g, {p_ub, p_vb, p_uc, p_vc, p_ca, p_rs})) {
-      const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
-      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-      hipLaunchKernelGGL(uvbke_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
-                         const_cast<T *>(p_ub), const_cast<T *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5, g, t);
-      return launch_status(K);
-    }
+  const int kc = kchunk_for(K, sizeof(T), d, kChunk);
+  const UvArgs<T> a{const_cast<T *>(p_ub), const_cast<T *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5};
+  {
+    // Recorded into a batch: the scalar body (same results as the vector one).
+    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+    if (batch_record(kUvbkeBatch<T>, a, g, t, t.n)) return OEB_OK;
+  }
+  constexpr int W = kVecW<T>;
+  if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_ub, p_vb, p_uc, p_vc, p_ca, p_rs})) {
+    const Tiles t = make_tiles(nx, ny, int(nz), W * kTileI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), W * kTileI, kTileJ));
+    hipLaunchKernelGGL(uvbke_vec_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
+                       const_cast<T *>(p_ub), const_cast<T *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5, g, t);
+    return launch_status(K);
   }
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-  hipLaunchKernelGGL(uvbke_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
-                     const_cast<T *>(p_ub), const_cast<T *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5, g, t);
+  hipLaunchKernelGGL(uvbke_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }
 

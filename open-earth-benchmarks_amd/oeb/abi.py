@@ -108,7 +108,8 @@ def signature(name: str, precision: str = "f64") -> str:
 EXPORTS = ["oeb_version", "oeb_status_string", "oeb_last_error"] + [
     f"oeb_{k}_{p}" for p in PRECISIONS for k in SIGNATURES
 ] + ["oeb_halo_fill_f64", "oeb_halo_fill_f32", "oeb_graph_begin", "oeb_graph_end", "oeb_graph_launch",
-     "oeb_graph_destroy"]
+     "oeb_graph_destroy", "oeb_batch_create", "oeb_batch_record", "oeb_batch_stop", "oeb_batch_launch",
+     "oeb_batch_info", "oeb_batch_destroy", "oeb_tune_set", "oeb_tune_get", "oeb_tune_clear"]
 
 
 class OebLibraryError(RuntimeError):
@@ -168,6 +169,18 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.oeb_graph_launch.argtypes = [C.c_void_p, C.c_void_p]
     lib.oeb_graph_destroy.restype = C.c_int
     lib.oeb_graph_destroy.argtypes = [C.c_void_p]
+    for fn, args in (("oeb_batch_create", [C.POINTER(C.c_void_p)]), ("oeb_batch_record", [C.c_void_p]),
+                     ("oeb_batch_stop", [C.c_void_p]), ("oeb_batch_launch", [C.c_void_p, C.c_void_p]),
+                     ("oeb_batch_info", [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+                     ("oeb_batch_destroy", [C.c_void_p])):
+        getattr(lib, fn).restype = C.c_int
+        getattr(lib, fn).argtypes = args
+    lib.oeb_tune_set.restype = C.c_int
+    lib.oeb_tune_set.argtypes = [C.c_char_p, C.c_int, PD, C.c_int, C.c_int]
+    lib.oeb_tune_get.restype = C.c_int
+    lib.oeb_tune_get.argtypes = [C.c_char_p, C.c_int, PD, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.oeb_tune_clear.restype = C.c_int
+    lib.oeb_tune_clear.argtypes = []
     if path is None:
         _lib = lib
     return lib

@@ -86,18 +86,22 @@ class Chain:
             if missing:
                 raise KeyError(f"chain: fields {missing} are not defined")
 
+    def run_op(self, op, stream=None) -> None:
+        """Enqueues one op of the program on `stream`."""
+        if isinstance(op, Halo):
+            for name in op.fields:
+                halo_fill(self.fields[name], self.domain, op.h, op.hk, op.mode, stream)
+        else:
+            spec = KERNELS[op.stencil]
+            arrays = [self.fields[op.bind[a.name]] if a.name in op.bind else None for a in spec.args]
+            sc = dict(spec.scalars)
+            sc.update(op.scalars)
+            stencils.call(op.stencil, arrays, [sc[n] for n, _ in spec.scalars], self.domain, stream)
+
     def step(self, stream=None) -> None:
         """Enqueues one step on `stream` (default: torch's current stream)."""
         for op in self.program:
-            if isinstance(op, Halo):
-                for name in op.fields:
-                    halo_fill(self.fields[name], self.domain, op.h, op.hk, op.mode, stream)
-            else:
-                spec = KERNELS[op.stencil]
-                arrays = [self.fields[op.bind[a.name]] if a.name in op.bind else None for a in spec.args]
-                sc = dict(spec.scalars)
-                sc.update(op.scalars)
-                stencils.call(op.stencil, arrays, [sc[n] for n, _ in spec.scalars], self.domain, stream)
+            self.run_op(op, stream)
 
     def capture(self, stream, steps: int = 1) -> Graph:
         """Records `steps` steps enqueued on `stream` (a created stream, not
@@ -133,6 +137,11 @@ FORWARD_WINDS = {
     "uvbke": ({"uc": "u3", "vc": "v3"}, {"ub": "u", "vb": "v"}),
 }
 FORWARD_REACH = {"hadvuv5th": 3, "hdiffsmag": 1, "fastwavesuv": 0, "uvbke": 1}
+# Scalars of the chain that differ from the drivers' values.  With the drivers'
+# dt = 10 (fastwavesuv) and dt5 = 112.5 (uvbke) each step multiplies the winds
+# by ~1e4 and hadvuv5th squares them, so the state overflows within a few
+# steps.  With these the winds settle at O(10) m/s.
+FORWARD_SCALARS = {"fastwavesuv": {"dt": 1e-3}, "uvbke": {"dt5": 9.0}}
 
 
 def forward_program() -> list:
@@ -143,7 +152,7 @@ def forward_program() -> list:
         bind = {a.name: f"{st}.{a.name}" for a in KERNELS[st].args if a.kind in ("in", "in1")}
         bind.update(ins)
         bind.update(outs)
-        prog.append(Call(st, bind))
+        prog.append(Call(st, bind, dict(FORWARD_SCALARS.get(st, {}))))
     return prog
 
 

@@ -78,3 +78,17 @@ def test_f32_256_against_f64(name):
         bmax, brms, bfrac = F32_BOUNDS.get((name, o), DEFAULT_BOUND)
         assert err <= bmax and rms <= brms and frac <= bfrac, (
             f"{name}.{o}: max {err:.2e} (bound {bmax:.0e}), rms {rms:.2e} ({brms:.0e}), frac {frac:.1e} ({bfrac:.0e})")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nx", [64, 66])
+@pytest.mark.parametrize("pairs", [0, 1])
+@pytest.mark.parametrize("name", ["laplace", "uvbke"])
+def test_f32_vector_path_switch(name, pairs, nx, monkeypatch):
+    """The 4-wide (16-byte) FP32 path, forced on or off with OEB_PAIRS_<K>,
+    and nx = 66, where it does not apply, agree bit for bit with the FP32
+    oracle."""
+    monkeypatch.setenv(f"OEB_PAIRS_{name.upper()}", str(pairs))
+    L, host = inputs(name, nx, 36, 7, "baseline", seed=9)
+    print(f"[gpu] f32 {name} pairs={pairs} nx={nx}", flush=True)
+    _check_exact(name, L, cast(name, host, "f32"))
