@@ -31,6 +31,12 @@ members (oeb_batch), eager and replayed from a HIP graph, first with the
 built-in k-chunks, then with the autotuner's choice per (stencil, member
 shape).  Gpts/s counts 4 stencils x all member points per step.
 
+adjoint: per stencil (hdiffsa, hdiffsmag, hadvuv5th, fvtp2d, fastwavesuv), the
+time of the hand-written adjoint against its forward at the same size; and the
+chain hdiffsa -> hadvuv5th -> fvtp2d -> fastwavesuv (oeb.adjoint_chain), 100
+steps checkpointed every 10, FP64 and FP32: forward time, forward + backward
+time (the backward recomputes each segment), their ratio, and peak memory.
+
 cpu_baseline: the reference's own kernels (OpenMP variants, serial DEFAULT for
 fvtp2d, which has no OpenMP variant), compiled by oracle/Makefile, timed on
 this host over a bounded sample of the same workload.
@@ -72,6 +78,8 @@ def parse_args(argv=None):
     p.add_argument("--no-chain", action="store_true", help="skip the chain block (eager vs HIP-graph steps)")
     p.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble block")
     p.add_argument("--members", type=int, default=48, help="members of the ensemble block")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint block")
+    p.add_argument("--adjoint-steps", type=int, default=100, help="steps of the checkpointed adjoint chain")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="also time a plain device copy of this many MiB (HBM counter calibration)")
     return p.parse_args(argv)
@@ -308,6 +316,81 @@ def ensemble_block(n_members: int, seed: int, steps: int = 10, reps: int = 3):
     }
 
 
+def adjoint_block(L, forms, seed, steps: int, segment: int = 10, reps: int = 5):
+    import torch
+
+    from oeb import autograd as AG
+    from oeb.adjoint_chain import AdjointChain
+    from oeb.generators import cast, generate
+    from oeb.layout import Layout1D
+    from oeb.spec import KERNELS
+    from oeb.storage import Storage1D, Storage3D
+
+    L1 = Layout1D(L.ny, L.h)
+    dom = (L.nx, L.ny, L.nz)
+    diff = {"hdiffsa": ("in",), "hdiffsmag": ("uin", "vin"), "hadvuv5th": ("uin", "vin"), "fvtp2d": ("q",),
+            "fastwavesuv": ("uin", "vin", "ppuv")}
+    fns = {"hdiffsa": AG.Hdiffsa, "hdiffsmag": AG.Hdiffsmag, "hadvuv5th": AG.Hadvuv5th, "fvtp2d": AG.Fvtp2d,
+           "fastwavesuv": AG.Fastwavesuv}
+    out = {"per_stencil": {}, "chain": {}}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return statistics.median(ts)
+
+    for prec in forms:
+        per = {}
+        for name, dnames in diff.items():
+            host = cast(name, generate(name, L, "baseline", seed), prec)
+            consts = {a.name: (Storage1D.to_device(L1, host[a.name]) if a.kind == "in1"
+                               else Storage3D.to_device(L, host[a.name]))
+                      for a in KERNELS[name].args if a.kind in ("in", "in1") and a.name not in dnames}
+            xs = [torch.from_numpy(host[n]).to("cuda").requires_grad_(True) for n in dnames]
+            ys = fns[name].apply(*xs, consts, L, dom)
+            ys = ys if isinstance(ys, tuple) else (ys,)
+            gs = [torch.ones_like(y) for y in ys]
+            with torch.no_grad():
+                f_ms = timed(lambda: fns[name].apply(*xs, consts, L, dom))
+            b_ms = timed(lambda: torch.autograd.grad(ys, xs, gs, retain_graph=True))
+            per[name] = {"forward_us": round(f_ms * 1e3, 2), "adjoint_us": round(b_ms * 1e3, 2),
+                         "adjoint_over_forward": round(b_ms / f_ms, 3)}
+            del xs, ys, gs, consts
+        out["per_stencil"][prec] = per
+
+        chain = AdjointChain(L, prec, seed)
+        with torch.no_grad():
+            fwd_ms = timed(lambda: chain.run(steps))
+
+        def fwd_bwd():
+            x = [t.clone().requires_grad_(True) for t in chain.state0]
+            q, u, v = chain.run(steps, x, segment=segment)
+            loss = (q * chain.interior_mask).sum() + (u * chain.interior_mask).sum() + (v * chain.interior_mask).sum()
+            return torch.autograd.grad(loss, x)
+
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        fb_ms = timed(fwd_bwd)
+        peak = torch.cuda.max_memory_allocated() - base
+        with torch.no_grad():
+            q, u, v = chain.run(steps)
+        out["chain"][prec] = {"steps": steps, "checkpoint_every": segment, "forward_ms": round(fwd_ms, 2),
+                              "forward_backward_ms": round(fb_ms, 2),
+                              "gradient_over_forward": round(fb_ms / fwd_ms, 3),
+                              "peak_extra_memory_gb": round(peak / 1e9, 2),
+                              "state_finite": bool(torch.isfinite(q).all() and torch.isfinite(u).all())}
+        del chain
+    return out
+
+
 def main(argv=None):
     args = parse_args(argv)
     if args.gpus != 1:
@@ -450,6 +533,8 @@ def main(argv=None):
         result["chain"] = chain_block(L, forms, args.seed)
     if not args.no_ensemble and args.members > 0:
         result["ensemble"] = ensemble_block(args.members, args.seed)
+    if not args.no_adjoint and args.adjoint_steps > 0:
+        result["adjoint"] = adjoint_block(L, forms, args.seed, args.adjoint_steps)
     if copy_line:
         result["device_copy"] = copy_line
     if not args.no_cpu_baseline:

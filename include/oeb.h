@@ -311,6 +311,85 @@ int oeb_graph_end(void *stream, oeb_graph **graph);
 int oeb_graph_launch(const oeb_graph *graph, void *stream);
 int oeb_graph_destroy(oeb_graph *graph);
 
+/* --------------------------------------------------------------- adjoints
+ * No counterpart in the reference.  oeb_<name>_adj_<f64|f32> computes the
+ * vector-Jacobian product of a stencil: given the forward inputs and the
+ * cotangents of its outputs (*_bar), it writes the cotangents of the
+ * differentiated inputs on the whole box the forward reads them on (halo
+ * included; other points are left untouched).  Coefficient fields are
+ * constants.  Limiter, upwind, clamp and PPM branches are frozen at the
+ * forward pass's choice.  Scratch arrays (same layout as the inputs) carry
+ * intermediate cotangents between the kernels of one call.
+ *   hdiffsa      d/d in on [-2, nx+2) x [-2, ny+2); scratch lap on
+ *                [-1, nx+1) x [-1, ny+1)
+ *   fastwavesuv  d/d uin, vin on the interior; d/d ppuv on
+ *                [0, nx+1) x [0, ny+1) x [-1, nz+1) (FP64 in both forms)
+ *   hadvuv5th    d/d uin, vin on [-3, nx+3) x [-3, ny+3); scratch: the eight
+ *                reference temporaries, on the interior
+ *   hdiffsmag    d/d uin, vin on [-1, nx+1) x [-1, ny+1); scratch work0..3 on
+ *                the interior
+ *   fvtp2d       d/d q on [-3, nx+3) x [-3, ny+3), from the cotangents of all
+ *                six outputs; q_i and q_j are the forward's outputs; scratch
+ *                work0..3 cover the regions of q_j, q_i, fx2 and fy2 */
+int oeb_hdiffsa_adj_f64(const oeb_domain *d, const oeb_s3 *in, const oeb_s3 *mask, const oeb_s1 *crlato,
+                        const oeb_s1 *crlatu, const oeb_s3 *out_bar, const oeb_s3 *in_bar, const oeb_s3 *lap,
+                        void *stream);
+int oeb_hdiffsa_adj_f32(const oeb_domain *d, const oeb_s3f *in, const oeb_s3f *mask, const oeb_s1f *crlato,
+                        const oeb_s1f *crlatu, const oeb_s3f *out_bar, const oeb_s3f *in_bar, const oeb_s3f *lap,
+                        void *stream);
+int oeb_fastwavesuv_adj_f64(const oeb_domain *d, const oeb_s3 *wgtfac, const oeb_s3 *ppuv, const oeb_s3 *hhl,
+                            const oeb_s3 *rho, const oeb_s1 *fx, const oeb_s3 *uout_bar, const oeb_s3 *vout_bar,
+                            const oeb_s3 *uin_bar, const oeb_s3 *vin_bar, const oeb_s3 *ppuv_bar, double edadlat,
+                            double dt, void *stream);
+int oeb_fastwavesuv_adj_f32(const oeb_domain *d, const oeb_s3f *wgtfac, const oeb_s3 *ppuv, const oeb_s3 *hhl,
+                            const oeb_s3f *rho, const oeb_s1f *fx, const oeb_s3f *uout_bar, const oeb_s3f *vout_bar,
+                            const oeb_s3f *uin_bar, const oeb_s3f *vin_bar, const oeb_s3 *ppuv_bar, float edadlat,
+                            float dt, void *stream);
+int oeb_hadvuv5th_adj_f64(const oeb_domain *d, const oeb_s3 *uin, const oeb_s3 *vin, const oeb_s1 *acrlat0,
+                          const oeb_s1 *acrlat1, const oeb_s1 *tgrlatda0, const oeb_s1 *tgrlatda1,
+                          const oeb_s3 *uout_bar, const oeb_s3 *vout_bar, const oeb_s3 *uin_bar,
+                          const oeb_s3 *vin_bar, const oeb_s3 *uatupos, const oeb_s3 *vatupos, const oeb_s3 *uatvpos,
+                          const oeb_s3 *vatvpos, const oeb_s3 *uavg, const oeb_s3 *vavg, const oeb_s3 *ures,
+                          const oeb_s3 *vres, double eddlat, double eddlon, void *stream);
+int oeb_hadvuv5th_adj_f32(const oeb_domain *d, const oeb_s3f *uin, const oeb_s3f *vin, const oeb_s1f *acrlat0,
+                          const oeb_s1f *acrlat1, const oeb_s1f *tgrlatda0, const oeb_s1f *tgrlatda1,
+                          const oeb_s3f *uout_bar, const oeb_s3f *vout_bar, const oeb_s3f *uin_bar,
+                          const oeb_s3f *vin_bar, const oeb_s3f *uatupos, const oeb_s3f *vatupos,
+                          const oeb_s3f *uatvpos, const oeb_s3f *vatvpos, const oeb_s3f *uavg, const oeb_s3f *vavg,
+                          const oeb_s3f *ures, const oeb_s3f *vres, float eddlat, float eddlon, void *stream);
+
+int oeb_hdiffsmag_adj_f64(const oeb_domain *d, const oeb_s3 *uin, const oeb_s3 *vin, const oeb_s3 *mask,
+                          const oeb_s1 *crlavo, const oeb_s1 *crlavu, const oeb_s1 *crlato, const oeb_s1 *crlatu,
+                          const oeb_s1 *acrlat0, const oeb_s3 *uout_bar, const oeb_s3 *vout_bar,
+                          const oeb_s3 *uin_bar, const oeb_s3 *vin_bar, const oeb_s3 *work0, const oeb_s3 *work1,
+                          const oeb_s3 *work2, const oeb_s3 *work3, double eddlat, double eddlon, double tau_smag,
+                          double weight_smag, void *stream);
+int oeb_hdiffsmag_adj_f32(const oeb_domain *d, const oeb_s3f *uin, const oeb_s3f *vin, const oeb_s3f *mask,
+                          const oeb_s1f *crlavo, const oeb_s1f *crlavu, const oeb_s1f *crlato, const oeb_s1f *crlatu,
+                          const oeb_s1f *acrlat0, const oeb_s3f *uout_bar, const oeb_s3f *vout_bar,
+                          const oeb_s3f *uin_bar, const oeb_s3f *vin_bar, const oeb_s3f *work0,
+                          const oeb_s3f *work1, const oeb_s3f *work2, const oeb_s3f *work3, float eddlat,
+                          float eddlon, float tau_smag, float weight_smag, void *stream);
+#pragma push_macro("p1")
+#pragma push_macro("p2")
+#undef p1
+#undef p2
+int oeb_fvtp2d_adj_f64(const oeb_domain *d, const oeb_s3 *q, const oeb_s3 *crx, const oeb_s3 *cry,
+                       const oeb_s3 *ra_x, const oeb_s3 *ra_y, const oeb_s3 *xfx, const oeb_s3 *yfx,
+                       const oeb_s3 *area, const oeb_s3 *q_i, const oeb_s3 *q_j, const oeb_s3 *q_i_bar,
+                       const oeb_s3 *q_j_bar, const oeb_s3 *fx1_bar, const oeb_s3 *fx2_bar, const oeb_s3 *fy1_bar,
+                       const oeb_s3 *fy2_bar, const oeb_s3 *q_bar, const oeb_s3 *work0, const oeb_s3 *work1,
+                       const oeb_s3 *work2, const oeb_s3 *work3, double p1, double p2, void *stream);
+int oeb_fvtp2d_adj_f32(const oeb_domain *d, const oeb_s3f *q, const oeb_s3f *crx, const oeb_s3f *cry,
+                       const oeb_s3f *ra_x, const oeb_s3f *ra_y, const oeb_s3f *xfx, const oeb_s3f *yfx,
+                       const oeb_s3f *area, const oeb_s3f *q_i, const oeb_s3f *q_j, const oeb_s3f *q_i_bar,
+                       const oeb_s3f *q_j_bar, const oeb_s3f *fx1_bar, const oeb_s3f *fx2_bar,
+                       const oeb_s3f *fy1_bar, const oeb_s3f *fy2_bar, const oeb_s3f *q_bar, const oeb_s3f *work0,
+                       const oeb_s3f *work1, const oeb_s3f *work2, const oeb_s3f *work3, float p1, float p2,
+                       void *stream);
+#pragma pop_macro("p2")
+#pragma pop_macro("p1")
+
 /* ---------------------------------------------------------------- tuning
  * No counterpart in the reference.  Each entry point picks the levels per
  * workgroup (k-chunk) and, where it has one, its 16-byte vector path from,

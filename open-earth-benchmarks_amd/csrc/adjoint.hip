@@ -500,6 +500,421 @@ int hadvuv5th_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
   return launch_status(K);
 }
 
+// ================================================================= hdiffsmag
+// Forward (kernels/hdiffsmag.h:205-263), differentiated in uin and vin:
+//   T(p)  = (v(j-1) - v) eddlat R - (u(i-1) - u) ac(j) eddlon
+//   S(p)  = (u(j+1) - u) eddlat R + (v(i+1) - v) ac(j) eddlon
+//   Qu    = 0.5 (T(i+1)^2 + T^2) + 0.5 (S(j-1)^2 + S^2)
+//   smag_u = clamp(tau sqrt(Qu) - w mask, 0, 0.5),  uout = u + smag_u lapu(u)
+//   Qv    = 0.5 (T(j+1)^2 + T^2) + 0.5 (S(i-1)^2 + S^2)
+//   smag_v = clamp(tau sqrt(Qv) - w mask, 0, 0.5),  vout = v + smag_v lapv(v)
+// Stage 1 (interior): gu = ubar lapu tau / (2 sqrt(Qu)) (0 when clamped), the
+// cotangent of Qu; gv alike; wu = smag_u ubar and wv = smag_v vbar, the
+// weights of the transposed Laplacians.  Stage 2 (on [-1, nx+1) x
+// [-1, ny+1)): T_bar(x) = T(x) (gu(x) + gu(x-i) + gv(x) + gv(x-j)),
+// S_bar(y) = S(y) (gu(y) + gu(y+j) + gv(y) + gv(y+i)), and the gathers of
+// T_bar, S_bar and the Laplacian weights onto u and v, plus ubar, vbar.
+template <class T>
+struct HsmAdj {
+  const T *uin, *vin, *mask, *vo, *vu, *lo, *lu, *ac, *ubar, *vbar;
+  int svo, svu, slo, slu, sac;
+  T eddlat, eddlon, tau, weight;
+  T *gu, *gv, *wu, *wv;  // scratch, interior
+  T *uinbar, *vinbar;
+};
+
+template <class T>
+__global__ void __launch_bounds__(kTI *kTJ) hdiffsmag_adj_stage1(HsmAdj<T> a, Geo g, BoxGrid b) {
+  int i, j;
+  if (!box_point(b, i, j)) return;
+  constexpr T rr = kEarthRadiusRecipAdj<T>;
+  const int sj = g.sj;
+  const int64_t at = int64_t(blockIdx.y) * g.sk + i + int64_t(j) * sj;
+  const T *u = a.uin + at;
+  const T *v = a.vin + at;
+  const T eddlat = a.eddlat, eddlon = a.eddlon;
+  const T ac_s = a.ac[int64_t(j - 1) * a.sac], ac_c = a.ac[int64_t(j) * a.sac], ac_n = a.ac[int64_t(j + 1) * a.sac];
+  // The forward's expressions (kernels/hdiffsmag.h:213-259), as hdiffsmag.hip forms them.
+  const T t0 = (v[-sj] - v[0]) * eddlat * rr - (u[-1] - u[0]) * ac_c * eddlon;
+  const T t_ip = (v[1 - sj] - v[1]) * eddlat * rr - (u[0] - u[1]) * ac_c * eddlon;
+  const T t_jp = (v[0] - v[sj]) * eddlat * rr - (u[sj - 1] - u[sj]) * ac_n * eddlon;
+  const T s0 = (u[sj] - u[0]) * eddlat * rr + (v[1] - v[0]) * ac_c * eddlon;
+  const T s_im = (u[sj - 1] - u[-1]) * eddlat * rr + (v[0] - v[-1]) * ac_c * eddlon;
+  const T s_jm = (u[0] - u[-sj]) * eddlat * rr + (v[1 - sj] - v[-sj]) * ac_s * eddlon;
+  const T tt0 = t0 * t0, tt_ip = t_ip * t_ip, tt_jp = t_jp * t_jp;
+  const T ss0 = s0 * s0, ss_im = s_im * s_im, ss_jm = s_jm * s_jm;
+  const T hdweight = a.weight * a.mask[at];
+  const T qu = T(0.5) * (tt_ip + tt0) + T(0.5) * (ss_jm + ss0);
+  const T qv = T(0.5) * (tt_jp + tt0) + T(0.5) * (ss_im + ss0);
+  const T su = a.tau * sqrt(qu) - hdweight, sv = a.tau * sqrt(qv) - hdweight;
+  // min(0.5, max(0, x)) with std::max(a, b) = a < b ? b : a and
+  // std::min(a, b) = b < a ? b : a: the result is x itself when 0 < x < 0.5.
+  const bool pass_u = (T(0) < su) && (su < T(0.5)), pass_v = (T(0) < sv) && (sv < T(0.5));
+  const T smag_u = pass_u ? su : ((T(0) < su) ? T(0.5) : T(0));
+  const T smag_v = pass_v ? sv : ((T(0) < sv) ? T(0.5) : T(0));
+  const T lapu = u[1] + u[-1] - T(2.0) * u[0] + a.lo[int64_t(j) * a.slo] * (u[sj] - u[0]) +
+                 a.lu[int64_t(j) * a.slu] * (u[-sj] - u[0]);
+  const T lapv = v[1] + v[-1] - T(2.0) * v[0] + a.vo[int64_t(j) * a.svo] * (v[sj] - v[0]) +
+                 a.vu[int64_t(j) * a.svu] * (v[-sj] - v[0]);
+  const T ub = a.ubar[at], vb = a.vbar[at];
+  a.gu[at] = pass_u ? ub * lapu * a.tau * T(0.5) / sqrt(qu) : T(0);
+  a.gv[at] = pass_v ? vb * lapv * a.tau * T(0.5) / sqrt(qv) : T(0);
+  a.wu[at] = smag_u * ub;
+  a.wv[at] = smag_v * vb;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kTI *kTJ) hdiffsmag_adj_stage2(HsmAdj<T> a, Geo g, BoxGrid b) {
+  int i, j;
+  if (!box_point(b, i, j)) return;
+  constexpr T rr = kEarthRadiusRecipAdj<T>;
+  const int sj = g.sj, nx = g.nx, ny = g.ny;
+  const int64_t at = int64_t(blockIdx.y) * g.sk + i + int64_t(j) * sj;
+  const T eddlat = a.eddlat, eddlon = a.eddlon;
+  auto inside = [&](int di, int dj) { return in_range(i + di, 0, nx) && in_range(j + dj, 0, ny); };
+  auto get = [&](const T *f, int di, int dj) { return inside(di, dj) ? f[at + di + int64_t(dj) * sj] : T(0); };
+  auto acr = [&](int jj) { return a.ac[int64_t(jj) * a.sac]; };
+  // T(x), S(x) at x = q + (di, dj).
+  auto Tx = [&](int di, int dj) {
+    const int64_t x = at + di + int64_t(dj) * sj;
+    return (a.vin[x - sj] - a.vin[x]) * eddlat * rr - (a.uin[x - 1] - a.uin[x]) * acr(j + dj) * eddlon;
+  };
+  auto Sx = [&](int di, int dj) {
+    const int64_t x = at + di + int64_t(dj) * sj;
+    return (a.uin[x + sj] - a.uin[x]) * eddlat * rr + (a.vin[x + 1] - a.vin[x]) * acr(j + dj) * eddlon;
+  };
+  // Cotangent weights of T(x) and S(x); zero where no output reads them.
+  auto tw = [&](int di, int dj) {
+    return get(a.gu, di, dj) + get(a.gu, di - 1, dj) + get(a.gv, di, dj) + get(a.gv, di, dj - 1);
+  };
+  auto sw = [&](int di, int dj) {
+    return get(a.gu, di, dj) + get(a.gu, di, dj + 1) + get(a.gv, di, dj) + get(a.gv, di + 1, dj);
+  };
+  auto Tbar = [&](int di, int dj) {
+    const T w = tw(di, dj);
+    return w != T(0) ? Tx(di, dj) * w : T(0);
+  };
+  auto Sbar = [&](int di, int dj) {
+    const T w = sw(di, dj);
+    return w != T(0) ? Sx(di, dj) * w : T(0);
+  };
+  T ru = inside(0, 0) ? a.ubar[at] : T(0);
+  T rv = inside(0, 0) ? a.vbar[at] : T(0);
+  // d(T_sqr)/d: T(q) reads u(q) (+ac(j) eddlon), v(q) (-eddlat R); T(q+i)
+  // reads u(q) (-ac(j) eddlon); T(q+j) reads v(q) (+eddlat R).
+  // (d Q / d T(x) = T(x): the 0.5 of Q cancels the square's 2.)
+  const T tq = Tbar(0, 0);
+  ru += tq * acr(j) * eddlon - Tbar(1, 0) * acr(j) * eddlon;
+  rv += Tbar(0, 1) * eddlat * rr - tq * eddlat * rr;
+  // S(q) reads u(q) (-eddlat R), v(q) (-ac(j) eddlon); S(q-j) reads u(q)
+  // (+eddlat R); S(q-i) reads v(q) (+ac(j) eddlon).
+  const T sq = Sbar(0, 0);
+  ru += Sbar(0, -1) * eddlat * rr - sq * eddlat * rr;
+  rv += Sbar(-1, 0) * acr(j) * eddlon - sq * acr(j) * eddlon;
+  // Transposed Laplacians: lapu(p) = u(p-1) + u(p+1) - 2u + lo(j)(u(j+1) - u) + lu(j)(u(j-1) - u).
+  if (inside(0, 0)) {
+    ru += get(a.wu, 0, 0) * (T(-2.0) - a.lo[int64_t(j) * a.slo] - a.lu[int64_t(j) * a.slu]);
+    rv += get(a.wv, 0, 0) * (T(-2.0) - a.vo[int64_t(j) * a.svo] - a.vu[int64_t(j) * a.svu]);
+  }
+  ru += get(a.wu, -1, 0) + get(a.wu, 1, 0);
+  rv += get(a.wv, -1, 0) + get(a.wv, 1, 0);
+  if (inside(0, -1)) {
+    ru += get(a.wu, 0, -1) * a.lo[int64_t(j - 1) * a.slo];
+    rv += get(a.wv, 0, -1) * a.vo[int64_t(j - 1) * a.svo];
+  }
+  if (inside(0, 1)) {
+    ru += get(a.wu, 0, 1) * a.lu[int64_t(j + 1) * a.slu];
+    rv += get(a.wv, 0, 1) * a.vu[int64_t(j + 1) * a.svu];
+  }
+  a.uinbar[at] = ru;
+  a.vinbar[at] = rv;
+}
+
+template <class T, class S3, class S1>
+int hdiffsmag_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S3 *mask, const S1 *crlavo,
+                      const S1 *crlavu, const S1 *crlato, const S1 *crlatu, const S1 *acrlat0, const S3 *uout_bar,
+                      const S3 *vout_bar, const S3 *uin_bar, const S3 *vin_bar, const S3 *const work[4], T eddlat,
+                      T eddlon, T tau_smag, T weight_smag, void *stream) {
+  const char *K = "hdiffsmag_adj";
+  OEB_TRY(check_domain(K, d));
+  const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
+  Geo g{int(nx), int(ny), int(nz), 0, 0};
+  const Box r0{0, nx, 0, ny, 0, nz}, r_uv{-1, nx + 1, -1, ny + 1, 0, nz};
+  HsmAdj<T> a;
+  a.eddlat = eddlat, a.eddlon = eddlon, a.tau = tau_smag, a.weight = weight_smag;
+  const T *p_ub, *p_vb, *p_w[4];
+  OEB_TRY(check3(K, "uin", uin, r_uv, g, true, &a.uin));
+  OEB_TRY(check3(K, "vin", vin, r_uv, g, false, &a.vin));
+  OEB_TRY(check3(K, "mask", mask, r0, g, false, &a.mask));
+  OEB_TRY(check3(K, "uout_bar", uout_bar, r0, g, false, &a.ubar));
+  OEB_TRY(check3(K, "vout_bar", vout_bar, r0, g, false, &a.vbar));
+  OEB_TRY(check3(K, "uin_bar", uin_bar, r_uv, g, false, &p_ub));
+  OEB_TRY(check3(K, "vin_bar", vin_bar, r_uv, g, false, &p_vb));
+  static const char *const kWork[4] = {"work0", "work1", "work2", "work3"};
+  for (int q = 0; q < 4; ++q) OEB_TRY(check3(K, kWork[q], work[q], r0, g, false, &p_w[q]));
+  OEB_TRY(check1(K, "crlavo", crlavo, 0, ny, &a.vo, &a.svo));
+  OEB_TRY(check1(K, "crlavu", crlavu, 0, ny, &a.vu, &a.svu));
+  OEB_TRY(check1(K, "crlato", crlato, 0, ny, &a.lo, &a.slo));
+  OEB_TRY(check1(K, "crlatu", crlatu, 0, ny, &a.lu, &a.slu));
+  OEB_TRY(check1(K, "acrlat0", acrlat0, -1, ny + 1, &a.ac, &a.sac));
+  AliasSet as;
+  as.add("uin", a.uin, r_uv, g, false);
+  as.add("vin", a.vin, r_uv, g, false);
+  as.add("mask", a.mask, r0, g, false);
+  as.add("uout_bar", a.ubar, r0, g, false);
+  as.add("vout_bar", a.vbar, r0, g, false);
+  as.add("uin_bar", p_ub, r_uv, g, true);
+  as.add("vin_bar", p_vb, r_uv, g, true);
+  for (int q = 0; q < 4; ++q) as.add(kWork[q], p_w[q], r0, g, true);
+  OEB_TRY(as.check(K));
+  a.gu = const_cast<T *>(p_w[0]), a.gv = const_cast<T *>(p_w[1]);
+  a.wu = const_cast<T *>(p_w[2]), a.wv = const_cast<T *>(p_w[3]);
+  a.uinbar = const_cast<T *>(p_ub);
+  a.vinbar = const_cast<T *>(p_vb);
+  const BoxGrid b1{0, 0, int(nx), int(ny)}, b2{-1, -1, int(nx + 2), int(ny + 2)};
+  hipLaunchKernelGGL(hdiffsmag_adj_stage1<T>, box_blocks(b1, int(nz)), dim3(kTI, kTJ), 0, hipStream_t(stream), a, g,
+                     b1);
+  OEB_TRY(launch_status(K));
+  hipLaunchKernelGGL(hdiffsmag_adj_stage2<T>, box_blocks(b2, int(nz)), dim3(kTI, kTJ), 0, hipStream_t(stream), a, g,
+                     b2);
+  return launch_status(K);
+}
+
+// ==================================================================== fvtp2d
+// Forward (kernels/fvtp2d.h:4-177), differentiated in q:
+//   fy2 = PPM_j(q, cry)                                 i [-3, nx+2) j [0, ny+1)
+//   q_i = (q area + yfx fy2 - yfx(j+1) fy2(j+1)) / ra_y   i [-3, nx+2) j [0, ny)
+//   fx1' = PPM_i(q_i, crx), fx1 = 0.5 (fx1' + fx2) xfx    [0, nx) x [0, ny)
+//   fx2 = PPM_i(q, crx)                                 i [0, nx+1)  j [-3, ny+2)
+//   q_j = (q area + xfx fx2 - xfx(i+1) fx2(i+1)) / ra_x   i [0, nx)    j [-3, ny+2)
+//   fy1' = PPM_j(q_j, cry), fy1 = 0.5 (fy1' + fy2) yfx    [0, nx) x [0, ny)
+// With S (the monotonicity switch) and the upwind side frozen, a PPM flux is
+// linear in its field: flux(m) = sum_{k=-3..2} w_k(m) f(m+k) (ppm_weights).
+// The adjoint runs the stages backwards, each a gather:
+//   K1  W0 = q_j_bar + PPM_j^T(0.5 yfx fy1_bar; q_j)        on q_j's region
+//   K2  W1 = q_i_bar + PPM_i^T(0.5 xfx fx1_bar; q_i)        on q_i's region
+//   K3  W2 = fx2_bar + 0.5 xfx fx1_bar + xfx (W0/ra_x - W0(i-1)/ra_x(i-1))   fx2's region
+//   K4  W3 = fy2_bar + 0.5 yfx fy1_bar + yfx (W1/ra_y - W1(j-1)/ra_y(j-1))   fy2's region
+//   K5  q_bar = area (W0/ra_x + W1/ra_y) + PPM_i^T(W2; q) + PPM_j^T(W3; q)   on q's box
+// q_i and q_j are the forward's outputs.  The FP32 forward forms q_i / q_j
+// with a compensated sum; the derivative is that of the same expression.
+
+// Weights w[k+3] of f(m+k), k = -3..2, in the flux at face m, from
+// f(m-3) .. f(m+2) and the Courant number c (kernels/fvtp2d.h:12, 20-34).
+template <class T>
+__device__ __forceinline__ void ppm_weights(const T f[6], T c, T p1, T p2, T w[6]) {
+  const T fm3 = f[0], fm2 = f[1], fm1 = f[2], f0 = f[3], fp1 = f[4], fp2 = f[5];
+  const T al_m = ((p1 * (fm2 + fm1)) + (p2 * (fm3 + f0)));
+  const T al_0 = ((p1 * (fm1 + f0)) + (p2 * (fm2 + fp1)));
+  const T al_p = ((p1 * (f0 + fp1)) + (p2 * (fm1 + fp2)));
+  const T almq_m = (al_m - fm1), br_m = (al_0 - fm1);
+  const T almq_0 = (al_0 - f0), br_0 = (al_p - f0);
+  const T smt_m = ((almq_m * br_m) < T(0)) ? T(1) : T(0);
+  const T smt_0 = ((almq_0 * br_0) < T(0)) ? T(1) : T(0);
+  const T S = (smt_m + (smt_0 * (smt_m == T(0) ? T(1) : T(0))));
+  if (c > T(0)) {
+    // flux = D f(m-1) + A al(m) + B al(m-1)
+    const T A = S * (T(1) - c) * (T(1) - c), B = -S * (T(1) - c) * c;
+    const T D = T(1) - S * (T(1) - c) * (T(1) - T(2) * c);
+    w[0] = B * p2;
+    w[1] = A * p2 + B * p1;
+    w[2] = D + A * p1 + B * p1;
+    w[3] = A * p1 + B * p2;
+    w[4] = A * p2;
+    w[5] = T(0);
+  } else {
+    // flux = D f(m) + A al(m) + B al(m+1)
+    const T A = S * (T(1) + c) * (T(1) + c), B = S * (T(1) + c) * c;
+    const T D = T(1) - S * (T(1) + c) * (T(1) + T(2) * c);
+    w[0] = T(0);
+    w[1] = A * p2;
+    w[2] = A * p1 + B * p2;
+    w[3] = D + A * p1 + B * p1;
+    w[4] = A * p2 + B * p1;
+    w[5] = B * p2;
+  }
+}
+
+template <class T>
+struct FvAdj {
+  const T *q, *crx, *cry, *ra_x, *ra_y, *xfx, *yfx, *area, *q_i, *q_j;
+  const T *qib, *qjb, *fx1b, *fx2b, *fy1b, *fy2b;
+  T p1, p2;
+  T *w0, *w1, *w2, *w3, *qbar;
+};
+
+// Regions (half-open, domain-relative) of the forward's outputs.
+struct FvRegions {
+  int nx, ny;
+  __device__ bool qi(int i, int j) const { return in_range(i, -3, nx + 2) && in_range(j, 0, ny); }
+  __device__ bool qj(int i, int j) const { return in_range(i, 0, nx) && in_range(j, -3, ny + 2); }
+  __device__ bool fx2(int i, int j) const { return in_range(i, 0, nx + 1) && in_range(j, -3, ny + 2); }
+  __device__ bool fy2(int i, int j) const { return in_range(i, -3, nx + 2) && in_range(j, 0, ny + 1); }
+  __device__ bool in(int i, int j) const { return in_range(i, 0, nx) && in_range(j, 0, ny); }
+};
+
+// sum over faces m = q-2 .. q+3 along stride s (m inside `face`) of
+// w_{q-m}(m) gbar(m), the weights taken from field f and Courant numbers c.
+// `g(m)` returns the flux cotangent at offset m (from q, along the axis).
+template <class T, class Face, class Gbar>
+__device__ __forceinline__ T ppm_transpose(const T *f, const T *c, int64_t at, int s, T p1, T p2, Face face,
+                                          Gbar gbar) {
+  T r = T(0);
+#pragma unroll
+  for (int o = -2; o <= 3; ++o) {  // face m = q + o; q is offset k = -o from m
+    if (!face(o)) continue;
+    const T gb = gbar(o);
+    if (gb == T(0)) continue;
+    const int64_t m = at + int64_t(o) * s;
+    T fv[6], w[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) fv[t] = f[m + int64_t(t - 3) * s];
+    ppm_weights(fv, c[m], p1, p2, w);
+    r += w[3 - o] * gb;
+  }
+  return r;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k1(FvAdj<T> a, Geo g, BoxGrid b) {
+  int i, j;
+  if (!box_point(b, i, j)) return;
+  const FvRegions R{g.nx, g.ny};
+  const int sj = g.sj;
+  const int64_t at = int64_t(blockIdx.y) * g.sk + i + int64_t(j) * sj;
+  const T r = ppm_transpose(
+      a.q_j, a.cry, at, sj, a.p1, a.p2, [&](int o) { return R.in(i, j + o); },
+      [&](int o) { return T(0.5) * a.yfx[at + int64_t(o) * sj] * a.fy1b[at + int64_t(o) * sj]; });
+  a.w0[at] = a.qjb[at] + r;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k2(FvAdj<T> a, Geo g, BoxGrid b) {
+  int i, j;
+  if (!box_point(b, i, j)) return;
+  const FvRegions R{g.nx, g.ny};
+  const int64_t at = int64_t(blockIdx.y) * g.sk + i + int64_t(j) * g.sj;
+  const T r = ppm_transpose(
+      a.q_i, a.crx, at, 1, a.p1, a.p2, [&](int o) { return R.in(i + o, j); },
+      [&](int o) { return T(0.5) * a.xfx[at + o] * a.fx1b[at + o]; });
+  a.w1[at] = a.qib[at] + r;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k3(FvAdj<T> a, Geo g, BoxGrid b) {
+  int i, j;
+  if (!box_point(b, i, j)) return;
+  const FvRegions R{g.nx, g.ny};
+  const int64_t at = int64_t(blockIdx.y) * g.sk + i + int64_t(j) * g.sj;
+  T r = a.fx2b[at];
+  if (R.in(i, j)) r += T(0.5) * a.xfx[at] * a.fx1b[at];
+  T d = T(0);
+  if (R.qj(i, j)) d += a.w0[at] / a.ra_x[at];
+  if (R.qj(i - 1, j)) d -= a.w0[at - 1] / a.ra_x[at - 1];
+  a.w2[at] = r + a.xfx[at] * d;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k4(FvAdj<T> a, Geo g, BoxGrid b) {
+  int i, j;
+  if (!box_point(b, i, j)) return;
+  const FvRegions R{g.nx, g.ny};
+  const int sj = g.sj;
+  const int64_t at = int64_t(blockIdx.y) * g.sk + i + int64_t(j) * sj;
+  T r = a.fy2b[at];
+  if (R.in(i, j)) r += T(0.5) * a.yfx[at] * a.fy1b[at];
+  T d = T(0);
+  if (R.qi(i, j)) d += a.w1[at] / a.ra_y[at];
+  if (R.qi(i, j - 1)) d -= a.w1[at - sj] / a.ra_y[at - sj];
+  a.w3[at] = r + a.yfx[at] * d;
+}
+
+template <class T>
+__global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k5(FvAdj<T> a, Geo g, BoxGrid b) {
+  int i, j;
+  if (!box_point(b, i, j)) return;
+  const FvRegions R{g.nx, g.ny};
+  const int sj = g.sj;
+  const int64_t at = int64_t(blockIdx.y) * g.sk + i + int64_t(j) * sj;
+  T r = T(0);
+  if (R.qj(i, j)) r += a.area[at] * (a.w0[at] / a.ra_x[at]);
+  if (R.qi(i, j)) r += a.area[at] * (a.w1[at] / a.ra_y[at]);
+  r += ppm_transpose(
+      a.q, a.crx, at, 1, a.p1, a.p2, [&](int o) { return R.fx2(i + o, j); }, [&](int o) { return a.w2[at + o]; });
+  r += ppm_transpose(
+      a.q, a.cry, at, sj, a.p1, a.p2, [&](int o) { return R.fy2(i, j + o); },
+      [&](int o) { return a.w3[at + int64_t(o) * sj]; });
+  a.qbar[at] = r;
+}
+
+template <class T, class S>
+int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, const S *ra_x, const S *ra_y,
+                   const S *xfx, const S *yfx, const S *area, const S *q_i, const S *q_j, const S *q_i_bar,
+                   const S *q_j_bar, const S *fx1_bar, const S *fx2_bar, const S *fy1_bar, const S *fy2_bar,
+                   const S *q_bar, const S *const work[4], T p1, T p2, void *stream) {
+  const char *K = "fvtp2d_adj";
+  OEB_TRY(check_domain(K, d));
+  const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
+  Geo g{int(nx), int(ny), int(nz), 0, 0};
+  const Box r0{0, nx, 0, ny, 0, nz};
+  const Box r_fy2{-3, nx + 2, 0, ny + 1, 0, nz}, r_fx2{0, nx + 1, -3, ny + 2, 0, nz};
+  const Box r_qi{-3, nx + 2, 0, ny, 0, nz}, r_qj{0, nx, -3, ny + 2, 0, nz};
+  const Box r_q{-3, nx + 3, -3, ny + 3, 0, nz}, r_area{-3, nx + 2, -3, ny + 2, 0, nz};
+  FvAdj<T> a;
+  a.p1 = p1, a.p2 = p2;
+  const T *p_qb, *p_w[4];
+  OEB_TRY(check3(K, "q", q, r_q, g, true, &a.q));
+  OEB_TRY(check3(K, "crx", crx, r_fx2, g, false, &a.crx));
+  OEB_TRY(check3(K, "cry", cry, r_fy2, g, false, &a.cry));
+  OEB_TRY(check3(K, "ra_x", ra_x, r_qj, g, false, &a.ra_x));
+  OEB_TRY(check3(K, "ra_y", ra_y, r_qi, g, false, &a.ra_y));
+  OEB_TRY(check3(K, "xfx", xfx, r_fx2, g, false, &a.xfx));
+  OEB_TRY(check3(K, "yfx", yfx, r_fy2, g, false, &a.yfx));
+  OEB_TRY(check3(K, "area", area, r_area, g, false, &a.area));
+  OEB_TRY(check3(K, "q_i", q_i, r_qi, g, false, &a.q_i));
+  OEB_TRY(check3(K, "q_j", q_j, r_qj, g, false, &a.q_j));
+  OEB_TRY(check3(K, "q_i_bar", q_i_bar, r_qi, g, false, &a.qib));
+  OEB_TRY(check3(K, "q_j_bar", q_j_bar, r_qj, g, false, &a.qjb));
+  OEB_TRY(check3(K, "fx1_bar", fx1_bar, r0, g, false, &a.fx1b));
+  OEB_TRY(check3(K, "fx2_bar", fx2_bar, r_fx2, g, false, &a.fx2b));
+  OEB_TRY(check3(K, "fy1_bar", fy1_bar, r0, g, false, &a.fy1b));
+  OEB_TRY(check3(K, "fy2_bar", fy2_bar, r_fy2, g, false, &a.fy2b));
+  OEB_TRY(check3(K, "q_bar", q_bar, r_q, g, false, &p_qb));
+  const Box wr[4] = {r_qj, r_qi, r_fx2, r_fy2};
+  static const char *const kWork[4] = {"work0", "work1", "work2", "work3"};
+  for (int t = 0; t < 4; ++t) OEB_TRY(check3(K, kWork[t], work[t], wr[t], g, false, &p_w[t]));
+  AliasSet as;
+  as.add("q_bar", p_qb, r_q, g, true);
+  for (int t = 0; t < 4; ++t) as.add(kWork[t], p_w[t], wr[t], g, true);
+  as.add("q", a.q, r_q, g, false);
+  as.add("q_i", a.q_i, r_qi, g, false);
+  as.add("q_j", a.q_j, r_qj, g, false);
+  as.add("q_i_bar", a.qib, r_qi, g, false);
+  as.add("q_j_bar", a.qjb, r_qj, g, false);
+  as.add("fx1_bar", a.fx1b, r0, g, false);
+  as.add("fx2_bar", a.fx2b, r_fx2, g, false);
+  as.add("fy1_bar", a.fy1b, r0, g, false);
+  as.add("fy2_bar", a.fy2b, r_fy2, g, false);
+  OEB_TRY(as.check(K));
+  a.w0 = const_cast<T *>(p_w[0]), a.w1 = const_cast<T *>(p_w[1]);
+  a.w2 = const_cast<T *>(p_w[2]), a.w3 = const_cast<T *>(p_w[3]);
+  a.qbar = const_cast<T *>(p_qb);
+  const BoxGrid bqj{0, -3, int(nx), int(ny + 5)}, bqi{-3, 0, int(nx + 5), int(ny)};
+  const BoxGrid bfx2{0, -3, int(nx + 1), int(ny + 5)}, bfy2{-3, 0, int(nx + 5), int(ny + 1)};
+  const BoxGrid bq{-3, -3, int(nx + 6), int(ny + 6)};
+  hipStream_t st = hipStream_t(stream);
+  hipLaunchKernelGGL(fvtp2d_adj_k1<T>, box_blocks(bqj, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqj);
+  OEB_TRY(launch_status(K));
+  hipLaunchKernelGGL(fvtp2d_adj_k2<T>, box_blocks(bqi, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqi);
+  OEB_TRY(launch_status(K));
+  hipLaunchKernelGGL(fvtp2d_adj_k3<T>, box_blocks(bfx2, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bfx2);
+  OEB_TRY(launch_status(K));
+  hipLaunchKernelGGL(fvtp2d_adj_k4<T>, box_blocks(bfy2, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bfy2);
+  OEB_TRY(launch_status(K));
+  hipLaunchKernelGGL(fvtp2d_adj_k5<T>, box_blocks(bq, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bq);
+  return launch_status(K);
+}
+
 }  // namespace
 }  // namespace oeb
 
@@ -555,4 +970,52 @@ extern "C" int oeb_hadvuv5th_adj_f32(const oeb_domain *d, const oeb_s3f *uin, co
   const oeb_s3f *const sc[8] = {uatupos, vatupos, uatvpos, vatvpos, uavg, vavg, ures, vres};
   return oeb::hadvuv5th_adj_run<float>(d, uin, vin, acrlat0, acrlat1, tgrlatda0, tgrlatda1, uout_bar, vout_bar,
                                        uin_bar, vin_bar, sc, eddlat, eddlon, stream);
+}
+
+extern "C" int oeb_hdiffsmag_adj_f64(const oeb_domain *d, const oeb_s3 *uin, const oeb_s3 *vin, const oeb_s3 *mask,
+                                     const oeb_s1 *crlavo, const oeb_s1 *crlavu, const oeb_s1 *crlato,
+                                     const oeb_s1 *crlatu, const oeb_s1 *acrlat0, const oeb_s3 *uout_bar,
+                                     const oeb_s3 *vout_bar, const oeb_s3 *uin_bar, const oeb_s3 *vin_bar,
+                                     const oeb_s3 *work0, const oeb_s3 *work1, const oeb_s3 *work2,
+                                     const oeb_s3 *work3, double eddlat, double eddlon, double tau_smag,
+                                     double weight_smag, void *stream) {
+  const oeb_s3 *const w[4] = {work0, work1, work2, work3};
+  return oeb::hdiffsmag_adj_run<double>(d, uin, vin, mask, crlavo, crlavu, crlato, crlatu, acrlat0, uout_bar,
+                                        vout_bar, uin_bar, vin_bar, w, eddlat, eddlon, tau_smag, weight_smag, stream);
+}
+
+extern "C" int oeb_hdiffsmag_adj_f32(const oeb_domain *d, const oeb_s3f *uin, const oeb_s3f *vin,
+                                     const oeb_s3f *mask, const oeb_s1f *crlavo, const oeb_s1f *crlavu,
+                                     const oeb_s1f *crlato, const oeb_s1f *crlatu, const oeb_s1f *acrlat0,
+                                     const oeb_s3f *uout_bar, const oeb_s3f *vout_bar, const oeb_s3f *uin_bar,
+                                     const oeb_s3f *vin_bar, const oeb_s3f *work0, const oeb_s3f *work1,
+                                     const oeb_s3f *work2, const oeb_s3f *work3, float eddlat, float eddlon,
+                                     float tau_smag, float weight_smag, void *stream) {
+  const oeb_s3f *const w[4] = {work0, work1, work2, work3};
+  return oeb::hdiffsmag_adj_run<float>(d, uin, vin, mask, crlavo, crlavu, crlato, crlatu, acrlat0, uout_bar,
+                                       vout_bar, uin_bar, vin_bar, w, eddlat, eddlon, tau_smag, weight_smag, stream);
+}
+
+extern "C" int oeb_fvtp2d_adj_f64(const oeb_domain *d, const oeb_s3 *q, const oeb_s3 *crx, const oeb_s3 *cry,
+                                  const oeb_s3 *ra_x, const oeb_s3 *ra_y, const oeb_s3 *xfx, const oeb_s3 *yfx,
+                                  const oeb_s3 *area, const oeb_s3 *q_i, const oeb_s3 *q_j, const oeb_s3 *q_i_bar,
+                                  const oeb_s3 *q_j_bar, const oeb_s3 *fx1_bar, const oeb_s3 *fx2_bar,
+                                  const oeb_s3 *fy1_bar, const oeb_s3 *fy2_bar, const oeb_s3 *q_bar,
+                                  const oeb_s3 *work0, const oeb_s3 *work1, const oeb_s3 *work2, const oeb_s3 *work3,
+                                  double p1, double p2, void *stream) {
+  const oeb_s3 *const w[4] = {work0, work1, work2, work3};
+  return oeb::fvtp2d_adj_run<double>(d, q, crx, cry, ra_x, ra_y, xfx, yfx, area, q_i, q_j, q_i_bar, q_j_bar,
+                                     fx1_bar, fx2_bar, fy1_bar, fy2_bar, q_bar, w, p1, p2, stream);
+}
+
+extern "C" int oeb_fvtp2d_adj_f32(const oeb_domain *d, const oeb_s3f *q, const oeb_s3f *crx, const oeb_s3f *cry,
+                                  const oeb_s3f *ra_x, const oeb_s3f *ra_y, const oeb_s3f *xfx, const oeb_s3f *yfx,
+                                  const oeb_s3f *area, const oeb_s3f *q_i, const oeb_s3f *q_j,
+                                  const oeb_s3f *q_i_bar, const oeb_s3f *q_j_bar, const oeb_s3f *fx1_bar,
+                                  const oeb_s3f *fx2_bar, const oeb_s3f *fy1_bar, const oeb_s3f *fy2_bar,
+                                  const oeb_s3f *q_bar, const oeb_s3f *work0, const oeb_s3f *work1,
+                                  const oeb_s3f *work2, const oeb_s3f *work3, float p1, float p2, void *stream) {
+  const oeb_s3f *const w[4] = {work0, work1, work2, work3};
+  return oeb::fvtp2d_adj_run<float>(d, q, crx, cry, ra_x, ra_y, xfx, yfx, area, q_i, q_j, q_i_bar, q_j_bar, fx1_bar,
+                                    fx2_bar, fy1_bar, fy2_bar, q_bar, w, p1, p2, stream);
 }

@@ -100,6 +100,23 @@ assert all(len(SIGNATURES[k]) == len(SIGNATURES_F32[k]) for k in SIGNATURES)
 
 PRECISIONS = ("f64", "f32")
 
+# Adjoint entry points oeb_<name>_adj_<precision>: argument names and kinds
+# after the domain ('3' / '1' arrays of the element type, 'D' FP64 arrays in
+# both forms, 's' a scalar of the element type).
+ADJOINTS = {
+    "hdiffsa": ("in mask crlato crlatu out_bar in_bar lap", "33113 33"),
+    "fastwavesuv": ("wgtfac ppuv hhl rho fx uout_bar vout_bar uin_bar vin_bar ppuv_bar edadlat dt",
+                    "3DD31 3333D ss"),
+    "hadvuv5th": ("uin vin acrlat0 acrlat1 tgrlatda0 tgrlatda1 uout_bar vout_bar uin_bar vin_bar uatupos vatupos "
+                  "uatvpos vatvpos uavg vavg ures vres eddlat eddlon", "33 1111 3333 33333333 ss"),
+    "hdiffsmag": ("uin vin mask crlavo crlavu crlato crlatu acrlat0 uout_bar vout_bar uin_bar vin_bar work0 work1 "
+                  "work2 work3 eddlat eddlon tau_smag weight_smag", "333 11111 3333 3333 ssss"),
+    "fvtp2d": ("q crx cry ra_x ra_y xfx yfx area q_i q_j q_i_bar q_j_bar fx1_bar fx2_bar fy1_bar fy2_bar q_bar "
+               "work0 work1 work2 work3 p1 p2", "33333333 33 333333 3 3333 ss"),
+}
+ADJOINTS = {k: (n.split(), kinds.replace(" ", "")) for k, (n, kinds) in ADJOINTS.items()}
+assert all(len(n) == len(kd) for n, kd in ADJOINTS.values())
+
 
 def signature(name: str, precision: str = "f64") -> str:
     return {"f64": SIGNATURES, "f32": SIGNATURES_F32}[precision][name]
@@ -109,7 +126,9 @@ EXPORTS = ["oeb_version", "oeb_status_string", "oeb_last_error"] + [
     f"oeb_{k}_{p}" for p in PRECISIONS for k in SIGNATURES
 ] + ["oeb_halo_fill_f64", "oeb_halo_fill_f32", "oeb_graph_begin", "oeb_graph_end", "oeb_graph_launch",
      "oeb_graph_destroy", "oeb_batch_create", "oeb_batch_record", "oeb_batch_stop", "oeb_batch_launch",
-     "oeb_batch_info", "oeb_batch_destroy", "oeb_tune_set", "oeb_tune_get", "oeb_tune_clear"]
+     "oeb_batch_info", "oeb_batch_destroy", "oeb_tune_set", "oeb_tune_get", "oeb_tune_clear"] + [
+    f"oeb_{k}_adj_{p}" for p in PRECISIONS for k in ADJOINTS
+]
 
 
 class OebLibraryError(RuntimeError):
@@ -181,6 +200,12 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.oeb_tune_get.argtypes = [C.c_char_p, C.c_int, PD, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.oeb_tune_clear.restype = C.c_int
     lib.oeb_tune_clear.argtypes = []
+    for prec in PRECISIONS:
+        for name, (_, kinds) in ADJOINTS.items():
+            fn = getattr(lib, f"oeb_{name}_adj_{prec}")
+            fn.restype = C.c_int
+            sc = C.c_double if prec == "f64" else C.c_float
+            fn.argtypes = [PD] + [sc if ch == "s" else (P1 if ch == "1" else P3) for ch in kinds] + [C.c_void_p]
     if path is None:
         _lib = lib
     return lib

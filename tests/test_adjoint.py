@@ -1,0 +1,219 @@
+"""Adjoints: the HIP adjoints behind oeb.autograd against torch.autograd
+through the ATen compositions of tests/aten_ref.py (FP64, 1e-12 relative to
+the largest gradient), and against FP64 central differences of the HIP
+forward (eps 1e-6) at 1000 seeded points, to 1e-7 relative to the largest
+gradient.
+
+Central differences: the points of one pass are spaced more than twice the
+stencil's reach apart, so each output is moved by at most one of them; the
+difference quotient of point q is sum_p ybar(p) (y+(p) - y-(p)) / (2 eps) over
+the outputs p within reach of q.  A perturbation can flip a frozen branch
+(limiter, upwind direction); such points are identified by the
+second difference and reported, and at most 1% may be excluded.
+"""
+import numpy as np
+import pytest
+
+import aten_ref as A
+from common import inputs
+from oeb import autograd as AG
+from oeb.generators import generate
+from oeb.layout import Layout, Layout1D
+from oeb.spec import KERNELS
+from oeb.storage import Storage1D, Storage3D
+
+pytestmark = pytest.mark.gpu
+
+SHAPE = (40, 36, 5)
+FD_POINTS = 1000
+FD_EPS = 1e-6
+FD_TOL = 1e-7
+ATEN_TOL = 1e-12
+
+
+def _dev(name, L, host, skip=()):
+    import torch
+
+    L1 = Layout1D(L.ny, L.h)
+    out = {}
+    for a in KERNELS[name].args:
+        if a.kind in ("in", "in1") and a.name not in skip:
+            out[a.name] = (Storage1D if a.kind == "in1" else Storage3D).to_device(L1 if a.kind == "in1" else L,
+                                                                                host[a.name])
+    return out
+
+
+def _t(a):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def _relmax(x, y):
+    import torch
+
+    return float((x - y).abs().max() / y.abs().max().clamp_min(1e-300))
+
+
+# ------------------------------------------------------------ the problems
+# Each returns (forward(*diff) -> tuple of outputs, ATen forward, diff inputs,
+# read boxes of the diff inputs as (i0, i1, j0, j1, k0, k1), reach).
+def _hdiffsa(seed):
+    L = Layout(*SHAPE, h=3, hk=1)
+    host = generate("hdiffsa", L, "baseline", seed)
+    c = _dev("hdiffsa", L, host, skip=("in",))
+    dom = (L.nx, L.ny, L.nz)
+    fwd = lambda x: (AG.Hdiffsa.apply(x, c, L, dom),)  # noqa: E731
+    ref = lambda x: (A.hdiffsa(x, _t(host["mask"]), _t(host["crlato"]), _t(host["crlatu"]), L),)  # noqa: E731
+    return L, fwd, ref, [_t(host["in"])], [(-2, L.nx + 2, -2, L.ny + 2, 0, L.nz)], 2
+
+
+def _hadvuv5th(seed):
+    L = Layout(*SHAPE, h=3, hk=1)
+    host = generate("hadvuv5th", L, "baseline", seed)
+    c = _dev("hadvuv5th", L, host, skip=("uin", "vin"))
+    dom = (L.nx, L.ny, L.nz)
+    sc = dict(KERNELS["hadvuv5th"].scalars)
+    fwd = lambda u, v: AG.Hadvuv5th.apply(u, v, c, L, dom)  # noqa: E731
+    ref = lambda u, v: A.hadvuv5th(u, v, _t(host["acrlat0"]), _t(host["acrlat1"]), _t(host["tgrlatda0"]),  # noqa
+                                   _t(host["tgrlatda1"]), L, sc["eddlat"], sc["eddlon"])
+    box = (-3, L.nx + 3, -3, L.ny + 3, 0, L.nz)
+    return L, fwd, ref, [_t(host["uin"]), _t(host["vin"])], [box, box], 3
+
+
+def _fastwavesuv(seed):
+    L = Layout(*SHAPE, h=3, hk=1)
+    host = generate("fastwavesuv", L, "baseline", seed)
+    c = _dev("fastwavesuv", L, host, skip=("uin", "vin", "ppuv"))
+    dom = (L.nx, L.ny, L.nz)
+    sc = dict(KERNELS["fastwavesuv"].scalars)
+    fwd = lambda u, v, p: AG.Fastwavesuv.apply(u, v, p, c, L, dom)  # noqa: E731
+    ref = lambda u, v, p: A.fastwavesuv(u, v, p, _t(host["utens"]), _t(host["vtens"]), _t(host["wgtfac"]),  # noqa
+                                        _t(host["hhl"]), _t(host["rho"]), _t(host["fx"]), L, sc["edadlat"],
+                                        sc["dt"])
+    inner = (0, L.nx, 0, L.ny, 0, L.nz)
+    return (L, fwd, ref, [_t(host["uin"]), _t(host["vin"]), _t(host["ppuv"])],
+            [inner, inner, (0, L.nx + 1, 0, L.ny + 1, -1, L.nz + 1)], 1)
+
+
+def _hdiffsmag(seed):
+    L = Layout(*SHAPE, h=3, hk=1)
+    host = generate("hdiffsmag", L, "baseline", seed)
+    c = _dev("hdiffsmag", L, host, skip=("uin", "vin"))
+    dom = (L.nx, L.ny, L.nz)
+    sc = dict(KERNELS["hdiffsmag"].scalars)
+    fwd = lambda u, v: AG.Hdiffsmag.apply(u, v, c, L, dom)  # noqa: E731
+    ref = lambda u, v: A.hdiffsmag(u, v, _t(host["mask"]), _t(host["crlavo"]), _t(host["crlavu"]),  # noqa: E731
+                                   _t(host["crlato"]), _t(host["crlatu"]), _t(host["acrlat0"]), L, sc["eddlat"],
+                                   sc["eddlon"], sc["tau_smag"], sc["weight_smag"])
+    box = (-1, L.nx + 1, -1, L.ny + 1, 0, L.nz)
+    return L, fwd, ref, [_t(host["uin"]), _t(host["vin"])], [box, box], 2
+
+
+def _fvtp2d(seed):
+    L = Layout(*SHAPE, h=3, hk=1)
+    host = generate("fvtp2d", L, "baseline", seed)
+    c = _dev("fvtp2d", L, host, skip=("q",))
+    dom = (L.nx, L.ny, L.nz)
+    sc = dict(KERNELS["fvtp2d"].scalars)
+    names = ("crx", "cry", "ra_x", "ra_y", "xfx", "yfx", "area")
+    fwd = lambda q: AG.Fvtp2d.apply(q, c, L, dom)  # noqa: E731
+    ref = lambda q: A.fvtp2d(q, *[_t(host[n]) for n in names], L, sc["p1"], sc["p2"])  # noqa: E731
+    return L, fwd, ref, [_t(host["q"])], [(-3, L.nx + 3, -3, L.ny + 3, 0, L.nz)], 6
+
+
+PROBLEMS = {"hdiffsa": _hdiffsa, "hadvuv5th": _hadvuv5th, "fastwavesuv": _fastwavesuv, "hdiffsmag": _hdiffsmag,
+            "fvtp2d": _fvtp2d}
+
+
+def _grads(fwd, xs, ybars):
+    import torch
+
+    xs = [x.clone().requires_grad_(True) for x in xs]
+    ys = fwd(*xs)
+    loss = sum((y * yb).sum() for y, yb in zip(ys, ybars))
+    return torch.autograd.grad(loss, xs)
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_adjoint_matches_aten_autograd(name):
+    import torch
+
+    L, fwd, ref, xs, boxes, _ = PROBLEMS[name](seed=31)
+    print(f"[gpu] adjoint vs autograd {name}", flush=True)
+    ys = fwd(*xs)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    ybars = [torch.randn(y.shape, generator=g, device="cuda", dtype=y.dtype) for y in ys]
+    ours = _grads(fwd, xs, ybars)
+    theirs = _grads(ref, xs, ybars)
+    # The forward outputs agree first (both FP64, same expressions).
+    for y, yr in zip(ys, ref(*xs)):
+        assert _relmax(y, yr) <= 1e-13
+    for a, b in zip(ours, theirs):
+        err = _relmax(a, b)
+        assert err <= ATEN_TOL, f"{name}: adjoint vs autograd rel max {err:.2e}"
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_adjoint_central_differences(name):
+    import torch
+
+    L, fwd, _, xs, boxes, reach = PROBLEMS[name](seed=32)
+    print(f"[gpu] adjoint vs central differences {name}", flush=True)
+    with torch.no_grad():
+        ys = fwd(*xs)
+    g = torch.Generator(device="cuda").manual_seed(6)
+    ybars = [torch.randn(y.shape, generator=g, device="cuda", dtype=y.dtype) for y in ys]
+    grads = _grads(fwd, xs, ybars)
+    gmax = max(float(gr.abs().max()) for gr in grads)
+    rng = np.random.default_rng(7)
+    spacing = 2 * reach + 1
+    checked, bad, flips = 0, [], 0
+    per_field = FD_POINTS // len(xs) + 1
+    for f, (x, box) in enumerate(zip(xs, boxes)):
+        i0, i1, j0, j1, k0, k1 = box
+        done = 0
+        while done < per_field:
+            oi, oj = rng.integers(0, spacing, size=2)
+            ii = np.arange(i0 + oi, i1, spacing)
+            jj = np.arange(j0 + oj, j1, spacing)
+            kk = rng.integers(k0, k1)
+            I, J = np.meshgrid(ii, jj)
+            I, J = I.ravel(), J.ravel()
+            sel = rng.permutation(len(I))[: per_field - done]
+            I, J = I[sel], J[sel]
+            idx = torch.from_numpy(L.index(I, J, kk).astype(np.int64)).to("cuda")
+            eps = FD_EPS * torch.clamp(x[idx].abs(), min=1.0)
+
+            def run(sign):
+                xp = [xx.clone() for xx in xs]
+                xp[f][idx] += sign * eps
+                with torch.no_grad():
+                    return fwd(*xp)
+
+            yp, ym, y0 = run(1.0), run(-1.0), ys
+            for n, q in enumerate(idx.tolist()):
+                fd = 0.0
+                curv = 0.0
+                for y_p, y_m, y_0, yb in zip(yp, ym, y0, ybars):
+                    v = lambda a: a.view(L.levels, L.rows, L.sj)  # noqa: E731
+                    k_, j_, i_ = np.unravel_index(q, (L.levels, L.rows, L.sj))
+                    sl = (slice(max(k_ - 1, 0), k_ + 2), slice(max(j_ - reach, 0), j_ + reach + 1),
+                          slice(max(i_ - reach, 0), i_ + reach + 1))
+                    fd += float((v(yb)[sl] * (v(y_p)[sl] - v(y_m)[sl])).sum())
+                    curv += float((v(yb)[sl] * (v(y_p)[sl] + v(y_m)[sl] - 2.0 * v(y_0)[sl])).abs().sum())
+                fd /= 2.0 * float(eps[n])
+                adj = float(grads[f][q])
+                checked += 1
+                if abs(fd - adj) > FD_TOL * gmax:
+                    # A frozen branch flipped under the perturbation: the
+                    # outputs jump, and the second difference shows it.
+                    if curv > 1e-3 * abs(fd) * float(eps[n]):
+                        flips += 1
+                    else:
+                        bad.append((f, q, fd, adj))
+            done += len(idx)
+    print(f"  {name}: {checked} points, {flips} branch flips excluded, {len(bad)} failures", flush=True)
+    assert checked >= FD_POINTS
+    assert flips <= 0.01 * checked
+    assert not bad, f"{name}: {len(bad)} points off by more than {FD_TOL:.0e} x max|grad|: {bad[:3]}"
