@@ -20,7 +20,9 @@ from .spec import KERNELS
 from .storage import Storage3D
 
 
-def _adjoint(name: str, fields: dict, scalars: dict, domain, precision: str) -> None:
+def _adjoint(name: str, fields: dict, scalars: dict, domain, precision: str, stream=None) -> None:
+    """Calls oeb_<name>_adj_<precision> with arrays looked up by argument name
+    (abi.ADJOINTS) on `stream` (default: torch's current stream)."""
     names, kinds = abi.ADJOINTS[name]
     lib = abi.load()
     nx, ny, nz = domain
@@ -31,7 +33,16 @@ def _adjoint(name: str, fields: dict, scalars: dict, domain, precision: str) -> 
             args.append((C.c_double if precision == "f64" else C.c_float)(float(scalars[n])))
         else:
             args.append(fields[n].ref())
-    abi.check(getattr(lib, f"oeb_{name}_adj_{precision}")(C.byref(dom), *args, stencils._stream(None)))
+    abi.check(getattr(lib, f"oeb_{name}_adj_{precision}")(C.byref(dom), *args, stencils._stream(stream)))
+
+
+def adjoint(name: str, fields: dict, scalars: dict | None = None, domain=None, precision: str = "f64",
+            stream=None) -> None:
+    """The adjoint entry point of stencil `name` on caller-provided buffers:
+    the forward inputs, the output cotangents *_bar, the input cotangents
+    (overwritten on the box the forward reads) and the scratch arrays, all
+    looked up by argument name."""
+    _adjoint(name, fields, _scalars(name, scalars), domain, precision, stream)
 
 
 def _scalars(name: str, scalars: dict | None) -> dict:
