@@ -145,6 +145,12 @@ int check_tiles(const char *kname, const Tiles &t, int64_t ni, int64_t nj, int n
   return OEB_OK;
 }
 
+int check_strips(const char *kname, const Strips &s) {
+  if (s.n < 1 || s.lanes > 1024 || int64_t(s.ns) * s.nj > (int64_t(1) << 30))
+    return fail(OEB_ERR_DOMAIN, std::string(kname) + ": domain too large");
+  return OEB_OK;
+}
+
 namespace {
 // Value of the environment variable <prefix><KNAME> (kernel name in upper
 // case) when it holds an integer in [lo, hi], else `dflt`.
@@ -208,6 +214,12 @@ bool pairs_for(const char *kname, size_t esize, const oeb_domain *d, bool dflt) 
   if (tune_lookup(kname, esize, d, &v) && v.vec >= 0) return v.vec != 0;
   return dflt;
 }
+
+int strip_rows_for(const char *kname, int dflt) { return env_tunable("OEB_ROWS_", kname, dflt, 1, 4096); }
+
+int strip_depth_for(const char *kname, int dflt) { return env_tunable("OEB_DEPTH_", kname, dflt, 1, 8); }
+
+bool strip_for(const char *kname, bool dflt) { return env_tunable("OEB_STRIP_", kname, dflt ? 1 : 0, 0, 1) != 0; }
 
 int launch_status(const char *kname) {
   const hipError_t e = hipGetLastError();

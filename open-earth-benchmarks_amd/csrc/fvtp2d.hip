@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn<T> a, FvOut<T> o, T p1
       const int gi = i0 - 3 + c, gj = j0 - 3 + r;
       sq[r][c] = (gi < nx + 3 && gj < ny + 3) ? q[gi + gj * sj] : T(0);
     }
-    __syncthreads();
+    lds_barrier();
 
     // B: fy2 on i [-3, nx+2) x j [0, ny+1) and fx2 on i [0, nx+1) x j [-3, ny+2).
     for (int e = tid; e < kYH * kYW; e += kNT) {
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn<T> a, FvOut<T> o, T p1
         if (own_i(gi) && own_j(gj)) st_nt(o.fx2 + plane + at, v);
       }
     }
-    __syncthreads();
+    lds_barrier();
 
     // C: q_i on i [-3, nx+2) x j [0, ny), q_j on i [0, nx) x j [-3, ny+2).
     for (int e = tid; e < kIH * kIW; e += kNT) {
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn<T> a, FvOut<T> o, T p1
         if (own_i(gi) && own_j(gj)) st_nt(o.q_j + plane + at, v);
       }
     }
-    __syncthreads();
+    lds_barrier();
 
     // D: fx1, fy1 on the tile's interior points.
     for (int e = tid; e < kTJ * kTI; e += kNT) {

@@ -133,7 +133,7 @@ __device__ __forceinline__ void hadvuv_body(const HadvArgs<T> &a, const Geo &g, 
 
   load(int64_t(c.k0) * g.sk);
   store(0);
-  __syncthreads();
+  lds_barrier();
   int cur = 0;
   for (int k = c.k0; k < c.k1; ++k) {
     const int64_t plane = int64_t(k) * g.sk;
@@ -162,7 +162,7 @@ __device__ __forceinline__ void hadvuv_body(const HadvArgs<T> &a, const Geo &g, 
     // Buffer cur^1 was last read before the previous barrier, so it can be
     // overwritten now; the barrier below publishes it for the next level.
     if (more) store(cur ^ 1);
-    __syncthreads();
+    lds_barrier();
     cur ^= 1;
   }
 }

@@ -56,6 +56,103 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
   }
 }
 
+// Strip form (Strips, h = 2): lane l owns column i = i0 - 2 + l.  Iteration t
+// handles row r = j0 - 1 + t: it publishes in(r) to LDS, forms lap(r) on
+// [i0 - 1, i0 + w + 1) from in(r) and its row neighbours (registers) and its
+// i neighbours (LDS), publishes lap(r), and from t = 2 on writes out(r - 1),
+// whose lap(r - 2), lap(r - 1), lap(r) and in(r - 2 .. r) are all at hand.
+// LDS holds 3 rows of in and 2 of lap; each row slot is rewritten only after
+// the barrier that follows its last read, so one barrier per row suffices.
+// Input and mask rows are loaded kDepth iterations ahead into register rings
+// (the loop is unrolled by kDepth so that ring slots are registers).  The
+// expressions are those of hdiffsa_kernel, term for term.
+template <class T, int kDepth>
+__global__ void __launch_bounds__(1024)
+    hdiffsa_strip_kernel(T *__restrict__ out, const T *__restrict__ in, const T *__restrict__ mask,
+                         const T *__restrict__ crlato, int so, const T *__restrict__ crlatu, int su, Geo g,
+                         Strips st) {
+  extern __shared__ __align__(16) unsigned char oeb_smem[];
+  T *s_in = reinterpret_cast<T *>(oeb_smem);  // 3 x lanes
+  T *s_lap = s_in + 3 * st.lanes;              // 2 x lanes
+  const StripCoord c = strip_coord(st, int(blockIdx.x), g.nx, g.ny);
+  const int l = int(threadIdx.x), L = st.lanes, sj = g.sj;
+  const bool lap_lane = l >= 1 && l < c.w + 3;  // lap on [i0 - 1, i0 + w + 1)
+  const bool out_lane = l >= 2 && l < c.w + 2;  // out on [i0, i0 + w)
+  const int64_t col = int64_t(c.k) * g.sk + (c.i0 - 2 + l);
+  const int r0 = c.j0 - 1, last_in = c.j1 + 1;  // in rows [r0 - 1, last_in]
+  // Loads are unconditional, with the address clamped into the array's valid
+  // box: a masked load (cond ? load : 0) makes the select wait for the load,
+  // which would drain the prefetch ring.  Values of lanes or rows outside the
+  // strip are never used.
+  const int64_t col_in = int64_t(c.k) * g.sk + (c.i0 - 2 + min(l, c.w + 3));
+  const int64_t col_mk = int64_t(c.k) * g.sk + (c.i0 + min(max(l - 2, 0), c.w - 1));
+  auto ld_in = [&](int row) -> T { return in[col_in + int64_t(min(row, last_in)) * sj]; };
+  auto ld_mask = [&](int row) -> T { return ld_nt(mask + col_mk + int64_t(min(max(row, c.j0), c.j1 - 1)) * sj); };
+  // Rows are prefetched a block of kDepth iterations ahead: while block t0
+  // runs on cin/cmk (in(r + 1) and mask(r - 1) of iteration t0 + u), the
+  // loads of block t0 + kDepth are in flight in nin/nmk.
+  T cin[kDepth], cmk[kDepth];
+#pragma unroll
+  for (int u = 0; u < kDepth; ++u) {
+    cin[u] = ld_in(r0 + 1 + u);
+    cmk[u] = ld_mask(r0 - 1 + u);
+  }
+  T x_m2 = T(0), x_m1 = ld_in(r0 - 1), x_0 = ld_in(r0);
+  T lap_s = T(0), lap_c = T(0);
+  const int nt = c.j1 - c.j0 + 2;
+  for (int t0 = 0; t0 < nt; t0 += kDepth) {
+    T nin[kDepth], nmk[kDepth];
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u) {
+      nin[u] = ld_in(r0 + 1 + kDepth + t0 + u);
+      nmk[u] = ld_mask(r0 - 1 + kDepth + t0 + u);
+    }
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u) {
+      const int t = t0 + u;
+      if (t < nt) {  // uniform across the workgroup
+        const int r = r0 + t;
+        const T x_p1 = cin[u], m_0 = cmk[u];
+        T *si = s_in + (t % 3) * L + l;
+        si[0] = x_0;
+        lds_barrier();
+        T lap_n = T(0);
+        if (lap_lane) {
+          const T lo = crlato[int64_t(r) * so], lu = crlatu[int64_t(r) * su];
+          lap_n = si[-1] + si[1] - T(2.0) * x_0 + lo * (x_p1 - x_0) + lu * (x_m1 - x_0);
+        }
+        s_lap[(t & 1) * L + l] = lap_n;
+        if (t >= 2 && out_lane) {
+          const int j = r - 1;
+          const T *ri = s_in + ((t - 1) % 3) * L + l;   // in(j)
+          const T *rl = s_lap + ((t - 1) & 1) * L + l;  // lap(j)
+          const T lo_s = crlato[int64_t(j - 1) * so], lo_c = crlato[int64_t(j) * so];
+          const T q0 = x_m1;
+          T flx_w = lap_c - rl[-1];
+          if (flx_w * (q0 - ri[-1]) > 0) flx_w = T(0);
+          T flx_e = rl[1] - lap_c;
+          if (flx_e * (ri[1] - q0) > 0) flx_e = T(0);
+          T fly_s = lo_s * (lap_c - lap_s);
+          if (fly_s * (q0 - x_m2) > 0) fly_s = T(0);
+          T fly_n = lo_c * (lap_n - lap_c);
+          if (fly_n * (x_0 - q0) > 0) fly_n = T(0);
+          st_nt(out + col + int64_t(j) * sj, q0 + (flx_w - flx_e + fly_s - fly_n) * m_0);
+        }
+        x_m2 = x_m1;
+        x_m1 = x_0;
+        x_0 = x_p1;
+        lap_s = lap_c;
+        lap_c = lap_n;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u) {
+      cin[u] = nin[u];
+      cmk[u] = nmk[u];
+    }
+  }
+}
+
 template <class T, class S3, class S1>
 int hdiffsa_run(const oeb_domain *d, const S3 *out, const S3 *in, const S3 *mask, const S1 *crlato,
                 const S1 *crlatu, void *stream) {
@@ -79,6 +176,18 @@ int hdiffsa_run(const oeb_domain *d, const S3 *out, const S3 *in, const S3 *mask
   as.add1("crlato", p_lo, -1, ny + 1, s_lo);
   as.add1("crlatu", p_lu, -1, ny + 1, s_lu);
   OEB_TRY(as.check(K));
+  if (strip_for(K, true)) {
+    const Strips st = make_strips(nx, ny, int(nz), 2, strip_rows_for(K, default_strip_rows(ny)));
+    OEB_TRY(check_strips(K, st));
+    // Measured at 256x256x80 and 512x512x80: prefetching 2 rows ahead is
+    // fastest in FP64 (4 and 8 rows: 1.2x and 1.8x slower), 4 in FP32.
+    const int depth = strip_depth_for(K, sizeof(T) == 8 ? 2 : 4);
+    auto kern = depth <= 2 ? hdiffsa_strip_kernel<T, 2> : depth <= 4 ? hdiffsa_strip_kernel<T, 4>
+                                                                     : hdiffsa_strip_kernel<T, 8>;
+    hipLaunchKernelGGL(kern, dim3(unsigned(st.n)), dim3(unsigned(st.lanes)), size_t(5) * st.lanes * sizeof(T),
+                       hipStream_t(stream), const_cast<T *>(p_out), p_in, p_mask, p_lo, s_lo, p_lu, s_lu, g, st);
+    return launch_status(K);
+  }
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kchunk_for(K, sizeof(T), d, kChunk));
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
   hipLaunchKernelGGL(hdiffsa_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),

@@ -173,6 +173,93 @@ __device__ __forceinline__ TileCoord tile_coord(const Tiles &t, int tile_i, int 
   return tile_coord_of(t, int(blockIdx.x), tile_i, tile_j, nz);
 }
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() is a
+// workgroup fence on all memory: before its s_barrier it waits for every
+// outstanding global load and store of the wave (s_waitcnt vmcnt(0)), which
+// drains the rows or levels a kernel has prefetched and makes each barrier
+// wait for the write-back of the outputs just stored.  The kernels that use
+// lds_barrier() share data between lanes through LDS only; global memory is
+// never written by one lane and read by another within a launch.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// ------------------------------------------------------------------ strips
+// Row-march layout of the horizontal stencils (those with no k reach).  A
+// workgroup owns the output columns [i0, i0 + w) of one level k over a chunk
+// of rows [j0, j1), with one lane per column of [i0 - h, i0 + w + h): the
+// strip and its i-halo.  It walks the rows in order, keeping each column's
+// j-neighbours in registers and exchanging i-neighbours through LDS, so an
+// input line is fetched once per row chunk.  A 64 x 4 tile fetches the lines
+// of its i- and j-halo again for every tile it borders, and those tiles run on
+// other XCDs (other L2s): 1.9-2.6x the algorithmic bytes for reach 1-3.  A
+// strip re-fetches only the rows of its j-halo, (rows + 2h) / rows, and one
+// line at each end of the row.  One strip spans nx when nx + 2h <= 1024.
+struct Strips {
+  int ns;     // strips across nx
+  int w;      // output columns per strip (the last may hold fewer)
+  int h;      // i-halo lanes each side
+  int lanes;  // workgroup size: w + 2h rounded up to a wavefront
+  int rows;   // rows per chunk
+  int nj;     // chunks across ny
+  int n;      // workgroups: ns * nj * nz
+};
+
+inline Strips make_strips(int64_t nx, int64_t ny, int nz, int h, int rows) {
+  Strips s;
+  s.h = h;
+  const int64_t maxw = 1024 - 2 * h;
+  s.ns = int((nx + maxw - 1) / maxw);
+  s.w = int((nx + s.ns - 1) / s.ns);
+  s.lanes = ((s.w + 2 * h + 63) / 64) * 64;
+  s.rows = rows < 1 ? 1 : rows;
+  s.nj = int((ny + s.rows - 1) / s.rows);
+  s.n = s.ns * s.nj * nz;
+  return s;
+}
+
+// Default rows per chunk: about 16 chunks across ny, between 8 and 32 rows.
+// At 256x256x80 and 512x512x80 that keeps ~1300 workgroups in flight; longer
+// chunks re-fetch fewer halo rows but leave CUs idle at the end of the launch.
+inline int default_strip_rows(int64_t ny) {
+  const int64_t r = ny / 16;
+  return int(r < 8 ? 8 : (r > 32 ? 32 : r));
+}
+
+struct StripCoord {
+  int i0, w, j0, j1, k;
+};
+
+__device__ __forceinline__ StripCoord strip_coord(const Strips &s, int id, int nx, int ny) {
+  StripCoord c;
+  const int si = id % s.ns;
+  const int rest = id / s.ns;
+  const int cj = rest % s.nj;
+  c.k = rest / s.nj;
+  c.i0 = si * s.w;
+  c.w = min(s.w, nx - c.i0);
+  c.j0 = cj * s.rows;
+  c.j1 = min(ny, c.j0 + s.rows);
+  return c;
+}
+
+// Fails with OEB_ERR_DOMAIN when a strip launch does not fit one grid.
+int check_strips(const char *kname, const Strips &s);
+
+// Rows per strip chunk for kernel `kname`: OEB_ROWS_<KNAME> when it holds an
+// integer in [1, 4096], else `dflt`.
+int strip_rows_for(const char *kname, int dflt);
+
+// Rows a strip kernel prefetches ahead (2, 4 or 8): OEB_DEPTH_<KNAME> when
+// it holds an integer in [1, 8], else `dflt`.
+int strip_depth_for(const char *kname, int dflt);
+
+// Whether kernel `kname` takes its strip path: OEB_STRIP_<KNAME> when it
+// holds 0 or 1, else `dflt`.
+bool strip_for(const char *kname, bool dflt);
+
 // ------------------------------------------------------------------ batches
 // A batch (oeb_batch_*) records the launches the entry points would make,
 // for many members of an ensemble, and replays each run of consecutive
