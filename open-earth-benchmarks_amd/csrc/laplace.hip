@@ -66,6 +66,80 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
   }
 }
 
+// Row-march form (Strips, no halo lanes): lane l owns the V columns
+// i = i0 + V l .. i + V - 1 of one level and walks the rows [j0, j1) of its
+// chunk.  Row j - 1 and j come from registers, row j + 1 is loaded (V wide),
+// and the i neighbours in(i - 1, j), in(i + V, j) are single loads that hit
+// the lines the adjacent lanes fetch.  A row of the input is thus fetched once
+// per chunk (plus the two halo rows).  Loads run D rows ahead; addresses of
+// lanes and rows outside the strip are clamped into it (never stored).  Same
+// arithmetic as laplace_kernel.
+template <class T, int V, int D>
+__global__ void __launch_bounds__(1024)
+    laplace_strip_kernel(const T *__restrict__ in, T *__restrict__ out, Geo g, Strips st) {
+  typedef typename Vec<T, V>::type VT;
+  const StripCoord c = strip_coord(st, int(blockIdx.x), g.nx, g.ny);
+  const int l = int(threadIdx.x), nl = c.w / V, sj = g.sj;
+  const bool out_lane = l < nl;
+  const int64_t col = int64_t(c.k) * g.sk + (c.i0 + V * min(l, nl - 1));
+  auto ldr = [&](int row) -> VT { return ldv<T, V>(in + col + int64_t(min(row, c.j1)) * sj); };
+  auto ldw = [&](int row) -> T { return in[col + int64_t(min(row, c.j1 - 1)) * sj - 1]; };
+  auto lde = [&](int row) -> T { return in[col + int64_t(min(row, c.j1 - 1)) * sj + V]; };
+  VT x_m1 = ldr(c.j0 - 1), x_0 = ldr(c.j0);
+  VT cin[D];
+  T cw[D], ce[D];
+#pragma unroll
+  for (int u = 0; u < D; ++u) {
+    cin[u] = ldr(c.j0 + 1 + u);
+    cw[u] = ldw(c.j0 + u);
+    ce[u] = lde(c.j0 + u);
+  }
+  const int nt = c.j1 - c.j0;
+  for (int t0 = 0; t0 < nt; t0 += D) {
+    VT nin[D];
+    T nw[D], ne[D];
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      nin[u] = ldr(c.j0 + 1 + D + t0 + u);
+      nw[u] = ldw(c.j0 + D + t0 + u);
+      ne[u] = lde(c.j0 + D + t0 + u);
+    }
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      if (t0 + u < nt) {
+        const VT x_p1 = cin[u];
+        VT r;
+#pragma unroll
+        for (int x = 0; x < V; ++x)
+          r[x] = lap_point<T>(x_0[x], x == 0 ? cw[u] : x_0[x - 1], x == V - 1 ? ce[u] : x_0[x + 1], x_p1[x],
+                              x_m1[x]);
+        if (out_lane) stv_nt<T, V>(out + col + int64_t(c.j0 + t0 + u) * sj, r);
+        x_m1 = x_0;
+        x_0 = x_p1;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      cin[u] = nin[u];
+      cw[u] = nw[u];
+      ce[u] = ne[u];
+    }
+  }
+}
+
+template <class T, int V>
+int laplace_strip_launch(const T *p_in, T *p_out, const Geo &g, const oeb_domain *d, hipStream_t stream) {
+  const char *K = "laplace";
+  Strips st = make_strips(d->nx, d->ny, int(d->nz), 0, 1, V);
+  st = make_strips(d->nx, d->ny, int(d->nz), 0, strip_rows_for(K, default_strip_rows(d->ny, st.lanes)), V);
+  OEB_TRY(check_strips(K, st));
+  const int depth = strip_depth_for(K, 2);
+  auto kern = depth <= 1 ? laplace_strip_kernel<T, V, 1>
+                         : (depth <= 2 ? laplace_strip_kernel<T, V, 2> : laplace_strip_kernel<T, V, 4>);
+  hipLaunchKernelGGL(kern, dim3(unsigned(st.n)), dim3(unsigned(st.lanes)), 0, stream, p_in, p_out, g, st);
+  return launch_status(K);
+}
+
 template <class T, class S>
 int laplace_run(const oeb_domain *d, const S *in, const S *out, void *stream) {
   const char *K = "laplace";
@@ -80,8 +154,13 @@ int laplace_run(const oeb_domain *d, const S *in, const S *out, void *stream) {
   as.add("in", p_in, r_in, g, false);
   as.add("out", p_out, r_out, g, true);
   OEB_TRY(as.check(K));
-  const int kc = kchunk_for(K, sizeof(T), d, kChunk);
   constexpr int W = kVecW<T>;
+  if (strip_for(K, true)) {
+    if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_in, p_out}))
+      return laplace_strip_launch<T, W>(p_in, const_cast<T *>(p_out), g, d, hipStream_t(stream));
+    return laplace_strip_launch<T, 1>(p_in, const_cast<T *>(p_out), g, d, hipStream_t(stream));
+  }
+  const int kc = kchunk_for(K, sizeof(T), d, kChunk);
   if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_in, p_out})) {
     const Tiles t = make_tiles(nx, ny, int(nz), W * kTileI, kTileJ, kc);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), W * kTileI, kTileJ));

@@ -200,32 +200,38 @@ __device__ __forceinline__ void lds_barrier() {
 struct Strips {
   int ns;     // strips across nx
   int w;      // output columns per strip (the last may hold fewer)
+  int v;      // columns per lane
   int h;      // i-halo lanes each side
-  int lanes;  // workgroup size: w + 2h rounded up to a wavefront
+  int lanes;  // workgroup size: w / v + 2h rounded up to a wavefront
   int rows;   // rows per chunk
   int nj;     // chunks across ny
   int n;      // workgroups: ns * nj * nz
 };
 
-inline Strips make_strips(int64_t nx, int64_t ny, int nz, int h, int rows) {
+// Strips of at most 1024 lanes of v columns (w a multiple of v), h halo
+// lanes each side.
+inline Strips make_strips(int64_t nx, int64_t ny, int nz, int h, int rows, int v = 1) {
   Strips s;
   s.h = h;
-  const int64_t maxw = 1024 - 2 * h;
+  s.v = v < 1 ? 1 : v;
+  const int64_t maxw = int64_t(s.v) * (1024 - 2 * h);
   s.ns = int((nx + maxw - 1) / maxw);
-  s.w = int((nx + s.ns - 1) / s.ns);
-  s.lanes = ((s.w + 2 * h + 63) / 64) * 64;
+  s.w = int(((nx + s.ns - 1) / s.ns + s.v - 1) / s.v * s.v);
+  s.lanes = ((s.w / s.v + 2 * h + 63) / 64) * 64;
   s.rows = rows < 1 ? 1 : rows;
   s.nj = int((ny + s.rows - 1) / s.rows);
   s.n = s.ns * s.nj * nz;
   return s;
 }
 
-// Default rows per chunk: about 16 chunks across ny, between 8 and 32 rows.
-// At 256x256x80 and 512x512x80 that keeps ~1300 workgroups in flight; longer
-// chunks re-fetch fewer halo rows but leave CUs idle at the end of the launch.
-inline int default_strip_rows(int64_t ny) {
-  const int64_t r = ny / 16;
-  return int(r < 8 ? 8 : (r > 32 ? 32 : r));
+// Default rows per chunk for a strip of `lanes` lanes: ny * lanes / 4096,
+// between 4 and 32, so that narrow strips (few waves per workgroup) get
+// shorter chunks and more workgroups.  Measured at 256x256x80 and
+// 512x512x80: longer chunks re-fetch fewer halo rows but leave CUs idle at
+// the end of the launch; shorter ones re-fetch more.
+inline int default_strip_rows(int64_t ny, int lanes) {
+  const int64_t r = ny * lanes / 4096;
+  return int(r < 4 ? 4 : (r > 32 ? 32 : r));
 }
 
 struct StripCoord {

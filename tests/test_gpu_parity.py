@@ -100,19 +100,19 @@ def test_path_switches(name, pairs, kchunk, monkeypatch):
     _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
 
 
-STRIPPED = ["hdiffsa"]
+STRIPPED = ["laplace", "hdiffsa"]
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("rows", [1, 5, 64])
 @pytest.mark.parametrize("strip", [0, 1])
-@pytest.mark.parametrize("shape", [(64, 36, 3), (1030, 9, 2)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("shape", [(64, 36, 3), (1030, 9, 2), (2100, 5, 2)], ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("name", STRIPPED)
 def test_strip_switches(name, shape, strip, rows, monkeypatch):
     """The row-march (strip) path and the tile path, forced through
     OEB_STRIP_<K>, with row chunks of one row, of a number that does not
-    divide ny, and of more than ny, and with nx wide enough for two strips,
-    agree bit for bit with the oracle."""
+    divide ny, and of more than ny, and with nx wide enough for two or three
+    strips, agree bit for bit with the oracle."""
     monkeypatch.setenv(f"OEB_STRIP_{name.upper()}", str(strip))
     monkeypatch.setenv(f"OEB_ROWS_{name.upper()}", str(rows))
     L, host = inputs(name, *shape, "baseline", seed=13)
