@@ -217,6 +217,10 @@ bool pairs_for(const char *kname, size_t esize, const oeb_domain *d, bool dflt) 
 
 int strip_rows_for(const char *kname, int dflt) { return env_tunable("OEB_ROWS_", kname, dflt, 1, 4096); }
 
+int env_choice(const char *prefix, const char *kname, int dflt, int lo, int hi) {
+  return env_tunable(prefix, kname, dflt, lo, hi);
+}
+
 int strip_depth_for(const char *kname, int dflt) { return env_tunable("OEB_DEPTH_", kname, dflt, 1, 8); }
 
 bool strip_for(const char *kname, bool dflt) { return env_tunable("OEB_STRIP_", kname, dflt ? 1 : 0, 0, 1) != 0; }

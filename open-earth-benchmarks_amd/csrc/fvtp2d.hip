@@ -85,7 +85,7 @@ struct FvOut {
   T *q_i, *q_j, *fx1, *fx2, *fy1, *fy2;
 };
 
-template <int kTJ, class T>
+template <int kTI, int kTJ, int kNT, class T>
 __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn<T> a, FvOut<T> o, T p1, T p2, Geo g, Tiles t) {
   // LDS tile extents (elements).
   constexpr int kQW = kTI + 6, kQH = kTJ + 6;  // q:   i [i0-3, i0+TI+3) x j [j0-3, j0+TJ+3)
@@ -192,6 +192,16 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn<T> a, FvOut<T> o, T p1
   }
 }
 
+template <int TI, int TJ, int NT, class T>
+int fvtp2d_launch(const char *K, const FvIn<T> &a, const FvOut<T> &o, T p1, T p2, const Geo &g,
+                  const oeb_domain *d, void *stream) {
+  const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
+  const Tiles t = make_tiles(nx, ny, int(nz), TI, TJ, kchunk_for(K, sizeof(T), d, kChunk));
+  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), TI, TJ));
+  hipLaunchKernelGGL((fvtp2d_kernel<TI, TJ, NT, T>), dim3(t.n), dim3(NT), 0, hipStream_t(stream), a, o, p1, p2, g, t);
+  return launch_status(K);
+}
+
 template <class T, class S>
 int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, const S *fx2, const S *fy1,
                const S *fy2, const S *q, const S *crx, const S *cry, const S *ra_x, const S *ra_y, const S *xfx,
@@ -246,11 +256,13 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   o.fy1 = const_cast<T *>(pfy1);
   o.fy2 = const_cast<T *>(pfy2);
 
-  const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJdefault, kchunk_for(K, sizeof(T), d, kChunk));
-  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, kTJdefault));
-  hipLaunchKernelGGL((fvtp2d_kernel<kTJdefault, T>), dim3(t.n), dim3(kNT), 0, hipStream_t(stream), a, o, p1, p2, g,
-                     t);
-  return launch_status(K);
+  switch (env_choice("OEB_TILE_", K, 0, 0, 4)) {
+    case 1: return fvtp2d_launch<128, 8, 512, T>(K, a, o, p1, p2, g, d, stream);
+    case 2: return fvtp2d_launch<64, 16, 256, T>(K, a, o, p1, p2, g, d, stream);
+    case 3: return fvtp2d_launch<128, 16, 512, T>(K, a, o, p1, p2, g, d, stream);
+    case 4: return fvtp2d_launch<128, 8, 256, T>(K, a, o, p1, p2, g, d, stream);
+    default: return fvtp2d_launch<kTI, kTJdefault, kNT, T>(K, a, o, p1, p2, g, d, stream);
+  }
 }
 
 }  // namespace

@@ -262,6 +262,10 @@ int strip_rows_for(const char *kname, int dflt);
 // it holds an integer in [1, 8], else `dflt`.
 int strip_depth_for(const char *kname, int dflt);
 
+// The environment variable <prefix><KNAME> when it holds an integer in
+// [lo, hi], else `dflt`.
+int env_choice(const char *prefix, const char *kname, int dflt, int lo, int hi);
+
 // Whether kernel `kname` takes its strip path: OEB_STRIP_<KNAME> when it
 // holds 0 or 1, else `dflt`.
 bool strip_for(const char *kname, bool dflt);
