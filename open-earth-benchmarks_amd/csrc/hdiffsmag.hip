@@ -96,6 +96,125 @@ hipError_t hdiffsmag_batch(const BatchMember *tab, int n, int total, hipStream_t
 template <class T>
 const BatchKind kHdiffsmagBatch{"hdiffsmag", hdiffsmag_batch<T>};
 
+// Row-march form (Strips, no halo lanes): lane l owns the V columns
+// i = i0 + V l .. i + V - 1 of one level and walks the rows [j0, j1) of its
+// chunk.  Rows j - 1, j, j + 1 of u and v are in registers (V wide, with the
+// single neighbours u(i - 1), u(i + V), v(i - 1), v(i + V) of each row), so
+// a row of each input is fetched once per chunk; the neighbour loads hit the
+// lines the adjacent lanes fetch.  Rows are loaded D rows ahead; addresses of
+// lanes and rows outside the strip are clamped into it (never stored).  The
+// expressions are those of hdiffsmag_body, term for term.
+template <class T, int V>
+struct SmagRow {
+  typename Vec<T, V>::type u, v;
+  T uw, ue, vw, ve;  // u(i - 1), u(i + V), v(i - 1), v(i + V)
+};
+
+template <class T, int V, int D>
+__global__ void __launch_bounds__(1024) hdiffsmag_strip_kernel(HsArgs<T> a, Geo g, Strips st) {
+  typedef typename Vec<T, V>::type VT;
+  typedef SmagRow<T, V> Row;
+  constexpr T rr = kEarthRadiusRecip<T>;
+  const T eddlat = a.eddlat, eddlon = a.eddlon, tau_smag = a.tau_smag, weight_smag = a.weight_smag;
+  const StripCoord c = strip_coord(st, int(blockIdx.x), g.nx, g.ny);
+  const int l = int(threadIdx.x), nl = c.w / V, sj = g.sj;
+  const bool out_lane = l < nl;
+  const int64_t col = int64_t(c.k) * g.sk + (c.i0 + V * min(l, nl - 1));
+  auto ldrow = [&](int r) -> Row {
+    const int64_t at = col + int64_t(min(r, c.j1)) * sj;  // u, v rows [j0 - 1, j1]
+    const T *pu = a.uin + at;
+    const T *pv = a.vin + at;
+    Row x;
+    x.u = ldv<T, V>(pu);
+    x.v = ldv<T, V>(pv);
+    x.uw = pu[-1];
+    x.ue = pu[V];
+    x.vw = pv[-1];
+    x.ve = pv[V];
+    return x;
+  };
+  auto ldmask = [&](int r) -> VT { return ldv_nt<T, V>(a.mask + col + int64_t(min(r, c.j1 - 1)) * sj); };
+  Row rs = ldrow(c.j0 - 1), r0 = ldrow(c.j0);
+  Row cr[D];  // row j + 1 of iteration t0 + u
+  VT cm[D];   // mask row j of iteration t0 + u
+#pragma unroll
+  for (int u = 0; u < D; ++u) {
+    cr[u] = ldrow(c.j0 + 1 + u);
+    cm[u] = ldmask(c.j0 + u);
+  }
+  const int nt = c.j1 - c.j0;
+  for (int t0 = 0; t0 < nt; t0 += D) {
+    Row nr[D];
+    VT nm[D];
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      nr[u] = ldrow(c.j0 + 1 + D + t0 + u);
+      nm[u] = ldmask(c.j0 + D + t0 + u);
+    }
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      if (t0 + u < nt) {
+        const int j = c.j0 + t0 + u;
+        const Row rn = cr[u];
+        const VT mk = cm[u];
+        const T ac_s = a.acrlat0[int64_t(j - 1) * a.s_ac], ac_c = a.acrlat0[int64_t(j) * a.s_ac],
+                ac_n = a.acrlat0[int64_t(j + 1) * a.s_ac];
+        const T lo = a.crlato[int64_t(j) * a.s_to], lu = a.crlatu[int64_t(j) * a.s_tu];
+        const T vo = a.crlavo[int64_t(j) * a.s_vo], vu = a.crlavu[int64_t(j) * a.s_vu];
+        VT ro, so;
+#pragma unroll
+        for (int x = 0; x < V; ++x) {
+          const T u_c = r0.u[x], u_w = x == 0 ? r0.uw : r0.u[x - 1], u_e = x == V - 1 ? r0.ue : r0.u[x + 1];
+          const T u_n = rn.u[x], u_s = rs.u[x], u_nw = x == 0 ? rn.uw : rn.u[x - 1];
+          const T v_c = r0.v[x], v_w = x == 0 ? r0.vw : r0.v[x - 1], v_e = x == V - 1 ? r0.ve : r0.v[x + 1];
+          const T v_n = rn.v[x], v_s = rs.v[x], v_se = x == V - 1 ? rs.ve : rs.v[x + 1];
+          const T t0v = (v_s - v_c) * eddlat * rr - (u_w - u_c) * ac_c * eddlon;
+          const T t_ip = (v_se - v_e) * eddlat * rr - (u_c - u_e) * ac_c * eddlon;
+          const T t_jp = (v_c - v_n) * eddlat * rr - (u_nw - u_n) * ac_n * eddlon;
+          const T s0 = (u_n - u_c) * eddlat * rr + (v_e - v_c) * ac_c * eddlon;
+          const T s_im = (u_nw - u_w) * eddlat * rr + (v_c - v_w) * ac_c * eddlon;
+          const T s_jm = (u_c - u_s) * eddlat * rr + (v_se - v_s) * ac_s * eddlon;
+          const T tt0 = t0v * t0v, tt_ip = t_ip * t_ip, tt_jp = t_jp * t_jp;
+          const T ss0 = s0 * s0, ss_im = s_im * s_im, ss_jm = s_jm * s_jm;
+          const T hdweight = weight_smag * mk[x];
+          T smag_u = tau_smag * sqrt(T(0.5) * (tt_ip + tt0) + T(0.5) * (ss_jm + ss0)) - hdweight;
+          smag_u = cxx_min(T(0.5), cxx_max(T(0.0), smag_u));
+          const T lapu = u_e + u_w - T(2.0) * u_c + lo * (u_n - u_c) + lu * (u_s - u_c);
+          ro[x] = u_c + smag_u * lapu;
+          T smag_v = tau_smag * sqrt(T(0.5) * (tt_jp + tt0) + T(0.5) * (ss_im + ss0)) - hdweight;
+          smag_v = cxx_min(T(0.5), cxx_max(T(0.0), smag_v));
+          const T lapv = v_e + v_w - T(2.0) * v_c + vo * (v_n - v_c) + vu * (v_s - v_c);
+          so[x] = v_c + smag_v * lapv;
+        }
+        if (out_lane) {
+          const int64_t at = col + int64_t(j) * sj;
+          stv_nt<T, V>(a.uout + at, ro);
+          stv_nt<T, V>(a.vout + at, so);
+        }
+        rs = r0;
+        r0 = rn;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      cr[u] = nr[u];
+      cm[u] = nm[u];
+    }
+  }
+}
+
+template <class T, int V>
+int hdiffsmag_strip_launch(const HsArgs<T> &a, const Geo &g, const oeb_domain *d, hipStream_t stream) {
+  const char *K = "hdiffsmag";
+  Strips st = make_strips(d->nx, d->ny, int(d->nz), 0, 1, V);
+  st = make_strips(d->nx, d->ny, int(d->nz), 0, strip_rows_for(K, default_strip_rows(d->ny, st.lanes)), V);
+  OEB_TRY(check_strips(K, st));
+  const int depth = strip_depth_for(K, 1);
+  auto kern = depth <= 1 ? hdiffsmag_strip_kernel<T, V, 1> : hdiffsmag_strip_kernel<T, V, 2>;
+  hipLaunchKernelGGL(kern, dim3(unsigned(st.n)), dim3(unsigned(st.lanes)), 0, stream, a, g, st);
+  return launch_status(K);
+}
+
 template <class T, class S3, class S1>
 int hdiffsmag_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *uin, const S3 *vin, const S3 *mask,
                   const S1 *crlavo, const S1 *crlavu, const S1 *crlato, const S1 *crlatu, const S1 *acrlat0,
@@ -135,6 +254,12 @@ int hdiffsmag_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 
   const HsArgs<T> a{const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_m, p_vo1, p_vu1, p_to1, p_tu1, p_ac,
                     s_vo, s_vu, s_to, s_tu, s_ac, eddlat, eddlon, tau_smag, weight_smag};
   if (batch_record(kHdiffsmagBatch<T>, a, g, t, t.n)) return OEB_OK;
+  if (strip_for(K, true)) {
+    constexpr int W = 16 / int(sizeof(T));
+    if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_uo, p_vo, p_ui, p_vi, p_m}))
+      return hdiffsmag_strip_launch<T, W>(a, g, d, hipStream_t(stream));
+    return hdiffsmag_strip_launch<T, 1>(a, g, d, hipStream_t(stream));
+  }
   hipLaunchKernelGGL(hdiffsmag_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }

@@ -100,7 +100,7 @@ def test_path_switches(name, pairs, kchunk, monkeypatch):
     _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
 
 
-STRIPPED = ["laplace", "hdiffsa"]
+STRIPPED = ["laplace", "hdiffsa", "hdiffsmag"]
 
 
 @pytest.mark.gpu
