@@ -129,6 +129,145 @@ __global__ void __launch_bounds__(kTI *kTJ) hdiffsa_adj_in(HsaAdj<T> a, Geo g, B
   a.inbar[at] = r;
 }
 
+// Fused row-march form, used when one strip spans nx (nx + 6 <= 1024): lane
+// l owns column i = l - 3 of [-3, nx + 3) of one level, and the workgroup
+// walks the in_bar rows [j0, j1) of its chunk of [-2, ny + 2).  With front row
+// r it forms lap(r) (on [-1, nx+1) x [-1, ny+1)), lap_bar(r - 1) and
+// in_bar(r - 2), so lap_bar never goes to memory: in, mask and out_bar are
+// read once per chunk and in_bar is written once.  Each lane keeps rows of
+// in, lap, lap_bar and mask x out_bar of its column in registers; the i
+// neighbours come from LDS rows: in (3 slots, published before the barrier
+// of iteration t and read in t and t + 1), mask x out_bar (2 slots, read in
+// t), lap and lap_bar (2 slots each, written after the barrier of t and read
+// in t + 1).  Values outside their boxes are exact zeros, as in the two-pass
+// form; the arithmetic is that of hdiffsa_adj_lap and hdiffsa_adj_in, term
+// for term.  Loads run two rows ahead, with addresses clamped into the
+// arrays' boxes; a clamped value is never used.
+template <class T>
+__global__ void __launch_bounds__(1024) hdiffsa_adj_strip(HsaAdj<T> a, Geo g, int rows, int nchunk) {
+  extern __shared__ __align__(16) unsigned char oeb_smem[];
+  const int L = int(blockDim.x);
+  T *s_in = reinterpret_cast<T *>(oeb_smem);  // 3 x L
+  T *s_mo = s_in + 3 * L;                      // 2 x L
+  T *s_lap = s_mo + 2 * L;                     // 2 x L
+  T *s_lb = s_lap + 2 * L;                     // 2 x L
+  const int nx = g.nx, ny = g.ny, sj = g.sj;
+  const int cj = int(blockIdx.x) % nchunk, k = int(blockIdx.x) / nchunk;
+  const int j0 = -2 + cj * rows, j1 = min(ny + 2, j0 + rows);
+  const int l = int(threadIdx.x), i = l - 3;
+  const bool col_in = i >= -2 && i < nx + 2;   // in_bar columns
+  const bool col_lap = i >= -1 && i < nx + 1;  // lap, lap_bar columns
+  const bool col_int = i >= 0 && i < nx;       // interior columns
+  const int64_t plane = int64_t(k) * g.sk;
+  const int64_t c_in = plane + min(max(i, -2), nx + 1), c_mo = plane + min(max(i, 0), nx - 1);
+  auto ld_in = [&](int r) -> T { return a.in[c_in + int64_t(min(max(r, -2), ny + 1)) * sj]; };
+  auto ld_mk = [&](int r) -> T { return a.mask[c_mo + int64_t(min(max(r, 0), ny - 1)) * sj]; };
+  auto ld_ob = [&](int r) -> T { return a.obar[c_mo + int64_t(min(max(r, 0), ny - 1)) * sj]; };
+  auto lo = [&](int jj) { return a.lo[int64_t(jj) * a.slo]; };
+  auto lu = [&](int jj) { return a.lu[int64_t(jj) * a.slu]; };
+  // mask x out_bar at (i, r), 0 outside the interior (hdiffsa_adj_lap's mo).
+  auto mo = [&](int r, T mk, T ob) { return (col_int && r >= 0 && r < ny) ? mk * ob : T(0); };
+  constexpr int D = 2;
+  const int r0 = j0 - 2, nt = j1 - j0 + 4;  // front rows r0 .. j1 + 1
+  T x_m2 = ld_in(r0 - 2), x_m1 = ld_in(r0 - 1), x_0 = ld_in(r0);
+  T mk_m2 = ld_mk(r0 - 2), ob_m2 = ld_ob(r0 - 2), mk_m1 = ld_mk(r0 - 1), ob_m1 = ld_ob(r0 - 1);
+  T lap_m2 = T(0), lap_m1 = T(0), lb_m3 = T(0), lb_m2 = T(0);
+  T cin[D], cmk[D], cob[D];  // in(r + 1), mask(r), out_bar(r) of iteration t0 + u
+#pragma unroll
+  for (int u = 0; u < D; ++u) {
+    cin[u] = ld_in(r0 + 1 + u);
+    cmk[u] = ld_mk(r0 + u);
+    cob[u] = ld_ob(r0 + u);
+  }
+  for (int t0 = 0; t0 < nt; t0 += D) {
+    T nin[D], nmk[D], nob[D];
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      nin[u] = ld_in(r0 + 1 + D + t0 + u);
+      nmk[u] = ld_mk(r0 + D + t0 + u);
+      nob[u] = ld_ob(r0 + D + t0 + u);
+    }
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      const int t = t0 + u;
+      if (t < nt) {  // uniform across the workgroup
+        const int r = r0 + t;
+        const T x_p1 = cin[u], mk_0 = cmk[u], ob_0 = cob[u];
+        T *si = s_in + (t % 3) * L + l;                // in row r
+        const T *si1 = s_in + ((t + 2) % 3) * L + l;   // in row r - 1
+        T *sm = s_mo + (t & 1) * L + l;                // mo row r - 1
+        const T mo_m2 = mo(r - 2, mk_m2, ob_m2), mo_m1 = mo(r - 1, mk_m1, ob_m1), mo_0 = mo(r, mk_0, ob_0);
+        si[0] = x_0;
+        sm[0] = mo_m1;
+        lds_barrier();
+        // lap(r), kernels/hdiffsa.h:147 (hsa_lap).
+        T lap_0 = T(0);
+        if (col_lap && r >= -1 && r < ny + 1)
+          lap_0 = si[-1] + si[1] - T(2.0) * x_0 + lo(r) * (x_p1 - x_0) + lu(r) * (x_m1 - x_0);
+        s_lap[(t & 1) * L + l] = lap_0;
+        // lap_bar(i, jj), jj = r - 1 (hdiffsa_adj_lap).
+        const int jj = r - 1;
+        T lb_m1 = T(0);
+        if (col_lap && jj >= -1 && jj < ny + 1) {
+          const T *sl = s_lap + ((t + 1) & 1) * L + l;  // lap row jj
+          const T l0 = lap_m1;
+          T rr = T(0);
+          if (jj >= 0 && jj < ny) {
+            if (i - 1 >= -1) {
+              const T f = l0 - sl[-1];
+              if (!(f * (x_m1 - si1[-1]) > 0)) rr += mo_m1 - sm[-1];
+            }
+            if (i < nx) {
+              const T f = sl[1] - l0;
+              if (!(f * (si1[1] - x_m1) > 0)) rr -= sm[1] - mo_m1;
+            }
+          }
+          if (i >= 0 && i < nx) {
+            if (jj - 1 >= -1) {
+              const T f = lo(jj - 1) * (l0 - lap_m2);
+              if (!(f * (x_m1 - x_m2) > 0)) rr += lo(jj - 1) * (mo_m1 - mo_m2);
+            }
+            if (jj < ny) {
+              const T f = lo(jj) * (lap_0 - l0);
+              if (!(f * (x_0 - x_m1) > 0)) rr -= lo(jj) * (mo_0 - mo_m1);
+            }
+          }
+          lb_m1 = rr;
+        }
+        s_lb[(t & 1) * L + l] = lb_m1;
+        // in_bar(i, q), q = r - 2 (hdiffsa_adj_in).
+        const int q = r - 2;
+        if (col_in && q >= j0 && q < j1) {
+          const T *sb = s_lb + ((t + 1) & 1) * L + l;  // lap_bar row q
+          T rr = (col_int && q >= 0 && q < ny) ? ob_m2 : T(0);
+          if (col_lap && q >= -1 && q < ny + 1) rr += lb_m2 * (T(-2.0) - lo(q) - lu(q));
+          rr += (i - 1 >= -1 ? sb[-1] : T(0)) + (i + 1 < nx + 1 ? sb[1] : T(0));
+          if (q - 1 >= -1 && q - 1 < ny + 1) rr += lb_m3 * lo(q - 1);
+          if (q + 1 >= -1 && q + 1 < ny + 1) rr += lb_m1 * lu(q + 1);
+          a.inbar[plane + i + int64_t(q) * sj] = rr;
+        }
+        x_m2 = x_m1;
+        x_m1 = x_0;
+        x_0 = x_p1;
+        mk_m2 = mk_m1;
+        ob_m2 = ob_m1;
+        mk_m1 = mk_0;
+        ob_m1 = ob_0;
+        lap_m2 = lap_m1;
+        lap_m1 = lap_0;
+        lb_m3 = lb_m2;
+        lb_m2 = lb_m1;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      cin[u] = nin[u];
+      cmk[u] = nmk[u];
+      cob[u] = nob[u];
+    }
+  }
+}
+
 template <class T, class S3, class S1>
 int hdiffsa_adj_run(const oeb_domain *d, const S3 *in, const S3 *mask, const S1 *crlato, const S1 *crlatu,
                     const S3 *out_bar, const S3 *in_bar, const S3 *lap, void *stream) {
@@ -155,6 +294,15 @@ int hdiffsa_adj_run(const oeb_domain *d, const S3 *in, const S3 *mask, const S1 
   OEB_TRY(as.check(K));
   a.inbar = const_cast<T *>(p_ib);
   a.lapbar = const_cast<T *>(p_lb);
+  if (nx + 6 <= 1024 && strip_for(K, true)) {
+    const int lanes = int((nx + 6 + 63) / 64 * 64);
+    const int rows = strip_rows_for(K, default_strip_rows(ny + 4, lanes));
+    const int nchunk = int((ny + 4 + rows - 1) / rows);
+    if (int64_t(nchunk) * nz > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
+    hipLaunchKernelGGL(hdiffsa_adj_strip<T>, dim3(unsigned(nchunk * nz)), dim3(unsigned(lanes)),
+                       size_t(9) * lanes * sizeof(T), hipStream_t(stream), a, g, rows, nchunk);
+    return launch_status(K);
+  }
   const BoxGrid b1{-1, -1, int(nx + 2), int(ny + 2)}, b2{-2, -2, int(nx + 4), int(ny + 4)};
   hipLaunchKernelGGL(hdiffsa_adj_lap<T>, box_blocks(b1, int(nz)), dim3(kTI, kTJ), 0, hipStream_t(stream), a, g, b1);
   OEB_TRY(launch_status(K));
@@ -183,70 +331,81 @@ struct FwAdj {
   double *ppbar;
 };
 
+// One lane per column (i, j) of [0, nx+1) x [0, ny+1) and a chunk of levels
+// [kb, ke) of the ppuv cotangent's levels [-1, nz+1) (grid.y = chunks).  Per
+// level the lane forms A(i, j), A(i-1, j), B(i, j), B(i, j-1) (the ppgu /
+// ppgv cotangents around it) and the two hhl ratios that weigh each, and from
+// them gc_bar(k) and the horizontal part of ppuv_bar(k); it carries gc_bar of
+// levels k-1, k, k+1 in registers.  A chunk starts one level early to have
+// gc_bar(kb - 1).  Interior columns also write uin_bar = uout_bar and
+// vin_bar = vout_bar on their interior levels.
 template <class T>
-__global__ void __launch_bounds__(kTI *kTJ) fastwavesuv_adj_uv(FwAdj<T> a, Geo g, BoxGrid b) {
-  int i, j;
-  if (!box_point(b, i, j)) return;
-  const int64_t at = int64_t(blockIdx.y) * g.sk + i + int64_t(j) * g.sj;
-  a.uinbar[at] = a.ubar[at];
-  a.vinbar[at] = a.vbar[at];
-}
-
-template <class T>
-__global__ void __launch_bounds__(kTI *kTJ) fastwavesuv_adj_pp(FwAdj<T> a, Geo g, BoxGrid b) {
+__global__ void __launch_bounds__(kTI *kTJ) fastwavesuv_adj_kernel(FwAdj<T> a, Geo g, BoxGrid b, int kc) {
   int i, j;
   if (!box_point(b, i, j)) return;
   const int sj = g.sj, nx = g.nx, ny = g.ny, nz = g.nz;
   const int64_t sk = g.sk, col = i + int64_t(j) * sj;
   const double dt = a.dt;
-  // Cotangents of ppgu(i+di, j+dj, k) and ppgv(...) (0 outside the interior).
-  auto A = [&](int di, int dj, int k) -> double {
-    if (!in_range(i + di, 0, nx) || !in_range(j + dj, 0, ny) || !in_range(k, 0, nz)) return 0.0;
-    const int64_t at = int64_t(k) * sk + col + di + int64_t(dj) * sj;
-    const T c = T(2.0) * a.fx[int64_t(j + dj) * a.fx_stride] / (a.rho[at + 1] + a.rho[at]);
-    return -dt * double(c) * double(a.ubar[at]);
+  const bool in_i = i < nx, in_j = j < ny, has_w = i >= 1 && in_j, has_s = j >= 1 && in_i;
+  const bool interior = in_i && in_j;
+  const T fx_c = in_j ? a.fx[int64_t(j) * a.fx_stride] : T(0);
+  const T fx_w = fx_c;  // A(i - 1, j) has the same row
+  // The cotangents and hhl ratios of one level (0 outside the interior).
+  struct Lv {
+    double gc, hs;  // gc_bar(k), (A(-1,0) - A(0,0)) + (B(0,-1) - B(0,0))
   };
-  auto B = [&](int di, int dj, int k) -> double {
-    if (!in_range(i + di, 0, nx) || !in_range(j + dj, 0, ny) || !in_range(k, 0, nz)) return 0.0;
-    const int64_t at = int64_t(k) * sk + col + di + int64_t(dj) * sj;
-    const T c = T(2.0) * a.edadlat / (a.rho[at + sj] + a.rho[at]);
-    return -dt * double(c) * double(a.vbar[at]);
-  };
-  // The hhl ratio of ppgu at (i+di, j+dj, k) (hu) and of ppgv (hv).
-  auto hu = [&](int di, int dj, int k) {
-    const int64_t at = int64_t(k) * sk + col + di + int64_t(dj) * sj, up = at + sk;
+  auto level = [&](int k) -> Lv {
+    Lv r{0.0, 0.0};
+    if (k < 0 || k >= nz) return r;
+    const int64_t at = int64_t(k) * sk + col, up = at + sk;
     const double *h = a.hhl;
-    return ((h[up] + h[at]) - (h[up + 1] + h[at + 1])) / ((h[up] - h[at]) + (h[up + 1] - h[at + 1]));
-  };
-  auto hv = [&](int di, int dj, int k) {
-    const int64_t at = int64_t(k) * sk + col + di + int64_t(dj) * sj, up = at + sk;
-    const double *h = a.hhl;
-    return ((h[up] + h[at]) - (h[up + sj] + h[at + sj])) / ((h[up] - h[at]) + (h[up + sj] - h[at + sj]));
-  };
-  // gc_bar(i, j, k): ppgu(i) and ppgu(i-1) read gc(i); ppgv(j) and ppgv(j-1) read gc(j).
-  auto gcbar = [&](int k) -> double {
-    if (!in_range(k, 0, nz)) return 0.0;
-    double r = 0.0;
-    if (i < nx && j < ny) r += 0.5 * hu(0, 0, k) * A(0, 0, k) + 0.5 * hv(0, 0, k) * B(0, 0, k);
-    if (i >= 1 && j < ny) r += 0.5 * hu(-1, 0, k) * A(-1, 0, k);
-    if (j >= 1 && i < nx) r += 0.5 * hv(0, -1, k) * B(0, -1, k);
+    double A0 = 0.0, Aw = 0.0, B0 = 0.0, Bs = 0.0;
+    if (interior) {
+      const T cu = T(2.0) * fx_c / (a.rho[at + 1] + a.rho[at]);
+      A0 = -dt * double(cu) * double(a.ubar[at]);
+      const T cv = T(2.0) * a.edadlat / (a.rho[at + sj] + a.rho[at]);
+      B0 = -dt * double(cv) * double(a.vbar[at]);
+      const double hu = ((h[up] + h[at]) - (h[up + 1] + h[at + 1])) / ((h[up] - h[at]) + (h[up + 1] - h[at + 1]));
+      const double hv =
+          ((h[up] + h[at]) - (h[up + sj] + h[at + sj])) / ((h[up] - h[at]) + (h[up + sj] - h[at + sj]));
+      r.gc += 0.5 * hu * A0 + 0.5 * hv * B0;
+    }
+    if (has_w) {
+      const int64_t aw = at - 1, uw = aw + sk;
+      const T cu = T(2.0) * fx_w / (a.rho[aw + 1] + a.rho[aw]);
+      Aw = -dt * double(cu) * double(a.ubar[aw]);
+      const double hu = ((h[uw] + h[aw]) - (h[uw + 1] + h[aw + 1])) / ((h[uw] - h[aw]) + (h[uw + 1] - h[aw + 1]));
+      r.gc += 0.5 * hu * Aw;
+    }
+    if (has_s) {
+      const int64_t as = at - sj, us = as + sk;
+      const T cv = T(2.0) * a.edadlat / (a.rho[as + sj] + a.rho[as]);
+      Bs = -dt * double(cv) * double(a.vbar[as]);
+      const double hv =
+          ((h[us] + h[as]) - (h[us + sj] + h[as + sj])) / ((h[us] - h[as]) + (h[us + sj] - h[as + sj]));
+      r.gc += 0.5 * hv * Bs;
+    }
+    r.hs = (Aw - A0) + (Bs - B0);
     return r;
   };
-  // gk(k) is read by gc(k) (-1) and gc(k-1) (+1).
-  double gc_km1 = 0.0, gc_k = gcbar(-1);  // gc cotangents of levels k-1 and k (k = -1 here)
-  double gc_kp1 = gcbar(0);
-  for (int k = -1; k <= nz; ++k) {
-    // gk_bar at levels k and k+1 (gk lives on [0, nz]).
-    const double gk_k = (k >= 0) ? (gc_km1 - gc_k) : 0.0;
-    const double gk_kp1 = (k + 1 <= nz) ? (gc_k - gc_kp1) : 0.0;
+  const int kb = -1 + int(blockIdx.y) * kc, ke = min(kb + kc, nz + 1);
+  Lv lm1 = level(kb - 1), l0 = level(kb);
+  for (int k = kb; k < ke; ++k) {
+    const Lv lp1 = level(k + 1);
+    // gk(k) is read by gc(k) (-1) and gc(k-1) (+1); gk lives on [0, nz].
+    const double gk_k = (k >= 0) ? (lm1.gc - l0.gc) : 0.0;
+    const double gk_kp1 = (k + 1 <= nz) ? (l0.gc - lp1.gc) : 0.0;
     const int64_t at = int64_t(k) * sk + col;
-    double r = (A(-1, 0, k) - A(0, 0, k)) + (B(0, -1, k) - B(0, 0, k));
+    double r = l0.hs;
     if (k >= 0) r += double(a.wgtfac[at]) * gk_k;
     if (k + 1 <= nz) r += (1.0 - double(a.wgtfac[at + sk])) * gk_kp1;
     a.ppbar[at] = r;
-    gc_km1 = gc_k;
-    gc_k = gc_kp1;
-    gc_kp1 = gcbar(k + 2);
+    if (interior && k >= 0 && k < nz) {
+      a.uinbar[at] = a.ubar[at];
+      a.vinbar[at] = a.vbar[at];
+    }
+    lm1 = l0;
+    l0 = lp1;
   }
 }
 
@@ -290,11 +449,11 @@ int fastwavesuv_adj_run(const oeb_domain *d, const S3 *wgtfac, const oeb_s3 *ppu
   a.uinbar = const_cast<T *>(p_ub);
   a.vinbar = const_cast<T *>(p_vb);
   a.ppbar = const_cast<double *>(p_pb);
-  const BoxGrid bi{0, 0, int(nx), int(ny)}, bp{0, 0, int(nx + 1), int(ny + 1)};
-  hipLaunchKernelGGL(fastwavesuv_adj_uv<T>, box_blocks(bi, int(nz)), dim3(kTI, kTJ), 0, hipStream_t(stream), a, g,
-                     bi);
-  OEB_TRY(launch_status(K));
-  hipLaunchKernelGGL(fastwavesuv_adj_pp<T>, box_blocks(bp, 1), dim3(kTI, kTJ), 0, hipStream_t(stream), a, g, bp);
+  const BoxGrid bp{0, 0, int(nx + 1), int(ny + 1)};
+  const int kc = kchunk_for(K, sizeof(T), d, 8);
+  const int nchunk = int((nz + 2 + kc - 1) / kc);
+  hipLaunchKernelGGL(fastwavesuv_adj_kernel<T>, box_blocks(bp, nchunk), dim3(kTI, kTJ), 0, hipStream_t(stream), a, g,
+                     bp, kc);
   return launch_status(K);
 }
 

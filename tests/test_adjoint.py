@@ -217,3 +217,28 @@ def test_adjoint_central_differences(name):
     assert checked >= FD_POINTS
     assert flips <= 0.01 * checked
     assert not bad, f"{name}: {len(bad)} points off by more than {FD_TOL:.0e} x max|grad|: {bad[:3]}"
+
+
+@pytest.mark.parametrize("rows", [1, 3, 64])
+@pytest.mark.parametrize("shape", [(40, 36, 5), (67, 9, 3)], ids=lambda s: "x".join(map(str, s)))
+def test_hdiffsa_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
+    """The fused row-march hdiffsa adjoint and the two-pass one (lap_bar
+    through memory), forced through OEB_STRIP_HDIFFSA_ADJ, give the same
+    doubles, at row chunks of one row, of a number that does not divide
+    ny + 4, and of more than ny + 4."""
+    import torch
+
+    L = Layout(*shape, h=3, hk=1)
+    host = generate("hdiffsa", L, "baseline", 41)
+    c = _dev("hdiffsa", L, host, skip=("in",))
+    dom = (L.nx, L.ny, L.nz)
+    x = _t(host["in"])
+    g = torch.Generator(device="cuda").manual_seed(3)
+    ybar = torch.randn(x.shape, generator=g, device="cuda", dtype=x.dtype)
+    print(f"[gpu] hdiffsa adjoint strip vs two-pass {shape} rows={rows}", flush=True)
+    out = []
+    for strip in (0, 1):
+        monkeypatch.setenv("OEB_STRIP_HDIFFSA_ADJ", str(strip))
+        monkeypatch.setenv("OEB_ROWS_HDIFFSA_ADJ", str(rows))
+        out.append(_grads(lambda v: (AG.Hdiffsa.apply(v, c, L, dom),), [x], [ybar])[0])
+    assert torch.equal(out[0], out[1])
