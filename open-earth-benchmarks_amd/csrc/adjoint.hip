@@ -611,6 +611,161 @@ __global__ void __launch_bounds__(kTI *kTJ) hadvuv5th_adj_stage2(HadvAdj<T> a, G
   a.vinbar[at] = rv;
 }
 
+// Fused row-march form, used when one strip spans nx (nx + 12 <= 640, so
+// that the FP64 form keeps its ~150 registers without spilling): lane
+// l owns column i = l - 6 of [-6, nx + 6) of one level; the workgroup walks
+// the output rows p of [j0 - 3, j1 + 3) for the uin_bar / vin_bar rows
+// [j0, j1) of its chunk of [-3, ny + 3).  Per row p it forms the stage-1
+// quantities of hadvuv5th_adj_stage1 at (i, p) (zero outside the interior)
+// and hands them on without memory:
+//  * along i (the x upwind stencils and the averages U, Uv, Vu), through one
+//    LDS row: lane i gathers the contributions of the outputs (i - 3 .. i + 3,
+//    p) to the inputs (i, p), and the 2x2 contributions of gu2 and gv to rows
+//    p + 1 and p - 1;
+//  * along j (the y upwind stencils, V and the rows above and below), by
+//    pushing them into a per-lane queue of the 7 input rows p - 3 .. p + 3 of
+//    its own column.
+// Row p - 3 of the queue is then complete and is written.  Rows of u and v
+// are loaded one row ahead; u(p + 1) and v(p) are published to LDS (3 slots)
+// for the x stencils of rows p and p + 1.  Two barriers per row: one
+// publishes the u, v row, the other the stage-1 row.  The terms are those of
+// the two-pass form; they are summed in another order.
+template <class T>
+__global__ void __launch_bounds__(640) hadvuv5th_adj_strip(HadvAdj<T> a, Geo g, int rows, int nchunk) {
+  extern __shared__ __align__(16) unsigned char oeb_smem[];
+  constexpr T rr = kEarthRadiusRecipAdj<T>;
+  const int L = int(blockDim.x);
+  T *s_u = reinterpret_cast<T *>(oeb_smem);  // 3 x L: u rows
+  T *s_v = s_u + 3 * L;                       // 3 x L: v rows
+  T *s_wux = s_v + 3 * L;                     // stage-1 row: eddlat ob |ua|
+  T *s_wvx = s_wux + L;                       // eddlat ob2 |ua2|
+  T *s_gu = s_wvx + L;
+  T *s_gu2 = s_gu + L;
+  T *s_gv = s_gu2 + L;
+  int *s_br = reinterpret_cast<int *>(s_gv + L);  // bit 0: ua > 0, bit 1: ua2 > 0
+  const int nx = g.nx, ny = g.ny, sj = g.sj;
+  const int cj = int(blockIdx.x) % nchunk, k = int(blockIdx.x) / nchunk;
+  const int j0 = -3 + cj * rows, j1 = min(ny + 3, j0 + rows);
+  const int l = int(threadIdx.x), i = l - 6;
+  const bool col_io = i >= -3 && i < nx + 3;  // uin_bar / vin_bar columns
+  const bool col_out = i >= 0 && i < nx;      // output (stage-1) columns
+  const int64_t plane = int64_t(k) * g.sk;
+  const int64_t c_uv = plane + min(max(i, -3), nx + 2), c_o = plane + min(max(i, 0), nx - 1);
+  auto ldu = [&](int r) -> T { return a.uin[c_uv + int64_t(min(max(r, -3), ny + 2)) * sj]; };
+  auto ldv_ = [&](int r) -> T { return a.vin[c_uv + int64_t(min(max(r, -3), ny + 2)) * sj]; };
+  auto ldub = [&](int r) -> T { return a.ubar[c_o + int64_t(min(max(r, 0), ny - 1)) * sj]; };
+  auto ldvb = [&](int r) -> T { return a.vbar[c_o + int64_t(min(max(r, 0), ny - 1)) * sj]; };
+  const int p0 = j0 - 3, nt = j1 - j0 + 6;  // output rows p0 .. j1 + 2
+  T ur[7], vr[7];                            // rows p - 3 .. p + 3
+#pragma unroll
+  for (int x = 0; x < 7; ++x) {
+    ur[x] = ldu(p0 - 3 + x);
+    vr[x] = ldv_(p0 - 3 + x);
+  }
+  T qu[7], qv[7];  // partial cotangents of input rows p - 3 .. p + 3
+#pragma unroll
+  for (int x = 0; x < 7; ++x) qu[x] = qv[x] = T(0);
+  // Slot 2 holds u(p0) and v(p0 - 1), the previous rows of iteration 0.
+  s_u[2 * L + l] = ur[3];
+  s_v[2 * L + l] = vr[2];
+  T nu = ldu(p0 + 4), nv = ldv_(p0 + 4), ob = ldub(p0), ob2 = ldvb(p0);
+  for (int t = 0; t < nt; ++t) {
+    const int p = p0 + t;
+    const T u_next = nu, v_next = nv, ob_c = ob, ob2_c = ob2;
+    nu = ldu(p + 5);
+    nv = ldv_(p + 5);
+    ob = ldub(p + 1);
+    ob2 = ldvb(p + 1);
+    T *su1 = s_u + (t % 3) * L;  // u row p + 1
+    T *sv0 = s_v + (t % 3) * L;  // v row p
+    su1[l] = ur[4];
+    sv0[l] = vr[3];
+    lds_barrier();
+    // Stage 1 at (i, p): hadvuv5th_adj_stage1, kernels/hadvuv5th.h:183-195.
+    T wux = T(0), wvx = T(0), wuy = T(0), wvy = T(0), gu = T(0), gv = T(0), gu2 = T(0), gv2 = T(0), tgt = T(0);
+    bool px = false, py = false, px2 = false, py2 = false;
+    if (col_out && p >= 0 && p < ny) {
+      const T *u = s_u + ((t + 2) % 3) * L + l;  // u row p
+      const T *u1 = su1 + l;                      // u row p + 1
+      const T *v = sv0 + l;                       // v row p
+      const T *vm = s_v + ((t + 2) % 3) * L + l;  // v row p - 1
+      const T a0 = a.a0[int64_t(p) * a.s0], a1 = a.a1[int64_t(p) * a.s1];
+      const T tg0 = a.t0[int64_t(p) * a.s2], tg1 = a.t1[int64_t(p) * a.s3];
+      const T U = (T(1.0) / T(3.0)) * (u[-1] + u[0] + u[1]);
+      const T Vu = T(0.25) * (v[1] + vm[1] + v[0] + vm[0]);
+      const T ua = a0 * U, va = rr * Vu;
+      px = ua > 0;
+      py = va > 0;
+      const T sx = s5(u, 1, px), sy = s5(&ur[3], 1, py);
+      wux = a.eddlat * ob_c * (px ? ua : -ua);
+      wuy = a.eddlon * ob_c * (py ? va : -va);
+      gu = a.eddlat * ob_c * (px ? sx : -sx) * a0 * (T(1.0) / T(3.0));
+      gv = a.eddlon * ob_c * (py ? sy : -sy) * rr * T(0.25) + ob_c * tg0 * u[0] * T(0.25);
+      tgt = ob_c * tg0 * Vu;
+      const T Uv = T(0.25) * (u[-1] + u[0] + u1[0] + u1[-1]);
+      const T V = T(1.0) / T(3.0) * (vr[2] + vr[3] + vr[4]);
+      const T ua2 = a1 * Uv, va2 = rr * V;
+      px2 = ua2 > 0;
+      py2 = va2 > 0;
+      const T sx2 = s5(v, 1, px2), sy2 = s5(&vr[3], 1, py2);
+      wvx = a.eddlat * ob2_c * (px2 ? ua2 : -ua2);
+      wvy = a.eddlon * ob2_c * (py2 ? va2 : -va2);
+      gu2 = a.eddlat * ob2_c * (px2 ? sx2 : -sx2) * a1 * T(0.25) - ob2_c * tg1 * Uv * T(0.5);
+      gv2 = a.eddlon * ob2_c * (py2 ? sy2 : -sy2) * rr * (T(1.0) / T(3.0));
+    }
+    s_wux[l] = wux;
+    s_wvx[l] = wvx;
+    s_gu[l] = gu;
+    s_gu2[l] = gu2;
+    s_gv[l] = gv;
+    s_br[l] = (px ? 1 : 0) | (px2 ? 2 : 0);
+    lds_barrier();
+    // Along i: row p of the inputs gathers from outputs (i - 3 .. i + 3, p).
+    if (col_io) {
+      T ru = T(0), rv = T(0);
+#pragma unroll
+      for (int kk = -3; kk <= 3; ++kk) {
+        const int br = s_br[l - kk];
+        ru += s_wux[l - kk] * c5<T>((br & 1) != 0, kk);
+        rv += s_wvx[l - kk] * c5<T>((br & 2) != 0, kk);
+      }
+      ru += s_gu[l - 1] + s_gu[l] + s_gu[l + 1];
+      const T g2 = s_gu2[l] + s_gu2[l + 1];  // to (i, p) and (i, p + 1)
+      const T hv = s_gv[l - 1] + s_gv[l];    // to (i, p) and (i, p - 1)
+      qu[3] += ru + g2 + tgt;
+      qu[4] += g2;
+      qv[3] += rv + hv;
+      qv[2] += hv;
+    }
+    // Along j: output (i, p) to inputs (i, p + kk) of its own column.
+#pragma unroll
+    for (int kk = -3; kk <= 3; ++kk) {
+      qu[3 + kk] += wuy * c5<T>(py, kk);
+      qv[3 + kk] += wvy * c5<T>(py2, kk);
+    }
+    qv[2] += gv2;
+    qv[3] += gv2;
+    qv[4] += gv2;
+    // Input row p - 3 has all its contributions.
+    const int q = p - 3;
+    if (col_io && q >= j0 && q < j1) {
+      const int64_t at = plane + i + int64_t(q) * sj;
+      a.uinbar[at] = qu[0];
+      a.vinbar[at] = qv[0];
+    }
+#pragma unroll
+    for (int x = 0; x < 6; ++x) {
+      qu[x] = qu[x + 1];
+      qv[x] = qv[x + 1];
+      ur[x] = ur[x + 1];
+      vr[x] = vr[x + 1];
+    }
+    qu[6] = qv[6] = T(0);
+    ur[6] = u_next;
+    vr[6] = v_next;
+  }
+}
+
 template <class T, class S3, class S1>
 int hadvuv5th_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S1 *acrlat0, const S1 *acrlat1,
                       const S1 *tgrlatda0, const S1 *tgrlatda1, const S3 *uout_bar, const S3 *vout_bar,
@@ -650,6 +805,15 @@ int hadvuv5th_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
   for (int q = 0; q < 8; ++q) *dst[q] = const_cast<T *>(p_s[q]);
   a.uinbar = const_cast<T *>(p_ub);
   a.vinbar = const_cast<T *>(p_vb);
+  if (nx + 12 <= 640 && strip_for(K, true)) {
+    const int lanes = int((nx + 12 + 63) / 64 * 64);
+    const int rows = strip_rows_for(K, default_strip_rows(ny + 6, lanes));
+    const int nchunk = int((ny + 6 + rows - 1) / rows);
+    if (int64_t(nchunk) * nz > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
+    hipLaunchKernelGGL(hadvuv5th_adj_strip<T>, dim3(unsigned(nchunk * nz)), dim3(unsigned(lanes)),
+                       size_t(lanes) * (11 * sizeof(T) + sizeof(int)), hipStream_t(stream), a, g, rows, nchunk);
+    return launch_status(K);
+  }
   const BoxGrid b1{0, 0, int(nx), int(ny)}, b2{-3, -3, int(nx + 6), int(ny + 6)};
   hipLaunchKernelGGL(hadvuv5th_adj_stage1<T>, box_blocks(b1, int(nz)), dim3(kTI, kTJ), 0, hipStream_t(stream), a, g,
                      b1);
