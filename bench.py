@@ -79,7 +79,8 @@ def parse_args(argv=None):
     p.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble block")
     p.add_argument("--members", type=int, default=48, help="members of the ensemble block")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint block")
-    p.add_argument("--adjoint-steps", type=int, default=100, help="steps of the checkpointed adjoint chain")
+    p.add_argument("--adjoint-steps", type=int, default=100,
+                   help="steps of the checkpointed adjoint chain (0: per-stencil adjoint timings only)")
     p.add_argument("--copy-mib", type=int, default=0,
                    help="also time a plain device copy of this many MiB (HBM counter calibration)")
     return p.parse_args(argv)
@@ -427,6 +428,8 @@ def adjoint_block(L, forms, seed, steps: int, segment: int = 10, reps: int = 5):
             del xs, ys, gs, consts
         out["per_stencil"][prec] = per
 
+        if steps <= 0:
+            continue
         chain = AdjointChain(L, prec, seed)
         with torch.no_grad():
             fwd_ms = timed(lambda: chain.run(steps))
@@ -594,7 +597,7 @@ def main(argv=None):
         result["chain"] = chain_block(L, forms, args.seed)
     if not args.no_ensemble and args.members > 0:
         result["ensemble"] = ensemble_block(args.members, args.seed)
-    if not args.no_adjoint and args.adjoint_steps > 0:
+    if not args.no_adjoint:
         result["adjoint"] = adjoint_block(L, forms, args.seed, args.adjoint_steps)
     if copy_line:
         result["device_copy"] = copy_line

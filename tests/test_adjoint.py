@@ -268,3 +268,29 @@ def test_hadvuv5th_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
         out.append(_grads(lambda u, v: AG.Hadvuv5th.apply(u, v, c, L, dom), xs, ybars))
     for a, b in zip(*out):
         assert _relmax(b, a) <= 1e-13
+
+
+@pytest.mark.parametrize("rows", [1, 3, 64])
+@pytest.mark.parametrize("shape", [(40, 36, 5), (67, 9, 3)], ids=lambda s: "x".join(map(str, s)))
+def test_hdiffsmag_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
+    """The fused row-march hdiffsmag adjoint and the two-pass one (gu, gv,
+    wu, wv through memory), forced through OEB_STRIP_HDIFFSMAG_ADJ, give the
+    same doubles, at row chunks of one row, of a number that does not divide
+    ny + 2, and of more than ny + 2."""
+    import torch
+
+    L = Layout(*shape, h=3, hk=1)
+    host = generate("hdiffsmag", L, "baseline", 47)
+    c = _dev("hdiffsmag", L, host, skip=("uin", "vin"))
+    dom = (L.nx, L.ny, L.nz)
+    xs = [_t(host["uin"]), _t(host["vin"])]
+    g = torch.Generator(device="cuda").manual_seed(8)
+    ybars = [torch.randn(x.shape, generator=g, device="cuda", dtype=x.dtype) for x in xs]
+    print(f"[gpu] hdiffsmag adjoint strip vs two-pass {shape} rows={rows}", flush=True)
+    out = []
+    for strip in (0, 1):
+        monkeypatch.setenv("OEB_STRIP_HDIFFSMAG_ADJ", str(strip))
+        monkeypatch.setenv("OEB_ROWS_HDIFFSMAG_ADJ", str(rows))
+        out.append(_grads(lambda u, v: AG.Hdiffsmag.apply(u, v, c, L, dom), xs, ybars))
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
