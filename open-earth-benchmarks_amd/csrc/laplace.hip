@@ -155,7 +155,11 @@ int laplace_run(const oeb_domain *d, const S *in, const S *out, void *stream) {
   as.add("out", p_out, r_out, g, true);
   OEB_TRY(as.check(K));
   constexpr int W = kVecW<T>;
-  if (strip_for(K, true)) {
+  // The row-march form is off by default: in the ten-stencil step (its input
+  // not in the MALL) the tile form measured 1.15x faster at 256x256x80 and
+  // 512x512x80; run back to back on one field (input in the MALL) the strip
+  // form measured 1.1-1.2x faster.  OEB_STRIP_LAPLACE=1 selects it.
+  if (strip_for(K, false)) {
     if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_in, p_out}))
       return laplace_strip_launch<T, W>(p_in, const_cast<T *>(p_out), g, d, hipStream_t(stream));
     return laplace_strip_launch<T, 1>(p_in, const_cast<T *>(p_out), g, d, hipStream_t(stream));
