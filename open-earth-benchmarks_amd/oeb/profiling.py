@@ -97,8 +97,11 @@ def pmc_per_kernel(db: Path, counter: str) -> dict:
 
 
 def stencil_of(kernel: str):
-    """Which stencil a device kernel belongs to (None for other kernels)."""
-    if kernel.startswith("hadvuv_kernel"):
+    """Which stencil an FP64 forward device kernel belongs to (None for the
+    FP32 instantiations, the adjoint kernels and other kernels)."""
+    if "float" in kernel or "_adj" in kernel:
+        return None
+    if kernel.startswith("hadvuv_"):
         return "hadvuv5th" if "<5" in kernel else "hadvuv"
     for name in ("laplace", "hdiffsa", "hdiffsmag", "fastwavesuv", "uvbke", "p_grad_c", "nh_p_grad", "fvtp2d"):
         if kernel.startswith(name + "_"):
