@@ -96,14 +96,31 @@ hipError_t hdiffsmag_batch(const BatchMember *tab, int n, int total, hipStream_t
 template <class T>
 const BatchKind kHdiffsmagBatch{"hdiffsmag", hdiffsmag_batch<T>};
 
-// Row-march form (Strips, no halo lanes): lane l owns the V columns
-// i = i0 + V l .. i + V - 1 of one level and walks the rows [j0, j1) of its
-// chunk.  Rows j - 1, j, j + 1 of u and v are in registers (V wide, with the
-// single neighbours u(i - 1), u(i + V), v(i - 1), v(i + V) of each row), so
-// a row of each input is fetched once per chunk; the neighbour loads hit the
-// lines the adjacent lanes fetch.  Rows are loaded D rows ahead; addresses of
-// lanes and rows outside the strip are clamped into it (never stored).  The
-// expressions are those of hdiffsmag_body, term for term.
+// Row-march form, cut into wave segments.  A wave owns the 64 V columns
+// i = c0 + V ln .. of one level (lane ln holds V of them) and walks the rows
+// [j0, j1) of its chunk.  Rows j - 1, j, j + 1 of u and v are in registers (V
+// wide, with the single neighbours u(i - 1), u(i + V), v(i - 1), v(i + V) of
+// each row), so a row of each input is fetched once per chunk; the neighbour
+// loads hit the lines the adjacent lanes fetch.  Waves share nothing, so four
+// of them make a 256-lane workgroup and the same body runs in the plain launch
+// and in a batched (persistent) one.  Rows are loaded D rows ahead; addresses
+// of lanes and rows outside the segment are clamped into it (never stored).
+// The expressions are those of hdiffsmag_body, term for term.
+// Segment layout in a Tiles: ti = segments per row, tj = row chunks,
+// kchunk = rows per chunk, nchunk = levels, n = workgroups.
+constexpr int kSegLanes = 64, kSegWaves = 4;
+
+inline Tiles make_segments(int64_t nx, int64_t ny, int nz, int v, int rows) {
+  Tiles t;
+  t.ti = int((nx + int64_t(kSegLanes) * v - 1) / (int64_t(kSegLanes) * v));
+  t.kchunk = rows < 1 ? 1 : rows;
+  t.tj = int((ny + t.kchunk - 1) / t.kchunk);
+  t.nchunk = nz;
+  const int64_t segs = int64_t(t.ti) * t.tj * nz;
+  t.n = int((segs + kSegWaves - 1) / kSegWaves);
+  return t;
+}
+
 template <class T, int V>
 struct SmagRow {
   typename Vec<T, V>::type u, v;
@@ -111,17 +128,21 @@ struct SmagRow {
 };
 
 template <class T, int V, int D>
-__global__ void __launch_bounds__(1024) hdiffsmag_strip_kernel(HsArgs<T> a, Geo g, Strips st) {
+__device__ __forceinline__ void hdiffsmag_seg_body(const HsArgs<T> &a, const Geo &g, const Tiles &t, int id) {
   typedef typename Vec<T, V>::type VT;
   typedef SmagRow<T, V> Row;
   constexpr T rr = kEarthRadiusRecip<T>;
+  const int tid = int(threadIdx.x) + int(blockDim.x) * int(threadIdx.y);
+  const int ln = tid & (kSegLanes - 1), seg = id * kSegWaves + tid / kSegLanes;
+  if (seg >= t.ti * t.tj * t.nchunk) return;  // a whole wave; no barriers follow
+  const int si = seg % t.ti, rest = seg / t.ti, cj = rest % t.tj, k = rest / t.tj;
+  const int c0 = si * kSegLanes * V, nl = min(kSegLanes, (g.nx - c0 + V - 1) / V);
+  const int j0 = cj * t.kchunk, j1 = min(g.ny, j0 + t.kchunk), sj = g.sj;
   const T eddlat = a.eddlat, eddlon = a.eddlon, tau_smag = a.tau_smag, weight_smag = a.weight_smag;
-  const StripCoord c = strip_coord(st, int(blockIdx.x), g.nx, g.ny);
-  const int l = int(threadIdx.x), nl = c.w / V, sj = g.sj;
-  const bool out_lane = l < nl;
-  const int64_t col = int64_t(c.k) * g.sk + (c.i0 + V * min(l, nl - 1));
+  const bool out_lane = ln < nl;
+  const int64_t col = int64_t(k) * g.sk + (c0 + V * min(ln, nl - 1));
   auto ldrow = [&](int r) -> Row {
-    const int64_t at = col + int64_t(min(r, c.j1)) * sj;  // u, v rows [j0 - 1, j1]
+    const int64_t at = col + int64_t(min(r, j1)) * sj;  // u, v rows [j0 - 1, j1]
     const T *pu = a.uin + at;
     const T *pv = a.vin + at;
     Row x;
@@ -133,28 +154,28 @@ __global__ void __launch_bounds__(1024) hdiffsmag_strip_kernel(HsArgs<T> a, Geo 
     x.ve = pv[V];
     return x;
   };
-  auto ldmask = [&](int r) -> VT { return ldv_nt<T, V>(a.mask + col + int64_t(min(r, c.j1 - 1)) * sj); };
-  Row rs = ldrow(c.j0 - 1), r0 = ldrow(c.j0);
+  auto ldmask = [&](int r) -> VT { return ldv_nt<T, V>(a.mask + col + int64_t(min(r, j1 - 1)) * sj); };
+  Row rs = ldrow(j0 - 1), r0 = ldrow(j0);
   Row cr[D];  // row j + 1 of iteration t0 + u
   VT cm[D];   // mask row j of iteration t0 + u
 #pragma unroll
   for (int u = 0; u < D; ++u) {
-    cr[u] = ldrow(c.j0 + 1 + u);
-    cm[u] = ldmask(c.j0 + u);
+    cr[u] = ldrow(j0 + 1 + u);
+    cm[u] = ldmask(j0 + u);
   }
-  const int nt = c.j1 - c.j0;
+  const int nt = j1 - j0;
   for (int t0 = 0; t0 < nt; t0 += D) {
     Row nr[D];
     VT nm[D];
 #pragma unroll
     for (int u = 0; u < D; ++u) {
-      nr[u] = ldrow(c.j0 + 1 + D + t0 + u);
-      nm[u] = ldmask(c.j0 + D + t0 + u);
+      nr[u] = ldrow(j0 + 1 + D + t0 + u);
+      nm[u] = ldmask(j0 + D + t0 + u);
     }
 #pragma unroll
     for (int u = 0; u < D; ++u) {
       if (t0 + u < nt) {
-        const int j = c.j0 + t0 + u;
+        const int j = j0 + t0 + u;
         const Row rn = cr[u];
         const VT mk = cm[u];
         const T ac_s = a.acrlat0[int64_t(j - 1) * a.s_ac], ac_c = a.acrlat0[int64_t(j) * a.s_ac],
@@ -203,15 +224,36 @@ __global__ void __launch_bounds__(1024) hdiffsmag_strip_kernel(HsArgs<T> a, Geo 
   }
 }
 
+template <class T, int V, int D>
+__global__ void __launch_bounds__(kSegLanes *kSegWaves) hdiffsmag_seg_kernel(HsArgs<T> a, Geo g, Tiles t) {
+  hdiffsmag_seg_body<T, V, D>(a, g, t, int(blockIdx.x));
+}
+
+template <class T, int V, int D>
+hipError_t hdiffsmag_seg_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
+  return launch_batch<HsArgs<T>, hdiffsmag_seg_body<T, V, D>>(tab, n, total, s, dim3(kSegLanes * kSegWaves));
+}
+template <class T, int V, int D>
+const BatchKind kHdiffsmagSegBatch{"hdiffsmag", hdiffsmag_seg_batch<T, V, D>};
+
+// Plain launch or batch record of the segment form with V columns per lane.
 template <class T, int V>
-int hdiffsmag_strip_launch(const HsArgs<T> &a, const Geo &g, const oeb_domain *d, hipStream_t stream) {
+int hdiffsmag_seg_run(const HsArgs<T> &a, const Geo &g, const oeb_domain *d, hipStream_t stream) {
   const char *K = "hdiffsmag";
-  Strips st = make_strips(d->nx, d->ny, int(d->nz), 0, 1, V);
-  st = make_strips(d->nx, d->ny, int(d->nz), 0, strip_rows_for(K, default_strip_rows(d->ny, st.lanes)), V);
-  OEB_TRY(check_strips(K, st));
-  const int depth = strip_depth_for(K, 1);
-  auto kern = depth <= 1 ? hdiffsmag_strip_kernel<T, V, 1> : hdiffsmag_strip_kernel<T, V, 2>;
-  hipLaunchKernelGGL(kern, dim3(unsigned(st.n)), dim3(unsigned(st.lanes)), 0, stream, a, g, st);
+  // Rows per chunk by the lanes that cover one row (default_strip_rows), so
+  // that narrow rows get short chunks and enough waves.
+  const int row_lanes = make_segments(d->nx, d->ny, int(d->nz), V, 1).ti * kSegLanes;
+  const int rows = strip_rows_for(K, default_strip_rows(d->ny, row_lanes));
+  const Tiles t = make_segments(d->nx, d->ny, int(d->nz), V, rows);
+  if (t.n < 1 || int64_t(t.ti) * t.tj * t.nchunk > (int64_t(1) << 30))
+    return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
+  if (strip_depth_for(K, 1) >= 2) {
+    if (batch_record(kHdiffsmagSegBatch<T, V, 2>, a, g, t, t.n)) return OEB_OK;
+    hipLaunchKernelGGL((hdiffsmag_seg_kernel<T, V, 2>), dim3(t.n), dim3(kSegLanes * kSegWaves), 0, stream, a, g, t);
+  } else {
+    if (batch_record(kHdiffsmagSegBatch<T, V, 1>, a, g, t, t.n)) return OEB_OK;
+    hipLaunchKernelGGL((hdiffsmag_seg_kernel<T, V, 1>), dim3(t.n), dim3(kSegLanes * kSegWaves), 0, stream, a, g, t);
+  }
   return launch_status(K);
 }
 
@@ -253,13 +295,13 @@ int hdiffsmag_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
   const HsArgs<T> a{const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_m, p_vo1, p_vu1, p_to1, p_tu1, p_ac,
                     s_vo, s_vu, s_to, s_tu, s_ac, eddlat, eddlon, tau_smag, weight_smag};
-  if (batch_record(kHdiffsmagBatch<T>, a, g, t, t.n)) return OEB_OK;
   if (strip_for(K, true)) {
     constexpr int W = 16 / int(sizeof(T));
     if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_uo, p_vo, p_ui, p_vi, p_m}))
-      return hdiffsmag_strip_launch<T, W>(a, g, d, hipStream_t(stream));
-    return hdiffsmag_strip_launch<T, 1>(a, g, d, hipStream_t(stream));
+      return hdiffsmag_seg_run<T, W>(a, g, d, hipStream_t(stream));
+    return hdiffsmag_seg_run<T, 1>(a, g, d, hipStream_t(stream));
   }
+  if (batch_record(kHdiffsmagBatch<T>, a, g, t, t.n)) return OEB_OK;
   hipLaunchKernelGGL(hdiffsmag_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }

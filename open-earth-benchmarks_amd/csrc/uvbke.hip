@@ -58,19 +58,18 @@ template <class T>
 constexpr int kVecW = 16 / int(sizeof(T));
 
 template <class T>
-__global__ void __launch_bounds__(kTileI *kTileJ)
-    uvbke_vec_kernel(T *__restrict__ ub, T *__restrict__ vb, const T *__restrict__ uc, const T *__restrict__ vc,
-                     const T *__restrict__ cosa, const T *__restrict__ rsina, double dt5, Geo g, Tiles t) {
+__device__ __forceinline__ void uvbke_vec_body(const UvArgs<T> &a, const Geo &g, const Tiles &t, int id) {
   constexpr int W = kVecW<T>;
-  const TileCoord c = tile_coord(t, W * kTileI, kTileJ, g.nz);
+  const TileCoord c = tile_coord_of(t, id, W * kTileI, kTileJ, g.nz);
   const int i = c.i0 + W * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
+  const double dt5 = a.dt5;
   int64_t at = int64_t(c.k0) * g.sk + (i + j * sj);
   for (int k = c.k0; k < c.k1; ++k, at += g.sk) {
-    const auto us = ldv<T, W>(uc + at - sj), u0 = ldv<T, W>(uc + at), v0 = ldv<T, W>(vc + at);
-    const T vw = vc[at - 1];
-    const auto ca = ldv_nt<T, W>(cosa + at), rs = ldv_nt<T, W>(rsina + at);
+    const auto us = ldv<T, W>(a.uc + at - sj), u0 = ldv<T, W>(a.uc + at), v0 = ldv<T, W>(a.vc + at);
+    const T vw = a.vc[at - 1];
+    const auto ca = ldv_nt<T, W>(a.cosa + at), rs = ldv_nt<T, W>(a.rsina + at);
     typename Vec<T, W>::type ro, rv;
 #pragma unroll
     for (int x = 0; x < W; ++x) {
@@ -79,10 +78,22 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
       ro[x] = T((dt5 * (usum - (vsum * ca[x]))) * rs[x]);
       rv[x] = T((dt5 * (vsum - (usum * ca[x]))) * rs[x]);
     }
-    stv_nt<T, W>(ub + at, ro);
-    stv_nt<T, W>(vb + at, rv);
+    stv_nt<T, W>(a.ub + at, ro);
+    stv_nt<T, W>(a.vb + at, rv);
   }
 }
+
+template <class T>
+__global__ void __launch_bounds__(kTileI *kTileJ) uvbke_vec_kernel(UvArgs<T> a, Geo g, Tiles t) {
+  uvbke_vec_body(a, g, t, int(blockIdx.x));
+}
+
+template <class T>
+hipError_t uvbke_vec_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
+  return launch_batch<UvArgs<T>, uvbke_vec_body<T>>(tab, n, total, s, dim3(kTileI, kTileJ));
+}
+template <class T>
+const BatchKind kUvbkeVecBatch{"uvbke", uvbke_vec_batch<T>};
 
 template <class T, class S>
 int uvbke_run(const oeb_domain *d, const S *ub, const S *vb, const S *uc, const S *vc, const S *cosa,
@@ -110,22 +121,17 @@ int uvbke_run(const oeb_domain *d, const S *ub, const S *vb, const S *uc, const 
   OEB_TRY(as.check(K));
   const int kc = kchunk_for(K, sizeof(T), d, kChunk);
   const UvArgs<T> a{const_cast<T *>(p_ub), const_cast<T *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5};
-  {
-    // Recorded into a batch: the scalar body (same results as the vector one).
-    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-    if (batch_record(kUvbkeBatch<T>, a, g, t, t.n)) return OEB_OK;
-  }
   constexpr int W = kVecW<T>;
   if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_ub, p_vb, p_uc, p_vc, p_ca, p_rs})) {
     const Tiles t = make_tiles(nx, ny, int(nz), W * kTileI, kTileJ, kc);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), W * kTileI, kTileJ));
-    hipLaunchKernelGGL(uvbke_vec_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
-                       const_cast<T *>(p_ub), const_cast<T *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5, g, t);
+    if (batch_record(kUvbkeVecBatch<T>, a, g, t, t.n)) return OEB_OK;
+    hipLaunchKernelGGL(uvbke_vec_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
     return launch_status(K);
   }
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+  if (batch_record(kUvbkeBatch<T>, a, g, t, t.n)) return OEB_OK;
   hipLaunchKernelGGL(uvbke_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }
