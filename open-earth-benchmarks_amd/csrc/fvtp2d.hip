@@ -256,14 +256,22 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   o.fy1 = const_cast<T *>(pfy1);
   o.fy2 = const_cast<T *>(pfy2);
 
-  // Tile shape.  Measured at 256x256x80 and 512x512x80 over 64x4, 64x8,
-  // 64x16, 128x8, 128x16 and 256x8: FP32 is fastest with 128x16 tiles of 512
-  // lanes (halo re-fetch 1.5x lower than 64x8, 1.2-1.3x faster); FP64 with
-  // 64x8 tiles of 256 lanes, whose 31 KB of LDS keep 5 workgroups (20 waves)
-  // on a CU.  Larger FP64 tiles hold fewer waves and ran 1.1-1.45x slower, the
-  // smaller 64x4 re-fetches more halo and ran 1.08x slower.
-  // OEB_TILE_FVTP2D = 1..6 forces one of the measured shapes.
-  switch (env_choice("OEB_TILE_", K, sizeof(T) == 4 ? 3 : 0, 0, 6)) {
+  // Tile shape and lanes, measured in the ten-stencil step at 256x256x80 and
+  // 512x512x80 over 64x4, 64x8, 64x16, 128x8, 128x16 and 256x8 tiles of 256,
+  // 384, 512 and 1024 lanes.  FP64: 64x16 tiles of 512 lanes (52 KB of LDS,
+  // 3 workgroups = 24 waves per CU) are fastest, 1.09x / 1.17x faster than
+  // 64x8 x 256 lanes (20 waves per CU); the four phases wait on global loads
+  // of their operands, and more waves hide that.  1024 lanes hold fewer
+  // workgroups and ran 1.2-1.3x slower.  FP32: 128x16 tiles of 512 lanes
+  // (half the bytes per tile, so less halo for the same LDS) are fastest.
+  // OEB_TILE_FVTP2D = 1..12 forces one of the measured shapes.
+  switch (env_choice("OEB_TILE_", K, sizeof(T) == 4 ? 3 : 8, 0, 12)) {
+    case 10: return fvtp2d_launch<64, 16, 1024, T>(K, a, o, p1, p2, g, d, stream);
+    case 11: return fvtp2d_launch<128, 16, 1024, T>(K, a, o, p1, p2, g, d, stream);
+    case 12: return fvtp2d_launch<128, 8, 1024, T>(K, a, o, p1, p2, g, d, stream);
+    case 7: return fvtp2d_launch<64, 8, 512, T>(K, a, o, p1, p2, g, d, stream);
+    case 8: return fvtp2d_launch<64, 16, 512, T>(K, a, o, p1, p2, g, d, stream);
+    case 9: return fvtp2d_launch<64, 8, 384, T>(K, a, o, p1, p2, g, d, stream);
     case 5: return fvtp2d_launch<64, 4, 256, T>(K, a, o, p1, p2, g, d, stream);
     case 6: return fvtp2d_launch<256, 8, 512, T>(K, a, o, p1, p2, g, d, stream);
     case 1: return fvtp2d_launch<128, 8, 512, T>(K, a, o, p1, p2, g, d, stream);
