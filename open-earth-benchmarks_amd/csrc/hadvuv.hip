@@ -62,7 +62,7 @@ __device__ __forceinline__ T upwind(const T *f, int sj, T ua, T va, T eddlat, T 
 // to the other buffer after the compute.  The loads' latency then overlaps the
 // arithmetic.  Against one buffer per level this measured 1.2-1.25x faster at
 // 256x256x80 and 512x512x80 (5th order).
-constexpr int kTI = 64, kTJ = 16, kNT = 256, kRows = kTJ / (kNT / kTI);
+constexpr int kTI = 64, kTJ = 16, kNT = 256;
 
 template <class T>
 struct HadvArgs {
@@ -74,8 +74,9 @@ struct HadvArgs {
 
 // Logical block `id` of a launch.  Ends with a barrier, so a persistent
 // workgroup may run it again at once (batched launches).
-template <int ORDER, class T>
+template <int ORDER, class T, int kNT = kNT>
 __device__ __forceinline__ void hadvuv_body(const HadvArgs<T> &a, const Geo &g, const Tiles &t, int id) {
+  constexpr int kRows = kTJ / (kNT / kTI);  // rows per lane
   constexpr int R = ORDER == 3 ? 2 : 3;
   constexpr int W = kTI + 2 * R, H = kTJ + 2 * R, kPer = (W * H + kNT - 1) / kNT;
   constexpr T rr = kEarthRadiusRecip<T>;
@@ -167,14 +168,22 @@ __device__ __forceinline__ void hadvuv_body(const HadvArgs<T> &a, const Geo &g, 
   }
 }
 
-template <int ORDER, class T>
-__global__ void __launch_bounds__(kNT) hadvuv_kernel(HadvArgs<T> a, Geo g, Tiles t) {
-  hadvuv_body<ORDER>(a, g, t, int(blockIdx.x));
+// Lanes per 64x16 tile, measured in the ten-stencil step at 256x256x80 and
+// 512x512x80 over 256, 512 and 1024: FP64 is fastest with 1024 (one row per
+// lane; 1.06-1.18x faster than 256), FP32 with 512 (1.05-1.2x faster).  More
+// waves per CU hide the latency of each level's staging.
+template <class T>
+constexpr int kHadvLanes = sizeof(T) == 8 ? 1024 : 512;
+
+template <int ORDER, class T, int NT>
+__global__ void __launch_bounds__(NT) hadvuv_kernel(HadvArgs<T> a, Geo g, Tiles t) {
+  hadvuv_body<ORDER, T, NT>(a, g, t, int(blockIdx.x));
 }
 
 template <int ORDER, class T>
 hipError_t hadvuv_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
-  return launch_batch<HadvArgs<T>, hadvuv_body<ORDER, T>>(tab, n, total, s, dim3(kTI, kNT / kTI));
+  constexpr int NT = kHadvLanes<T>;
+  return launch_batch<HadvArgs<T>, hadvuv_body<ORDER, T, NT>, NT>(tab, n, total, s, dim3(kTI, NT / kTI));
 }
 template <int ORDER, class T>
 const BatchKind kHadvuvBatch{ORDER == 3 ? "hadvuv" : "hadvuv5th", hadvuv_batch<ORDER, T>};
@@ -213,7 +222,14 @@ int hadv_run(const char *K, const oeb_domain *d, const S3 *uout, const S3 *vout,
   const HadvArgs<T> a{const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_a0, p_a1, p_t0, p_t1,
                       s0, s1, s2, s3, eddlat, eddlon};
   if (batch_record(kHadvuvBatch<ORDER, T>, a, g, t, t.n)) return OEB_OK;
-  hipLaunchKernelGGL((hadvuv_kernel<ORDER, T>), dim3(t.n), dim3(kTI, kNT / kTI), 0, hipStream_t(stream), a, g, t);
+  // Lanes per 64x16 tile: 256 (4 rows each), 512 (2) or 1024 (1).
+  const int nt = env_choice("OEB_LANES_", K, kHadvLanes<T>, 256, 1024);
+  if (nt >= 1024)
+    hipLaunchKernelGGL((hadvuv_kernel<ORDER, T, 1024>), dim3(t.n), dim3(kTI, 16), 0, hipStream_t(stream), a, g, t);
+  else if (nt >= 512)
+    hipLaunchKernelGGL((hadvuv_kernel<ORDER, T, 512>), dim3(t.n), dim3(kTI, 8), 0, hipStream_t(stream), a, g, t);
+  else
+    hipLaunchKernelGGL((hadvuv_kernel<ORDER, T, 256>), dim3(t.n), dim3(kTI, 4), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }
 

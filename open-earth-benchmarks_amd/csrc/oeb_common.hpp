@@ -348,8 +348,8 @@ __device__ __forceinline__ int batch_find_lds(const int *firsts, int n, int b) {
 }
 
 // The persistent kernel of a run: Body(args, geo, tiles, logical block id).
-template <class A, void (*Body)(const A &, const Geo &, const Tiles &, int)>
-__global__ void __launch_bounds__(256) batch_kernel(const BatchMember *__restrict__ tab, int n, int total) {
+template <class A, void (*Body)(const A &, const Geo &, const Tiles &, int), int NT = 256>
+__global__ void __launch_bounds__(NT) batch_kernel(const BatchMember *__restrict__ tab, int n, int total) {
   __shared__ int firsts[kBatchLdsMembers];
   const bool cached = n <= kBatchLdsMembers;
   if (cached) {
@@ -367,11 +367,11 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchMember *__restric
 // in [1, 2^30], else kBatchGrid.
 int batch_grid();
 
-template <class A, void (*Body)(const A &, const Geo &, const Tiles &, int)>
+template <class A, void (*Body)(const A &, const Geo &, const Tiles &, int), int NT = 256>
 hipError_t launch_batch(const BatchMember *table, int n, int total, hipStream_t stream, dim3 block) {
   const int cap = batch_grid();
   const int grid = total < cap ? total : cap;
-  hipLaunchKernelGGL((batch_kernel<A, Body>), dim3(grid), block, 0, stream, table, n, total);
+  hipLaunchKernelGGL((batch_kernel<A, Body, NT>), dim3(grid), block, 0, stream, table, n, total);
   return hipGetLastError();
 }
 
