@@ -1418,6 +1418,138 @@ __global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k5(FvAdj<T> a, Geo g, Box
   a.qbar[at] = r;
 }
 
+// Tiled form of the transposed PPM gathers (K1, K2, K5).  A workgroup takes
+// a 64 x 8 tile of one level of the output box.  Per axis it forms, once for
+// every face m whose flux reaches the tile, the six products w_k(m) gbar(m)
+// (ppm_weights) into LDS; each point then adds the six products that name
+// it, in the order ppm_transpose adds them (faces q-2 .. q+3; a face outside
+// its region, or with a zero cotangent, gives 0).  ppm_transpose recomputes
+// the weights of six faces per point instead.  K5 runs the x axis, then the y
+// axis, through the same LDS.
+constexpr int kPX = 64, kPY = 8, kPNT = 256;
+constexpr int kPPts = kPX * kPY / kPNT;  // points per lane
+
+template <class T>
+struct PpmLds {
+  T p[6][kPX * (kPY + 5)];
+};
+
+// Products of the faces of the tile (i0, j0) along AXIS (0: i, 1: j).
+template <int AXIS, class T, class Face, class Gbar>
+__device__ __forceinline__ void ppm_products(PpmLds<T> &L, const T *f, const T *c, int64_t plane, int sj, int i0,
+                                             int j0, T p1, T p2, Face face, Gbar gbar) {
+  constexpr int FW = AXIS == 0 ? kPX + 5 : kPX, FH = AXIS == 0 ? kPY : kPY + 5;
+  const int s = AXIS == 0 ? 1 : sj;
+  for (int e = int(threadIdx.x); e < FW * FH; e += kPNT) {
+    const int r = e / FW, cc = e % FW;
+    const int fi = (AXIS == 0 ? i0 - 2 : i0) + cc, fj = (AXIS == 0 ? j0 : j0 - 2) + r;
+    T pk[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    if (face(fi, fj)) {
+      const int64_t m = plane + fi + int64_t(fj) * sj;
+      const T gb = gbar(m);
+      if (gb != T(0)) {
+        T fv[6], w[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) fv[t] = f[m + int64_t(t - 3) * s];
+        ppm_weights(fv, c[m], p1, p2, w);
+#pragma unroll
+        for (int t = 0; t < 6; ++t) pk[t] = w[t] * gb;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 6; ++t) L.p[t][e] = pk[t];
+  }
+}
+
+// Sum over faces q-2 .. q+3 along AXIS of the products that name the tile
+// point (lx, ly).
+template <int AXIS, class T>
+__device__ __forceinline__ T ppm_gather(const PpmLds<T> &L, int lx, int ly) {
+  constexpr int FW = AXIS == 0 ? kPX + 5 : kPX;
+  T r = T(0);
+#pragma unroll
+  for (int o = -2; o <= 3; ++o) {
+    const int idx = AXIS == 0 ? ly * FW + (lx + o + 2) : (ly + o + 2) * FW + lx;
+    r += L.p[3 - o][idx];
+  }
+  return r;
+}
+
+template <class T, int MODE>
+__global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxGrid b) {
+  __shared__ PpmLds<T> L;
+  const int ntx = (b.ni + kPX - 1) / kPX;
+  const int i0 = b.ilo + int(blockIdx.x % ntx) * kPX, j0 = b.jlo + int(blockIdx.x / ntx) * kPY;
+  const int sj = g.sj;
+  const int64_t plane = int64_t(blockIdx.y) * g.sk;
+  const FvRegions R{g.nx, g.ny};
+  const T p1 = a.p1, p2 = a.p2;
+  auto point = [&](int u, int &lx, int &ly) {
+    const int e = int(threadIdx.x) + u * kPNT;
+    lx = e % kPX;
+    ly = e / kPX;
+    return in_range(i0 + lx, b.ilo, b.ilo + b.ni) && in_range(j0 + ly, b.jlo, b.jlo + b.nj);
+  };
+  if (MODE == 1) {  // K1
+    ppm_products<1>(L, a.q_j, a.cry, plane, sj, i0, j0, p1, p2, [&](int i, int j) { return R.in(i, j); },
+                    [&](int64_t m) { return T(0.5) * a.yfx[m] * a.fy1b[m]; });
+    lds_barrier();
+#pragma unroll
+    for (int u = 0; u < kPPts; ++u) {
+      int lx, ly;
+      if (point(u, lx, ly)) {
+        const int64_t at = plane + (i0 + lx) + int64_t(j0 + ly) * sj;
+        a.w0[at] = a.qjb[at] + ppm_gather<1>(L, lx, ly);
+      }
+    }
+  } else if (MODE == 2) {  // K2
+    ppm_products<0>(L, a.q_i, a.crx, plane, sj, i0, j0, p1, p2, [&](int i, int j) { return R.in(i, j); },
+                    [&](int64_t m) { return T(0.5) * a.xfx[m] * a.fx1b[m]; });
+    lds_barrier();
+#pragma unroll
+    for (int u = 0; u < kPPts; ++u) {
+      int lx, ly;
+      if (point(u, lx, ly)) {
+        const int64_t at = plane + (i0 + lx) + int64_t(j0 + ly) * sj;
+        a.w1[at] = a.qib[at] + ppm_gather<0>(L, lx, ly);
+      }
+    }
+  } else {  // K5
+    ppm_products<0>(L, a.q, a.crx, plane, sj, i0, j0, p1, p2, [&](int i, int j) { return R.fx2(i, j); },
+                    [&](int64_t m) { return a.w2[m]; });
+    lds_barrier();
+    T rx[kPPts];
+#pragma unroll
+    for (int u = 0; u < kPPts; ++u) {
+      int lx, ly;
+      point(u, lx, ly);
+      rx[u] = ppm_gather<0>(L, lx, ly);
+    }
+    lds_barrier();
+    ppm_products<1>(L, a.q, a.cry, plane, sj, i0, j0, p1, p2, [&](int i, int j) { return R.fy2(i, j); },
+                    [&](int64_t m) { return a.w3[m]; });
+    lds_barrier();
+#pragma unroll
+    for (int u = 0; u < kPPts; ++u) {
+      int lx, ly;
+      if (point(u, lx, ly)) {
+        const int i = i0 + lx, j = j0 + ly;
+        const int64_t at = plane + i + int64_t(j) * sj;
+        T r = T(0);
+        if (R.qj(i, j)) r += a.area[at] * (a.w0[at] / a.ra_x[at]);
+        if (R.qi(i, j)) r += a.area[at] * (a.w1[at] / a.ra_y[at]);
+        r += rx[u];
+        r += ppm_gather<1>(L, lx, ly);
+        a.qbar[at] = r;
+      }
+    }
+  }
+}
+
+inline dim3 ptile_blocks(const BoxGrid &b, int nz) {
+  return dim3(unsigned(((b.ni + kPX - 1) / kPX) * ((b.nj + kPY - 1) / kPY)), unsigned(nz));
+}
+
 template <class T, class S>
 int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, const S *ra_x, const S *ra_y,
                    const S *xfx, const S *yfx, const S *area, const S *q_i, const S *q_j, const S *q_i_bar,
@@ -1474,15 +1606,27 @@ int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, 
   const BoxGrid bfx2{0, -3, int(nx + 1), int(ny + 5)}, bfy2{-3, 0, int(nx + 5), int(ny + 1)};
   const BoxGrid bq{-3, -3, int(nx + 6), int(ny + 6)};
   hipStream_t st = hipStream_t(stream);
-  hipLaunchKernelGGL(fvtp2d_adj_k1<T>, box_blocks(bqj, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqj);
-  OEB_TRY(launch_status(K));
-  hipLaunchKernelGGL(fvtp2d_adj_k2<T>, box_blocks(bqi, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqi);
+  // The tiled transposes (fvtp2d_adj_ptile) by default; OEB_TILE_FVTP2D_ADJ=0
+  // selects the per-point gathers.
+  const bool tiled = env_choice("OEB_TILE_", K, 1, 0, 1) != 0;
+  if (tiled) {
+    hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 1>), ptile_blocks(bqj, int(nz)), dim3(kPNT), 0, st, a, g, bqj);
+    OEB_TRY(launch_status(K));
+    hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 2>), ptile_blocks(bqi, int(nz)), dim3(kPNT), 0, st, a, g, bqi);
+  } else {
+    hipLaunchKernelGGL(fvtp2d_adj_k1<T>, box_blocks(bqj, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqj);
+    OEB_TRY(launch_status(K));
+    hipLaunchKernelGGL(fvtp2d_adj_k2<T>, box_blocks(bqi, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqi);
+  }
   OEB_TRY(launch_status(K));
   hipLaunchKernelGGL(fvtp2d_adj_k3<T>, box_blocks(bfx2, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bfx2);
   OEB_TRY(launch_status(K));
   hipLaunchKernelGGL(fvtp2d_adj_k4<T>, box_blocks(bfy2, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bfy2);
   OEB_TRY(launch_status(K));
-  hipLaunchKernelGGL(fvtp2d_adj_k5<T>, box_blocks(bq, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bq);
+  if (tiled)
+    hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 5>), ptile_blocks(bq, int(nz)), dim3(kPNT), 0, st, a, g, bq);
+  else
+    hipLaunchKernelGGL(fvtp2d_adj_k5<T>, box_blocks(bq, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bq);
   return launch_status(K);
 }
 
