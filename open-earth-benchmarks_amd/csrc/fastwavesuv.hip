@@ -101,19 +101,42 @@ hipError_t fastwavesuv_batch(const BatchMember *tab, int n, int total, hipStream
 template <class T>
 const BatchKind kFastwavesuvBatch{"fastwavesuv", fastwavesuv_batch<T>};
 
-// Two adjacent columns (i, i+1) per lane, i even, 16-byte accesses.  The east
-// neighbour of column i is column i+1 of the pair; that of column i+1 is a
-// single load at i+2.  Suffix P: the pair at (i, i+1); Q: the pair at row j+1;
-// X: column i+2 at row j.
-__global__ void __launch_bounds__(kTileI *kTileJ)
-    fastwavesuv_pair_kernel(FwArgs<double> a, Geo g, Tiles t) {
-  const double edadlat = a.edadlat, dt = a.dt;
-  const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
+// Two adjacent columns (i, i+1) per lane, i even: the element-type fields
+// move as pairs (16 bytes in FP64, 8 in FP32), the FP64 ppuv and hhl as
+// 16-byte pairs.  The east neighbour of column i is column i+1 of the pair;
+// that of column i+1 is a single load at i+2.  Suffix P: the pair at (i, i+1);
+// Q: the pair at row j+1; X: column i+2 at row j.
+template <class T>
+struct Pr {
+  T x, y;
+};
+template <class T>
+__device__ __forceinline__ Pr<T> ldp(const T *p) {
+  const typename Vec<T, 2>::type v = ldv<T, 2>(p);
+  return Pr<T>{v[0], v[1]};
+}
+template <class T>
+__device__ __forceinline__ Pr<T> ldp_nt(const T *p) {
+  const typename Vec<T, 2>::type v = ldv_nt<T, 2>(p);
+  return Pr<T>{v[0], v[1]};
+}
+template <class T>
+__device__ __forceinline__ void stp_nt(T *p, T x, T y) {
+  typename Vec<T, 2>::type v;
+  v[0] = x;
+  v[1] = y;
+  stv_nt<T, 2>(p, v);
+}
+
+template <class T>
+__device__ __forceinline__ void fastwavesuv_pair_body(const FwArgs<T> &a, const Geo &g, const Tiles &t, int id) {
+  const T edadlat = a.edadlat, dt = a.dt;
+  const TileCoord c = tile_coord_of(t, id, kPairI, kTileJ, g.nz);
   const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
   const int64_t sk = g.sk;
-  const double fxj = a.fx[int64_t(j) * a.fx_stride];
+  const T fxj = a.fx[int64_t(j) * a.fx_stride];
   int64_t at = int64_t(c.k0) * sk + (i + j * sj);
   const int64_t dn = at - sk;
   d2 ppP = ld2(a.ppuv + at), ppQ = ld2(a.ppuv + at + sj);
@@ -121,7 +144,7 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
   d2 gkP, gkQ;
   double gkX;
   {
-    const d2 wP = ld2(a.wgtfac + at), wQ = ld2(a.wgtfac + at + sj);
+    const Pr<T> wP = ldp(a.wgtfac + at), wQ = ldp(a.wgtfac + at + sj);
     const d2 mP = ld2(a.ppuv + dn), mQ = ld2(a.ppuv + dn + sj);
     gkP = d2{ppgk_of(wP.x, ppP.x, mP.x), ppgk_of(wP.y, ppP.y, mP.y)};
     gkQ = d2{ppgk_of(wQ.x, ppQ.x, mQ.x), ppgk_of(wQ.y, ppQ.y, mQ.y)};
@@ -133,30 +156,42 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
     const int64_t up = at + sk;
     const d2 ppP1 = ld2(a.ppuv + up), ppQ1 = ld2(a.ppuv + up + sj);
     const double ppX1 = a.ppuv[up + 2];
-    const d2 wP = ld2(a.wgtfac + up), wQ = ld2(a.wgtfac + up + sj);
-    const double wX = a.wgtfac[up + 2];
+    const Pr<T> wP = ldp(a.wgtfac + up), wQ = ldp(a.wgtfac + up + sj);
+    const T wX = a.wgtfac[up + 2];
     const d2 gkP1{ppgk_of(wP.x, ppP1.x, ppP.x), ppgk_of(wP.y, ppP1.y, ppP.y)};
     const d2 gkQ1{ppgk_of(wQ.x, ppQ1.x, ppQ.x), ppgk_of(wQ.y, ppQ1.y, ppQ.y)};
     const double gkX1 = ppgk_of(wX, ppX1, ppX);
     const d2 hP1 = ld2(a.hhl + up), hQ1 = ld2(a.hhl + up + sj);
     const double hX1 = a.hhl[up + 2];
-    const d2 rP = ld2(a.rho + at), rQ = ld2(a.rho + at + sj);
-    const double rX = a.rho[at + 2];
-    const d2 ui = ld2_nt(a.uin + at), vi = ld2_nt(a.vin + at);
-    const d2 ut = ld2_nt(a.utens + at), vt = ld2_nt(a.vtens + at);
-    double u0, v0, u1, v1;
+    const Pr<T> rP = ldp(a.rho + at), rQ = ldp(a.rho + at + sj);
+    const T rX = a.rho[at + 2];
+    const Pr<T> ui = ldp_nt(a.uin + at), vi = ldp_nt(a.vin + at);
+    const Pr<T> ut = ldp_nt(a.utens + at), vt = ldp_nt(a.vtens + at);
+    T u0, v0, u1, v1;
     fw_point(ppP.x, ppP.y, ppQ.x, CEN{gkP.x, gkP.y, gkQ.x, gkP1.x, gkP1.y, gkQ1.x},
              CEN{hP.x, hP.y, hQ.x, hP1.x, hP1.y, hQ1.x}, rP.x, rP.y, rQ.x, ui.x, vi.x, ut.x, vt.x, fxj, edadlat, dt,
              u0, v0);
     fw_point(ppP.y, ppX, ppQ.y, CEN{gkP.y, gkX, gkQ.y, gkP1.y, gkX1, gkQ1.y}, CEN{hP.y, hX, hQ.y, hP1.y, hX1, hQ1.y},
              rP.y, rX, rQ.y, ui.y, vi.y, ut.y, vt.y, fxj, edadlat, dt, u1, v1);
-    st2_nt(a.uout + at, u0, u1);
-    st2_nt(a.vout + at, v0, v1);
+    stp_nt(a.uout + at, u0, u1);
+    stp_nt(a.vout + at, v0, v1);
     ppP = ppP1, ppQ = ppQ1, ppX = ppX1;
     gkP = gkP1, gkQ = gkQ1, gkX = gkX1;
     hP = hP1, hQ = hQ1, hX = hX1;
   }
 }
+
+template <class T>
+__global__ void __launch_bounds__(kTileI *kTileJ) fastwavesuv_pair_kernel(FwArgs<T> a, Geo g, Tiles t) {
+  fastwavesuv_pair_body(a, g, t, int(blockIdx.x));
+}
+
+template <class T>
+hipError_t fastwavesuv_pair_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
+  return launch_batch<FwArgs<T>, fastwavesuv_pair_body<T>>(tab, n, total, s, dim3(kTileI, kTileJ));
+}
+template <class T>
+const BatchKind kFastwavesuvPairBatch{"fastwavesuv", fastwavesuv_pair_batch<T>};
 
 template <class T, class S3, class S1>
 int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *uin, const S3 *vin,
@@ -201,23 +236,18 @@ int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S
   a.edadlat = edadlat;
   a.dt = dt;
   const int kc = kchunk_for(K, sizeof(T), d, kChunk);
-  {
-    // Recorded into a batch: the scalar body (same results as the paired one).
-    const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-    if (batch_record(kFastwavesuvBatch<T>, a, g, t, t.n)) return OEB_OK;
-  }
-  if constexpr (sizeof(T) == 8) {
-    if (pairs_for(K, sizeof(T), d, true) && nx % 2 == 0 &&
-        pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.utens, a.vtens, a.wgtfac, a.ppuv, a.hhl, a.rho})) {
-      const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
-      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-      hipLaunchKernelGGL(fastwavesuv_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
-      return launch_status(K);
-    }
+  if (pairs_for(K, sizeof(T), d, true) && nx % 2 == 0 &&
+      pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.utens, a.vtens, a.wgtfac, a.ppuv, a.hhl, a.rho})) {
+    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+    if (batch_record(kFastwavesuvPairBatch<T>, a, g, t, t.n)) return OEB_OK;
+    hipLaunchKernelGGL(fastwavesuv_pair_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g,
+                       t);
+    return launch_status(K);
   }
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
+  if (batch_record(kFastwavesuvBatch<T>, a, g, t, t.n)) return OEB_OK;
   hipLaunchKernelGGL(fastwavesuv_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }
