@@ -183,7 +183,9 @@ class Fvtp2d(torch.autograd.Function):
         f = dict(consts, q=Storage3D(L, q), q_i=Storage3D(L, qi), q_j=Storage3D(L, qj), q_bar=Storage3D(L, qb))
         for n, gr in zip(FVTP2D_OUTPUTS, grads):
             f[f"{n}_bar"] = Storage3D(L, (gr if gr is not None else torch.zeros_like(q)).contiguous())
+        # The adjoint writes every point of a work array that it reads (each
+        # on its output's region), so they need no clearing.
         for n in ("work0", "work1", "work2", "work3"):
-            f[n] = Storage3D(L, torch.zeros_like(q))
+            f[n] = Storage3D(L, torch.empty_like(q))
         _adjoint("fvtp2d", f, _scalars("fvtp2d", scalars), domain, Storage3D(L, q).precision)
         return qb, None, None, None, None
