@@ -21,6 +21,8 @@
 // the tile touches that edge.  So every point is written exactly once.
 // FP32 form: the PPM stages are those of the reference's float build; the flux
 // divergences of q_i and q_j are compensated (flux_div below).
+// A second form, fvtp2d_strip_kernel, marches rows instead of taking tiles (the
+// FP32 default); both give the same bits.
 // Algorithmic traffic: 8 inputs + 6 outputs = 112 B per interior point (FP64),
 // 56 B (FP32).
 #include "oeb_common.hpp"
@@ -192,6 +194,173 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn<T> a, FvOut<T> o, T p1
   }
 }
 
+// One row's loads for the strip form, for lead row L of the march (rows
+// clamped into the ranges the strip reads).
+template <class T>
+struct FvRow {
+  T q;          // q(i, L)
+  T cry, yfx;   // at (i, L-2): fy2 and fy1 of row L-2
+  T ra_y;       // at (i, L-3): q_i of row L-3
+  T crx, xfx;   // at (i, L)
+  T area;       // at (i, L)
+  T ra_x;       // at (i, L-1): q_j of row L-1
+};
+
+// Strip form (Strips, h = 3): a workgroup owns one level, the columns
+// [i0, i0 + w) and the rows [j0, j1); lane l holds column i = i0 - 3 + l.
+// Iteration t handles the lead row L = j0 - 3 + t, from j0 - 3 to j1 + 2:
+//   before the barrier, from the lane's own column (registers):
+//     fy2(L-2) = PPM_j(q rows L-5 .. L, cry)                   (:9-37)
+//     q_i(L-3) from fy2(L-3), fy2(L-2)                          (:39-51)
+//     and it publishes q(L), q_i(L-3), crx(L), xfx(L) and area(L) to LDS;
+//   after it, with the row's i neighbours from LDS:
+//     fx2(L) = PPM_i(q row L, crx), into LDS                   (:84-113)
+//     q_j(L-1) from fx2(L-1) at faces i and i+1                 (:115-127)
+//     fy1(L-3) = PPM_j(q_j rows L-6 .. L-1, cry), and its flux  (:129-162, :173-176)
+//     fx1(L-3) = PPM_i(q_i row L-3, crx), and its flux          (:53-82, :166-170)
+// q and q_i rows take two LDS slots each; crx, xfx, area and fx2 take rings of
+// four, which also hold a lane's own values of rows L-3 .. L-1 until fx1,
+// q_i and q_j use them (so they need no registers).  q_j lags fx2 by a row so
+// that face i+1 of row L-1 is read from the ring after a barrier instead of
+// being computed a second time.  Each slot is rewritten only after the
+// barrier that follows its last read by another lane, so one barrier per row
+// suffices.  The expressions are those of fvtp2d_kernel (ppm_face,
+// flux_div), so the results are the same bits.  Rows are loaded kDepth
+// iterations ahead, a block at a time, with addresses clamped into the box
+// the strip reads (a masked load would make its select wait and drain the
+// prefetch).  A tile re-fetches the halo of every input for every tile it
+// borders; a strip re-fetches only its chunk's five halo rows and one line at
+// each row end.
+template <class T, int kDepth, int kMaxLanes>
+__global__ void __launch_bounds__(kMaxLanes)
+    fvtp2d_strip_kernel(FvIn<T> a, FvOut<T> o, T p1, T p2, Geo g, Strips st) {
+  extern __shared__ __align__(16) unsigned char oeb_smem[];
+  const int NL = st.lanes;
+  T *s_q = reinterpret_cast<T *>(oeb_smem);  // 2 slots: q(L)
+  T *s_qi = s_q + 2 * NL;                     // 2 slots: q_i(L-3)
+  T *s_cx = s_qi + 2 * NL;                    // 4 slots: crx(L)
+  T *s_xf = s_cx + 4 * NL;                    // 4 slots: xfx(L)
+  T *s_ar = s_xf + 4 * NL;                    // 4 slots: area(L)
+  T *s_f2 = s_ar + 4 * NL;                    // 4 slots: fx2(L)
+  const StripCoord c = strip_coord(st, int(blockIdx.x), g.nx, g.ny);
+  const int l = int(threadIdx.x), sj = g.sj, w = c.w, j0 = c.j0, j1 = c.j1;
+  const int gi = c.i0 - 3 + l;
+  const bool first_i = c.i0 == 0, last_i = c.i0 + w >= g.nx;
+  const bool first_j = j0 == 0, last_j = j1 >= g.ny;
+  const bool y_lane = l < w + 5;            // fy2, q_i on [i0 - 3, i0 + w + 2)
+  const bool x_lane = l >= 3 && l < w + 3;  // q_j, fx1, fy1 on [i0, i0 + w)
+  const bool f_lane = l >= 3 && l < w + 4;  // fx2 on faces [i0, i0 + w]
+  // Extended-region points: a strip owns its own columns and rows, plus what
+  // lies beyond the domain on the side of a first or last strip / chunk.
+  const bool own_col = (gi >= c.i0 || first_i) && (gi < c.i0 + w || last_i);
+  auto own_row = [&](int r) { return (r >= j0 || first_j) && (r < j1 || last_j); };
+  const int64_t plane = int64_t(c.k) * g.sk;
+  const int64_t col = plane + gi;
+  const int64_t cq = plane + (c.i0 - 3 + min(l, w + 5));          // q lanes [0, w + 6)
+  const int64_t cy = plane + (c.i0 - 3 + min(l, w + 4));          // y lanes [0, w + 5)
+  const int64_t cx = plane + (c.i0 + min(max(l - 3, 0), w));      // faces [i0, i0 + w]
+  const int64_t cr = plane + (c.i0 + min(max(l - 3, 0), w - 1));  // x lanes [3, w + 3)
+  auto ld = [&](int L) -> FvRow<T> {
+    FvRow<T> r;
+    const int64_t rq = int64_t(min(L, j1 + 2)) * sj;               // q rows [j0 - 3, j1 + 2]
+    const int64_t r2 = int64_t(min(max(L - 2, j0), j1)) * sj;      // rows [j0, j1]
+    const int64_t r3 = int64_t(min(max(L - 3, j0), j1 - 1)) * sj;  // rows [j0, j1)
+    const int64_t rx = int64_t(min(L, j1 + 1)) * sj;               // rows [j0 - 3, j1 + 1]
+    const int64_t r1 = int64_t(min(max(L - 1, j0 - 3), j1 + 1)) * sj;
+    r.q = a.q[cq + rq];
+    r.cry = a.cry[cy + r2];
+    r.yfx = a.yfx[cy + r2];
+    r.ra_y = ld_nt(a.ra_y + cy + r3);
+    r.crx = a.crx[cx + rx];
+    r.xfx = a.xfx[cx + rx];
+    r.area = a.area[cy + rx];
+    r.ra_x = ld_nt(a.ra_x + cr + r1);
+    return r;
+  };
+  FvRow<T> cur[kDepth];
+#pragma unroll
+  for (int u = 0; u < kDepth; ++u) cur[u] = ld(j0 - 3 + u);
+  // Column windows: q of rows L-5 .. L-1, q_j of rows L-6 .. L-2; fy2, yfx and
+  // cry of row L-3.
+  T q5 = T(0), q4 = T(0), q3 = T(0), q2 = T(0), q1 = T(0);
+  T j6 = T(0), j5 = T(0), j4 = T(0), j3 = T(0), j2 = T(0);
+  T fy2p = T(0), yfxp = T(0), cryp = T(0);
+  const int nt = j1 - j0 + 6;
+  for (int t0 = 0; t0 < nt; t0 += kDepth) {
+    FvRow<T> nxt[kDepth];
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u) nxt[u] = ld(j0 - 3 + t0 + kDepth + u);
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u) {
+      const int t = t0 + u;
+      if (t < nt) {  // uniform across the workgroup
+        const int L = j0 - 3 + t;
+        const FvRow<T> &r = cur[u];
+        const T q0 = r.q;
+        const int s2 = (t & 1) * NL + l;        // this row's slot of s_q, s_qi
+        const int s4 = (t & 3) * NL + l;        // this row's slot of the rings
+        const int s4m1 = ((t + 3) & 3) * NL + l;  // row L-1's slot of the rings
+        const int s4m3 = ((t + 1) & 3) * NL + l;  // row L-3's slot of the rings
+        // fy2(L-2) and q_i(L-3) from the column.
+        T fy2n = T(0), qi = T(0);
+        if (L - 2 >= j0 && y_lane) {
+          fy2n = ppm_face(q5, q4, q3, q2, q1, q0, r.cry, p1, p2);
+          if (own_col && own_row(L - 2)) st_nt(o.fy2 + col + int64_t(L - 2) * sj, fy2n);
+          if (L - 3 >= j0) {
+            qi = flux_div(q3, s_ar[s4m3], yfxp, fy2p, r.yfx, fy2n, r.ra_y);
+            if (own_col) st_nt(o.q_i + col + int64_t(L - 3) * sj, qi);
+          }
+        }
+        s_q[s2] = q0;
+        s_qi[s2] = qi;
+        s_cx[s4] = r.crx;
+        s_xf[s4] = r.xfx;
+        s_ar[s4] = r.area;
+        lds_barrier();
+        // fx2(L) from the q row; q_j(L-1) from fx2(L-1) at faces i and i+1.
+        if (L <= j1 + 1 && f_lane) {
+          const T *sq = s_q + s2;
+          const T fx2 = ppm_face(sq[-3], sq[-2], sq[-1], sq[0], sq[1], sq[2], r.crx, p1, p2);
+          s_f2[s4] = fx2;
+          if (own_row(L) && (l < w + 3 || last_i)) st_nt(o.fx2 + col + int64_t(L) * sj, fx2);
+        }
+        T qj = T(0);
+        if (L - 1 >= j0 - 3 && L - 1 <= j1 + 1 && x_lane) {
+          qj = flux_div(q1, s_ar[s4m1], s_xf[s4m1], s_f2[s4m1], s_xf[s4m1 + 1], s_f2[s4m1 + 1], r.ra_x);
+          if (own_row(L - 1)) st_nt(o.q_j + col + int64_t(L - 1) * sj, qj);
+        }
+        if (L - 3 >= j0 && x_lane) {
+          const T fy1 = ppm_face(j6, j5, j4, j3, j2, qj, cryp, p1, p2);
+          st_nt(o.fy1 + col + int64_t(L - 3) * sj, ((T(0.5) * (fy1 + fy2p)) * yfxp));
+        }
+        if (L - 3 >= j0 && x_lane) {
+          const T *si = s_qi + s2;
+          const T fx1 = ppm_face(si[-3], si[-2], si[-1], si[0], si[1], si[2], s_cx[s4m3], p1, p2);
+          st_nt(o.fx1 + col + int64_t(L - 3) * sj, ((T(0.5) * (fx1 + s_f2[s4m3])) * s_xf[s4m3]));
+        }
+        q5 = q4, q4 = q3, q3 = q2, q2 = q1, q1 = q0;
+        j6 = j5, j5 = j4, j4 = j3, j3 = j2, j2 = qj;
+        fy2p = fy2n, yfxp = r.yfx, cryp = r.cry;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kDepth; ++u) cur[u] = nxt[u];
+  }
+}
+
+template <class T, int kDepth>
+int fvtp2d_strip_launch(const char *K, const FvIn<T> &a, const FvOut<T> &o, T p1, T p2, const Geo &g,
+                        const Strips &st, void *stream) {
+  const size_t lds = size_t(20) * st.lanes * sizeof(T);
+  if (st.lanes <= 512)
+    hipLaunchKernelGGL((fvtp2d_strip_kernel<T, kDepth, 512>), dim3(unsigned(st.n)), dim3(unsigned(st.lanes)), lds,
+                       hipStream_t(stream), a, o, p1, p2, g, st);
+  else
+    hipLaunchKernelGGL((fvtp2d_strip_kernel<T, kDepth, 1024>), dim3(unsigned(st.n)), dim3(unsigned(st.lanes)), lds,
+                       hipStream_t(stream), a, o, p1, p2, g, st);
+  return launch_status(K);
+}
+
 template <int TI, int TJ, int NT, class T>
 int fvtp2d_launch(const char *K, const FvIn<T> &a, const FvOut<T> &o, T p1, T p2, const Geo &g,
                   const oeb_domain *d, void *stream) {
@@ -255,6 +424,25 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   o.fx2 = const_cast<T *>(pfx2);
   o.fy1 = const_cast<T *>(pfy1);
   o.fy2 = const_cast<T *>(pfy2);
+
+  // Strips by default in FP32, tiles in FP64; OEB_STRIP_FVTP2D forces either.
+  // Measured in the ten-stencil step at 256x256x80 and 512x512x80: FP32
+  // strips of 320 lanes take 93 / 318 us against 118 / 396 us for the best
+  // tiles (1.25x).  FP64 strips read 1.5x fewer bytes than the tiles
+  // (rocprofv3 FETCH_SIZE at 256x256x80: 445 MB against 671 MB), but at 109
+  // VGPRs a SIMD holds only 4 of their waves, each row a chain of dependent
+  // FP64 operations between two barriers: 163 / 596 us at best (256 lanes,
+  // 32 rows) against 163 / 572 us for the tiles, which run at 5.6 TB/s of
+  // fetched plus written bytes.
+  if (strip_for(K, sizeof(T) == 4)) {
+    // Strips of at most 512 lanes (OEB_LANES_FVTP2D: 64 .. 1024).
+    const int max_lanes = env_choice("OEB_LANES_", K, 512, 64, 1024);
+    Strips st = make_strips(nx, ny, int(nz), 3, 1, 1, max_lanes);
+    st = make_strips(nx, ny, int(nz), 3, strip_rows_for(K, default_strip_rows(ny, st.lanes)), 1, max_lanes);
+    OEB_TRY(check_strips(K, st));
+    if (strip_depth_for(K, 1) >= 2) return fvtp2d_strip_launch<T, 2>(K, a, o, p1, p2, g, st, stream);
+    return fvtp2d_strip_launch<T, 1>(K, a, o, p1, p2, g, st, stream);
+  }
 
   // Tile shape and lanes, measured in the ten-stencil step at 256x256x80 and
   // 512x512x80 over 64x4, 64x8, 64x16, 128x8, 128x16 and 256x8 tiles of 256,

@@ -208,13 +208,13 @@ struct Strips {
   int n;      // workgroups: ns * nj * nz
 };
 
-// Strips of at most 1024 lanes of v columns (w a multiple of v), h halo
-// lanes each side.
-inline Strips make_strips(int64_t nx, int64_t ny, int nz, int h, int rows, int v = 1) {
+// Strips of at most max_lanes (<= 1024) lanes of v columns (w a multiple of
+// v), h halo lanes each side.
+inline Strips make_strips(int64_t nx, int64_t ny, int nz, int h, int rows, int v = 1, int max_lanes = 1024) {
   Strips s;
   s.h = h;
   s.v = v < 1 ? 1 : v;
-  const int64_t maxw = int64_t(s.v) * (1024 - 2 * h);
+  const int64_t maxw = int64_t(s.v) * (max_lanes - 2 * h);
   s.ns = int((nx + maxw - 1) / maxw);
   s.w = int(((nx + s.ns - 1) / s.ns + s.v - 1) / s.v * s.v);
   s.lanes = ((s.w / s.v + 2 * h + 63) / 64) * 64;

@@ -92,3 +92,34 @@ def test_f32_vector_path_switch(name, pairs, nx, monkeypatch):
     L, host = inputs(name, nx, 36, 7, "baseline", seed=9)
     print(f"[gpu] f32 {name} pairs={pairs} nx={nx}", flush=True)
     _check_exact(name, L, cast(name, host, "f32"))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rows", [1, 5, 64])
+@pytest.mark.parametrize("strip", [0, 1])
+@pytest.mark.parametrize("shape", [(64, 36, 3), (1030, 9, 2)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("name", ["hdiffsa", "hdiffsmag", "fvtp2d"])
+def test_f32_strip_switches(name, shape, strip, rows, monkeypatch):
+    """The FP32 row-march (strip) and tile paths, forced through
+    OEB_STRIP_<K>, with row chunks of one row, of a number that does not
+    divide ny, and of more than ny, agree bit for bit with the FP32 oracle."""
+    monkeypatch.setenv(f"OEB_STRIP_{name.upper()}", str(strip))
+    monkeypatch.setenv(f"OEB_ROWS_{name.upper()}", str(rows))
+    L, host = inputs(name, *shape, "baseline", seed=14)
+    print(f"[gpu] f32 {name} strip={strip} rows={rows} {shape}", flush=True)
+    _check_exact(name, L, cast(name, host, "f32"))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("depth", [1, 2])
+@pytest.mark.parametrize("lanes", [64, 128, 512, 1024])
+def test_f32_fvtp2d_strip_shapes(lanes, depth, monkeypatch):
+    """fvtp2d's FP32 strips of at most 64 .. 1024 lanes (one to five strips
+    across nx = 300), with one and two rows loaded ahead, agree bit for bit
+    with the FP32 oracle."""
+    monkeypatch.setenv("OEB_STRIP_FVTP2D", "1")
+    monkeypatch.setenv("OEB_LANES_FVTP2D", str(lanes))
+    monkeypatch.setenv("OEB_DEPTH_FVTP2D", str(depth))
+    L, host = inputs("fvtp2d", 300, 23, 2, "stress", seed=15)
+    print(f"[gpu] f32 fvtp2d strip lanes={lanes} depth={depth}", flush=True)
+    _check_exact("fvtp2d", L, cast("fvtp2d", host, "f32"))

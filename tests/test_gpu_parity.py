@@ -100,7 +100,7 @@ def test_path_switches(name, pairs, kchunk, monkeypatch):
     _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
 
 
-STRIPPED = ["laplace", "hdiffsa", "hdiffsmag"]
+STRIPPED = ["laplace", "hdiffsa", "hdiffsmag", "fvtp2d"]
 
 
 @pytest.mark.gpu
@@ -118,3 +118,19 @@ def test_strip_switches(name, shape, strip, rows, monkeypatch):
     L, host = inputs(name, *shape, "baseline", seed=13)
     print(f"[gpu] {name} strip={strip} rows={rows} {shape}", flush=True)
     _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("depth", [1, 2])
+@pytest.mark.parametrize("lanes", [64, 128, 512, 1024])
+@pytest.mark.parametrize("family", ["baseline", "stress"])
+def test_fvtp2d_strip_shapes(family, lanes, depth, monkeypatch):
+    """fvtp2d's strips of at most 64 .. 1024 lanes (one to five strips across
+    nx = 300), with one and two rows loaded ahead, agree bit for bit with the
+    oracle."""
+    monkeypatch.setenv("OEB_STRIP_FVTP2D", "1")
+    monkeypatch.setenv("OEB_LANES_FVTP2D", str(lanes))
+    monkeypatch.setenv("OEB_DEPTH_FVTP2D", str(depth))
+    L, host = inputs("fvtp2d", 300, 23, 2, family, seed=15)
+    print(f"[gpu] fvtp2d strip {family} lanes={lanes} depth={depth}", flush=True)
+    _check("fvtp2d", L, host, run_gpu("fvtp2d", copy(host), L), require_exact=True)
