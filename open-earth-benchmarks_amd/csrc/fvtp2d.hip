@@ -433,7 +433,11 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   // VGPRs a SIMD holds only 4 of their waves, each row a chain of dependent
   // FP64 operations between two barriers: 163 / 596 us at best (256 lanes,
   // 32 rows) against 163 / 572 us for the tiles, which run at 5.6 TB/s of
-  // fetched plus written bytes.
+  // fetched plus written bytes.  A third form, 64-column bands marching down
+  // chunks of rows in 16-row blocks that hand their shared rows on in
+  // registers (no j-halo re-fetched or recomputed), fetched 599 MB and took
+  // 164 / 570 us: the tiles' extra bytes are mostly the i-halo (a 64-wide FP64
+  // row with 3 halo points each side spans 6 cache lines instead of 4).
   if (strip_for(K, sizeof(T) == 4)) {
     // Strips of at most 512 lanes (OEB_LANES_FVTP2D: 64 .. 1024).
     const int max_lanes = env_choice("OEB_LANES_", K, 512, 64, 1024);

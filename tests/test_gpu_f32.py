@@ -123,3 +123,4 @@ def test_f32_fvtp2d_strip_shapes(lanes, depth, monkeypatch):
     L, host = inputs("fvtp2d", 300, 23, 2, "stress", seed=15)
     print(f"[gpu] f32 fvtp2d strip lanes={lanes} depth={depth}", flush=True)
     _check_exact("fvtp2d", L, cast("fvtp2d", host, "f32"))
+
