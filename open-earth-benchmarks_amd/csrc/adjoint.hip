@@ -142,7 +142,10 @@ __global__ void __launch_bounds__(kTI *kTJ) hdiffsa_adj_in(HsaAdj<T> a, Geo g, B
 // in t + 1).  Values outside their boxes are exact zeros, as in the two-pass
 // form; the arithmetic is that of hdiffsa_adj_lap and hdiffsa_adj_in, term
 // for term.  Loads run two rows ahead, with addresses clamped into the
-// arrays' boxes; a clamped value is never used.
+// arrays' boxes; a clamped value is never used.  crlato and crlatu of the
+// chunk's rows are copied to LDS tables first: loaded where they are used,
+// under lane conditions, they would be vector loads whose waits also wait for
+// the rows loaded ahead.
 template <class T>
 __global__ void __launch_bounds__(1024) hdiffsa_adj_strip(HsaAdj<T> a, Geo g, int rows, int nchunk) {
   extern __shared__ __align__(16) unsigned char oeb_smem[];
@@ -151,10 +154,18 @@ __global__ void __launch_bounds__(1024) hdiffsa_adj_strip(HsaAdj<T> a, Geo g, in
   T *s_mo = s_in + 3 * L;                      // 2 x L
   T *s_lap = s_mo + 2 * L;                     // 2 x L
   T *s_lb = s_lap + 2 * L;                     // 2 x L
+  T *s_lo = s_lb + 2 * L;                      // rows + 7: crlato of rows [j0 - 5, j1 + 2)
+  T *s_lu = s_lo + rows + 7;                   // crlatu alike
   const int nx = g.nx, ny = g.ny, sj = g.sj;
   const int cj = int(blockIdx.x) % nchunk, k = int(blockIdx.x) / nchunk;
   const int j0 = -2 + cj * rows, j1 = min(ny + 2, j0 + rows);
   const int l = int(threadIdx.x), i = l - 3;
+  const int tb = j0 - 5;  // first row of the tables; read after the first barrier
+  for (int e = l; e < j1 - j0 + 7; e += L) {
+    const int64_t row = min(max(tb + e, -1), ny);  // crlato, crlatu live on [-1, ny + 1)
+    s_lo[e] = a.lo[row * a.slo];
+    s_lu[e] = a.lu[row * a.slu];
+  }
   const bool col_in = i >= -2 && i < nx + 2;   // in_bar columns
   const bool col_lap = i >= -1 && i < nx + 1;  // lap, lap_bar columns
   const bool col_int = i >= 0 && i < nx;       // interior columns
@@ -163,8 +174,8 @@ __global__ void __launch_bounds__(1024) hdiffsa_adj_strip(HsaAdj<T> a, Geo g, in
   auto ld_in = [&](int r) -> T { return a.in[c_in + int64_t(min(max(r, -2), ny + 1)) * sj]; };
   auto ld_mk = [&](int r) -> T { return a.mask[c_mo + int64_t(min(max(r, 0), ny - 1)) * sj]; };
   auto ld_ob = [&](int r) -> T { return a.obar[c_mo + int64_t(min(max(r, 0), ny - 1)) * sj]; };
-  auto lo = [&](int jj) { return a.lo[int64_t(jj) * a.slo]; };
-  auto lu = [&](int jj) { return a.lu[int64_t(jj) * a.slu]; };
+  auto lo = [&](int jj) { return s_lo[jj - tb]; };
+  auto lu = [&](int jj) { return s_lu[jj - tb]; };
   // mask x out_bar at (i, r), 0 outside the interior (hdiffsa_adj_lap's mo).
   auto mo = [&](int r, T mk, T ob) { return (col_int && r >= 0 && r < ny) ? mk * ob : T(0); };
   constexpr int D = 2;
@@ -297,11 +308,13 @@ int hdiffsa_adj_run(const oeb_domain *d, const S3 *in, const S3 *mask, const S1 
   if (nx + 6 <= 1024 && strip_for(K, true)) {
     const int lanes = int((nx + 6 + 63) / 64 * 64);
     // Row chunks measured at 256x256x80: 32 rows fastest in FP64, 16 in FP32.
-    const int rows = strip_rows_for(K, sizeof(T) == 8 ? 32 : 16);
+    // At most 256 rows, which bounds the crlato / crlatu tables in LDS.
+    const int rows = std::min(strip_rows_for(K, sizeof(T) == 8 ? 32 : 16), 256);
     const int nchunk = int((ny + 4 + rows - 1) / rows);
     if (int64_t(nchunk) * nz > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
     hipLaunchKernelGGL(hdiffsa_adj_strip<T>, dim3(unsigned(nchunk * nz)), dim3(unsigned(lanes)),
-                       size_t(9) * lanes * sizeof(T), hipStream_t(stream), a, g, rows, nchunk);
+                       (size_t(9) * lanes + 2 * size_t(rows + 7)) * sizeof(T), hipStream_t(stream), a, g, rows,
+                       nchunk);
     return launch_status(K);
   }
   const BoxGrid b1{-1, -1, int(nx + 2), int(ny + 2)}, b2{-2, -2, int(nx + 4), int(ny + 4)};
@@ -997,7 +1010,10 @@ __global__ void __launch_bounds__(kTI *kTJ) hdiffsmag_adj_stage2(HsmAdj<T> a, Ge
 // read in later ones; the j neighbours are rows of the lane's own column in
 // registers.  One barrier per row.  The expressions are those of the
 // two-pass form, term for term.  Loads run D rows ahead, with addresses
-// clamped into the arrays' boxes; a clamped value is never used.
+// clamped into the arrays' boxes; a clamped value is never used.  The
+// chunk's rows of the five 1-D arrays are copied to LDS tables first (as in
+// hdiffsa_adj_strip: loads under lane conditions would wait for the rows
+// loaded ahead).
 template <class T, int D>
 __global__ void __launch_bounds__(1024) hdiffsmag_adj_strip(HsmAdj<T> a, Geo g, int rows, int nchunk) {
   extern __shared__ __align__(16) unsigned char oeb_smem[];
@@ -1011,10 +1027,30 @@ __global__ void __launch_bounds__(1024) hdiffsmag_adj_strip(HsmAdj<T> a, Geo g, 
   T *s_wv = s_wu + 3 * L;
   T *s_tb = s_wv + 3 * L;  // 2 x L each: T_bar, S_bar rows
   T *s_sb = s_tb + 2 * L;
+  const int nt1 = rows + 9;  // table rows [j0 - 6, j1 + 3)
+  T *s_ac = s_sb + 2 * L;     // acrlat0
+  T *s_lo = s_ac + nt1;       // crlato
+  T *s_lu = s_lo + nt1;       // crlatu
+  T *s_vo = s_lu + nt1;       // crlavo
+  T *s_vu = s_vo + nt1;       // crlavu
   const int nx = g.nx, ny = g.ny, sj = g.sj;
   const int cj = int(blockIdx.x) % nchunk, k = int(blockIdx.x) / nchunk;
   const int j0 = -1 + cj * rows, j1 = min(ny + 1, j0 + rows);
   const int l = int(threadIdx.x), i = l - 2;
+  const int tb = j0 - 6;  // first row of the tables; read after the first barrier
+  for (int e = l; e < j1 - j0 + 9; e += L) {
+    const int64_t ra = min(max(tb + e, -1), ny);     // acrlat0 lives on [-1, ny + 1)
+    const int64_t rl = min(max(tb + e, 0), ny - 1);  // the others on [0, ny)
+    s_ac[e] = a.ac[ra * a.sac];
+    s_lo[e] = a.lo[rl * a.slo];
+    s_lu[e] = a.lu[rl * a.slu];
+    s_vo[e] = a.vo[rl * a.svo];
+    s_vu[e] = a.vu[rl * a.svu];
+  }
+  auto lo = [&](int jj) { return s_lo[jj - tb]; };
+  auto lu = [&](int jj) { return s_lu[jj - tb]; };
+  auto vo = [&](int jj) { return s_vo[jj - tb]; };
+  auto vu = [&](int jj) { return s_vu[jj - tb]; };
   const bool col_uv = i >= -1 && i < nx + 1;  // uin_bar, vin_bar columns
   const bool col_int = i >= 0 && i < nx;
   const bool col_t = i >= 0 && i <= nx;    // T with a possibly non-zero weight
@@ -1025,7 +1061,7 @@ __global__ void __launch_bounds__(1024) hdiffsmag_adj_strip(HsmAdj<T> a, Geo g, 
   auto ldu = [&](int r) -> T { return a.uin[c_uv + int64_t(min(max(r, -1), ny)) * sj]; };
   auto ldv_ = [&](int r) -> T { return a.vin[c_uv + int64_t(min(max(r, -1), ny)) * sj]; };
   auto ldo = [&](const T *f, int r) -> T { return f[c_o + int64_t(min(max(r, 0), ny - 1)) * sj]; };
-  auto acr = [&](int jj) { return a.ac[int64_t(jj) * a.sac]; };
+  auto acr = [&](int jj) { return s_ac[jj - tb]; };
   const int r0 = j0 - 2, nt = j1 - j0 + 5;  // front rows r0 .. j1 + 2
   // Own-column rows (index: row relative to the front row r).
   T u_m2 = ldu(r0 - 2), u_m1 = ldu(r0 - 1), v_m2 = ldv_(r0 - 2), v_m1 = ldv_(r0 - 1);
@@ -1066,10 +1102,8 @@ __global__ void __launch_bounds__(1024) hdiffsmag_adj_strip(HsmAdj<T> a, Geo g, 
       const bool pass_u = (T(0) < su) && (su < T(0.5)), pass_v = (T(0) < sv) && (sv < T(0.5));
       const T smag_u = pass_u ? su : ((T(0) < su) ? T(0.5) : T(0));
       const T smag_v = pass_v ? sv : ((T(0) < sv) ? T(0.5) : T(0));
-      const T lapu = U1[1] + U1[-1] - T(2.0) * u_m1 + a.lo[int64_t(p) * a.slo] * (u_0 - u_m1) +
-                     a.lu[int64_t(p) * a.slu] * (u_m2 - u_m1);
-      const T lapv = V1[1] + V1[-1] - T(2.0) * v_m1 + a.vo[int64_t(p) * a.svo] * (v_0 - v_m1) +
-                     a.vu[int64_t(p) * a.svu] * (v_m2 - v_m1);
+      const T lapu = U1[1] + U1[-1] - T(2.0) * u_m1 + lo(p) * (u_0 - u_m1) + lu(p) * (u_m2 - u_m1);
+      const T lapv = V1[1] + V1[-1] - T(2.0) * v_m1 + vo(p) * (v_0 - v_m1) + vu(p) * (v_m2 - v_m1);
       gu_m1 = pass_u ? ub_m1 * lapu * a.tau * T(0.5) / sqrt(qu) : T(0);
       gv_m1 = pass_v ? vb_m1 * lapv * a.tau * T(0.5) / sqrt(qv) : T(0);
       wu_m1 = smag_u * ub_m1;
@@ -1112,18 +1146,18 @@ __global__ void __launch_bounds__(1024) hdiffsmag_adj_strip(HsmAdj<T> a, Geo g, 
       ru += sb_m4 * eddlat * rr - sq * eddlat * rr;
       rv += SB[-1] * ac * eddlon - sq * ac * eddlon;
       if (in0) {
-        ru += wu_m3 * (T(-2.0) - a.lo[int64_t(q) * a.slo] - a.lu[int64_t(q) * a.slu]);
-        rv += wv_m3 * (T(-2.0) - a.vo[int64_t(q) * a.svo] - a.vu[int64_t(q) * a.svu]);
+        ru += wu_m3 * (T(-2.0) - lo(q) - lu(q));
+        rv += wv_m3 * (T(-2.0) - vo(q) - vu(q));
       }
       ru += (i - 1 >= 0 && i - 1 < nx ? WU[-1] : T(0)) + (i + 1 >= 0 && i + 1 < nx ? WU[1] : T(0));
       rv += (i - 1 >= 0 && i - 1 < nx ? WV[-1] : T(0)) + (i + 1 >= 0 && i + 1 < nx ? WV[1] : T(0));
       if (col_int && q - 1 >= 0 && q - 1 < ny) {
-        ru += wu_m4 * a.lo[int64_t(q - 1) * a.slo];
-        rv += wv_m4 * a.vo[int64_t(q - 1) * a.svo];
+        ru += wu_m4 * lo(q - 1);
+        rv += wv_m4 * vo(q - 1);
       }
       if (col_int && q + 1 >= 0 && q + 1 < ny) {
-        ru += wu_m2 * a.lu[int64_t(q + 1) * a.slu];
-        rv += wv_m2 * a.vu[int64_t(q + 1) * a.svu];
+        ru += wu_m2 * lu(q + 1);
+        rv += wv_m2 * vu(q + 1);
       }
       const int64_t at = plane + i + int64_t(q) * sj;
       a.uinbar[at] = ru;
@@ -1234,12 +1268,14 @@ int hdiffsmag_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
     // Measured at 256x256x80: FP64 fastest with 16-row chunks and rows
     // loaded one row ahead (two rows ahead spills registers), FP32 with
     // 32-row chunks loaded two rows ahead.
-    const int rows = strip_rows_for(K, sizeof(T) == 8 ? 16 : 32);
+    // At most 256 rows, which bounds the 1-D tables in LDS.
+    const int rows = std::min(strip_rows_for(K, sizeof(T) == 8 ? 16 : 32), 256);
     const int nchunk = int((ny + 2 + rows - 1) / rows);
     if (int64_t(nchunk) * nz > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
     auto kern = strip_depth_for(K, sizeof(T) == 8 ? 1 : 2) >= 2 ? hdiffsmag_adj_strip<T, 2> : hdiffsmag_adj_strip<T, 1>;
-    hipLaunchKernelGGL(kern, dim3(unsigned(nchunk * nz)), dim3(unsigned(lanes)), size_t(22) * lanes * sizeof(T),
-                       hipStream_t(stream), a, g, rows, nchunk);
+    hipLaunchKernelGGL(kern, dim3(unsigned(nchunk * nz)), dim3(unsigned(lanes)),
+                       (size_t(22) * lanes + 5 * size_t(rows + 9)) * sizeof(T), hipStream_t(stream), a, g, rows,
+                       nchunk);
     return launch_status(K);
   }
   const BoxGrid b1{0, 0, int(nx), int(ny)}, b2{-1, -1, int(nx + 2), int(ny + 2)};
