@@ -540,6 +540,18 @@ __device__ __forceinline__ T s5(const T *f, int s, bool plus) {
           T(-1.0) / T(4.0) * f[2 * s] + T(1.0) / T(30.0) * f[3 * s]);
 }
 
+// s5 on seven values f(-3) .. f(3) passed by value (the same terms in the same
+// order), for register windows: taking a window's address would place it in
+// scratch memory.
+template <class T>
+__device__ __forceinline__ T s5v(T fm3, T fm2, T fm1, T f0, T fp1, T fp2, T fp3, bool plus) {
+  if (plus)
+    return (T(1.0) / T(30.0) * fm3 + T(-1.0) / T(4.0) * fm2 + fm1 + T(-1.0) / T(3.0) * f0 + T(-1.0) / T(2.0) * fp1 +
+            T(1.0) / T(20.0) * fp2);
+  return (T(1.0) / T(20.0) * fm2 + T(-1.0) / T(2.0) * fm1 + T(-1.0) / T(3.0) * f0 + fp1 + T(-1.0) / T(4.0) * fp2 +
+          T(1.0) / T(30.0) * fp3);
+}
+
 template <class T>
 __global__ void __launch_bounds__(kTI *kTJ) hadvuv5th_adj_stage1(HadvAdj<T> a, Geo g, BoxGrid b) {
   int i, j;
@@ -705,7 +717,7 @@ __global__ void __launch_bounds__(640) hadvuv5th_adj_strip(HadvAdj<T> a, Geo g, 
       const T ua = a0 * U, va = rr * Vu;
       px = ua > 0;
       py = va > 0;
-      const T sx = s5(u, 1, px), sy = s5(&ur[3], 1, py);
+      const T sx = s5(u, 1, px), sy = s5v(ur[0], ur[1], ur[2], ur[3], ur[4], ur[5], ur[6], py);
       wux = a.eddlat * ob_c * (px ? ua : -ua);
       wuy = a.eddlon * ob_c * (py ? va : -va);
       gu = a.eddlat * ob_c * (px ? sx : -sx) * a0 * (T(1.0) / T(3.0));
@@ -716,7 +728,7 @@ __global__ void __launch_bounds__(640) hadvuv5th_adj_strip(HadvAdj<T> a, Geo g, 
       const T ua2 = a1 * Uv, va2 = rr * V;
       px2 = ua2 > 0;
       py2 = va2 > 0;
-      const T sx2 = s5(v, 1, px2), sy2 = s5(&vr[3], 1, py2);
+      const T sx2 = s5(v, 1, px2), sy2 = s5v(vr[0], vr[1], vr[2], vr[3], vr[4], vr[5], vr[6], py2);
       wvx = a.eddlat * ob2_c * (px2 ? ua2 : -ua2);
       wvy = a.eddlon * ob2_c * (py2 ? va2 : -va2);
       gu2 = a.eddlat * ob2_c * (px2 ? sx2 : -sx2) * a1 * T(0.25) - ob2_c * tg1 * Uv * T(0.5);
