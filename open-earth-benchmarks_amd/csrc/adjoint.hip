@@ -1471,9 +1471,11 @@ __global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k5(FvAdj<T> a, Geo g, Box
 // every face m whose flux reaches the tile, the six products w_k(m) gbar(m)
 // (ppm_weights) into LDS; each point then adds the six products that name
 // it, in the order ppm_transpose adds them (faces q-2 .. q+3; a face outside
-// its region, or with a zero cotangent, gives 0).  ppm_transpose recomputes
+// its region gives 0, one with a zero cotangent zero products).  ppm_transpose recomputes
 // the weights of six faces per point instead.  K5 runs the x axis, then the y
-// axis, through the same LDS.
+// axis, through the same LDS, and forms each face's cotangent W2 or W3 (K3,
+// K4) where it forms the face's products, so the tiled form is three launches
+// and W2, W3 never go to memory.
 constexpr int kPX = 64, kPY = 8, kPNT = 256;
 constexpr int kPPts = kPX * kPY / kPNT;  // points per lane
 
@@ -1493,16 +1495,16 @@ __device__ __forceinline__ void ppm_products(PpmLds<T> &L, const T *f, const T *
     const int fi = (AXIS == 0 ? i0 - 2 : i0) + cc, fj = (AXIS == 0 ? j0 : j0 - 2) + r;
     T pk[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
     if (face(fi, fj)) {
+      // No early exit on a zero cotangent: the field loads then do not wait
+      // for the cotangent's loads (a zero cotangent gives zero products).
       const int64_t m = plane + fi + int64_t(fj) * sj;
-      const T gb = gbar(m);
-      if (gb != T(0)) {
-        T fv[6], w[6];
+      const T gb = gbar(m, fi, fj);
+      T fv[6], w[6];
 #pragma unroll
-        for (int t = 0; t < 6; ++t) fv[t] = f[m + int64_t(t - 3) * s];
-        ppm_weights(fv, c[m], p1, p2, w);
+      for (int t = 0; t < 6; ++t) fv[t] = f[m + int64_t(t - 3) * s];
+      ppm_weights(fv, c[m], p1, p2, w);
 #pragma unroll
-        for (int t = 0; t < 6; ++t) pk[t] = w[t] * gb;
-      }
+      for (int t = 0; t < 6; ++t) pk[t] = w[t] * gb;
     }
 #pragma unroll
     for (int t = 0; t < 6; ++t) L.p[t][e] = pk[t];
@@ -1540,7 +1542,7 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxG
   };
   if (MODE == 1) {  // K1
     ppm_products<1>(L, a.q_j, a.cry, plane, sj, i0, j0, p1, p2, [&](int i, int j) { return R.in(i, j); },
-                    [&](int64_t m) { return T(0.5) * a.yfx[m] * a.fy1b[m]; });
+                    [&](int64_t m, int, int) { return T(0.5) * a.yfx[m] * a.fy1b[m]; });
     lds_barrier();
 #pragma unroll
     for (int u = 0; u < kPPts; ++u) {
@@ -1552,7 +1554,7 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxG
     }
   } else if (MODE == 2) {  // K2
     ppm_products<0>(L, a.q_i, a.crx, plane, sj, i0, j0, p1, p2, [&](int i, int j) { return R.in(i, j); },
-                    [&](int64_t m) { return T(0.5) * a.xfx[m] * a.fx1b[m]; });
+                    [&](int64_t m, int, int) { return T(0.5) * a.xfx[m] * a.fx1b[m]; });
     lds_barrier();
 #pragma unroll
     for (int u = 0; u < kPPts; ++u) {
@@ -1562,9 +1564,26 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxG
         a.w1[at] = a.qib[at] + ppm_gather<0>(L, lx, ly);
       }
     }
-  } else {  // K5
-    ppm_products<0>(L, a.q, a.crx, plane, sj, i0, j0, p1, p2, [&](int i, int j) { return R.fx2(i, j); },
-                    [&](int64_t m) { return a.w2[m]; });
+  } else {  // K5, with K3 and K4 formed per face (w2 and w3 never go to memory)
+    // W2 at face (i, j): fvtp2d_adj_k3's expression.
+    auto w2_at = [&](int64_t m, int i, int j) -> T {
+      T r = a.fx2b[m];
+      if (R.in(i, j)) r += T(0.5) * a.xfx[m] * a.fx1b[m];
+      T d = T(0);
+      if (R.qj(i, j)) d += a.w0[m] / a.ra_x[m];
+      if (R.qj(i - 1, j)) d -= a.w0[m - 1] / a.ra_x[m - 1];
+      return r + a.xfx[m] * d;
+    };
+    // W3 at face (i, j): fvtp2d_adj_k4's expression.
+    auto w3_at = [&](int64_t m, int i, int j) -> T {
+      T r = a.fy2b[m];
+      if (R.in(i, j)) r += T(0.5) * a.yfx[m] * a.fy1b[m];
+      T d = T(0);
+      if (R.qi(i, j)) d += a.w1[m] / a.ra_y[m];
+      if (R.qi(i, j - 1)) d -= a.w1[m - sj] / a.ra_y[m - sj];
+      return r + a.yfx[m] * d;
+    };
+    ppm_products<0>(L, a.q, a.crx, plane, sj, i0, j0, p1, p2, [&](int i, int j) { return R.fx2(i, j); }, w2_at);
     lds_barrier();
     T rx[kPPts];
 #pragma unroll
@@ -1574,8 +1593,7 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxG
       rx[u] = ppm_gather<0>(L, lx, ly);
     }
     lds_barrier();
-    ppm_products<1>(L, a.q, a.cry, plane, sj, i0, j0, p1, p2, [&](int i, int j) { return R.fy2(i, j); },
-                    [&](int64_t m) { return a.w3[m]; });
+    ppm_products<1>(L, a.q, a.cry, plane, sj, i0, j0, p1, p2, [&](int i, int j) { return R.fy2(i, j); }, w3_at);
     lds_barrier();
 #pragma unroll
     for (int u = 0; u < kPPts; ++u) {
@@ -1656,25 +1674,27 @@ int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, 
   hipStream_t st = hipStream_t(stream);
   // The tiled transposes (fvtp2d_adj_ptile) by default; OEB_TILE_FVTP2D_ADJ=0
   // selects the per-point gathers.
+  // The tiled form (fvtp2d_adj_ptile: three launches, K3 and K4 formed inside
+  // K5) by default; OEB_TILE_FVTP2D_ADJ=0 selects the five per-point passes,
+  // with W2 and W3 through work2 and work3.
   const bool tiled = env_choice("OEB_TILE_", K, 1, 0, 1) != 0;
   if (tiled) {
     hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 1>), ptile_blocks(bqj, int(nz)), dim3(kPNT), 0, st, a, g, bqj);
     OEB_TRY(launch_status(K));
     hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 2>), ptile_blocks(bqi, int(nz)), dim3(kPNT), 0, st, a, g, bqi);
-  } else {
-    hipLaunchKernelGGL(fvtp2d_adj_k1<T>, box_blocks(bqj, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqj);
     OEB_TRY(launch_status(K));
-    hipLaunchKernelGGL(fvtp2d_adj_k2<T>, box_blocks(bqi, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqi);
+    hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 5>), ptile_blocks(bq, int(nz)), dim3(kPNT), 0, st, a, g, bq);
+    return launch_status(K);
   }
+  hipLaunchKernelGGL(fvtp2d_adj_k1<T>, box_blocks(bqj, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqj);
+  OEB_TRY(launch_status(K));
+  hipLaunchKernelGGL(fvtp2d_adj_k2<T>, box_blocks(bqi, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqi);
   OEB_TRY(launch_status(K));
   hipLaunchKernelGGL(fvtp2d_adj_k3<T>, box_blocks(bfx2, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bfx2);
   OEB_TRY(launch_status(K));
   hipLaunchKernelGGL(fvtp2d_adj_k4<T>, box_blocks(bfy2, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bfy2);
   OEB_TRY(launch_status(K));
-  if (tiled)
-    hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 5>), ptile_blocks(bq, int(nz)), dim3(kPNT), 0, st, a, g, bq);
-  else
-    hipLaunchKernelGGL(fvtp2d_adj_k5<T>, box_blocks(bq, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bq);
+  hipLaunchKernelGGL(fvtp2d_adj_k5<T>, box_blocks(bq, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bq);
   return launch_status(K);
 }
 
