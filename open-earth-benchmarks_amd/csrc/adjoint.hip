@@ -1017,7 +1017,7 @@ __global__ void __launch_bounds__(kTI *kTJ) hdiffsmag_adj_stage2(HsmAdj<T> a, Ge
 //  * forms T_bar and S_bar at (i, r - 2) from the gu, gv of rows r - 3 ..
 //    r - 1;
 //  * gathers uin_bar, vin_bar at (i, r - 3) (hdiffsmag_adj_stage2).
-// The i neighbours of gu, gv, wu, wv (3 slots), T_bar, S_bar (2 slots) and
+// The i neighbours of gu, gv (2 slots), wu, wv (3 slots), T_bar, S_bar (2 slots) and
 // u, v come from LDS rows, published after the barrier of one iteration and
 // read in later ones; the j neighbours are rows of the lane's own column in
 // registers.  One barrier per row.  The expressions are those of the
@@ -1033,9 +1033,9 @@ __global__ void __launch_bounds__(1024) hdiffsmag_adj_strip(HsmAdj<T> a, Geo g, 
   const int L = int(blockDim.x);
   T *s_u = reinterpret_cast<T *>(oeb_smem);  // 3 x L: u rows
   T *s_v = s_u + 3 * L;                       // 3 x L: v rows
-  T *s_gu = s_v + 3 * L;                      // 3 x L each: gu, gv, wu, wv rows
-  T *s_gv = s_gu + 3 * L;
-  T *s_wu = s_gv + 3 * L;
+  T *s_gu = s_v + 3 * L;                      // 2 x L each: gu, gv rows (read one iteration later)
+  T *s_gv = s_gu + 2 * L;
+  T *s_wu = s_gv + 2 * L;                     // 3 x L each: wu, wv rows (read two iterations later)
   T *s_wv = s_wu + 3 * L;
   T *s_tb = s_wv + 3 * L;  // 2 x L each: T_bar, S_bar rows
   T *s_sb = s_tb + 2 * L;
@@ -1121,8 +1121,8 @@ __global__ void __launch_bounds__(1024) hdiffsmag_adj_strip(HsmAdj<T> a, Geo g, 
       wu_m1 = smag_u * ub_m1;
       wv_m1 = smag_v * vb_m1;
     }
-    s_gu[s0 * L + l] = gu_m1;
-    s_gv[s0 * L + l] = gv_m1;
+    s_gu[(t & 1) * L + l] = gu_m1;
+    s_gv[(t & 1) * L + l] = gv_m1;
     s_wu[s0 * L + l] = wu_m1;
     s_wv[s0 * L + l] = wv_m1;
     // T(i, r) and S(i, r - 1) (the stage-2 lambdas Tx, Sx).
@@ -1132,8 +1132,8 @@ __global__ void __launch_bounds__(1024) hdiffsmag_adj_strip(HsmAdj<T> a, Geo g, 
     // T_bar, S_bar at (i, r - 2): weights from gu, gv of rows r - 3 .. r - 1.
     T tb_m2 = T(0), sb_m2 = T(0);
     if (l >= 1 && l < L - 1) {
-      const T *GU = s_gu + s1 * L + l;  // gu row r - 2 (published in the previous iteration)
-      const T *GV = s_gv + s1 * L + l;  // gv row r - 2
+      const T *GU = s_gu + ((t + 1) & 1) * L + l;  // gu row r - 2 (published in the previous iteration)
+      const T *GV = s_gv + ((t + 1) & 1) * L + l;  // gv row r - 2
       const T tw = GU[0] + GU[-1] + GV[0] + gv_m3;
       const T sw = GU[0] + gu_m1 + GV[0] + GV[1];
       tb_m2 = tw != T(0) ? t_m2 * tw : T(0);
@@ -1286,7 +1286,7 @@ int hdiffsmag_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
     if (int64_t(nchunk) * nz > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
     auto kern = strip_depth_for(K, sizeof(T) == 8 ? 1 : 2) >= 2 ? hdiffsmag_adj_strip<T, 2> : hdiffsmag_adj_strip<T, 1>;
     hipLaunchKernelGGL(kern, dim3(unsigned(nchunk * nz)), dim3(unsigned(lanes)),
-                       (size_t(22) * lanes + 5 * size_t(rows + 9)) * sizeof(T), hipStream_t(stream), a, g, rows,
+                       (size_t(20) * lanes + 5 * size_t(rows + 9)) * sizeof(T), hipStream_t(stream), a, g, rows,
                        nchunk);
     return launch_status(K);
   }
