@@ -72,9 +72,10 @@ __global__ void __launch_bounds__(kTileI *kTileJ) nh_p_grad_kernel(NhArgs<T> a, 
   }
 }
 
-// Two adjacent columns (i, i+1) per lane, i even, 16-byte accesses.  The east
-// neighbour of column i is column i+1 of the pair; that of column i+1 is a
-// single load at i+2.
+// Two adjacent columns (i, i+1) per lane, i even: the element-type fields
+// move as pairs (16 bytes in FP64, 8 in FP32), the FP64 gz, pp and pk3 as
+// 16-byte pairs.  The east neighbour of column i is column i+1 of the pair;
+// that of column i+1 is a single load at i+2.
 struct PairCol {
   d2 C, N, C1, N1;   // pair at (i, i+1), rows j and j+1, levels k and k+1
   double E, E1;      // column i+2, row j, levels k and k+1
@@ -86,8 +87,8 @@ struct PairCol {
   __device__ void roll() { C = C1, N = N1, E = E1; }
 };
 
-__global__ void __launch_bounds__(kTileI *kTileJ)
-    nh_p_grad_pair_kernel(NhArgs<double> a, double dt, Geo g, Tiles t) {
+template <class T>
+__global__ void __launch_bounds__(kTileI *kTileJ) nh_p_grad_pair_kernel(NhArgs<T> a, T dt, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
   const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
@@ -103,14 +104,19 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
     gz.load_up(a.gz, up, sj);
     pk.load_up(a.pk3, up, sj);
     pp.load_up(a.pp, up, sj);
-    const d2 w1C = ld2(a.wk1 + at), w1N = ld2(a.wk1 + at + sj);
-    const double w1E = a.wk1[at + 2];
-    const d2 ui = ld2_nt(a.uin + at), vi = ld2_nt(a.vin + at), rx = ld2_nt(a.rdx + at), ry = ld2_nt(a.rdy + at);
-    double u0, v0, u1, v1;
-    nh_point(gz.lo(), pk.lo(), pp.lo(), w1C.x, w1C.y, w1N.x, ui.x, vi.x, rx.x, ry.x, dt, u0, v0);
-    nh_point(gz.hi(), pk.hi(), pp.hi(), w1C.y, w1E, w1N.y, ui.y, vi.y, rx.y, ry.y, dt, u1, v1);
-    st2_nt(a.uout + at, u0, u1);
-    st2_nt(a.vout + at, v0, v1);
+    typedef typename Vec<T, 2>::type P;
+    const P w1C = ldv<T, 2>(a.wk1 + at), w1N = ldv<T, 2>(a.wk1 + at + sj);
+    const T w1E = a.wk1[at + 2];
+    const P ui = ldv_nt<T, 2>(a.uin + at), vi = ldv_nt<T, 2>(a.vin + at);
+    const P rx = ldv_nt<T, 2>(a.rdx + at), ry = ldv_nt<T, 2>(a.rdy + at);
+    T u0, v0, u1, v1;
+    nh_point(gz.lo(), pk.lo(), pp.lo(), w1C[0], w1C[1], w1N[0], ui[0], vi[0], rx[0], ry[0], dt, u0, v0);
+    nh_point(gz.hi(), pk.hi(), pp.hi(), w1C[1], w1E, w1N[1], ui[1], vi[1], rx[1], ry[1], dt, u1, v1);
+    P uo, vo;
+    uo[0] = u0, uo[1] = u1;
+    vo[0] = v0, vo[1] = v1;
+    stv_nt<T, 2>(a.uout + at, uo);
+    stv_nt<T, 2>(a.vout + at, vo);
     gz.roll();
     pk.roll();
     pp.roll();
@@ -154,15 +160,16 @@ int nh_p_grad_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 
   a.uout = const_cast<T *>(p_uo);
   a.vout = const_cast<T *>(p_vo);
   const int kc = kchunk_for(K, sizeof(T), d, kChunk);
-  if constexpr (sizeof(T) == 8) {
-    if (pairs_for(K, sizeof(T), d, false) && nx % 2 == 0 &&
-        pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdx, a.rdy, a.gz, a.pp, a.pk3, a.wk1})) {
-      const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
-      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-      hipLaunchKernelGGL(nh_p_grad_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt, g,
-                         t);
-      return launch_status(K);
-    }
+  // Column pairs by default in the FP32 form only.  Measured in the
+  // ten-stencil step at 256x256x80 and 512x512x80: FP32 pairs take
+  // 63 / 223 us against 68 / 241 us for single columns; FP64 pairs were slower.
+  if (pairs_for(K, sizeof(T), d, sizeof(T) == 4) && nx % 2 == 0 &&
+      pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdx, a.rdy, a.gz, a.pp, a.pk3, a.wk1})) {
+    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+    hipLaunchKernelGGL(nh_p_grad_pair_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt, g,
+                       t);
+    return launch_status(K);
   }
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));

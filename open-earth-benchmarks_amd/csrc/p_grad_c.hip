@@ -61,10 +61,11 @@ __global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_kernel(PgcArgs<T> a, 
   }
 }
 
-// Two adjacent columns (i, i+1) per lane, i even, 16-byte accesses.  For
-// column i+1 the west neighbour is column i of the pair.
-__global__ void __launch_bounds__(kTileI *kTileJ)
-    p_grad_c_pair_kernel(PgcArgs<double> a, double dt2, Geo g, Tiles t) {
+// Two adjacent columns (i, i+1) per lane, i even: the element-type fields
+// move as pairs (16 bytes in FP64, 8 in FP32), the FP64 gz and pkc as 16-byte
+// pairs.  For column i+1 the west neighbour is column i of the pair.
+template <class T>
+__global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_pair_kernel(PgcArgs<T> a, T dt2, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
   const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
@@ -81,16 +82,20 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
     const double gzW1 = a.gz[up - 1];
     const d2 pkC1 = ld2(a.pkc + up), pkS1 = ld2(a.pkc + up - sj);
     const double pkW1 = a.pkc[up - 1];
-    const d2 dpC = ld2(a.delpc + at), dpS = ld2(a.delpc + at - sj);
-    const double dpW = a.delpc[at - 1];
-    const d2 ui = ld2_nt(a.uin + at), vi = ld2_nt(a.vin + at), rx = ld2_nt(a.rdxc + at), ry = ld2_nt(a.rdyc + at);
-    double u0, v0, u1, v1;
-    pgc_point(gzC.x, gzW, gzS.x, gzC1.x, gzW1, gzS1.x, pkC.x, pkW, pkS.x, pkC1.x, pkW1, pkS1.x, dpC.x, dpW, dpS.x,
-              ui.x, vi.x, rx.x, ry.x, dt2, u0, v0);
-    pgc_point(gzC.y, gzC.x, gzS.y, gzC1.y, gzC1.x, gzS1.y, pkC.y, pkC.x, pkS.y, pkC1.y, pkC1.x, pkS1.y, dpC.y, dpC.x,
-              dpS.y, ui.y, vi.y, rx.y, ry.y, dt2, u1, v1);
-    st2_nt(a.uout + at, u0, u1);
-    st2_nt(a.vout + at, v0, v1);
+    const typename Vec<T, 2>::type dpC = ldv<T, 2>(a.delpc + at), dpS = ldv<T, 2>(a.delpc + at - sj);
+    const T dpW = a.delpc[at - 1];
+    const typename Vec<T, 2>::type ui = ldv_nt<T, 2>(a.uin + at), vi = ldv_nt<T, 2>(a.vin + at);
+    const typename Vec<T, 2>::type rx = ldv_nt<T, 2>(a.rdxc + at), ry = ldv_nt<T, 2>(a.rdyc + at);
+    typename Vec<T, 2>::type uo, vo;
+    T u0, v0, u1, v1;
+    pgc_point(gzC.x, gzW, gzS.x, gzC1.x, gzW1, gzS1.x, pkC.x, pkW, pkS.x, pkC1.x, pkW1, pkS1.x, dpC[0], dpW, dpS[0],
+              ui[0], vi[0], rx[0], ry[0], dt2, u0, v0);
+    pgc_point(gzC.y, gzC.x, gzS.y, gzC1.y, gzC1.x, gzS1.y, pkC.y, pkC.x, pkS.y, pkC1.y, pkC1.x, pkS1.y, dpC[1], dpC[0],
+              dpS[1], ui[1], vi[1], rx[1], ry[1], dt2, u1, v1);
+    uo[0] = u0, uo[1] = u1;
+    vo[0] = v0, vo[1] = v1;
+    stv_nt<T, 2>(a.uout + at, uo);
+    stv_nt<T, 2>(a.vout + at, vo);
     gzC = gzC1, gzS = gzS1, gzW = gzW1;
     pkC = pkC1, pkS = pkS1, pkW = pkW1;
   }
@@ -130,15 +135,16 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
   a.uout = const_cast<T *>(p_uo);
   a.vout = const_cast<T *>(p_vo);
   const int kc = kchunk_for(K, sizeof(T), d, kChunk);
-  if constexpr (sizeof(T) == 8) {
-    if (pairs_for(K, sizeof(T), d, false) && nx % 2 == 0 &&
-        pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdxc, a.rdyc, a.delpc, a.gz, a.pkc})) {
-      const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
-      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-      hipLaunchKernelGGL(p_grad_c_pair_kernel, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g,
-                         t);
-      return launch_status(K);
-    }
+  // Column pairs by default in the FP32 form only.  Measured in the
+  // ten-stencil step at 256x256x80 and 512x512x80: FP32 pairs take
+  // 52 / 189 us against 55 / 203 us for single columns; FP64 pairs were slower.
+  if (pairs_for(K, sizeof(T), d, sizeof(T) == 4) && nx % 2 == 0 &&
+      pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdxc, a.rdyc, a.delpc, a.gz, a.pkc})) {
+    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+    hipLaunchKernelGGL(p_grad_c_pair_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g,
+                       t);
+    return launch_status(K);
   }
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));

@@ -81,13 +81,14 @@ def test_f32_256_against_f64(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nx", [64, 66])
+@pytest.mark.parametrize("nx", [64, 66, 67])
 @pytest.mark.parametrize("pairs", [0, 1])
-@pytest.mark.parametrize("name", ["laplace", "uvbke"])
+@pytest.mark.parametrize("name", ["laplace", "uvbke", "fastwavesuv", "p_grad_c", "nh_p_grad"])
 def test_f32_vector_path_switch(name, pairs, nx, monkeypatch):
-    """The 4-wide (16-byte) FP32 path, forced on or off with OEB_PAIRS_<K>,
-    and nx = 66, where it does not apply, agree bit for bit with the FP32
-    oracle."""
+    """The vector paths of the FP32 form (4-wide for laplace and uvbke,
+    column pairs with FP64 island pairs for the others), forced on or off
+    with OEB_PAIRS_<K>, and nx where they do not apply (66 for the 4-wide
+    paths, 67 for all), agree bit for bit with the FP32 oracle."""
     monkeypatch.setenv(f"OEB_PAIRS_{name.upper()}", str(pairs))
     L, host = inputs(name, nx, 36, 7, "baseline", seed=9)
     print(f"[gpu] f32 {name} pairs={pairs} nx={nx}", flush=True)
