@@ -456,7 +456,8 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   // of their operands, and more waves hide that.  1024 lanes hold fewer
   // workgroups and ran 1.2-1.3x slower.  FP32: 128x16 tiles of 512 lanes
   // (half the bytes per tile, so less halo for the same LDS) are fastest.
-  // OEB_TILE_FVTP2D = 1..12 forces one of the measured shapes.
+  // OEB_TILE_FVTP2D = 1..12 forces one of the measured shapes, 0 the 64x8
+  // tiles of 256 lanes.
   switch (env_choice("OEB_TILE_", K, sizeof(T) == 4 ? 3 : 8, 0, 12)) {
     case 10: return fvtp2d_launch<64, 16, 1024, T>(K, a, o, p1, p2, g, d, stream);
     case 11: return fvtp2d_launch<128, 16, 1024, T>(K, a, o, p1, p2, g, d, stream);
