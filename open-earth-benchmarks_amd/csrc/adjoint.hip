@@ -637,25 +637,37 @@ __global__ void __launch_bounds__(kTI *kTJ) hadvuv5th_adj_stage2(HadvAdj<T> a, G
   a.vinbar[at] = rv;
 }
 
+// Stage 1 of hadvuv5th_adj_stage1 at one point (i, p) of the row march:
+// the quantities the gathers need, 0 outside the interior.
+template <class T>
+struct HadvSt1 {
+  T wux, wvx, wuy, wvy;  // eddlat ob |ua|, eddlat ob2 |ua2|, eddlon ob |va|, eddlon ob2 |va2|
+  T gu, gv, gu2, gv2, tgt;
+  int br;        // bit 0: ua > 0, bit 1: ua2 > 0
+  bool py, py2;  // va > 0, va2 > 0
+};
+
 // Fused row-march form, used when one strip spans nx (nx + 12 <= 640, so
-// that the FP64 form keeps its ~150 registers without spilling): lane
-// l owns column i = l - 6 of [-6, nx + 6) of one level; the workgroup walks
-// the output rows p of [j0 - 3, j1 + 3) for the uin_bar / vin_bar rows
-// [j0, j1) of its chunk of [-3, ny + 3).  Per row p it forms the stage-1
-// quantities of hadvuv5th_adj_stage1 at (i, p) (zero outside the interior)
-// and hands them on without memory:
-//  * along i (the x upwind stencils and the averages U, Uv, Vu), through one
+// that the FP64 form keeps its registers without spilling): lane l owns
+// column i = l - 6 of [-6, nx + 6) of one level; the workgroup walks the
+// output rows p of [j0 - 3, j1 + 3) for the uin_bar / vin_bar rows [j0, j1)
+// of its chunk of [-3, ny + 3).  Per row p it gathers the contributions of
+// the stage-1 quantities of hadvuv5th_adj_stage1 at (i, p) without memory:
+//  * along i (the x upwind stencils and the averages U, Uv, Vu), from one
 //    LDS row: lane i gathers the contributions of the outputs (i - 3 .. i + 3,
 //    p) to the inputs (i, p), and the 2x2 contributions of gu2 and gv to rows
 //    p + 1 and p - 1;
 //  * along j (the y upwind stencils, V and the rows above and below), by
 //    pushing them into a per-lane queue of the 7 input rows p - 3 .. p + 3 of
 //    its own column.
-// Row p - 3 of the queue is then complete and is written.  Rows of u and v
-// are loaded D rows ahead; u(p + 1) and v(p) are published to LDS (3 slots)
-// for the x stencils of rows p and p + 1.  Two barriers per row: one
-// publishes the u, v row, the other the stage-1 row.  The terms are those of
-// the two-pass form; they are summed in another order.
+// Row p - 3 of the queue is then complete and is written.  Stage 1 runs one
+// row ahead: iteration t publishes, before its one barrier, the u row p + 2
+// and v row p + 1 (3 LDS slots each) and the stage-1 row p formed in
+// iteration t - 1 (2 slots); after the barrier it gathers row p and forms
+// stage 1 of row p + 1, two independent chains.  acrlat0/1 and tgrlatda0/1 of
+// the chunk's rows come from LDS tables.  Rows of u and v are loaded D rows
+// ahead.  The terms are those of the two-pass form; they are summed in
+// another order.
 template <class T, int D>
 __global__ void __launch_bounds__(640) hadvuv5th_adj_strip(HadvAdj<T> a, Geo g, int rows, int nchunk) {
   extern __shared__ __align__(16) unsigned char oeb_smem[];
@@ -663,26 +675,70 @@ __global__ void __launch_bounds__(640) hadvuv5th_adj_strip(HadvAdj<T> a, Geo g, 
   const int L = int(blockDim.x);
   T *s_u = reinterpret_cast<T *>(oeb_smem);  // 3 x L: u rows
   T *s_v = s_u + 3 * L;                       // 3 x L: v rows
-  T *s_wux = s_v + 3 * L;                     // stage-1 row: eddlat ob |ua|
-  T *s_wvx = s_wux + L;                       // eddlat ob2 |ua2|
-  T *s_gu = s_wvx + L;
-  T *s_gu2 = s_gu + L;
-  T *s_gv = s_gu2 + L;
-  int *s_br = reinterpret_cast<int *>(s_gv + L);  // bit 0: ua > 0, bit 1: ua2 > 0
+  T *s_wux = s_v + 3 * L;                     // 2 x L each: stage-1 rows
+  T *s_wvx = s_wux + 2 * L;
+  T *s_gu = s_wvx + 2 * L;
+  T *s_gu2 = s_gu + 2 * L;
+  T *s_gv = s_gu2 + 2 * L;
+  T *s_a0 = s_gv + 2 * L;  // rows + 10 each: rows [j0 - 6, j1 + 4)
+  T *s_a1 = s_a0 + rows + 10;
+  T *s_t0 = s_a1 + rows + 10;
+  T *s_t1 = s_t0 + rows + 10;
+  int *s_br = reinterpret_cast<int *>(s_t1 + rows + 10);  // 2 x L
   const int nx = g.nx, ny = g.ny, sj = g.sj;
   const int cj = int(blockIdx.x) % nchunk, k = int(blockIdx.x) / nchunk;
   const int j0 = -3 + cj * rows, j1 = min(ny + 3, j0 + rows);
   const int l = int(threadIdx.x), i = l - 6;
   const bool col_io = i >= -3 && i < nx + 3;  // uin_bar / vin_bar columns
   const bool col_out = i >= 0 && i < nx;      // output (stage-1) columns
+  const int tb = j0 - 6;                      // first row of the tables
+  for (int e = l; e < j1 - j0 + 10; e += L) {
+    const int64_t r = min(max(tb + e, 0), ny - 1);  // the 1-D arrays live on [0, ny)
+    s_a0[e] = a.a0[r * a.s0];
+    s_a1[e] = a.a1[r * a.s1];
+    s_t0[e] = a.t0[r * a.s2];
+    s_t1[e] = a.t1[r * a.s3];
+  }
   const int64_t plane = int64_t(k) * g.sk;
   const int64_t c_uv = plane + min(max(i, -3), nx + 2), c_o = plane + min(max(i, 0), nx - 1);
   auto ldu = [&](int r) -> T { return a.uin[c_uv + int64_t(min(max(r, -3), ny + 2)) * sj]; };
   auto ldv_ = [&](int r) -> T { return a.vin[c_uv + int64_t(min(max(r, -3), ny + 2)) * sj]; };
   auto ldub = [&](int r) -> T { return a.ubar[c_o + int64_t(min(max(r, 0), ny - 1)) * sj]; };
   auto ldvb = [&](int r) -> T { return a.vbar[c_o + int64_t(min(max(r, 0), ny - 1)) * sj]; };
+  // Stage 1 at (i, p) (kernels/hadvuv5th.h:183-195), given the LDS rows u(p),
+  // u(p + 1), v(p), v(p - 1) and the column windows w (rows p - 3 .. p + 3).
+  auto stage1 = [&](int p, const T *u, const T *u1, const T *v, const T *vm, const T (&ur)[7], const T (&vr)[7],
+                    T ob_c, T ob2_c) {
+    HadvSt1<T> S{};
+    if (col_out && p >= 0 && p < ny) {
+      const T a0 = s_a0[p - tb], a1 = s_a1[p - tb], tg0 = s_t0[p - tb], tg1 = s_t1[p - tb];
+      const T U = (T(1.0) / T(3.0)) * (u[-1] + u[0] + u[1]);
+      const T Vu = T(0.25) * (v[1] + vm[1] + v[0] + vm[0]);
+      const T ua = a0 * U, va = rr * Vu;
+      const bool px = ua > 0;
+      S.py = va > 0;
+      const T sx = s5(u, 1, px), sy = s5v(ur[0], ur[1], ur[2], ur[3], ur[4], ur[5], ur[6], S.py);
+      S.wux = a.eddlat * ob_c * (px ? ua : -ua);
+      S.wuy = a.eddlon * ob_c * (S.py ? va : -va);
+      S.gu = a.eddlat * ob_c * (px ? sx : -sx) * a0 * (T(1.0) / T(3.0));
+      S.gv = a.eddlon * ob_c * (S.py ? sy : -sy) * rr * T(0.25) + ob_c * tg0 * u[0] * T(0.25);
+      S.tgt = ob_c * tg0 * Vu;
+      const T Uv = T(0.25) * (u[-1] + u[0] + u1[0] + u1[-1]);
+      const T V = T(1.0) / T(3.0) * (vr[2] + vr[3] + vr[4]);
+      const T ua2 = a1 * Uv, va2 = rr * V;
+      const bool px2 = ua2 > 0;
+      S.py2 = va2 > 0;
+      const T sx2 = s5(v, 1, px2), sy2 = s5v(vr[0], vr[1], vr[2], vr[3], vr[4], vr[5], vr[6], S.py2);
+      S.wvx = a.eddlat * ob2_c * (px2 ? ua2 : -ua2);
+      S.wvy = a.eddlon * ob2_c * (S.py2 ? va2 : -va2);
+      S.gu2 = a.eddlat * ob2_c * (px2 ? sx2 : -sx2) * a1 * T(0.25) - ob2_c * tg1 * Uv * T(0.5);
+      S.gv2 = a.eddlon * ob2_c * (S.py2 ? sy2 : -sy2) * rr * (T(1.0) / T(3.0));
+      S.br = (px ? 1 : 0) | (px2 ? 2 : 0);
+    }
+    return S;
+  };
   const int p0 = j0 - 3, nt = j1 - j0 + 6;  // output rows p0 .. j1 + 2
-  T ur[7], vr[7];                            // rows p - 3 .. p + 3
+  T ur[7], vr[7];                            // rows p0 - 3 .. p0 + 3, then one row ahead
 #pragma unroll
   for (int x = 0; x < 7; ++x) {
     ur[x] = ldu(p0 - 3 + x);
@@ -691,69 +747,46 @@ __global__ void __launch_bounds__(640) hadvuv5th_adj_strip(HadvAdj<T> a, Geo g, 
   T qu[7], qv[7];  // partial cotangents of input rows p - 3 .. p + 3
 #pragma unroll
   for (int x = 0; x < 7; ++x) qu[x] = qv[x] = T(0);
-  // Slot 2 holds u(p0) and v(p0 - 1), the previous rows of iteration 0.
-  s_u[2 * L + l] = ur[3];
-  s_v[2 * L + l] = vr[2];
-  // One iteration, given the rows it loads: u, v at p + 4 and out_bar at p.
-  auto step = [&](int t, T u_next, T v_next, T ob_c, T ob2_c) {
+  // Stage 1 of row p0 from slots 1 (u(p0), v(p0 - 1)) and 2 (u(p0 + 1), v(p0)).
+  s_u[1 * L + l] = ur[3];
+  s_u[2 * L + l] = ur[4];
+  s_v[1 * L + l] = vr[2];
+  s_v[2 * L + l] = vr[3];
+  lds_barrier();
+  HadvSt1<T> S = stage1(p0, s_u + L + l, s_u + 2 * L + l, s_v + 2 * L + l, s_v + L + l, ur, vr, ldub(p0), ldvb(p0));
+#pragma unroll
+  for (int x = 0; x < 6; ++x) {
+    ur[x] = ur[x + 1];
+    vr[x] = vr[x + 1];
+  }
+  ur[6] = ldu(p0 + 4);
+  vr[6] = ldv_(p0 + 4);
+  // One iteration, given the rows it loads: u, v at p + 5 and out_bar at p + 1.
+  auto step = [&](int t, T u_next, T v_next, T ob_n, T ob2_n) {
     const int p = p0 + t;
-    T *su1 = s_u + (t % 3) * L;  // u row p + 1
-    T *sv0 = s_v + (t % 3) * L;  // v row p
-    su1[l] = ur[4];
-    sv0[l] = vr[3];
-    lds_barrier();
-    // Stage 1 at (i, p): hadvuv5th_adj_stage1, kernels/hadvuv5th.h:183-195.
-    T wux = T(0), wvx = T(0), wuy = T(0), wvy = T(0), gu = T(0), gv = T(0), gu2 = T(0), gv2 = T(0), tgt = T(0);
-    bool px = false, py = false, px2 = false, py2 = false;
-    if (col_out && p >= 0 && p < ny) {
-      const T *u = s_u + ((t + 2) % 3) * L + l;  // u row p
-      const T *u1 = su1 + l;                      // u row p + 1
-      const T *v = sv0 + l;                       // v row p
-      const T *vm = s_v + ((t + 2) % 3) * L + l;  // v row p - 1
-      const T a0 = a.a0[int64_t(p) * a.s0], a1 = a.a1[int64_t(p) * a.s1];
-      const T tg0 = a.t0[int64_t(p) * a.s2], tg1 = a.t1[int64_t(p) * a.s3];
-      const T U = (T(1.0) / T(3.0)) * (u[-1] + u[0] + u[1]);
-      const T Vu = T(0.25) * (v[1] + vm[1] + v[0] + vm[0]);
-      const T ua = a0 * U, va = rr * Vu;
-      px = ua > 0;
-      py = va > 0;
-      const T sx = s5(u, 1, px), sy = s5v(ur[0], ur[1], ur[2], ur[3], ur[4], ur[5], ur[6], py);
-      wux = a.eddlat * ob_c * (px ? ua : -ua);
-      wuy = a.eddlon * ob_c * (py ? va : -va);
-      gu = a.eddlat * ob_c * (px ? sx : -sx) * a0 * (T(1.0) / T(3.0));
-      gv = a.eddlon * ob_c * (py ? sy : -sy) * rr * T(0.25) + ob_c * tg0 * u[0] * T(0.25);
-      tgt = ob_c * tg0 * Vu;
-      const T Uv = T(0.25) * (u[-1] + u[0] + u1[0] + u1[-1]);
-      const T V = T(1.0) / T(3.0) * (vr[2] + vr[3] + vr[4]);
-      const T ua2 = a1 * Uv, va2 = rr * V;
-      px2 = ua2 > 0;
-      py2 = va2 > 0;
-      const T sx2 = s5(v, 1, px2), sy2 = s5v(vr[0], vr[1], vr[2], vr[3], vr[4], vr[5], vr[6], py2);
-      wvx = a.eddlat * ob2_c * (px2 ? ua2 : -ua2);
-      wvy = a.eddlon * ob2_c * (py2 ? va2 : -va2);
-      gu2 = a.eddlat * ob2_c * (px2 ? sx2 : -sx2) * a1 * T(0.25) - ob2_c * tg1 * Uv * T(0.5);
-      gv2 = a.eddlon * ob2_c * (py2 ? sy2 : -sy2) * rr * (T(1.0) / T(3.0));
-    }
-    s_wux[l] = wux;
-    s_wvx[l] = wvx;
-    s_gu[l] = gu;
-    s_gu2[l] = gu2;
-    s_gv[l] = gv;
-    s_br[l] = (px ? 1 : 0) | (px2 ? 2 : 0);
+    const int s3 = (t % 3) * L + l, s3m = ((t + 2) % 3) * L + l, s2 = (t & 1) * L + l;
+    s_u[s3] = ur[4];  // u row p + 2 (ur holds rows p - 2 .. p + 4)
+    s_v[s3] = vr[3];  // v row p + 1
+    s_wux[s2] = S.wux;
+    s_wvx[s2] = S.wvx;
+    s_gu[s2] = S.gu;
+    s_gu2[s2] = S.gu2;
+    s_gv[s2] = S.gv;
+    s_br[s2] = S.br;
     lds_barrier();
     // Along i: row p of the inputs gathers from outputs (i - 3 .. i + 3, p).
     if (col_io) {
       T ru = T(0), rv = T(0);
 #pragma unroll
       for (int kk = -3; kk <= 3; ++kk) {
-        const int br = s_br[l - kk];
-        ru += s_wux[l - kk] * c5<T>((br & 1) != 0, kk);
-        rv += s_wvx[l - kk] * c5<T>((br & 2) != 0, kk);
+        const int br = s_br[s2 - kk];
+        ru += s_wux[s2 - kk] * c5<T>((br & 1) != 0, kk);
+        rv += s_wvx[s2 - kk] * c5<T>((br & 2) != 0, kk);
       }
-      ru += s_gu[l - 1] + s_gu[l] + s_gu[l + 1];
-      const T g2 = s_gu2[l] + s_gu2[l + 1];  // to (i, p) and (i, p + 1)
-      const T hv = s_gv[l - 1] + s_gv[l];    // to (i, p) and (i, p - 1)
-      qu[3] += ru + g2 + tgt;
+      ru += s_gu[s2 - 1] + s_gu[s2] + s_gu[s2 + 1];
+      const T g2 = s_gu2[s2] + s_gu2[s2 + 1];  // to (i, p) and (i, p + 1)
+      const T hv = s_gv[s2 - 1] + s_gv[s2];    // to (i, p) and (i, p - 1)
+      qu[3] += ru + g2 + S.tgt;
       qu[4] += g2;
       qv[3] += rv + hv;
       qv[2] += hv;
@@ -761,12 +794,15 @@ __global__ void __launch_bounds__(640) hadvuv5th_adj_strip(HadvAdj<T> a, Geo g, 
     // Along j: output (i, p) to inputs (i, p + kk) of its own column.
 #pragma unroll
     for (int kk = -3; kk <= 3; ++kk) {
-      qu[3 + kk] += wuy * c5<T>(py, kk);
-      qv[3 + kk] += wvy * c5<T>(py2, kk);
+      qu[3 + kk] += S.wuy * c5<T>(S.py, kk);
+      qv[3 + kk] += S.wvy * c5<T>(S.py2, kk);
     }
-    qv[2] += gv2;
-    qv[3] += gv2;
-    qv[4] += gv2;
+    qv[2] += S.gv2;
+    qv[3] += S.gv2;
+    qv[4] += S.gv2;
+    // Stage 1 of row p + 1 from u(p + 1) (slot of t - 1), u(p + 2), v(p + 1) and
+    // v(p) (slot of t - 1).
+    S = stage1(p + 1, s_u + s3m, s_u + s3, s_v + s3, s_v + s3m, ur, vr, ob_n, ob2_n);
     // Input row p - 3 has all its contributions.
     const int q = p - 3;
     if (col_io && q >= j0 && q < j1) {
@@ -789,19 +825,19 @@ __global__ void __launch_bounds__(640) hadvuv5th_adj_strip(HadvAdj<T> a, Geo g, 
   T cu[D], cv[D], cb[D], cc[D];
 #pragma unroll
   for (int u = 0; u < D; ++u) {
-    cu[u] = ldu(p0 + 4 + u);
-    cv[u] = ldv_(p0 + 4 + u);
-    cb[u] = ldub(p0 + u);
-    cc[u] = ldvb(p0 + u);
+    cu[u] = ldu(p0 + 5 + u);
+    cv[u] = ldv_(p0 + 5 + u);
+    cb[u] = ldub(p0 + 1 + u);
+    cc[u] = ldvb(p0 + 1 + u);
   }
   for (int t0 = 0; t0 < nt; t0 += D) {
     T nu[D], nv[D], nb[D], nc[D];
 #pragma unroll
     for (int u = 0; u < D; ++u) {
-      nu[u] = ldu(p0 + 4 + D + t0 + u);
-      nv[u] = ldv_(p0 + 4 + D + t0 + u);
-      nb[u] = ldub(p0 + D + t0 + u);
-      nc[u] = ldvb(p0 + D + t0 + u);
+      nu[u] = ldu(p0 + 5 + D + t0 + u);
+      nv[u] = ldv_(p0 + 5 + D + t0 + u);
+      nb[u] = ldub(p0 + 1 + D + t0 + u);
+      nc[u] = ldvb(p0 + 1 + D + t0 + u);
     }
 #pragma unroll
     for (int u = 0; u < D; ++u)
@@ -857,14 +893,16 @@ int hadvuv5th_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
   a.vinbar = const_cast<T *>(p_vb);
   if (nx + 12 <= 640 && strip_for(K, true)) {
     const int lanes = int((nx + 12 + 63) / 64 * 64);
-    // Row chunks measured at 256x256x80 over 8-64: 32 rows fastest in both forms.
-    const int rows = strip_rows_for(K, 32);
+    // Row chunks measured at 256x256x80 over 8-64: 32 rows fastest in both
+    // forms.  At most 256 rows, which bounds the 1-D tables in LDS.
+    const int rows = std::min(strip_rows_for(K, 32), 256);
     const int nchunk = int((ny + 6 + rows - 1) / rows);
     if (int64_t(nchunk) * nz > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
     // Rows loaded one ahead: two ahead spills registers and measured slower.
     auto kern = strip_depth_for(K, 1) >= 2 ? hadvuv5th_adj_strip<T, 2> : hadvuv5th_adj_strip<T, 1>;
     hipLaunchKernelGGL(kern, dim3(unsigned(nchunk * nz)), dim3(unsigned(lanes)),
-                       size_t(lanes) * (11 * sizeof(T) + sizeof(int)), hipStream_t(stream), a, g, rows, nchunk);
+                       (size_t(lanes) * 16 + 4 * size_t(rows + 10)) * sizeof(T) + size_t(lanes) * 2 * sizeof(int),
+                       hipStream_t(stream), a, g, rows, nchunk);
     return launch_status(K);
   }
   const BoxGrid b1{0, 0, int(nx), int(ny)}, b2{-3, -3, int(nx + 6), int(ny + 6)};
