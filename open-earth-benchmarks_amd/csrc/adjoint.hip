@@ -667,9 +667,12 @@ struct HadvSt1 {
 // stage 1 of row p + 1, two independent chains.  acrlat0/1 and tgrlatda0/1 of
 // the chunk's rows come from LDS tables.  Rows of u and v are loaded D rows
 // ahead.  The terms are those of the two-pass form; they are summed in
-// another order.
+// another order.  The FP64 form needs 133 registers, which would hold 3
+// waves per SIMD; asked for 4 (amdgpu_waves_per_eu), it keeps 128 and spills
+// 24 bytes per lane, and measured 1.1x faster at 256x256x80.
 template <class T, int D>
-__global__ void __launch_bounds__(640) hadvuv5th_adj_strip(HadvAdj<T> a, Geo g, int rows, int nchunk) {
+__global__ void __launch_bounds__(640) __attribute__((amdgpu_waves_per_eu(4)))
+    hadvuv5th_adj_strip(HadvAdj<T> a, Geo g, int rows, int nchunk) {
   extern __shared__ __align__(16) unsigned char oeb_smem[];
   constexpr T rr = kEarthRadiusRecipAdj<T>;
   const int L = int(blockDim.x);
