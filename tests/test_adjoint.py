@@ -244,13 +244,15 @@ def test_hdiffsa_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
     assert torch.equal(out[0], out[1])
 
 
+@pytest.mark.parametrize("depth", [1, 2])
 @pytest.mark.parametrize("rows", [1, 4, 64])
 @pytest.mark.parametrize("shape", [(40, 36, 5), (67, 9, 3)], ids=lambda s: "x".join(map(str, s)))
-def test_hadvuv5th_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
+def test_hadvuv5th_adjoint_strip_matches_two_pass(shape, rows, depth, monkeypatch):
     """The fused row-march hadvuv5th adjoint (x contributions gathered from
-    LDS, y contributions pushed down each column) and the two-pass one (eight
-    scratch arrays), forced through OEB_STRIP_HADVUV5TH_ADJ, agree to 1e-13 of
-    the largest cotangent: the same terms, summed in another order."""
+    LDS, y contributions pushed down each column), with rows loaded one and
+    two ahead, and the two-pass one (eight scratch arrays), forced through
+    OEB_STRIP_HADVUV5TH_ADJ, agree to 1e-13 of the largest cotangent: the same
+    terms, summed in another order."""
     import torch
 
     L = Layout(*shape, h=3, hk=1)
@@ -260,8 +262,9 @@ def test_hadvuv5th_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
     xs = [_t(host["uin"]), _t(host["vin"])]
     g = torch.Generator(device="cuda").manual_seed(4)
     ybars = [torch.randn(x.shape, generator=g, device="cuda", dtype=x.dtype) for x in xs]
-    print(f"[gpu] hadvuv5th adjoint strip vs two-pass {shape} rows={rows}", flush=True)
+    print(f"[gpu] hadvuv5th adjoint strip vs two-pass {shape} rows={rows} depth={depth}", flush=True)
     out = []
+    monkeypatch.setenv("OEB_DEPTH_HADVUV5TH_ADJ", str(depth))
     for strip in (0, 1):
         monkeypatch.setenv("OEB_STRIP_HADVUV5TH_ADJ", str(strip))
         monkeypatch.setenv("OEB_ROWS_HADVUV5TH_ADJ", str(rows))
