@@ -1356,6 +1356,9 @@ int hdiffsmag_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
 //   K3  W2 = fx2_bar + 0.5 xfx fx1_bar + xfx (W0/ra_x - W0(i-1)/ra_x(i-1))   fx2's region
 //   K4  W3 = fy2_bar + 0.5 yfx fy1_bar + yfx (W1/ra_y - W1(j-1)/ra_y(j-1))   fy2's region
 //   K5  q_bar = area (W0/ra_x + W1/ra_y) + PPM_i^T(W2; q) + PPM_j^T(W3; q)   on q's box
+// W0 and W1 are used only through W0/ra_x and W1/ra_y, so K1 and K2 store
+// those quotients (work0, work1): each is divided once, and every later use
+// reads the same double.
 // q_i and q_j are the forward's outputs.  The FP32 forward forms q_i / q_j
 // with a compensated sum; the derivative is that of the same expression.
 
@@ -1445,7 +1448,7 @@ __global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k1(FvAdj<T> a, Geo g, Box
   const T r = ppm_transpose(
       a.q_j, a.cry, at, sj, a.p1, a.p2, [&](int o) { return R.in(i, j + o); },
       [&](int o) { return T(0.5) * a.yfx[at + int64_t(o) * sj] * a.fy1b[at + int64_t(o) * sj]; });
-  a.w0[at] = a.qjb[at] + r;
+  a.w0[at] = (a.qjb[at] + r) / a.ra_x[at];
 }
 
 template <class T>
@@ -1457,7 +1460,7 @@ __global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k2(FvAdj<T> a, Geo g, Box
   const T r = ppm_transpose(
       a.q_i, a.crx, at, 1, a.p1, a.p2, [&](int o) { return R.in(i + o, j); },
       [&](int o) { return T(0.5) * a.xfx[at + o] * a.fx1b[at + o]; });
-  a.w1[at] = a.qib[at] + r;
+  a.w1[at] = (a.qib[at] + r) / a.ra_y[at];
 }
 
 template <class T>
@@ -1469,8 +1472,8 @@ __global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k3(FvAdj<T> a, Geo g, Box
   T r = a.fx2b[at];
   if (R.in(i, j)) r += T(0.5) * a.xfx[at] * a.fx1b[at];
   T d = T(0);
-  if (R.qj(i, j)) d += a.w0[at] / a.ra_x[at];
-  if (R.qj(i - 1, j)) d -= a.w0[at - 1] / a.ra_x[at - 1];
+  if (R.qj(i, j)) d += a.w0[at];
+  if (R.qj(i - 1, j)) d -= a.w0[at - 1];
   a.w2[at] = r + a.xfx[at] * d;
 }
 
@@ -1484,8 +1487,8 @@ __global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k4(FvAdj<T> a, Geo g, Box
   T r = a.fy2b[at];
   if (R.in(i, j)) r += T(0.5) * a.yfx[at] * a.fy1b[at];
   T d = T(0);
-  if (R.qi(i, j)) d += a.w1[at] / a.ra_y[at];
-  if (R.qi(i, j - 1)) d -= a.w1[at - sj] / a.ra_y[at - sj];
+  if (R.qi(i, j)) d += a.w1[at];
+  if (R.qi(i, j - 1)) d -= a.w1[at - sj];
   a.w3[at] = r + a.yfx[at] * d;
 }
 
@@ -1497,8 +1500,8 @@ __global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k5(FvAdj<T> a, Geo g, Box
   const int sj = g.sj;
   const int64_t at = int64_t(blockIdx.y) * g.sk + i + int64_t(j) * sj;
   T r = T(0);
-  if (R.qj(i, j)) r += a.area[at] * (a.w0[at] / a.ra_x[at]);
-  if (R.qi(i, j)) r += a.area[at] * (a.w1[at] / a.ra_y[at]);
+  if (R.qj(i, j)) r += a.area[at] * a.w0[at];
+  if (R.qi(i, j)) r += a.area[at] * a.w1[at];
   r += ppm_transpose(
       a.q, a.crx, at, 1, a.p1, a.p2, [&](int o) { return R.fx2(i + o, j); }, [&](int o) { return a.w2[at + o]; });
   r += ppm_transpose(
@@ -1590,7 +1593,7 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxG
       int lx, ly;
       if (point(u, lx, ly)) {
         const int64_t at = plane + (i0 + lx) + int64_t(j0 + ly) * sj;
-        a.w0[at] = a.qjb[at] + ppm_gather<1>(L, lx, ly);
+        a.w0[at] = (a.qjb[at] + ppm_gather<1>(L, lx, ly)) / a.ra_x[at];
       }
     }
   } else if (MODE == 2) {  // K2
@@ -1602,7 +1605,7 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxG
       int lx, ly;
       if (point(u, lx, ly)) {
         const int64_t at = plane + (i0 + lx) + int64_t(j0 + ly) * sj;
-        a.w1[at] = a.qib[at] + ppm_gather<0>(L, lx, ly);
+        a.w1[at] = (a.qib[at] + ppm_gather<0>(L, lx, ly)) / a.ra_y[at];
       }
     }
   } else {  // K5, with K3 and K4 formed per face (w2 and w3 never go to memory)
@@ -1611,8 +1614,8 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxG
       T r = a.fx2b[m];
       if (R.in(i, j)) r += T(0.5) * a.xfx[m] * a.fx1b[m];
       T d = T(0);
-      if (R.qj(i, j)) d += a.w0[m] / a.ra_x[m];
-      if (R.qj(i - 1, j)) d -= a.w0[m - 1] / a.ra_x[m - 1];
+      if (R.qj(i, j)) d += a.w0[m];
+      if (R.qj(i - 1, j)) d -= a.w0[m - 1];
       return r + a.xfx[m] * d;
     };
     // W3 at face (i, j): fvtp2d_adj_k4's expression.
@@ -1620,8 +1623,8 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxG
       T r = a.fy2b[m];
       if (R.in(i, j)) r += T(0.5) * a.yfx[m] * a.fy1b[m];
       T d = T(0);
-      if (R.qi(i, j)) d += a.w1[m] / a.ra_y[m];
-      if (R.qi(i, j - 1)) d -= a.w1[m - sj] / a.ra_y[m - sj];
+      if (R.qi(i, j)) d += a.w1[m];
+      if (R.qi(i, j - 1)) d -= a.w1[m - sj];
       return r + a.yfx[m] * d;
     };
     ppm_products<0>(L, a.q, a.crx, plane, sj, i0, j0, p1, p2, [&](int i, int j) { return R.fx2(i, j); }, w2_at);
@@ -1643,8 +1646,8 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxG
         const int i = i0 + lx, j = j0 + ly;
         const int64_t at = plane + i + int64_t(j) * sj;
         T r = T(0);
-        if (R.qj(i, j)) r += a.area[at] * (a.w0[at] / a.ra_x[at]);
-        if (R.qi(i, j)) r += a.area[at] * (a.w1[at] / a.ra_y[at]);
+        if (R.qj(i, j)) r += a.area[at] * a.w0[at];
+        if (R.qi(i, j)) r += a.area[at] * a.w1[at];
         r += rx[u];
         r += ppm_gather<1>(L, lx, ly);
         a.qbar[at] = r;
