@@ -1569,11 +1569,13 @@ __device__ __forceinline__ T ppm_gather(const PpmLds<T> &L, int lx, int ly) {
   return r;
 }
 
+// Tile `blk` (of the box b) of one level of stage MODE, with the LDS of its
+// workgroup.
 template <class T, int MODE>
-__global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxGrid b) {
-  __shared__ PpmLds<T> L;
+__device__ __forceinline__ void fvtp2d_adj_ptile_body(PpmLds<T> &L, const FvAdj<T> &a, const Geo &g,
+                                                      const BoxGrid &b, int blk) {
   const int ntx = (b.ni + kPX - 1) / kPX;
-  const int i0 = b.ilo + int(blockIdx.x % ntx) * kPX, j0 = b.jlo + int(blockIdx.x / ntx) * kPY;
+  const int i0 = b.ilo + (blk % ntx) * kPX, j0 = b.jlo + (blk / ntx) * kPY;
   const int sj = g.sj;
   const int64_t plane = int64_t(blockIdx.y) * g.sk;
   const FvRegions R{g.nx, g.ny};
@@ -1656,9 +1658,27 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxG
   }
 }
 
-inline dim3 ptile_blocks(const BoxGrid &b, int nz) {
-  return dim3(unsigned(((b.ni + kPX - 1) / kPX) * ((b.nj + kPY - 1) / kPY)), unsigned(nz));
+template <class T, int MODE>
+__global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxGrid b) {
+  __shared__ PpmLds<T> L;
+  fvtp2d_adj_ptile_body<T, MODE>(L, a, g, b, int(blockIdx.x));
 }
+
+inline int ptile_count(const BoxGrid &b) { return ((b.ni + kPX - 1) / kPX) * ((b.nj + kPY - 1) / kPY); }
+
+// K1 and K2 in one launch: the first n1 tiles of each level run K1 on b1,
+// the others K2 on b2.  They read different fields and write work0 and work1.
+template <class T>
+__global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile12(FvAdj<T> a, Geo g, BoxGrid b1, BoxGrid b2, int n1) {
+  __shared__ PpmLds<T> L;
+  const int blk = int(blockIdx.x);
+  if (blk < n1)
+    fvtp2d_adj_ptile_body<T, 1>(L, a, g, b1, blk);
+  else
+    fvtp2d_adj_ptile_body<T, 2>(L, a, g, b2, blk - n1);
+}
+
+inline dim3 ptile_blocks(const BoxGrid &b, int nz) { return dim3(unsigned(ptile_count(b)), unsigned(nz)); }
 
 template <class T, class S>
 int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, const S *ra_x, const S *ra_y,
@@ -1718,14 +1738,14 @@ int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, 
   hipStream_t st = hipStream_t(stream);
   // The tiled transposes (fvtp2d_adj_ptile) by default; OEB_TILE_FVTP2D_ADJ=0
   // selects the per-point gathers.
-  // The tiled form (fvtp2d_adj_ptile: three launches, K3 and K4 formed inside
-  // K5) by default; OEB_TILE_FVTP2D_ADJ=0 selects the five per-point passes,
+  // The tiled form (fvtp2d_adj_ptile: K1 and K2 in one launch, then K5, which
+  // forms K3 and K4 inside) by default; OEB_TILE_FVTP2D_ADJ=0 selects the five per-point passes,
   // with W2 and W3 through work2 and work3.
   const bool tiled = env_choice("OEB_TILE_", K, 1, 0, 1) != 0;
   if (tiled) {
-    hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 1>), ptile_blocks(bqj, int(nz)), dim3(kPNT), 0, st, a, g, bqj);
-    OEB_TRY(launch_status(K));
-    hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 2>), ptile_blocks(bqi, int(nz)), dim3(kPNT), 0, st, a, g, bqi);
+    const int n1 = ptile_count(bqj), n2 = ptile_count(bqi);
+    hipLaunchKernelGGL(fvtp2d_adj_ptile12<T>, dim3(unsigned(n1 + n2), unsigned(nz)), dim3(kPNT), 0, st, a, g, bqj,
+                       bqi, n1);
     OEB_TRY(launch_status(K));
     hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 5>), ptile_blocks(bq, int(nz)), dim3(kPNT), 0, st, a, g, bq);
     return launch_status(K);
