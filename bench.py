@@ -99,6 +99,28 @@ def _traffic(size: int, nz: int, kernel: str):
         return None
 
 
+def host_info() -> dict:
+    """The host the CPU baseline ran on: CPU model, logical CPUs, physical
+    cores and sockets (from /proc/cpuinfo, where present)."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import binding as B
+
+    model, sockets = None, set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "model name" and model is None:
+                    model = val
+                elif key == "physical id":
+                    sockets.add(val)
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "physical_cores": B.cpu_count_physical(),
+            "sockets": len(sockets) or None}
+
+
 def cpu_baseline(names, hosts, L, budget_s: float):
     """Times the reference's kernels (or the oracle when the reference build is
     absent) on this host over whole steps of the same workload."""
@@ -137,6 +159,7 @@ def cpu_baseline(names, hosts, L, budget_s: float):
         "kind": kind,
         "sample": f"{steps} step(s) of {len(names)} stencils at {L.nx}x{L.ny}x{L.nz}, {what}, {threads} thread(s)",
         "per_stencil_ms": {n: round(1e3 * statistics.median(v), 3) for n, v in per.items()},
+        "host": host_info(),
     }
 
 
