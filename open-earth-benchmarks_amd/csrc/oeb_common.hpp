@@ -49,18 +49,29 @@ struct Box {
 };
 
 // Tile layout of a column launch: logical id -> (tile_i, tile_j, chunk),
-// tile_i fastest.
+// tile_i fastest.  With xcd = g > 1, g consecutive level chunks are
+// interleaved between the rows of tiles: id = ci + ti (sub + g (cj + tj kk)),
+// chunk = g kk + sub.  Workgroup ids go round-robin to the 8 XCDs, so with
+// ti g = 8 a tile and the tile below it (id + 8) run on the same XCD, and the
+// halo rows they share are fetched into that XCD's L2 once.
 struct Tiles {
   int ti, tj, nchunk, kchunk, n;
+  int xcd = 1;
 };
 
-inline Tiles make_tiles(int64_t ni, int64_t nj, int nz, int tile_i, int tile_j, int kchunk) {
+// Whether kernel `kname` interleaves level chunks (Tiles::xcd): OEB_XCD_<KNAME>
+// when it holds 0 or 1, else `dflt`.
+bool xcd_for(const char *kname, bool dflt);
+
+inline Tiles make_tiles(int64_t ni, int64_t nj, int nz, int tile_i, int tile_j, int kchunk,
+                        bool interleave = false) {
   Tiles t;
   t.ti = int((ni + tile_i - 1) / tile_i);
   t.tj = int((nj + tile_j - 1) / tile_j);
   t.kchunk = kchunk < 1 ? 1 : kchunk;
   t.nchunk = (nz + t.kchunk - 1) / t.kchunk;
   t.n = t.ti * t.tj * t.nchunk;
+  if (interleave && t.ti < 8 && 8 % t.ti == 0 && t.nchunk % (8 / t.ti) == 0) t.xcd = 8 / t.ti;
   return t;
 }
 
@@ -158,9 +169,14 @@ struct TileCoord {
 // Tile coordinates of logical block `id` of a launch (tile_i fastest).
 __device__ __forceinline__ TileCoord tile_coord_of(const Tiles &t, int id, int tile_i, int tile_j, int nz) {
   const int ci = id % t.ti;
-  const int rest = id / t.ti;
+  int rest = id / t.ti;
+  int sub = 0;
+  if (t.xcd > 1) {
+    sub = rest % t.xcd;
+    rest /= t.xcd;
+  }
   const int cj = rest % t.tj;
-  const int ck = rest / t.tj;
+  const int ck = (rest / t.tj) * t.xcd + sub;
   TileCoord c;
   c.i0 = ci * tile_i;
   c.j0 = cj * tile_j;

@@ -125,3 +125,14 @@ def test_f32_fvtp2d_strip_shapes(lanes, depth, monkeypatch):
     print(f"[gpu] f32 fvtp2d strip lanes={lanes} depth={depth}", flush=True)
     _check_exact("fvtp2d", L, cast("fvtp2d", host, "f32"))
 
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("xcd", [0, 1])
+@pytest.mark.parametrize("shape", [(64, 36, 8), (200, 9, 4), (256, 20, 6)], ids=lambda s: "x".join(map(str, s)))
+def test_f32_laplace_xcd_interleave(shape, xcd, monkeypatch):
+    """FP32 laplace with and without the level-chunk interleave agrees bit
+    for bit with the FP32 oracle."""
+    monkeypatch.setenv("OEB_XCD_LAPLACE", str(xcd))
+    L, host = inputs("laplace", *shape, "baseline", seed=23)
+    print(f"[gpu] f32 laplace xcd={xcd} {shape}", flush=True)
+    _check_exact("laplace", L, cast("laplace", host, "f32"))

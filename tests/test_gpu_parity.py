@@ -121,6 +121,20 @@ def test_strip_switches(name, shape, strip, rows, monkeypatch):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("xcd", [0, 1])
+@pytest.mark.parametrize("shape", [(64, 36, 8), (200, 9, 4), (256, 20, 6), (66, 17, 2)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_laplace_xcd_interleave(shape, xcd, monkeypatch):
+    """laplace with level chunks interleaved between tile rows (OEB_XCD_LAPLACE)
+    and without, on shapes where the interleave applies (8 or 4 chunks per
+    group) and where it does not (chunks not a multiple of the group, or the
+    scalar path at nx = 66), agrees bit for bit with the oracle."""
+    monkeypatch.setenv("OEB_XCD_LAPLACE", str(xcd))
+    L, host = inputs("laplace", *shape, "baseline", seed=22)
+    print(f"[gpu] laplace xcd={xcd} {shape}", flush=True)
+    _check("laplace", L, host, run_gpu("laplace", copy(host), L), require_exact=True)
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("depth", [1, 2])
 @pytest.mark.parametrize("lanes", [64, 128, 512, 1024])
 @pytest.mark.parametrize("family", ["baseline", "stress"])
