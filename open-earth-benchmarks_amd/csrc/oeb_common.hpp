@@ -14,7 +14,8 @@
 // deals consecutive workgroup ids round-robin to the 8 XCDs, so neighbouring
 // tiles run at the same time on different XCDs.  Their shared halo lines are
 // then served by the MALL, which all XCDs share.  Remapping ids so that each
-// XCD owns a contiguous run measured 3-10% slower on every stencil shape tried.
+// XCD owns a contiguous run measured 3-10% slower on every stencil shape tried;
+// laplace interleaves level chunks between tile rows instead (Tiles::xcd).
 //
 // Memory access.  Outputs are written once and never read back, so they are
 // stored non-temporally (st_nt).  Inputs read only at the centre point are
