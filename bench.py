@@ -20,9 +20,9 @@ bit), with the fraction of points whose error exceeds 1e-5 of the output's
 max-norm.
 
 chain: the forward chain hadvuv5th -> hdiffsmag -> fastwavesuv -> uvbke with
-periodic halo refills (oeb.chain), 10 steps at the same size, enqueued
-eagerly and replayed as one recorded HIP graph; ms per step for each, per
-form.
+periodic halo refills (oeb.chain: one launch refills both winds), 10 steps
+at the same size, enqueued eagerly and replayed as one recorded HIP graph;
+ms per step for each, per form.
 
 ensemble: 48 members, nx and ny drawn from {96,...,512} and nz from
 {60, 80, 137}, FP32 with FP64 islands, 10 steps of the same chain: one launch
@@ -221,9 +221,12 @@ def chain_block(L, forms, seed, steps: int = 10, reps: int = 5):
     """ms per step of the forward chain, eager and as a replayed graph."""
     import torch
 
-    from oeb.chain import Chain, forward_host_fields, forward_program, to_device
+    from oeb.chain import Chain, Halo, forward_host_fields, forward_program, to_device
 
-    out = {"stencils": ["hadvuv5th", "hdiffsmag", "fastwavesuv", "uvbke"], "halo_fills_per_step": 6,
+    prog = forward_program()
+    out = {"stencils": ["hadvuv5th", "hdiffsmag", "fastwavesuv", "uvbke"],
+           "halo_refills_per_step": sum(1 for op in prog if isinstance(op, Halo)),
+           "fields_refilled_per_step": sum(len(op.fields) for op in prog if isinstance(op, Halo)),
            "steps_per_measurement": steps}
     side = torch.cuda.Stream()
     for prec in forms:

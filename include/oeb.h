@@ -282,19 +282,29 @@ int oeb_fvtp2d_f32(const oeb_domain *d, const oeb_s3f *q_i, const oeb_s3f *q_j, 
  * No counterpart in the reference: its kernels write only their output
  * regions and only the drivers' fillMath (include/util.h:155-163) writes
  * halos.  A chain of stencils, whose outputs feed the next stencil's inputs,
- * refills the halo the next stencil reads with these.  Two launches on
- * `stream`:
+ * refills the halo the next stencil reads with these.  One launch on
+ * `stream` writes
  *   - the i/j ring [-halo_ij, nx+halo_ij) x [-halo_ij, ny+halo_ij) minus the
- *     interior, on levels [0, nz): OEB_HALO_PERIODIC copies the interior
- *     point at the wrapped index (i mod nx, j mod ny), OEB_HALO_ZERO_GRADIENT
- *     the one at the clamped index;
+ *     interior, on levels [0, nz), and
  *   - the k-halo planes [-halo_k, 0) and [nz, nz+halo_k) over the whole i/j
- *     box: copies of planes 0 and nz-1, in both modes.
- * halo_ij and halo_k lie in [0, 16]; the descriptor must cover the box. */
+ *     box.
+ * Each of these points takes the value of the interior point at the wrapped
+ * (OEB_HALO_PERIODIC: i mod nx, j mod ny) or clamped (OEB_HALO_ZERO_GRADIENT)
+ * i/j index on the clamped level (so the k-halo planes are copies of planes 0
+ * and nz-1, rings included, in both modes).
+ * halo_ij and halo_k lie in [0, 16]; the descriptor must cover the box.
+ * oeb_halo_fill_n refills 1..8 fields of one layout in one launch (what a
+ * chain does before a stencil that reads several refilled fields); the
+ * fields' boxes must not overlap.  Same result as one oeb_halo_fill per
+ * field. */
 #define OEB_HALO_PERIODIC 0
 #define OEB_HALO_ZERO_GRADIENT 1
 int oeb_halo_fill_f64(const oeb_domain *d, const oeb_s3 *field, int halo_ij, int halo_k, int mode, void *stream);
 int oeb_halo_fill_f32(const oeb_domain *d, const oeb_s3f *field, int halo_ij, int halo_k, int mode, void *stream);
+int oeb_halo_fill_n_f64(const oeb_domain *d, const oeb_s3 *const *fields, int nfields, int halo_ij, int halo_k,
+                        int mode, void *stream);
+int oeb_halo_fill_n_f32(const oeb_domain *d, const oeb_s3f *const *fields, int nfields, int halo_ij, int halo_k,
+                        int mode, void *stream);
 
 /* ----------------------------------------------------------------- graphs
  * No counterpart in the reference.  Records every launch the oeb_* entry
@@ -408,7 +418,7 @@ int oeb_tune_clear(void);
  * Runs one stencil over many members of an ensemble in ONE launch.
  *   oeb_batch_record(b)  from now on, on this host thread, the batchable
  *                        entry points (hadvuv, hadvuv5th, hdiffsmag,
- *                        fastwavesuv, uvbke and oeb_halo_fill, both
+ *                        fastwavesuv, uvbke and oeb_halo_fill(_n), both
  *                        precisions) validate their arguments as usual and
  *                        record their launch into b instead of launching;
  *                        the stream argument is ignored.  Other entry points

@@ -5,20 +5,26 @@
 // only the drivers' fillMath (include/util.h:155-163) ever writes a halo.  A
 // chain of stencils, whose outputs feed the next stencil's inputs, has to
 // refill the halo a stencil reads before it runs.  oeb_halo_fill_* does that
-// in two launches on the caller's stream:
+// for up to kMaxFields fields of one layout in ONE launch on the caller's
+// stream, over
 //   1. the i/j ring [-h, nx+h) x [-h, ny+h) minus the interior, on levels
-//      [0, nz): periodic (wrapped index) or zero-gradient (clamped index);
-//   2. the k-halo planes [-hk, 0) and [nz, nz+hk) over the whole i/j box:
-//      copies of planes 0 and nz-1 (zero-gradient in k whatever the mode,
-//      since launch 1 has already filled those planes' rings).
-// Both are pure copies, HBM-bound and small next to the stencils: the ring is
-// 2h(nx + ny + 2h) points per level.
+//      [0, nz), and
+//   2. the k-halo planes [-hk, 0) and [nz, nz+hk) over the whole i/j box.
+// Every halo point copies the interior point at its wrapped (periodic) or
+// clamped (zero-gradient) i/j index on its clamped level (zero-gradient in k
+// whatever the mode).  Sources are interior points only, so no point of the
+// launch reads what another writes and one launch does both parts.  It is a
+// pure copy of 2h(nx + ny + 2h) points per level plus 2hk planes, HBM-bound
+// and small next to the stencils; at the sizes of a chain its time is the
+// fixed cost of a launch, which one launch for all fields of a refill pays
+// once.
 #include "oeb_common.hpp"
 
 namespace oeb {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kMaxFields = 8;
 
 __device__ __forceinline__ int wrap(int a, int n) {
   const int r = a % n;
@@ -46,47 +52,41 @@ __device__ __forceinline__ void ring_point(int64_t e, int nx, int ny, int h, int
 }
 
 template <class T>
-struct RingArgs {
-  T *f;
-  int nx, ny, nz, h, sj, periodic;
-  int64_t sk, ring;
-};
-
-template <class T>
-struct KArgs {
-  T *f;
-  int nx, ny, nz, h, hk, sj;
+struct HaloArgs {
+  T *f[kMaxFields];
+  int nx, ny, nz, h, hk, sj, periodic;
+  int bpf;         // logical blocks per field
   int64_t sk;
+  int64_t ring;    // ring points per level
+  int64_t nring;   // ring * nz: ring points of all levels, before the k planes
+  int64_t npts;    // halo points per field
 };
 
-// Logical block `id`: ring elements [id*256, id*256 + 256) of all levels.
+// Logical block `id`: halo points [b*256, b*256 + 256) of field id / bpf,
+// b = id % bpf; the ring points of levels 0..nz-1 first, then the 2hk k-halo
+// planes (-hk..-1, then nz..nz+hk-1), each the whole i/j box.
 template <class T>
-__device__ __forceinline__ void halo_ring_body(const RingArgs<T> &a, const Geo &, const Tiles &, int id) {
-  const int64_t e = int64_t(id) * kThreads + threadIdx.x;
-  if (e >= a.ring * a.nz) return;
-  const int k = int(e / a.ring);
-  int i, j;
-  ring_point(e % a.ring, a.nx, a.ny, a.h, i, j);
+__device__ __forceinline__ void halo_body(const HaloArgs<T> &a, const Geo &, const Tiles &, int id) {
+  const int fld = id / a.bpf;
+  const int64_t e = int64_t(id - fld * a.bpf) * kThreads + threadIdx.x;
+  if (e >= a.npts) return;
+  int i, j, k;
+  if (e < a.nring) {
+    k = int(e / a.ring);
+    ring_point(e - int64_t(k) * a.ring, a.nx, a.ny, a.h, i, j);
+  } else {
+    const int64_t w = a.nx + 2 * a.h, box = w * (a.ny + 2 * a.h);
+    const int64_t f = e - a.nring;
+    const int q = int(f / box);
+    const int64_t p = f - int64_t(q) * box;
+    k = q < a.hk ? q - a.hk : a.nz + (q - a.hk);
+    i = -a.h + int(p % w), j = -a.h + int(p / w);
+  }
   const int si = a.periodic ? wrap(i, a.nx) : clampi(i, a.nx);
   const int sjj = a.periodic ? wrap(j, a.ny) : clampi(j, a.ny);
-  const int64_t plane = int64_t(k) * a.sk;
-  a.f[plane + i + int64_t(j) * a.sj] = a.f[plane + si + int64_t(sjj) * a.sj];
-}
-
-// Logical block `id`: elements [id*256, id*256 + 256) of the 2*hk k-halo
-// planes (each the whole i/j box).
-template <class T>
-__device__ __forceinline__ void halo_k_body(const KArgs<T> &a, const Geo &, const Tiles &, int id) {
-  const int64_t w = a.nx + 2 * a.h, box = w * (a.ny + 2 * a.h);
-  const int64_t e = int64_t(id) * kThreads + threadIdx.x;
-  if (e >= box * 2 * a.hk) return;
-  const int64_t p = e % box;
-  const int q = int(e / box);  // 0 .. 2hk-1: planes -hk..-1, then nz..nz+hk-1
-  const int k = q < a.hk ? q - a.hk : a.nz + (q - a.hk);
-  const int ks = q < a.hk ? 0 : a.nz - 1;
-  const int i = -a.h + int(p % w), j = -a.h + int(p / w);
-  const int64_t at = i + int64_t(j) * a.sj;
-  a.f[int64_t(k) * a.sk + at] = a.f[int64_t(ks) * a.sk + at];
+  const int ks = clampi(k, a.nz);
+  T *f = a.f[fld];
+  f[int64_t(k) * a.sk + i + int64_t(j) * a.sj] = f[int64_t(ks) * a.sk + si + int64_t(sjj) * a.sj];
 }
 
 template <class A, void (*Body)(const A &, const Geo &, const Tiles &, int)>
@@ -95,54 +95,53 @@ __global__ void __launch_bounds__(kThreads) halo_kernel(A a, int blocks) {
 }
 
 template <class T>
-hipError_t halo_ring_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
-  return launch_batch<RingArgs<T>, halo_ring_body<T>>(tab, n, total, s, dim3(kThreads));
+hipError_t halo_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
+  return launch_batch<HaloArgs<T>, halo_body<T>>(tab, n, total, s, dim3(kThreads));
 }
 template <class T>
-hipError_t halo_k_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
-  return launch_batch<KArgs<T>, halo_k_body<T>>(tab, n, total, s, dim3(kThreads));
-}
-template <class T>
-const BatchKind kHaloRingBatch{"halo_fill (ring)", halo_ring_batch<T>};
-template <class T>
-const BatchKind kHaloKBatch{"halo_fill (k planes)", halo_k_batch<T>};
+const BatchKind kHaloBatch{"halo_fill", halo_batch<T>};
 
 int grid_for(int64_t blocks) { return int(blocks < 8192 ? (blocks < 1 ? 1 : blocks) : 8192); }
 
+const char *const kFieldName[kMaxFields] = {"fields[0]", "fields[1]", "fields[2]", "fields[3]",
+                                            "fields[4]", "fields[5]", "fields[6]", "fields[7]"};
+
 template <class T, class S>
-int halo_run(const oeb_domain *d, const S *field, int halo_ij, int halo_k, int mode, void *stream) {
-  const char *K = "halo_fill";
+int halo_run(const char *K, const oeb_domain *d, const S *const *fields, int nfields, int halo_ij, int halo_k,
+             int mode, void *stream) {
   OEB_TRY(check_domain(K, d));
+  if (fields == nullptr) return fail(OEB_ERR_NULL, std::string(K) + ": fields is NULL");
+  if (nfields < 1 || nfields > kMaxFields)
+    return fail(OEB_ERR_ARG, std::string(K) + ": nfields must lie in [1, " + std::to_string(kMaxFields) + "]");
   if (halo_ij < 0 || halo_ij > 16 || halo_k < 0 || halo_k > 16)
-    return fail(OEB_ERR_ARG, "halo_fill: halo_ij and halo_k must lie in [0, 16]");
+    return fail(OEB_ERR_ARG, std::string(K) + ": halo_ij and halo_k must lie in [0, 16]");
   if (mode != OEB_HALO_PERIODIC && mode != OEB_HALO_ZERO_GRADIENT)
-    return fail(OEB_ERR_ARG, "halo_fill: mode must be OEB_HALO_PERIODIC or OEB_HALO_ZERO_GRADIENT");
+    return fail(OEB_ERR_ARG, std::string(K) + ": mode must be OEB_HALO_PERIODIC or OEB_HALO_ZERO_GRADIENT");
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz, h = halo_ij, hk = halo_k;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box b{-h, nx + h, -h, ny + h, -hk, nz + hk};
-  const T *p;
-  OEB_TRY(check3(K, "field", field, b, g, true, &p));
-  T *f = const_cast<T *>(p);
-  if (h > 0) {
-    const int64_t ring = 2 * h * (nx + 2 * h) + ny * 2 * h;
-    const int64_t blocks = (ring * nz + kThreads - 1) / kThreads;
-    if (blocks > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, "halo_fill: domain too large");
-    const RingArgs<T> a{f, int(nx), int(ny), int(nz), int(h), g.sj, mode == OEB_HALO_PERIODIC ? 1 : 0, g.sk, ring};
-    if (!batch_record(kHaloRingBatch<T>, a, g, Tiles{}, int(blocks))) {
-      hipLaunchKernelGGL((halo_kernel<RingArgs<T>, halo_ring_body<T>>), dim3(grid_for(blocks)), dim3(kThreads), 0,
-                         hipStream_t(stream), a, int(blocks));
-      OEB_TRY(launch_status(K));
-    }
+  HaloArgs<T> a{};
+  AliasSet as;
+  for (int n = 0; n < nfields; ++n) {
+    const T *p;
+    OEB_TRY(check3(K, nfields == 1 ? "field" : kFieldName[n], fields[n], b, g, n == 0, &p));
+    as.add(kFieldName[n], p, b, g, true);
+    a.f[n] = const_cast<T *>(p);
   }
-  if (hk > 0) {
-    const int64_t blocks = ((nx + 2 * h) * (ny + 2 * h) * 2 * hk + kThreads - 1) / kThreads;
-    if (blocks > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, "halo_fill: domain too large");
-    const KArgs<T> a{f, int(nx), int(ny), int(nz), int(h), int(hk), g.sj, g.sk};
-    if (!batch_record(kHaloKBatch<T>, a, g, Tiles{}, int(blocks))) {
-      hipLaunchKernelGGL((halo_kernel<KArgs<T>, halo_k_body<T>>), dim3(grid_for(blocks)), dim3(kThreads), 0,
-                         hipStream_t(stream), a, int(blocks));
-      OEB_TRY(launch_status(K));
-    }
+  OEB_TRY(as.check(K));
+  const int64_t ring = 2 * h * (nx + 2 * h) + ny * 2 * h;
+  const int64_t npts = ring * nz + (nx + 2 * h) * (ny + 2 * h) * 2 * hk;
+  if (npts == 0) return OEB_OK;
+  const int64_t bpf = (npts + kThreads - 1) / kThreads;
+  if (bpf * nfields > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
+  a.nx = int(nx), a.ny = int(ny), a.nz = int(nz), a.h = int(h), a.hk = int(hk), a.sj = g.sj;
+  a.periodic = mode == OEB_HALO_PERIODIC ? 1 : 0;
+  a.bpf = int(bpf), a.sk = g.sk, a.ring = ring, a.nring = ring * nz, a.npts = npts;
+  const int blocks = int(bpf * nfields);
+  if (!batch_record(kHaloBatch<T>, a, g, Tiles{}, blocks)) {
+    hipLaunchKernelGGL((halo_kernel<HaloArgs<T>, halo_body<T>>), dim3(grid_for(blocks)), dim3(kThreads), 0,
+                       hipStream_t(stream), a, blocks);
+    OEB_TRY(launch_status(K));
   }
   return OEB_OK;
 }
@@ -152,12 +151,22 @@ int halo_run(const oeb_domain *d, const S *field, int halo_ij, int halo_k, int m
 
 extern "C" int oeb_halo_fill_f64(const oeb_domain *d, const oeb_s3 *field, int halo_ij, int halo_k, int mode,
                                  void *stream) {
-  return oeb::halo_run<double>(d, field, halo_ij, halo_k, mode, stream);
+  return oeb::halo_run<double>("halo_fill", d, &field, 1, halo_ij, halo_k, mode, stream);
 }
 
 extern "C" int oeb_halo_fill_f32(const oeb_domain *d, const oeb_s3f *field, int halo_ij, int halo_k, int mode,
                                  void *stream) {
-  return oeb::halo_run<float>(d, field, halo_ij, halo_k, mode, stream);
+  return oeb::halo_run<float>("halo_fill", d, &field, 1, halo_ij, halo_k, mode, stream);
+}
+
+extern "C" int oeb_halo_fill_n_f64(const oeb_domain *d, const oeb_s3 *const *fields, int nfields, int halo_ij,
+                                   int halo_k, int mode, void *stream) {
+  return oeb::halo_run<double>("halo_fill_n", d, fields, nfields, halo_ij, halo_k, mode, stream);
+}
+
+extern "C" int oeb_halo_fill_n_f32(const oeb_domain *d, const oeb_s3f *const *fields, int nfields, int halo_ij,
+                                   int halo_k, int mode, void *stream) {
+  return oeb::halo_run<float>("halo_fill_n", d, fields, nfields, halo_ij, halo_k, mode, stream);
 }
 
 // ------------------------------------------------------------------ graphs

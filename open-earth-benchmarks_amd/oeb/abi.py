@@ -124,9 +124,9 @@ def signature(name: str, precision: str = "f64") -> str:
 
 EXPORTS = ["oeb_version", "oeb_status_string", "oeb_last_error"] + [
     f"oeb_{k}_{p}" for p in PRECISIONS for k in SIGNATURES
-] + ["oeb_halo_fill_f64", "oeb_halo_fill_f32", "oeb_graph_begin", "oeb_graph_end", "oeb_graph_launch",
-     "oeb_graph_destroy", "oeb_batch_create", "oeb_batch_record", "oeb_batch_stop", "oeb_batch_launch",
-     "oeb_batch_info", "oeb_batch_destroy", "oeb_tune_set", "oeb_tune_get", "oeb_tune_clear"] + [
+] + ["oeb_halo_fill_f64", "oeb_halo_fill_f32", "oeb_halo_fill_n_f64", "oeb_halo_fill_n_f32",
+     "oeb_graph_begin", "oeb_graph_end", "oeb_graph_launch", "oeb_graph_destroy", "oeb_batch_create",
+     "oeb_batch_record", "oeb_batch_stop", "oeb_batch_launch", "oeb_batch_info", "oeb_batch_destroy", "oeb_tune_set", "oeb_tune_get", "oeb_tune_clear"] + [
     f"oeb_{k}_adj_{p}" for p in PRECISIONS for k in ADJOINTS
 ]
 
@@ -180,6 +180,9 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
         fn = getattr(lib, f"oeb_halo_fill_{prec}")
         fn.restype = C.c_int
         fn.argtypes = [PD, P3, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        fn = getattr(lib, f"oeb_halo_fill_n_{prec}")
+        fn.restype = C.c_int
+        fn.argtypes = [PD, C.POINTER(P3), C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
     lib.oeb_graph_begin.restype = C.c_int
     lib.oeb_graph_begin.argtypes = [C.c_void_p]
     lib.oeb_graph_end.restype = C.c_int

@@ -6,7 +6,8 @@ The reference couples nothing: each kernel writes only its output region
 (kernels/<name>.h), and only the drivers' fillMath (include/util.h:155-163)
 writes halos.  A chain maps each stencil's outputs onto the next one's inputs
 by name, and refills, before each call, the halo that call reads
-(oeb_halo_fill_*: periodic or zero-gradient in i/j, zero-gradient in k).
+(oeb_halo_fill_n_*: periodic or zero-gradient in i/j, zero-gradient in k;
+one launch for all fields of a refill).
 
 The program is plain data (Halo and Call records), so the same step runs on
 the GPU here and, in the tests, on the host with the oracle.
@@ -51,6 +52,20 @@ def halo_fill(f: Storage3D, domain, h: int, hk: int = 0, mode: int = abi.OEB_HAL
     abi.check(fn(C.byref(dom), f.ref(), int(h), int(hk), int(mode), stencils._stream(stream)))
 
 
+def halo_fill_n(fs: list, domain, h: int, hk: int = 0, mode: int = abi.OEB_HALO_PERIODIC, stream=None) -> None:
+    """Refills the halos of 1..8 fields of one layout and precision in one
+    launch (oeb_halo_fill_n_*); the same result as halo_fill on each."""
+    lib = abi.load()
+    nx, ny, nz = domain
+    dom = abi.OebDomain(nx, ny, nz)
+    precs = {f.precision for f in fs}
+    if len(precs) != 1:
+        raise ValueError(f"halo_fill_n: fields of one precision expected, got {sorted(precs)}")
+    fn = getattr(lib, f"oeb_halo_fill_n_{precs.pop()}")
+    ptrs = (abi.P3 * len(fs))(*[C.pointer(f.desc) for f in fs])
+    abi.check(fn(C.byref(dom), ptrs, len(fs), int(h), int(hk), int(mode), stencils._stream(stream)))
+
+
 class Graph:
     """A recorded step (oeb_graph_*).  launch() replays it on a stream."""
 
@@ -89,8 +104,7 @@ class Chain:
     def run_op(self, op, stream=None) -> None:
         """Enqueues one op of the program on `stream`."""
         if isinstance(op, Halo):
-            for name in op.fields:
-                halo_fill(self.fields[name], self.domain, op.h, op.hk, op.mode, stream)
+            halo_fill_n([self.fields[n] for n in op.fields], self.domain, op.h, op.hk, op.mode, stream)
         else:
             spec = KERNELS[op.stencil]
             arrays = [self.fields[op.bind[a.name]] if a.name in op.bind else None for a in spec.args]

@@ -1,19 +1,24 @@
-"""Chains: the halo update (oeb_halo_fill_*), the forward workload chain
-hadvuv5th -> hdiffsmag -> fastwavesuv -> uvbke, and its HIP-graph replay.
+"""Chains: the halo update (oeb_halo_fill_*, oeb_halo_fill_n_*), the forward
+workload chain hadvuv5th -> hdiffsmag -> fastwavesuv -> uvbke, and its
+HIP-graph replay.
 
-CPU: the host halo fill against its definition, and the host chain (oracle
-plus host halo fill) staying bounded.  GPU: the halo fill and the chain, bit
-for bit against the host, and a recorded step replayed several times, with
-eager work and synchronisation in between, against eager steps.
+CPU: the host halo fill against its definition, the host chain (oracle plus
+host halo fill) staying bounded, and the argument checks of the multi-field
+fill.  GPU: the halo fills and the chain, bit for bit against the host, and a
+recorded step replayed several times, with eager work and synchronisation in
+between, against eager steps.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
 from chain_host import halo_fill_host, run_host
 from common import copy
 from oeb import abi
-from oeb.chain import Chain, forward_host_fields, forward_program, halo_fill, to_device
+from oeb.chain import Chain, forward_host_fields, forward_program, halo_fill, halo_fill_n, to_device
 from oeb.layout import Layout
+from oeb.storage import Storage3D
 
 STATE = ("u", "v", "u1", "v1", "u2", "v2", "u3", "v3")
 
@@ -68,6 +73,60 @@ def test_halo_fill_gpu(case, precision):
     b = a.copy()
     halo_fill_host(b, L, h, hk, mode)
     assert np.array_equal(d.to_host().view(np.uint8), b.view(np.uint8))
+
+
+def test_halo_fill_n_rejects_bad_calls():
+    """Validation only (host arrays, nothing launched): the field count, the
+    fields' layouts, overlap between fields, halo widths and mode."""
+    L = Layout(8, 8, 2, h=3, hk=1, align=1)
+    a, b = Storage3D(L, np.zeros(L.size)), Storage3D(L, np.zeros(L.size))
+    dom = (8, 8, 2)
+    for fs, kw, status in [
+        ([a] * 9, {}, abi.OEB_ERR_ARG),
+        ([a, a], {}, abi.OEB_ERR_ALIAS),
+        ([a, Storage3D(Layout(8, 8, 2, h=3, hk=1, align=16), np.zeros(Layout(8, 8, 2, h=3, hk=1, align=16).size))], {},
+         abi.OEB_ERR_LAYOUT),
+        ([a, b], {"h": 4}, abi.OEB_ERR_LAYOUT),
+        ([a, b], {"h": 1, "hk": 2}, abi.OEB_ERR_LAYOUT),
+        ([a, b], {"h": 17}, abi.OEB_ERR_ARG),
+        ([a, b], {"h": 1, "mode": 7}, abi.OEB_ERR_ARG),
+    ]:
+        args = {"h": 3, "hk": 0, "mode": abi.OEB_HALO_PERIODIC}
+        args.update(kw)
+        with pytest.raises(abi.OebError) as e:
+            halo_fill_n(fs, dom, args["h"], args["hk"], args["mode"])
+        assert e.value.status == status, (len(fs), kw, e.value.message)
+    lib = abi.load()
+    ptrs = (abi.P3 * 1)(C.pointer(a.desc))
+    assert lib.oeb_halo_fill_n_f64(C.byref(abi.OebDomain(8, 8, 2)), ptrs, 0, 1, 0, 0, None) == abi.OEB_ERR_ARG
+    assert lib.oeb_halo_fill_n_f64(C.byref(abi.OebDomain(8, 8, 2)), None, 1, 1, 0, 0, None) == abi.OEB_ERR_NULL
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("case", HALO_CASES + [(33, 20, 7, 2, 2, abi.OEB_HALO_PERIODIC)],
+                         ids=lambda c: "x".join(map(str, c[:3])) + f"h{c[3]}k{c[4]}m{c[5]}")
+@pytest.mark.parametrize("nfields", [1, 3, 8])
+def test_halo_fill_n_gpu(case, precision, nfields):
+    """nfields fields refilled in one launch, each bit for bit equal to the
+    host fill of that field alone; fields of one layout may sit at any
+    offsets of one buffer as long as their boxes do not overlap."""
+    import torch
+
+    nx, ny, nz, h, hk, mode = case
+    L = Layout(nx, ny, nz, h=3, hk=2, align=16)
+    rng = np.random.default_rng(5 + nfields)
+    dt = np.float64 if precision == "f64" else np.float32
+    hosts = [rng.standard_normal(L.size).astype(dt) for _ in range(nfields)]
+    print(f"[gpu] halo_fill_n {case} {precision} x{nfields}", flush=True)
+    big = torch.from_numpy(np.concatenate(hosts)).to("cuda")
+    fs = [Storage3D(L, big[n * L.size:(n + 1) * L.size]) for n in range(nfields)]
+    halo_fill_n(fs, (nx, ny, nz), h, hk, mode)
+    torch.cuda.synchronize()
+    for n in range(nfields):
+        b = hosts[n].copy()
+        halo_fill_host(b, L, h, hk, mode)
+        assert np.array_equal(fs[n].to_host().view(np.uint8), b.view(np.uint8)), f"field {n}"
 
 
 def _state(fields):

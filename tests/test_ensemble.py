@@ -70,7 +70,7 @@ def test_batch_matches_plain_and_host(precision):
     b = batched.record()
     try:
         records, launches = b.info()
-        assert records == len(SHAPES) * (2 * 3 + 4) and launches == 7
+        assert records == len(SHAPES) * (3 + 4) and launches == 7  # 3 two-field refills, 4 stencils
         for _ in range(3):
             plain.step_plain()
             b.launch()
