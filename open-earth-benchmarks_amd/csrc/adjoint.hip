@@ -540,6 +540,19 @@ __device__ __forceinline__ T s5(const T *f, int s, bool plus) {
           T(-1.0) / T(4.0) * f[2 * s] + T(1.0) / T(30.0) * f[3 * s]);
 }
 
+// s5 with both branches formed from the seven values f(-3s) .. f(3s), read
+// once, and one of them selected: lanes of a wave whose branches differ run
+// one path instead of both in turn.  The same terms in the same order as s5.
+template <class T>
+__device__ __forceinline__ T s5sel(const T *f, int s, bool plus) {
+  const T fm3 = f[-3 * s], fm2 = f[-2 * s], fm1 = f[-s], f0 = f[0], fp1 = f[s], fp2 = f[2 * s], fp3 = f[3 * s];
+  const T up = (T(1.0) / T(30.0) * fm3 + T(-1.0) / T(4.0) * fm2 + fm1 + T(-1.0) / T(3.0) * f0 +
+                T(-1.0) / T(2.0) * fp1 + T(1.0) / T(20.0) * fp2);
+  const T dn = (T(1.0) / T(20.0) * fm2 + T(-1.0) / T(2.0) * fm1 + T(-1.0) / T(3.0) * f0 + fp1 +
+                T(-1.0) / T(4.0) * fp2 + T(1.0) / T(30.0) * fp3);
+  return plus ? up : dn;
+}
+
 // s5 on seven values f(-3) .. f(3) passed by value (the same terms in the same
 // order), for register windows: taking a window's address would place it in
 // scratch memory.
@@ -647,8 +660,9 @@ struct HadvSt1 {
   bool py, py2;  // va > 0, va2 > 0
 };
 
-// Fused row-march form, used when one strip spans nx (nx + 12 <= 640, so
-// that the FP64 form keeps its registers without spilling): lane l owns
+// Fused row-march form (the FP32 default; FP64 takes the tiled form below),
+// used when one strip spans nx (nx + 12 <= 640, so that the FP64 form keeps
+// its registers without spilling): lane l owns
 // column i = l - 6 of [-6, nx + 6) of one level; the workgroup walks the
 // output rows p of [j0 - 3, j1 + 3) for the uin_bar / vin_bar rows [j0, j1)
 // of its chunk of [-3, ny + 3).  Per row p it gathers the contributions of
@@ -855,6 +869,213 @@ __global__ void __launch_bounds__(640) __attribute__((amdgpu_waves_per_eu(4)))
   }
 }
 
+// Tiled form: a workgroup takes a 64 x TJ tile of one level at a time, for
+// each level of its chunk.
+//  1. u and v on the tile plus 6 points each side go to LDS (addresses
+//     clamped into the box [-3, nx+3) x [-3, ny+3); a value outside it feeds
+//     no interior stage-1 point).  The next level's are loaded into
+//     registers during step 3 of this one.
+//  2. Stage 1 of hadvuv5th_adj_stage1 (the same expressions) at every interior
+//     point of the tile plus 3 each side, into LDS fields of their own: the
+//     four upwind weights, gu, gv, gu2, gv2, the metric term and the four
+//     branch bits.
+//  3. Every uin_bar / vin_bar point the tile owns gathers exactly as
+//     hadvuv5th_adj_stage2 does (same terms, same order, so both forms give
+//     the same bits), with the stage-1 values read from LDS.  A tile owns its
+//     own points, plus the halo beyond the domain where it touches an edge,
+//     so every point of the box is written once.
+// Two barriers per tile level: after step 1 (u, v written; the last level's
+// gathers done with the fields) and after step 2 (fields written; stage 1
+// done with u and v).  The stage-1 region is 1.5 (TJ = 16) to 1.9 (TJ = 8)
+// times the tile, so stage 1 is formed that much more often than in the
+// two-pass form, but nothing but uin_bar and vin_bar goes to memory.
+template <class T, int TJ>
+struct HadvTileLds {
+  static constexpr int kW2 = kTI + 12, kH2 = TJ + 12;  // u, v: [i0-6, i0+70) x [j0-6, j0+TJ+6)
+  static constexpr int kW1 = kTI + 6, kH1 = TJ + 6;    // stage 1: [i0-3, i0+67) x [j0-3, j0+TJ+3)
+  static constexpr int kN2 = kW2 * kH2, kN1 = kW1 * kH1;
+  static constexpr size_t kBytes = (2 * size_t(kN2) + 9 * size_t(kN1)) * sizeof(T) + size_t(kN1) * sizeof(int);
+};
+
+// At least 4 waves per SIMD: at most 128 registers.
+template <class T, int TJ, int NT>
+__global__ void __launch_bounds__(NT, 4) hadvuv5th_adj_tile(HadvAdj<T> a, Geo g, Tiles t) {
+  using Lds = HadvTileLds<T, TJ>;
+  constexpr int kW2 = Lds::kW2, kN2 = Lds::kN2, kW1 = Lds::kW1, kN1 = Lds::kN1;
+  constexpr int kLoad = (kN2 + NT - 1) / NT;
+  __shared__ __align__(16) unsigned char smem[Lds::kBytes];
+  T *s_u = reinterpret_cast<T *>(smem), *s_v = s_u + kN2;
+  T *s_wux = s_v + kN2, *s_wvx = s_wux + kN1, *s_wuy = s_wvx + kN1, *s_wvy = s_wuy + kN1;
+  T *s_gu = s_wvy + kN1, *s_gv = s_gu + kN1, *s_gu2 = s_gv + kN1, *s_gv2 = s_gu2 + kN1;
+  T *s_tgt = s_gv2 + kN1;
+  int *s_br = reinterpret_cast<int *>(s_tgt + kN1);
+  constexpr T rr = kEarthRadiusRecipAdj<T>;
+  const TileCoord tc = tile_coord(t, kTI, TJ, g.nz);
+  const int i0 = tc.i0, j0 = tc.j0;
+  const int nx = g.nx, ny = g.ny, sj = g.sj;
+  const int tid = int(threadIdx.x);
+  // Owned uin_bar / vin_bar points: the tile, plus the halo beyond an edge.
+  const int oi0 = i0 == 0 ? -3 : i0, oi1 = i0 + kTI >= nx ? nx + 3 : i0 + kTI;
+  const int oj0 = j0 == 0 ? -3 : j0, oj1 = j0 + TJ >= ny ? ny + 3 : j0 + TJ;
+  const int ow = oi1 - oi0, on = ow * (oj1 - oj0);
+  const bool far = i0 >= 3 && i0 + kTI + 3 <= nx && j0 >= 3 && j0 + TJ + 3 <= ny;
+  auto inside = [&](int i, int j) { return in_range(i, 0, nx) && in_range(j, 0, ny); };
+  // u, v of level k with a 6-point halo, into registers.
+  T lu[kLoad], lv[kLoad];
+  auto load_uv = [&](int k) {
+    const int64_t plane = int64_t(k) * g.sk;
+#pragma unroll
+    for (int n = 0; n < kLoad; ++n) {
+      const int e = tid + n * NT;
+      const int gi = min(max(i0 - 6 + e % kW2, -3), nx + 2), gj = min(max(j0 - 6 + e / kW2, -3), ny + 2);
+      const int64_t at = plane + gi + int64_t(gj) * sj;
+      lu[n] = a.uin[at];
+      lv[n] = a.vin[at];
+    }
+  };
+  load_uv(tc.k0);
+
+  for (int k = tc.k0; k < tc.k1; ++k) {
+    const int64_t plane = int64_t(k) * g.sk;
+    // 1. u, v to LDS.
+#pragma unroll
+    for (int n = 0; n < kLoad; ++n) {
+      const int e = tid + n * NT;
+      if (e < kN2) {
+        s_u[e] = lu[n];
+        s_v[e] = lv[n];
+      }
+    }
+    lds_barrier();
+
+    // 2. Stage 1 (kernels/hadvuv5th.h:183-195).
+    for (int e = tid; e < kN1; e += NT) {
+      const int c1 = e % kW1, r1 = e / kW1;
+      const int pi = i0 - 3 + c1, pj = j0 - 3 + r1;
+      T f0 = T(0), f1 = T(0), f2 = T(0), f3 = T(0), f4 = T(0), f5 = T(0), f6 = T(0), f7 = T(0), f8 = T(0);
+      int br = 0;
+      if (inside(pi, pj)) {
+        const int64_t at = plane + pi + int64_t(pj) * sj;
+        const T ob = a.ubar[at], ob2 = a.vbar[at];
+        const T a0 = a.a0[int64_t(pj) * a.s0], a1 = a.a1[int64_t(pj) * a.s1];
+        const T tg0 = a.t0[int64_t(pj) * a.s2], tg1 = a.t1[int64_t(pj) * a.s3];
+        const int e2 = (r1 + 3) * kW2 + c1 + 3;
+        const T *u = s_u + e2, *v = s_v + e2;
+        const T U = (T(1.0) / T(3.0)) * (u[-1] + u[0] + u[1]);
+        const T Vu = T(0.25) * (v[1] + v[1 - kW2] + v[0] + v[-kW2]);
+        const T ua = a0 * U, va = rr * Vu;
+        const bool px = ua > 0, py = va > 0;
+        const T sx = s5sel(u, 1, px), sy = s5sel(u, kW2, py);
+        f0 = a.eddlat * ob * (px ? ua : -ua);
+        f2 = a.eddlon * ob * (py ? va : -va);
+        f4 = a.eddlat * ob * (px ? sx : -sx) * a0 * (T(1.0) / T(3.0));
+        f5 = a.eddlon * ob * (py ? sy : -sy) * rr * T(0.25) + ob * tg0 * u[0] * T(0.25);
+        f8 = ob * tg0 * Vu;
+        const T Uv = T(0.25) * (u[-1] + u[0] + u[kW2] + u[kW2 - 1]);
+        const T V = T(1.0) / T(3.0) * (v[-kW2] + v[0] + v[kW2]);
+        const T ua2 = a1 * Uv, va2 = rr * V;
+        const bool px2 = ua2 > 0, py2 = va2 > 0;
+        const T sx2 = s5sel(v, 1, px2), sy2 = s5sel(v, kW2, py2);
+        f1 = a.eddlat * ob2 * (px2 ? ua2 : -ua2);
+        f3 = a.eddlon * ob2 * (py2 ? va2 : -va2);
+        f6 = a.eddlat * ob2 * (px2 ? sx2 : -sx2) * a1 * T(0.25) - ob2 * tg1 * Uv * T(0.5);
+        f7 = a.eddlon * ob2 * (py2 ? sy2 : -sy2) * rr * (T(1.0) / T(3.0));
+        br = (px ? 1 : 0) | (px2 ? 2 : 0) | (py ? 4 : 0) | (py2 ? 8 : 0);
+      }
+      s_wux[e] = f0;
+      s_wvx[e] = f1;
+      s_wuy[e] = f2;
+      s_wvy[e] = f3;
+      s_gu[e] = f4;
+      s_gv[e] = f5;
+      s_gu2[e] = f6;
+      s_gv2[e] = f7;
+      s_tgt[e] = f8;
+      s_br[e] = br;
+    }
+    lds_barrier();
+
+    // 3. The gathers of hadvuv5th_adj_stage2 at the owned points, with the
+    // next level's u and v in flight.  A tile 3 or more points from every
+    // edge owns only its own points, all of whose neighbours are interior:
+    // the same terms in the same order without the tests.
+    if (k + 1 < tc.k1) load_uv(k + 1);
+    if (far) {
+      for (int e = tid; e < kTI * TJ; e += NT) {
+        const int i = i0 + e % kTI, j = j0 + e / kTI;
+        const int c = (e % kTI + 3) + (e / kTI + 3) * kW1;
+        T ru = T(0), rv = T(0);
+#pragma unroll
+        for (int kk = -3; kk <= 3; ++kk) {
+          const int q = c - kk, b = s_br[q];
+          ru += s_wux[q] * c5<T>((b & 1) != 0, kk);
+          rv += s_wvx[q] * c5<T>((b & 2) != 0, kk);
+          const int qy = c - kk * kW1, by = s_br[qy];
+          ru += s_wuy[qy] * c5<T>((by & 4) != 0, kk);
+          rv += s_wvy[qy] * c5<T>((by & 8) != 0, kk);
+        }
+        ru += s_gu[c - 1];
+        ru += s_gu[c];
+        ru += s_gu[c + 1];
+        ru += s_gu2[c];
+        ru += s_gu2[c + 1];
+        ru += s_gu2[c - kW1];
+        ru += s_gu2[c - kW1 + 1];
+        ru += s_tgt[c];
+        rv += s_gv[c - 1];
+        rv += s_gv[c - 1 + kW1];
+        rv += s_gv[c];
+        rv += s_gv[c + kW1];
+        rv += s_gv2[c - kW1];
+        rv += s_gv2[c];
+        rv += s_gv2[c + kW1];
+        const int64_t at = plane + i + int64_t(j) * sj;
+        a.uinbar[at] = ru;
+        a.vinbar[at] = rv;
+      }
+      continue;
+    }
+    for (int e = tid; e < on; e += NT) {
+      const int i = oi0 + e % ow, j = oj0 + e / ow;
+      const int c = (i - i0 + 3) + (j - j0 + 3) * kW1;  // (i, j) in the stage-1 region
+      T ru = T(0), rv = T(0);
+#pragma unroll
+      for (int kk = -3; kk <= 3; ++kk) {
+        if (inside(i - kk, j)) {
+          const int q = c - kk;
+          const int b = s_br[q];
+          ru += s_wux[q] * c5<T>((b & 1) != 0, kk);
+          rv += s_wvx[q] * c5<T>((b & 2) != 0, kk);
+        }
+        if (inside(i, j - kk)) {
+          const int q = c - kk * kW1;
+          const int b = s_br[q];
+          ru += s_wuy[q] * c5<T>((b & 4) != 0, kk);
+          rv += s_wvy[q] * c5<T>((b & 8) != 0, kk);
+        }
+      }
+      for (int m = -1; m <= 1; ++m)
+        if (inside(i + m, j)) ru += s_gu[c + m];
+      if (inside(i, j)) ru += s_gu2[c];
+      if (inside(i + 1, j)) ru += s_gu2[c + 1];
+      if (inside(i, j - 1)) ru += s_gu2[c - kW1];
+      if (inside(i + 1, j - 1)) ru += s_gu2[c - kW1 + 1];
+      if (inside(i, j)) ru += s_tgt[c];
+      if (inside(i - 1, j)) rv += s_gv[c - 1];
+      if (inside(i - 1, j + 1)) rv += s_gv[c - 1 + kW1];
+      if (inside(i, j)) rv += s_gv[c];
+      if (inside(i, j + 1)) rv += s_gv[c + kW1];
+      for (int m = -1; m <= 1; ++m)
+        if (inside(i, j + m)) rv += s_gv2[c + m * kW1];
+      const int64_t at = plane + i + int64_t(j) * sj;
+      a.uinbar[at] = ru;
+      a.vinbar[at] = rv;
+    }
+    // The next level's step 1 writes only u and v, which this step does not
+    // read; the barrier after it orders this step before the next step 2.
+  }
+}
+
 template <class T, class S3, class S1>
 int hadvuv5th_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S1 *acrlat0, const S1 *acrlat1,
                       const S1 *tgrlatda0, const S1 *tgrlatda1, const S3 *uout_bar, const S3 *vout_bar,
@@ -894,6 +1115,27 @@ int hadvuv5th_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
   for (int q = 0; q < 8; ++q) *dst[q] = const_cast<T *>(p_s[q]);
   a.uinbar = const_cast<T *>(p_ub);
   a.vinbar = const_cast<T *>(p_vb);
+  // OEB_TILE_HADVUV5TH_ADJ: 1 = the tiled form, 0 = the row march (or the
+  // two-pass form).  FP64 takes 64x16 tiles of 1024 lanes (151 KB of LDS, one
+  // workgroup per CU); FP32 64x8 tiles of 512 lanes (51 KB), or 64x16 of 1024
+  // with OEB_TILE_HADVUV5TH_ADJ=2.  Measured at 256x256x80: FP64 tiles 173 us
+  // (4 levels per workgroup; 180 at 2) against 211 for the row march; FP32
+  // tiles 107 us against 86, so FP32 keeps the row march.
+  const int tile = env_choice("OEB_TILE_", K, sizeof(T) == 8 ? 1 : 0, 0, 2);
+  if (tile != 0) {
+    const int kc = kchunk_for(K, sizeof(T), d, 4);
+    if (sizeof(T) == 4 && tile == 1) {
+      const Tiles t = make_tiles(nx, ny, int(nz), kTI, 8, kc);
+      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 8));
+      if constexpr (sizeof(T) == 4)
+        hipLaunchKernelGGL((hadvuv5th_adj_tile<T, 8, 512>), dim3(t.n), dim3(512), 0, hipStream_t(stream), a, g, t);
+    } else {
+      const Tiles t = make_tiles(nx, ny, int(nz), kTI, 16, kc);
+      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 16));
+      hipLaunchKernelGGL((hadvuv5th_adj_tile<T, 16, 1024>), dim3(t.n), dim3(1024), 0, hipStream_t(stream), a, g, t);
+    }
+    return launch_status(K);
+  }
   if (nx + 12 <= 640 && strip_for(K, true)) {
     const int lanes = int((nx + 12 + 63) / 64 * 64);
     // Row chunks measured at 256x256x80 over 8-64: 32 rows fastest in both

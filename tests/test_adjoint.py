@@ -264,6 +264,7 @@ def test_hadvuv5th_adjoint_strip_matches_two_pass(shape, rows, depth, monkeypatc
     ybars = [torch.randn(x.shape, generator=g, device="cuda", dtype=x.dtype) for x in xs]
     print(f"[gpu] hadvuv5th adjoint strip vs two-pass {shape} rows={rows} depth={depth}", flush=True)
     out = []
+    monkeypatch.setenv("OEB_TILE_HADVUV5TH_ADJ", "0")
     monkeypatch.setenv("OEB_DEPTH_HADVUV5TH_ADJ", str(depth))
     for strip in (0, 1):
         monkeypatch.setenv("OEB_STRIP_HADVUV5TH_ADJ", str(strip))
@@ -271,6 +272,42 @@ def test_hadvuv5th_adjoint_strip_matches_two_pass(shape, rows, depth, monkeypatc
         out.append(_grads(lambda u, v: AG.Hadvuv5th.apply(u, v, c, L, dom), xs, ybars))
     for a, b in zip(*out):
         assert _relmax(b, a) <= 1e-13
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("kchunk", [1, 2, 3])
+@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("shape", [(40, 36, 5), (67, 9, 3), (130, 21, 2), (64, 32, 4), (3, 2, 2), (200, 40, 2)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_hadvuv5th_adjoint_tile_matches_two_pass(shape, tile, kchunk, precision, monkeypatch):
+    """The tiled hadvuv5th adjoint (stage 1 of a tile and its 3-point halo
+    in LDS, then the two-pass form's gathers), forced through
+    OEB_TILE_HADVUV5TH_ADJ (FP64: 64x16 tiles; FP32: 64x8 and 64x16) with
+    1-3 levels per workgroup, and the two-pass form give the same bits: the
+    same expressions, the same terms in the same order.  Shapes with edge
+    tiles narrower than a tile, several tiles each way, a 3x2 domain, and
+    tiles 3 or more points from every edge (200x40: the gathers without
+    edge tests)."""
+    import torch
+
+    from oeb.generators import cast
+
+    L = Layout(*shape, h=3, hk=1)
+    host = cast("hadvuv5th", generate("hadvuv5th", L, "baseline", 53), precision)
+    c = _dev("hadvuv5th", L, host, skip=("uin", "vin"))
+    dom = (L.nx, L.ny, L.nz)
+    xs = [_t(host["uin"]), _t(host["vin"])]
+    g = torch.Generator(device="cuda").manual_seed(12)
+    ybars = [torch.randn(x.shape, generator=g, device="cuda", dtype=x.dtype) for x in xs]
+    print(f"[gpu] hadvuv5th adjoint tile {tile} vs two-pass {shape} kchunk={kchunk} {precision}", flush=True)
+    out = []
+    monkeypatch.setenv("OEB_STRIP_HADVUV5TH_ADJ", "0")
+    monkeypatch.setenv("OEB_KCHUNK_HADVUV5TH_ADJ", str(kchunk))
+    for t in (0, tile):
+        monkeypatch.setenv("OEB_TILE_HADVUV5TH_ADJ", str(t))
+        out.append(_grads(lambda u, v: AG.Hadvuv5th.apply(u, v, c, L, dom), xs, ybars))
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
 
 
 @pytest.mark.parametrize("rows", [1, 3, 64])
