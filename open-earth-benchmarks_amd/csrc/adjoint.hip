@@ -1517,6 +1517,180 @@ __global__ void __launch_bounds__(1024) hdiffsmag_adj_strip(HsmAdj<T> a, Geo g, 
   }
 }
 
+// Tiled form: a workgroup takes a 64 x TJ tile of one level at a time, for
+// each level of its chunk, in four steps with a barrier after each of the
+// first three:
+//  1. u and v on the tile plus 3 points each side to LDS (addresses clamped
+//     into the box [-1, nx+1) x [-1, ny+1)); the next level's are loaded
+//     into registers during step 4 of this one.
+//  2. Stage 1 of hdiffsmag_adj_stage1 (gu, gv, wu, wv) at the interior points
+//     of the tile plus 2 each side, 0 elsewhere.
+//  3. T_bar and S_bar of hdiffsmag_adj_stage2 at the tile plus 1 each side
+//     (the points whose weight can be non-zero for the points the tile
+//     owns; outside that region they are 0).
+//  4. The gathers of hdiffsmag_adj_stage2 at the points the tile owns: its
+//     own, plus the halo beyond the domain where it touches an edge.
+// The same expressions, terms and order as the two-pass form, so both forms
+// give the same bits.  Only uin_bar and vin_bar go to memory.
+template <class T, int TJ>
+struct HsmTileLds {
+  static constexpr int kW2 = kTI + 6, kH2 = TJ + 6;  // u, v: [i0-3, i0+67) x [j0-3, j0+TJ+3)
+  static constexpr int kW1 = kTI + 4, kH1 = TJ + 4;  // stage 1: [i0-2, i0+66) x [j0-2, j0+TJ+2)
+  static constexpr int kW0 = kTI + 2, kH0 = TJ + 2;  // T_bar, S_bar: [i0-1, i0+65) x [j0-1, j0+TJ+1)
+  static constexpr int kN2 = kW2 * kH2, kN1 = kW1 * kH1, kN0 = kW0 * kH0;
+  static constexpr size_t kBytes = (2 * size_t(kN2) + 4 * size_t(kN1) + 2 * size_t(kN0)) * sizeof(T);
+};
+
+template <class T, int TJ, int NT>
+__global__ void __launch_bounds__(NT) hdiffsmag_adj_tile(HsmAdj<T> a, Geo g, Tiles t) {
+  using Lds = HsmTileLds<T, TJ>;
+  constexpr int kW2 = Lds::kW2, kN2 = Lds::kN2, kW1 = Lds::kW1, kN1 = Lds::kN1, kW0 = Lds::kW0, kH0 = Lds::kH0,
+                kN0 = Lds::kN0;
+  constexpr int kLoad = (kN2 + NT - 1) / NT;
+  __shared__ __align__(16) unsigned char smem[Lds::kBytes];
+  T *s_u = reinterpret_cast<T *>(smem), *s_v = s_u + kN2;
+  T *s_gu = s_v + kN2, *s_gv = s_gu + kN1, *s_wu = s_gv + kN1, *s_wv = s_wu + kN1;
+  T *s_tb = s_wv + kN1, *s_sb = s_tb + kN0;
+  constexpr T rr = kEarthRadiusRecipAdj<T>;
+  const T eddlat = a.eddlat, eddlon = a.eddlon;
+  const TileCoord tc = tile_coord(t, kTI, TJ, g.nz);
+  const int i0 = tc.i0, j0 = tc.j0;
+  const int nx = g.nx, ny = g.ny, sj = g.sj;
+  const int tid = int(threadIdx.x);
+  const int oi0 = i0 == 0 ? -1 : i0, oi1 = i0 + kTI >= nx ? nx + 1 : i0 + kTI;
+  const int oj0 = j0 == 0 ? -1 : j0, oj1 = j0 + TJ >= ny ? ny + 1 : j0 + TJ;
+  const int ow = oi1 - oi0, on = ow * (oj1 - oj0);
+  auto inside = [&](int i, int j) { return in_range(i, 0, nx) && in_range(j, 0, ny); };
+  auto acr = [&](int jj) { return a.ac[int64_t(jj) * a.sac]; };
+  T lu[kLoad], lv[kLoad];
+  auto load_uv = [&](int k) {
+    const int64_t plane = int64_t(k) * g.sk;
+#pragma unroll
+    for (int n = 0; n < kLoad; ++n) {
+      const int e = tid + n * NT;
+      const int gi = min(max(i0 - 3 + e % kW2, -1), nx), gj = min(max(j0 - 3 + e / kW2, -1), ny);
+      const int64_t at = plane + gi + int64_t(gj) * sj;
+      lu[n] = a.uin[at];
+      lv[n] = a.vin[at];
+    }
+  };
+  load_uv(tc.k0);
+
+  for (int k = tc.k0; k < tc.k1; ++k) {
+    const int64_t plane = int64_t(k) * g.sk;
+    // 1. u, v to LDS.
+#pragma unroll
+    for (int n = 0; n < kLoad; ++n) {
+      const int e = tid + n * NT;
+      if (e < kN2) {
+        s_u[e] = lu[n];
+        s_v[e] = lv[n];
+      }
+    }
+    lds_barrier();
+
+    // 2. Stage 1 (kernels/hdiffsmag.h:213-259), as hdiffsmag_adj_stage1.
+    for (int e = tid; e < kN1; e += NT) {
+      const int c1 = e % kW1, r1 = e / kW1;
+      const int i = i0 - 2 + c1, j = j0 - 2 + r1;
+      T gu = T(0), gv = T(0), wu = T(0), wv = T(0);
+      if (inside(i, j)) {
+        const int64_t at = plane + i + int64_t(j) * sj;
+        const T *u = s_u + (r1 + 1) * kW2 + c1 + 1, *v = s_v + (r1 + 1) * kW2 + c1 + 1;
+        const T ac_s = acr(j - 1), ac_c = acr(j), ac_n = acr(j + 1);
+        const T t0 = (v[-kW2] - v[0]) * eddlat * rr - (u[-1] - u[0]) * ac_c * eddlon;
+        const T t_ip = (v[1 - kW2] - v[1]) * eddlat * rr - (u[0] - u[1]) * ac_c * eddlon;
+        const T t_jp = (v[0] - v[kW2]) * eddlat * rr - (u[kW2 - 1] - u[kW2]) * ac_n * eddlon;
+        const T s0 = (u[kW2] - u[0]) * eddlat * rr + (v[1] - v[0]) * ac_c * eddlon;
+        const T s_im = (u[kW2 - 1] - u[-1]) * eddlat * rr + (v[0] - v[-1]) * ac_c * eddlon;
+        const T s_jm = (u[0] - u[-kW2]) * eddlat * rr + (v[1 - kW2] - v[-kW2]) * ac_s * eddlon;
+        const T tt0 = t0 * t0, tt_ip = t_ip * t_ip, tt_jp = t_jp * t_jp;
+        const T ss0 = s0 * s0, ss_im = s_im * s_im, ss_jm = s_jm * s_jm;
+        const T hdweight = a.weight * a.mask[at];
+        const T qu = T(0.5) * (tt_ip + tt0) + T(0.5) * (ss_jm + ss0);
+        const T qv = T(0.5) * (tt_jp + tt0) + T(0.5) * (ss_im + ss0);
+        const T su = a.tau * sqrt(qu) - hdweight, sv = a.tau * sqrt(qv) - hdweight;
+        const bool pass_u = (T(0) < su) && (su < T(0.5)), pass_v = (T(0) < sv) && (sv < T(0.5));
+        const T smag_u = pass_u ? su : ((T(0) < su) ? T(0.5) : T(0));
+        const T smag_v = pass_v ? sv : ((T(0) < sv) ? T(0.5) : T(0));
+        const T lapu = u[1] + u[-1] - T(2.0) * u[0] + a.lo[int64_t(j) * a.slo] * (u[kW2] - u[0]) +
+                       a.lu[int64_t(j) * a.slu] * (u[-kW2] - u[0]);
+        const T lapv = v[1] + v[-1] - T(2.0) * v[0] + a.vo[int64_t(j) * a.svo] * (v[kW2] - v[0]) +
+                       a.vu[int64_t(j) * a.svu] * (v[-kW2] - v[0]);
+        const T ub = a.ubar[at], vb = a.vbar[at];
+        gu = pass_u ? ub * lapu * a.tau * T(0.5) / sqrt(qu) : T(0);
+        gv = pass_v ? vb * lapv * a.tau * T(0.5) / sqrt(qv) : T(0);
+        wu = smag_u * ub;
+        wv = smag_v * vb;
+      }
+      s_gu[e] = gu;
+      s_gv[e] = gv;
+      s_wu[e] = wu;
+      s_wv[e] = wv;
+    }
+    lds_barrier();
+
+    // 3. T_bar, S_bar (hdiffsmag_adj_stage2's Tbar and Sbar).
+    for (int e = tid; e < kN0; e += NT) {
+      const int c0 = e % kW0, r0 = e / kW0;
+      const int x1 = (r0 + 1) * kW1 + c0 + 1;  // in the stage-1 region
+      const int x2 = (r0 + 2) * kW2 + c0 + 2;  // in the u, v region
+      const int xj = j0 - 1 + r0;
+      const T tw = s_gu[x1] + s_gu[x1 - 1] + s_gv[x1] + s_gv[x1 - kW1];
+      const T sw = s_gu[x1] + s_gu[x1 + kW1] + s_gv[x1] + s_gv[x1 + 1];
+      T tb = T(0), sb = T(0);
+      if (tw != T(0))
+        tb = ((s_v[x2 - kW2] - s_v[x2]) * eddlat * rr - (s_u[x2 - 1] - s_u[x2]) * acr(xj) * eddlon) * tw;
+      if (sw != T(0))
+        sb = ((s_u[x2 + kW2] - s_u[x2]) * eddlat * rr + (s_v[x2 + 1] - s_v[x2]) * acr(xj) * eddlon) * sw;
+      s_tb[e] = tb;
+      s_sb[e] = sb;
+    }
+    lds_barrier();
+
+    // 4. The gathers at the owned points, with the next level's u and v in
+    // flight.  T_bar and S_bar are 0 outside their region (no weight there).
+    if (k + 1 < tc.k1) load_uv(k + 1);
+    for (int e = tid; e < on; e += NT) {
+      const int i = oi0 + e % ow, j = oj0 + e / ow;
+      const int ci = i - i0, cj = j - j0;
+      auto bar = [&](const T *f, int di, int dj) {
+        const int x = ci + di + 1, y = cj + dj + 1;
+        return (x >= 0 && x < kW0 && y >= 0 && y < kH0) ? f[x + y * kW0] : T(0);
+      };
+      const int c1 = (ci + 2) + (cj + 2) * kW1;
+      auto w1 = [&](const T *f, int di, int dj) { return inside(i + di, j + dj) ? f[c1 + di + dj * kW1] : T(0); };
+      const int64_t at = plane + i + int64_t(j) * sj;
+      T ru = inside(i, j) ? a.ubar[at] : T(0);
+      T rv = inside(i, j) ? a.vbar[at] : T(0);
+      const T tq = bar(s_tb, 0, 0);
+      ru += tq * acr(j) * eddlon - bar(s_tb, 1, 0) * acr(j) * eddlon;
+      rv += bar(s_tb, 0, 1) * eddlat * rr - tq * eddlat * rr;
+      const T sq = bar(s_sb, 0, 0);
+      ru += bar(s_sb, 0, -1) * eddlat * rr - sq * eddlat * rr;
+      rv += bar(s_sb, -1, 0) * acr(j) * eddlon - sq * acr(j) * eddlon;
+      if (inside(i, j)) {
+        ru += w1(s_wu, 0, 0) * (T(-2.0) - a.lo[int64_t(j) * a.slo] - a.lu[int64_t(j) * a.slu]);
+        rv += w1(s_wv, 0, 0) * (T(-2.0) - a.vo[int64_t(j) * a.svo] - a.vu[int64_t(j) * a.svu]);
+      }
+      ru += w1(s_wu, -1, 0) + w1(s_wu, 1, 0);
+      rv += w1(s_wv, -1, 0) + w1(s_wv, 1, 0);
+      if (inside(i, j - 1)) {
+        ru += w1(s_wu, 0, -1) * a.lo[int64_t(j - 1) * a.slo];
+        rv += w1(s_wv, 0, -1) * a.vo[int64_t(j - 1) * a.svo];
+      }
+      if (inside(i, j + 1)) {
+        ru += w1(s_wu, 0, 1) * a.lu[int64_t(j + 1) * a.slu];
+        rv += w1(s_wv, 0, 1) * a.vu[int64_t(j + 1) * a.svu];
+      }
+      a.uinbar[at] = ru;
+      a.vinbar[at] = rv;
+    }
+    // The next level's step 1 writes only u and v, which this step does not
+    // read; the barrier after it orders this step before the next step 2.
+  }
+}
+
 template <class T, class S3, class S1>
 int hdiffsmag_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S3 *mask, const S1 *crlavo,
                       const S1 *crlavu, const S1 *crlato, const S1 *crlatu, const S1 *acrlat0, const S3 *uout_bar,
@@ -1558,6 +1732,26 @@ int hdiffsmag_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
   a.wu = const_cast<T *>(p_w[2]), a.wv = const_cast<T *>(p_w[3]);
   a.uinbar = const_cast<T *>(p_ub);
   a.vinbar = const_cast<T *>(p_vb);
+  // OEB_TILE_HDIFFSMAG_ADJ: 1 = 64x8 tiles of 512 lanes, 2 = 64x16 tiles of
+  // 1024 lanes, 0 = the row march (or the two-pass form).  Measured at
+  // 256x256x80 with 4 levels per workgroup: FP64 64x16 tiles 146-148 us, 64x8
+  // 149-155, row march 158-160; FP32 tiles 85-95 us against 72 for the row
+  // march.  Loading mask, ubar and vbar of the next level during the gathers
+  // raised the registers to 117 and took 170 us.
+  const int tile = env_choice("OEB_TILE_", K, sizeof(T) == 8 ? 2 : 0, 0, 2);
+  if (tile != 0) {
+    const int kc = kchunk_for(K, sizeof(T), d, 4);
+    if (tile == 1) {
+      const Tiles t = make_tiles(nx, ny, int(nz), kTI, 8, kc);
+      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 8));
+      hipLaunchKernelGGL((hdiffsmag_adj_tile<T, 8, 512>), dim3(t.n), dim3(512), 0, hipStream_t(stream), a, g, t);
+    } else {
+      const Tiles t = make_tiles(nx, ny, int(nz), kTI, 16, kc);
+      OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 16));
+      hipLaunchKernelGGL((hdiffsmag_adj_tile<T, 16, 1024>), dim3(t.n), dim3(1024), 0, hipStream_t(stream), a, g, t);
+    }
+    return launch_status(K);
+  }
   if (nx + 4 <= 1024 && strip_for(K, true)) {
     const int lanes = int((nx + 4 + 63) / 64 * 64);
     // Measured at 256x256x80: FP64 fastest with 16-row chunks and rows

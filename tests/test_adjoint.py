@@ -310,6 +310,40 @@ def test_hadvuv5th_adjoint_tile_matches_two_pass(shape, tile, kchunk, precision,
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("kchunk", [1, 3])
+@pytest.mark.parametrize("tile", [1, 2])
+@pytest.mark.parametrize("shape", [(40, 36, 5), (67, 9, 3), (130, 21, 2), (64, 32, 4), (3, 2, 2), (200, 40, 2)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_hdiffsmag_adjoint_tile_matches_two_pass(shape, tile, kchunk, precision, monkeypatch):
+    """The tiled hdiffsmag adjoint (stage 1, then T_bar and S_bar, in LDS,
+    then the two-pass form's gathers), forced through OEB_TILE_HDIFFSMAG_ADJ
+    with 64x8 and 64x16 tiles and 1 or 3 levels per workgroup, and the
+    two-pass form give the same bits: the same expressions, terms and order.
+    Shapes with edge tiles narrower than a tile, several tiles each way and
+    a 3x2 domain."""
+    import torch
+
+    from oeb.generators import cast
+
+    L = Layout(*shape, h=3, hk=1)
+    host = cast("hdiffsmag", generate("hdiffsmag", L, "baseline", 59), precision)
+    c = _dev("hdiffsmag", L, host, skip=("uin", "vin"))
+    dom = (L.nx, L.ny, L.nz)
+    xs = [_t(host["uin"]), _t(host["vin"])]
+    g = torch.Generator(device="cuda").manual_seed(14)
+    ybars = [torch.randn(x.shape, generator=g, device="cuda", dtype=x.dtype) for x in xs]
+    print(f"[gpu] hdiffsmag adjoint tile {tile} vs two-pass {shape} kchunk={kchunk} {precision}", flush=True)
+    out = []
+    monkeypatch.setenv("OEB_STRIP_HDIFFSMAG_ADJ", "0")
+    monkeypatch.setenv("OEB_KCHUNK_HDIFFSMAG_ADJ", str(kchunk))
+    for t in (0, tile):
+        monkeypatch.setenv("OEB_TILE_HDIFFSMAG_ADJ", str(t))
+        out.append(_grads(lambda u, v: AG.Hdiffsmag.apply(u, v, c, L, dom), xs, ybars))
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
+
+
 @pytest.mark.parametrize("rows", [1, 3, 64])
 @pytest.mark.parametrize("shape", [(40, 36, 5), (67, 9, 3)], ids=lambda s: "x".join(map(str, s)))
 def test_hdiffsmag_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
@@ -328,6 +362,7 @@ def test_hdiffsmag_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
     ybars = [torch.randn(x.shape, generator=g, device="cuda", dtype=x.dtype) for x in xs]
     print(f"[gpu] hdiffsmag adjoint strip vs two-pass {shape} rows={rows}", flush=True)
     out = []
+    monkeypatch.setenv("OEB_TILE_HDIFFSMAG_ADJ", "0")
     for strip in (0, 1):
         monkeypatch.setenv("OEB_STRIP_HDIFFSMAG_ADJ", str(strip))
         monkeypatch.setenv("OEB_ROWS_HDIFFSMAG_ADJ", str(rows))
