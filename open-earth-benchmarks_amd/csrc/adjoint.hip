@@ -1130,7 +1130,8 @@ int hadvuv5th_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
       if constexpr (sizeof(T) == 4)
         hipLaunchKernelGGL((hadvuv5th_adj_tile<T, 8, 512>), dim3(t.n), dim3(512), 0, hipStream_t(stream), a, g, t);
     } else {
-      const Tiles t = make_tiles(nx, ny, int(nz), kTI, 16, kc);
+      const int order = xcd_mode_for(K, 0);
+      const Tiles t = make_tiles(nx, ny, int(nz), kTI, 16, kc, order == 1, order == 2);
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 16));
       hipLaunchKernelGGL((hadvuv5th_adj_tile<T, 16, 1024>), dim3(t.n), dim3(1024), 0, hipStream_t(stream), a, g, t);
     }
@@ -1746,7 +1747,8 @@ int hdiffsmag_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 8));
       hipLaunchKernelGGL((hdiffsmag_adj_tile<T, 8, 512>), dim3(t.n), dim3(512), 0, hipStream_t(stream), a, g, t);
     } else {
-      const Tiles t = make_tiles(nx, ny, int(nz), kTI, 16, kc);
+      const int order = xcd_mode_for(K, 2);  // Tiles::irow: 143 us against 145
+      const Tiles t = make_tiles(nx, ny, int(nz), kTI, 16, kc, order == 1, order == 2);
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 16));
       hipLaunchKernelGGL((hdiffsmag_adj_tile<T, 16, 1024>), dim3(t.n), dim3(1024), 0, hipStream_t(stream), a, g, t);
     }

@@ -223,6 +223,8 @@ int env_choice(const char *prefix, const char *kname, int dflt, int lo, int hi) 
 
 bool xcd_for(const char *kname, bool dflt) { return env_tunable("OEB_XCD_", kname, dflt ? 1 : 0, 0, 1) != 0; }
 
+int xcd_mode_for(const char *kname, int dflt) { return env_tunable("OEB_XCD_", kname, dflt, 0, 2); }
+
 int batch_grid() { return env_tunable("OEB_BATCH_", "grid", kBatchGrid, 1, long(1) << 30); }
 
 int strip_depth_for(const char *kname, int dflt) { return env_tunable("OEB_DEPTH_", kname, dflt, 1, 8); }

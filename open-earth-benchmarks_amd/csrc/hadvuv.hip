@@ -220,7 +220,10 @@ int hadv_run(const char *K, const oeb_domain *d, const S3 *uout, const S3 *vout,
   // Levels per workgroup: 4, and 5 for FP32 hadvuv, which measured 1.12x
   // faster with 5 at 256x256x80 (the same at 512x512x80).
   const int kc_dflt = (ORDER == 3 && sizeof(T) == 4) ? 5 : kChunk;
-  const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJ, kchunk_for(K, sizeof(T), d, kc_dflt));
+  // The tiles of a row on one XCD (Tiles::irow): FP64 hadvuv 141 us against
+  // 145 at 512x512x80, hadvuv5th 1% faster, the same or better at 256x256x80.
+  const int order = xcd_mode_for(K, 2);
+  const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJ, kchunk_for(K, sizeof(T), d, kc_dflt), order == 1, order == 2);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, kTJ));
   const HadvArgs<T> a{const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_a0, p_a1, p_t0, p_t1,
                       s0, s1, s2, s3, eddlat, eddlon};

@@ -55,17 +55,25 @@ struct Box {
 // chunk = g kk + sub.  Workgroup ids go round-robin to the 8 XCDs, so with
 // ti g = 8 a tile and the tile below it (id + 8) run on the same XCD, and the
 // halo rows they share are fetched into that XCD's L2 once.
+// With irow set instead, the ti tiles of one row of one level chunk go to one
+// XCD, in consecutive slots of its dispatch order: id = x + 8 (ci + ti q) for
+// row r = x + 8 q (r = cj + tj chunk), so that tiles side by side fetch the
+// cache lines they share into one L2.
 struct Tiles {
   int ti, tj, nchunk, kchunk, n;
   int xcd = 1;
+  int irow = 0;
 };
 
 // Whether kernel `kname` interleaves level chunks (Tiles::xcd): OEB_XCD_<KNAME>
 // when it holds 0 or 1, else `dflt`.
 bool xcd_for(const char *kname, bool dflt);
+// The tile order of kernel `kname`: OEB_XCD_<KNAME> when it holds 0 (rows of
+// tiles in order), 1 (Tiles::xcd) or 2 (Tiles::irow), else `dflt`.
+int xcd_mode_for(const char *kname, int dflt);
 
 inline Tiles make_tiles(int64_t ni, int64_t nj, int nz, int tile_i, int tile_j, int kchunk,
-                        bool interleave = false) {
+                        bool interleave = false, bool irow = false) {
   Tiles t;
   t.ti = int((ni + tile_i - 1) / tile_i);
   t.tj = int((nj + tile_j - 1) / tile_j);
@@ -73,6 +81,7 @@ inline Tiles make_tiles(int64_t ni, int64_t nj, int nz, int tile_i, int tile_j, 
   t.nchunk = (nz + t.kchunk - 1) / t.kchunk;
   t.n = t.ti * t.tj * t.nchunk;
   if (interleave && t.ti < 8 && 8 % t.ti == 0 && t.nchunk % (8 / t.ti) == 0) t.xcd = 8 / t.ti;
+  if (irow && (int64_t(t.tj) * t.nchunk) % 8 == 0) t.irow = 1;
   return t;
 }
 
@@ -169,15 +178,23 @@ struct TileCoord {
 
 // Tile coordinates of logical block `id` of a launch (tile_i fastest).
 __device__ __forceinline__ TileCoord tile_coord_of(const Tiles &t, int id, int tile_i, int tile_j, int nz) {
-  const int ci = id % t.ti;
-  int rest = id / t.ti;
-  int sub = 0;
-  if (t.xcd > 1) {
-    sub = rest % t.xcd;
-    rest /= t.xcd;
+  int ci, cj, ck;
+  if (t.irow) {
+    const int s = id >> 3, r = (id & 7) + 8 * (s / t.ti);
+    ci = s % t.ti;
+    cj = r % t.tj;
+    ck = r / t.tj;
+  } else {
+    ci = id % t.ti;
+    int rest = id / t.ti;
+    int sub = 0;
+    if (t.xcd > 1) {
+      sub = rest % t.xcd;
+      rest /= t.xcd;
+    }
+    cj = rest % t.tj;
+    ck = (rest / t.tj) * t.xcd + sub;
   }
-  const int cj = rest % t.tj;
-  const int ck = (rest / t.tj) * t.xcd + sub;
   TileCoord c;
   c.i0 = ci * tile_i;
   c.j0 = cj * tile_j;

@@ -135,6 +135,22 @@ def test_laplace_xcd_interleave(shape, xcd, monkeypatch):
     _check("laplace", L, host, run_gpu("laplace", copy(host), L), require_exact=True)
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("order", [0, 2])
+@pytest.mark.parametrize("name", ["fvtp2d", "hadvuv", "hadvuv5th"])
+@pytest.mark.parametrize("shape", [(200, 64, 8), (130, 32, 16), (67, 9, 3)], ids=lambda s: "x".join(map(str, s)))
+def test_tile_row_order(shape, name, order, monkeypatch):
+    """The tiled stencils with the tiles of a row on one XCD (OEB_XCD_<K>=2,
+    Tiles::irow) and in row order, on shapes where that order applies (4 and
+    3 tiles per row, 8 or 16 rows of tiles and level chunks) and where it
+    does not, agree bit for bit with the oracle."""
+    monkeypatch.setenv(f"OEB_XCD_{name.upper()}", str(order))
+    monkeypatch.setenv("OEB_STRIP_FVTP2D", "0")
+    L, host = inputs(name, *shape, "baseline", seed=24)
+    print(f"[gpu] {name} tile order {order} {shape}", flush=True)
+    _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("depth", [1, 2])
 @pytest.mark.parametrize("lanes", [64, 128, 512, 1024])
 @pytest.mark.parametrize("family", ["baseline", "stress"])
