@@ -170,15 +170,17 @@ int laplace_run(const oeb_domain *d, const S *in, const S *out, void *stream) {
   // at 256x256x80 (from run to run) and 1.11x at 512x512x80 in FP64,
   // 1.05-1.16x in FP32.  The other tiled stencils measured no gain from it.
   // OEB_XCD_LAPLACE=0 turns it off.
-  const bool xcd = xcd_for(K, true);
+  // Tiles::irow (OEB_XCD_LAPLACE=2) measured 1.04x slower.
+  const int xo = xcd_mode_for(K, 1);
+  const bool xcd = xo == 1;
   if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_in, p_out})) {
-    const Tiles t = make_tiles(nx, ny, int(nz), W * kTileI, kTileJ, kc, xcd);
+    const Tiles t = make_tiles(nx, ny, int(nz), W * kTileI, kTileJ, kc, xcd, xo == 2);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), W * kTileI, kTileJ));
     hipLaunchKernelGGL(laplace_vec_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), p_in,
                        const_cast<T *>(p_out), g, t);
     return launch_status(K);
   }
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xcd);
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xcd, xo == 2);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
   hipLaunchKernelGGL(laplace_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), p_in,
                      const_cast<T *>(p_out), g, t);
