@@ -136,7 +136,7 @@ def test_laplace_xcd_interleave(shape, xcd, monkeypatch):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("order", [0, 2])
-@pytest.mark.parametrize("name", ["fvtp2d", "hadvuv", "hadvuv5th"])
+@pytest.mark.parametrize("name", ["fvtp2d", "hadvuv", "hadvuv5th", "fastwavesuv", "p_grad_c", "nh_p_grad"])
 @pytest.mark.parametrize("shape", [(200, 64, 8), (130, 32, 16), (67, 9, 3)], ids=lambda s: "x".join(map(str, s)))
 def test_tile_row_order(shape, name, order, monkeypatch):
     """The tiled stencils with the tiles of a row on one XCD (OEB_XCD_<K>=2,
