@@ -235,7 +235,9 @@ int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S
   a.vout = const_cast<T *>(p_vo);
   a.edadlat = edadlat;
   a.dt = dt;
-  const int kc = kchunk_for(K, sizeof(T), d, kChunk);
+  // Levels per workgroup: 8, and 4 in the FP32 form, which measured 56 / 203 us against 59 / 204
+  // with 8 at 256x256x80 / 512x512x80 (the ten-stencil step); FP64 measured the same.
+  const int kc = kchunk_for(K, sizeof(T), d, sizeof(T) == 4 ? 4 : kChunk);
   // The tiles of a row on one XCD (Tiles::irow): FP32 204 / 58 us against 209 / 60
   // at 512x512x80 / 256x256x80, FP64 the same (OEB_XCD_FASTWAVESUV=0 for row order).
   const int xo = xcd_mode_for(K, 2);

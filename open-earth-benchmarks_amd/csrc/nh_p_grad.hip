@@ -159,7 +159,9 @@ int nh_p_grad_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 
   OEB_TRY(as.check(K));
   a.uout = const_cast<T *>(p_uo);
   a.vout = const_cast<T *>(p_vo);
-  const int kc = kchunk_for(K, sizeof(T), d, kChunk);
+  // Levels per workgroup: 8, and 4 in the FP32 form, which measured 60 / 219 us against 62 / 219
+  // with 8 at 256x256x80 / 512x512x80 (the ten-stencil step); FP64 measured the same.
+  const int kc = kchunk_for(K, sizeof(T), d, sizeof(T) == 4 ? 4 : kChunk);
   // The tiles of a row on one XCD (Tiles::irow): FP32 219 / 62 us against 223 / 63
   // at 512x512x80 / 256x256x80, FP64 the same (OEB_XCD_NH_P_GRAD=0 for row order).
   const int xo = xcd_mode_for(K, 2);
