@@ -1958,7 +1958,10 @@ __global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k5(FvAdj<T> a, Geo g, Box
 // axis, through the same LDS, and forms each face's cotangent W2 or W3 (K3,
 // K4) where it forms the face's products, so the tiled form is three launches
 // and W2, W3 never go to memory.
-constexpr int kPX = 64, kPY = 8, kPNT = 256;
+// 64 x 16 tiles of 512 lanes (64.5 KB of LDS in FP64, two workgroups per CU):
+// 393 us for the FP64 adjoint at 256x256x80 against 408 with 64 x 8 tiles of
+// 256 lanes, which form the products of 13 rows for 8 instead of 21 for 16.
+constexpr int kPX = 64, kPY = 16, kPNT = 512;
 constexpr int kPPts = kPX * kPY / kPNT;  // points per lane
 
 template <class T>
@@ -2011,11 +2014,11 @@ __device__ __forceinline__ T ppm_gather(const PpmLds<T> &L, int lx, int ly) {
 // workgroup.
 template <class T, int MODE>
 __device__ __forceinline__ void fvtp2d_adj_ptile_body(PpmLds<T> &L, const FvAdj<T> &a, const Geo &g,
-                                                      const BoxGrid &b, int blk) {
+                                                      const BoxGrid &b, int blk, int k) {
   const int ntx = (b.ni + kPX - 1) / kPX;
   const int i0 = b.ilo + (blk % ntx) * kPX, j0 = b.jlo + (blk / ntx) * kPY;
   const int sj = g.sj;
-  const int64_t plane = int64_t(blockIdx.y) * g.sk;
+  const int64_t plane = int64_t(k) * g.sk;
   const FvRegions R{g.nx, g.ny};
   const T p1 = a.p1, p2 = a.p2;
   auto point = [&](int u, int &lx, int &ly) {
@@ -2096,10 +2099,22 @@ __device__ __forceinline__ void fvtp2d_adj_ptile_body(PpmLds<T> &L, const FvAdj<
   }
 }
 
+// With irow set, the tiles of one row of one level go to one XCD, in
+// consecutive slots of its dispatch order (as Tiles::irow): linear block
+// x + 8 (ci + ntx q) takes row r = x + 8 q (r = tile row + nty level).  The
+// host sets it only when nty nz is a multiple of 8.
 template <class T, int MODE>
-__global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxGrid b) {
+__global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxGrid b, int irow) {
   __shared__ PpmLds<T> L;
-  fvtp2d_adj_ptile_body<T, MODE>(L, a, g, b, int(blockIdx.x));
+  int blk = int(blockIdx.x), k = int(blockIdx.y);
+  if (irow) {
+    const int ntx = (b.ni + kPX - 1) / kPX, nty = (b.nj + kPY - 1) / kPY;
+    const int lin = int(blockIdx.x) + int(gridDim.x) * int(blockIdx.y);
+    const int q = lin >> 3, r = (lin & 7) + 8 * (q / ntx);
+    blk = q % ntx + ntx * (r % nty);
+    k = r / nty;
+  }
+  fvtp2d_adj_ptile_body<T, MODE>(L, a, g, b, blk, k);
 }
 
 inline int ptile_count(const BoxGrid &b) { return ((b.ni + kPX - 1) / kPX) * ((b.nj + kPY - 1) / kPY); }
@@ -2111,9 +2126,9 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile12(FvAdj<T> a, Geo g, Bo
   __shared__ PpmLds<T> L;
   const int blk = int(blockIdx.x);
   if (blk < n1)
-    fvtp2d_adj_ptile_body<T, 1>(L, a, g, b1, blk);
+    fvtp2d_adj_ptile_body<T, 1>(L, a, g, b1, blk, int(blockIdx.y));
   else
-    fvtp2d_adj_ptile_body<T, 2>(L, a, g, b2, blk - n1);
+    fvtp2d_adj_ptile_body<T, 2>(L, a, g, b2, blk - n1, int(blockIdx.y));
 }
 
 inline dim3 ptile_blocks(const BoxGrid &b, int nz) { return dim3(unsigned(ptile_count(b)), unsigned(nz)); }
@@ -2185,7 +2200,11 @@ int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, 
     hipLaunchKernelGGL(fvtp2d_adj_ptile12<T>, dim3(unsigned(n1 + n2), unsigned(nz)), dim3(kPNT), 0, st, a, g, bqj,
                        bqi, n1);
     OEB_TRY(launch_status(K));
-    hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 5>), ptile_blocks(bq, int(nz)), dim3(kPNT), 0, st, a, g, bq);
+    // K5 with the tiles of a row on one XCD (OEB_XCD_FVTP2D_ADJ=2, the default;
+    // 0 for row order): 408 us against 411 for the FP64 adjoint with 64 x 8 tiles.
+    const int nty = (bq.nj + kPY - 1) / kPY;
+    const int irow = (xcd_mode_for(K, 2) == 2 && (int64_t(nty) * nz) % 8 == 0) ? 1 : 0;
+    hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 5>), ptile_blocks(bq, int(nz)), dim3(kPNT), 0, st, a, g, bq, irow);
     return launch_status(K);
   }
   hipLaunchKernelGGL(fvtp2d_adj_k1<T>, box_blocks(bqj, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqj);
