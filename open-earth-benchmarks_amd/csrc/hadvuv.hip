@@ -218,8 +218,12 @@ int hadv_run(const char *K, const oeb_domain *d, const S3 *uout, const S3 *vout,
   as.add1("tgrlatda1", p_t1, 0, ny, s3);
   OEB_TRY(as.check(K));
   // Levels per workgroup: 4, and 5 for FP32 hadvuv, which measured 1.12x
-  // faster with 5 at 256x256x80 (the same at 512x512x80).
-  const int kc_dflt = (ORDER == 3 && sizeof(T) == 4) ? 5 : kChunk;
+  // faster with 5 at 256x256x80 (the same at 512x512x80).  FP64 takes 8 when
+  // a level holds 256 tiles or more: at 512x512x80 (256 tiles) hadvuv measured
+  // 136 us against 141 and hadvuv5th 157 against 166; at 256x256x80 (64
+  // tiles) 8 levels measured slower than 4.
+  const int64_t level_tiles = ((nx + kTI - 1) / kTI) * ((ny + kTJ - 1) / kTJ);
+  const int kc_dflt = sizeof(T) == 4 ? (ORDER == 3 ? 5 : kChunk) : (level_tiles >= 256 ? 8 : kChunk);
   // The tiles of a row on one XCD (Tiles::irow): FP64 hadvuv 141 us against
   // 145 at 512x512x80, hadvuv5th 1% faster, the same or better at 256x256x80.
   const int order = xcd_mode_for(K, 2);

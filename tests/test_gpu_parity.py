@@ -135,6 +135,19 @@ def test_laplace_xcd_interleave(shape, xcd, monkeypatch):
     _check("laplace", L, host, run_gpu("laplace", copy(host), L), require_exact=True)
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kchunk", [1, 3, 8, 80])
+@pytest.mark.parametrize("name", ["hadvuv", "hadvuv5th"])
+def test_hadv_level_chunks(name, kchunk, monkeypatch):
+    """hadvuv and hadvuv5th at level chunks that divide nz, do not divide
+    it, and exceed it (the next level's rows are loaded during the current
+    one) agree bit for bit with the oracle."""
+    monkeypatch.setenv(f"OEB_KCHUNK_{name.upper()}", str(kchunk))
+    L, host = inputs(name, 130, 36, 11, "baseline", seed=25)
+    print(f"[gpu] {name} kchunk={kchunk}", flush=True)
+    _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("order", [0, 2])
 @pytest.mark.parametrize("name", ["fvtp2d", "hadvuv", "hadvuv5th", "fastwavesuv", "p_grad_c", "nh_p_grad"])
 @pytest.mark.parametrize("shape", [(200, 64, 8), (130, 32, 16), (67, 9, 3)], ids=lambda s: "x".join(map(str, s)))
