@@ -339,14 +339,23 @@ def ensemble_block(n_members: int, seed: int, steps: int = 10, reps: int = 3):
         "batched_graph_untuned_ms_per_step": round(untuned_graph_ms, 3),
         "batched_graph_ms_per_step": round(graph_ms, 3),
         "autotune": {"choices": len(choices), "seconds": round(tune_s, 2),
-                     "kchunk_histogram": {str(k): sum(1 for v in choices.values() if v["kchunk"] == k)
-                                          for k in sorted({v["kchunk"] for v in choices.values()})}},
+                     "non_default": sum(1 for v in choices.values() if v["knobs"]),
+                     "knob_histogram": _knob_histogram(choices)},
         "plain_gpts_s": gpts(plain_ms),
         "batched_graph_gpts_s": gpts(graph_ms),
         "speedup_graph_vs_plain": round(plain_ms / graph_ms, 3),
         "setup_s": round(setup, 2),
         "state_finite": finite,
     }
+
+
+def _knob_histogram(choices: dict) -> dict:
+    """How often each knob value was chosen over the default."""
+    h: dict = {}
+    for v in choices.values():
+        for k, x in v["knobs"].items():
+            h[f"{k}={x}"] = h.get(f"{k}={x}", 0) + 1
+    return dict(sorted(h.items()))
 
 
 def adjoint_block(L, forms, seed, steps: int, segment: int = 10, reps: int = 5):

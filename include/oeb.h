@@ -401,14 +401,33 @@ int oeb_fvtp2d_adj_f32(const oeb_domain *d, const oeb_s3f *q, const oeb_s3f *crx
 #pragma pop_macro("p1")
 
 /* ---------------------------------------------------------------- tuning
- * No counterpart in the reference.  Each entry point picks the levels per
- * workgroup (k-chunk) and, where it has one, its 16-byte vector path from,
- * in order: the environment (OEB_KCHUNK_<KERNEL>, OEB_PAIRS_<KERNEL>), this
- * process-wide table, a built-in default.  The table is keyed by kernel name
- * ("hadvuv5th", "laplace", ...), element size (8 or 4 bytes) and the
- * domain's (nx, ny, nz); the project's autotuner (oeb.tune) fills it.
- * kchunk 0 and vector -1 mean "not set"; setting both removes the entry.
- * Thread-safe. */
+ * No counterpart in the reference: its drivers fix every loop order at
+ * compile time.  Each entry point has launch parameters ("knobs") with
+ * built-in defaults, which a process-wide table may override per kernel name
+ * ("hadvuv5th", "fvtp2d", "hdiffsmag_adj", ...), element size and domain.
+ * The project's autotuner (oeb.tune) times the candidates and fills the
+ * table.  Knobs (a kernel ignores those it does not have):
+ *   "kchunk" [1, 4096]  levels per workgroup of a tile launch
+ *   "vector" [0, 1]     1: the 16-byte vector / column-pair body
+ *   "form"   [0, 1]     1: the row-march (strip) form, 0: the tile form
+ *   "rows"   [1, 4096]  rows per strip chunk
+ *   "depth"  [1, 8]     rows a strip loads ahead
+ *   "order"  [0, 3]     tile order: 0 rows in order, 1 level chunks
+ *                       interleaved between tile rows, 2 the tiles of a row
+ *                       on one XCD, 3 whole level chunks per XCD
+ *   "tile"   [0, 12]    tile shape (kernel-specific index)
+ *   "lanes"  [64, 1024] workgroup size
+ *   "grid"   [1, 2^30]  workgroups of a batched launch (kernel "batch")
+ * oeb_tune_knob_set stores `value` for (kernel, element_bytes, d); value -1
+ * removes it.  element_bytes 0 stands for both precisions and d == NULL for
+ * every domain.  A call uses the most specific entry that sets the knob: its
+ * domain and element size, then every domain, then both precisions.  A value
+ * a kernel's launch cannot use falls back to that kernel's default.
+ * oeb_tune_knob_get reports the entry for exactly that key (-1 if unset).
+ * oeb_tune_set / oeb_tune_get handle "kchunk" (0 = unset) and "vector" (-1 =
+ * unset) together.  Thread-safe. */
+int oeb_tune_knob_set(const char *kernel, int element_bytes, const oeb_domain *d, const char *knob, int value);
+int oeb_tune_knob_get(const char *kernel, int element_bytes, const oeb_domain *d, const char *knob, int *value);
 int oeb_tune_set(const char *kernel, int element_bytes, const oeb_domain *d, int kchunk, int vector);
 int oeb_tune_get(const char *kernel, int element_bytes, const oeb_domain *d, int *kchunk, int *vector);
 int oeb_tune_clear(void);

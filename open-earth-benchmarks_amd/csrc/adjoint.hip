@@ -305,12 +305,12 @@ int hdiffsa_adj_run(const oeb_domain *d, const S3 *in, const S3 *mask, const S1 
   OEB_TRY(as.check(K));
   a.inbar = const_cast<T *>(p_ib);
   a.lapbar = const_cast<T *>(p_lb);
-  if (nx + 6 <= 1024 && strip_for(K, true)) {
+  if (nx + 6 <= 1024 && knob_on(K, sizeof(T), d, KNOB_FORM, true)) {
     const int lanes = int((nx + 6 + 63) / 64 * 64);
     // Row chunks measured at 256x256x80 over 16-64: 24 rows fastest in both
     // forms (FP64 58.5 us against 61 with 32; FP32 45.8 us against 48.5 with
     // 16).  At most 256 rows, which bounds the crlato / crlatu tables in LDS.
-    const int rows = std::min(strip_rows_for(K, 24), 256);
+    const int rows = std::min(knob(K, sizeof(T), d, KNOB_ROWS, 24), 256);
     const int nchunk = int((ny + 4 + rows - 1) / rows);
     if (int64_t(nchunk) * nz > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
     hipLaunchKernelGGL(hdiffsa_adj_strip<T>, dim3(unsigned(nchunk * nz)), dim3(unsigned(lanes)),
@@ -465,7 +465,7 @@ int fastwavesuv_adj_run(const oeb_domain *d, const S3 *wgtfac, const oeb_s3 *ppu
   a.vinbar = const_cast<T *>(p_vb);
   a.ppbar = const_cast<double *>(p_pb);
   const BoxGrid bp{0, 0, int(nx + 1), int(ny + 1)};
-  const int kc = kchunk_for(K, sizeof(T), d, 8);
+  const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, 8);
   const int nchunk = int((nz + 2 + kc - 1) / kc);
   hipLaunchKernelGGL(fastwavesuv_adj_kernel<T>, box_blocks(bp, nchunk), dim3(kTI, kTJ), 0, hipStream_t(stream), a, g,
                      bp, kc);
@@ -1116,37 +1116,37 @@ int hadvuv5th_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
   for (int q = 0; q < 8; ++q) *dst[q] = const_cast<T *>(p_s[q]);
   a.uinbar = const_cast<T *>(p_ub);
   a.vinbar = const_cast<T *>(p_vb);
-  // OEB_TILE_HADVUV5TH_ADJ: 1 = the tiled form, 0 = the row march (or the
+  // the tile knob: 1 = the tiled form, 0 = the row march (or the
   // two-pass form).  FP64 takes 64x16 tiles of 1024 lanes (151 KB of LDS, one
   // workgroup per CU); FP32 64x8 tiles of 512 lanes (51 KB), or 64x16 of 1024
-  // with OEB_TILE_HADVUV5TH_ADJ=2.  Measured at 256x256x80: FP64 tiles 173 us
+  // with tile knob 2.  Measured at 256x256x80: FP64 tiles 173 us
   // (4 levels per workgroup; 180 at 2) against 211 for the row march; FP32
   // tiles 107 us against 86, so FP32 keeps the row march.
-  const int tile = env_choice("OEB_TILE_", K, sizeof(T) == 8 ? 1 : 0, 0, 2);
+  const int tile = knob(K, sizeof(T), d, KNOB_TILE, sizeof(T) == 8 ? 1 : 0);
   if (tile != 0) {
-    const int kc = kchunk_for(K, sizeof(T), d, 4);
+    const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, 4);
     if (sizeof(T) == 4 && tile == 1) {
       const Tiles t = make_tiles(nx, ny, int(nz), kTI, 8, kc);
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 8));
       if constexpr (sizeof(T) == 4)
         hipLaunchKernelGGL((hadvuv5th_adj_tile<T, 8, 512>), dim3(t.n), dim3(512), 0, hipStream_t(stream), a, g, t);
     } else {
-      const int order = xcd_mode_for(K, 0);
-      const Tiles t = make_tiles(nx, ny, int(nz), kTI, 16, kc, order == 1, order == 2);
+      const int order = knob(K, sizeof(T), d, KNOB_ORDER, 0);
+      const Tiles t = make_tiles(nx, ny, int(nz), kTI, 16, kc, order);
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 16));
       hipLaunchKernelGGL((hadvuv5th_adj_tile<T, 16, 1024>), dim3(t.n), dim3(1024), 0, hipStream_t(stream), a, g, t);
     }
     return launch_status(K);
   }
-  if (nx + 12 <= 640 && strip_for(K, true)) {
+  if (nx + 12 <= 640 && knob_on(K, sizeof(T), d, KNOB_FORM, true)) {
     const int lanes = int((nx + 12 + 63) / 64 * 64);
     // Row chunks measured at 256x256x80 over 8-64: 32 rows fastest in both
     // forms.  At most 256 rows, which bounds the 1-D tables in LDS.
-    const int rows = std::min(strip_rows_for(K, 32), 256);
+    const int rows = std::min(knob(K, sizeof(T), d, KNOB_ROWS, 32), 256);
     const int nchunk = int((ny + 6 + rows - 1) / rows);
     if (int64_t(nchunk) * nz > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
     // Rows loaded one ahead: two ahead spills registers and measured slower.
-    auto kern = strip_depth_for(K, 1) >= 2 ? hadvuv5th_adj_strip<T, 2> : hadvuv5th_adj_strip<T, 1>;
+    auto kern = knob(K, sizeof(T), d, KNOB_DEPTH, 1) >= 2 ? hadvuv5th_adj_strip<T, 2> : hadvuv5th_adj_strip<T, 1>;
     hipLaunchKernelGGL(kern, dim3(unsigned(nchunk * nz)), dim3(unsigned(lanes)),
                        (size_t(lanes) * 16 + 4 * size_t(rows + 10)) * sizeof(T) + size_t(lanes) * 2 * sizeof(int),
                        hipStream_t(stream), a, g, rows, nchunk);
@@ -1734,37 +1734,37 @@ int hdiffsmag_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
   a.wu = const_cast<T *>(p_w[2]), a.wv = const_cast<T *>(p_w[3]);
   a.uinbar = const_cast<T *>(p_ub);
   a.vinbar = const_cast<T *>(p_vb);
-  // OEB_TILE_HDIFFSMAG_ADJ: 1 = 64x8 tiles of 512 lanes, 2 = 64x16 tiles of
+  // the tile knob: 1 = 64x8 tiles of 512 lanes, 2 = 64x16 tiles of
   // 1024 lanes, 0 = the row march (or the two-pass form).  Measured at
   // 256x256x80 with 4 levels per workgroup: FP64 64x16 tiles 146-148 us, 64x8
   // 149-155, row march 158-160; FP32 tiles 85-95 us against 72 for the row
   // march.  Loading mask, ubar and vbar of the next level during the gathers
   // raised the registers to 117 and took 170 us.
-  const int tile = env_choice("OEB_TILE_", K, sizeof(T) == 8 ? 2 : 0, 0, 2);
+  const int tile = knob(K, sizeof(T), d, KNOB_TILE, sizeof(T) == 8 ? 2 : 0);
   if (tile != 0) {
-    const int kc = kchunk_for(K, sizeof(T), d, 4);
+    const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, 4);
     if (tile == 1) {
       const Tiles t = make_tiles(nx, ny, int(nz), kTI, 8, kc);
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 8));
       hipLaunchKernelGGL((hdiffsmag_adj_tile<T, 8, 512>), dim3(t.n), dim3(512), 0, hipStream_t(stream), a, g, t);
     } else {
-      const int order = xcd_mode_for(K, 2);  // Tiles::irow: 143 us against 145
-      const Tiles t = make_tiles(nx, ny, int(nz), kTI, 16, kc, order == 1, order == 2);
+      const int order = knob(K, sizeof(T), d, KNOB_ORDER, 2);  // Tiles::irow: 143 us against 145
+      const Tiles t = make_tiles(nx, ny, int(nz), kTI, 16, kc, order);
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 16));
       hipLaunchKernelGGL((hdiffsmag_adj_tile<T, 16, 1024>), dim3(t.n), dim3(1024), 0, hipStream_t(stream), a, g, t);
     }
     return launch_status(K);
   }
-  if (nx + 4 <= 1024 && strip_for(K, true)) {
+  if (nx + 4 <= 1024 && knob_on(K, sizeof(T), d, KNOB_FORM, true)) {
     const int lanes = int((nx + 4 + 63) / 64 * 64);
     // Measured at 256x256x80: FP64 fastest with 16-row chunks and rows
     // loaded one row ahead (two rows ahead spills registers), FP32 with
     // 32-row chunks loaded two rows ahead.
     // At most 256 rows, which bounds the 1-D tables in LDS.
-    const int rows = std::min(strip_rows_for(K, sizeof(T) == 8 ? 16 : 32), 256);
+    const int rows = std::min(knob(K, sizeof(T), d, KNOB_ROWS, sizeof(T) == 8 ? 16 : 32), 256);
     const int nchunk = int((ny + 2 + rows - 1) / rows);
     if (int64_t(nchunk) * nz > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
-    auto kern = strip_depth_for(K, sizeof(T) == 8 ? 1 : 2) >= 2 ? hdiffsmag_adj_strip<T, 2> : hdiffsmag_adj_strip<T, 1>;
+    auto kern = knob(K, sizeof(T), d, KNOB_DEPTH, sizeof(T) == 8 ? 1 : 2) >= 2 ? hdiffsmag_adj_strip<T, 2> : hdiffsmag_adj_strip<T, 1>;
     hipLaunchKernelGGL(kern, dim3(unsigned(nchunk * nz)), dim3(unsigned(lanes)),
                        (size_t(20) * lanes + 5 * size_t(rows + 9)) * sizeof(T), hipStream_t(stream), a, g, rows,
                        nchunk);
@@ -2190,21 +2190,21 @@ int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, 
   const BoxGrid bfx2{0, -3, int(nx + 1), int(ny + 5)}, bfy2{-3, 0, int(nx + 5), int(ny + 1)};
   const BoxGrid bq{-3, -3, int(nx + 6), int(ny + 6)};
   hipStream_t st = hipStream_t(stream);
-  // The tiled transposes (fvtp2d_adj_ptile) by default; OEB_TILE_FVTP2D_ADJ=0
+  // The tiled transposes (fvtp2d_adj_ptile) by default; tile knob 0
   // selects the per-point gathers.
   // The tiled form (fvtp2d_adj_ptile: K1 and K2 in one launch, then K5, which
-  // forms K3 and K4 inside) by default; OEB_TILE_FVTP2D_ADJ=0 selects the five per-point passes,
+  // forms K3 and K4 inside) by default; tile knob 0 selects the five per-point passes,
   // with W2 and W3 through work2 and work3.
-  const bool tiled = env_choice("OEB_TILE_", K, 1, 0, 1) != 0;
+  const bool tiled = knob(K, sizeof(T), d, KNOB_TILE, 1) != 0;
   if (tiled) {
     const int n1 = ptile_count(bqj), n2 = ptile_count(bqi);
     hipLaunchKernelGGL(fvtp2d_adj_ptile12<T>, dim3(unsigned(n1 + n2), unsigned(nz)), dim3(kPNT), 0, st, a, g, bqj,
                        bqi, n1);
     OEB_TRY(launch_status(K));
-    // K5 with the tiles of a row on one XCD (OEB_XCD_FVTP2D_ADJ=2, the default;
+    // K5 with the tiles of a row on one XCD (order knob 2, the default;
     // 0 for row order): 408 us against 411 for the FP64 adjoint with 64 x 8 tiles.
     const int nty = (bq.nj + kPY - 1) / kPY;
-    const int irow = (xcd_mode_for(K, 2) == 2 && (int64_t(nty) * nz) % 8 == 0) ? 1 : 0;
+    const int irow = (knob(K, sizeof(T), d, KNOB_ORDER, 2) == 2 && (int64_t(nty) * nz) % 8 == 0) ? 1 : 0;
     hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 5>), ptile_blocks(bq, int(nz)), dim3(kPNT), 0, st, a, g, bq, irow);
     return launch_status(K);
   }

@@ -152,21 +152,9 @@ int check_strips(const char *kname, const Strips &s) {
 }
 
 namespace {
-// Value of the environment variable <prefix><KNAME> (kernel name in upper
-// case) when it holds an integer in [lo, hi], else `dflt`.
-int env_tunable(const char *prefix, const char *kname, int dflt, long lo, long hi) {
-  std::string var = prefix;
-  for (const char *c = kname; *c; ++c) var += char(std::toupper(static_cast<unsigned char>(*c)));
-  const char *v = std::getenv(var.c_str());
-  if (v == nullptr) return dflt;
-  char *end = nullptr;
-  const long n = std::strtol(v, &end, 10);
-  return (end != v && *end == '\0' && n >= lo && n <= hi) ? int(n) : dflt;
-}
-}  // namespace
-
-namespace {
-// The tuning table: (kernel, element size, nx, ny, nz) -> (k-chunk, vector path).
+// The tuning table: (kernel, element size, nx, ny, nz) -> one value per knob
+// (-1: not set).  Element size 0 stands for both precisions and a domain of
+// (0, 0, 0) for every domain.
 struct TuneKey {
   std::string kernel;
   size_t esize;
@@ -183,53 +171,55 @@ struct TuneHash {
   }
 };
 struct TuneVal {
-  int kchunk;  // 0: not set
-  int vec;     // -1: not set
+  int v[KNOB_COUNT];
+  TuneVal() {
+    for (int &x : v) x = -1;
+  }
+  bool empty() const {
+    for (int x : v)
+      if (x >= 0) return false;
+    return true;
+  }
 };
 std::mutex g_tune_mu;
 std::unordered_map<TuneKey, TuneVal, TuneHash> g_tune;
 
-bool tune_lookup(const char *kname, size_t esize, const oeb_domain *d, TuneVal *v) {
-  std::lock_guard<std::mutex> lock(g_tune_mu);
-  if (g_tune.empty()) return false;
-  const auto it = g_tune.find(TuneKey{kname, esize, d->nx, d->ny, d->nz});
-  if (it == g_tune.end()) return false;
-  *v = it->second;
-  return true;
+struct KnobInfo {
+  const char *name;
+  int lo, hi;
+};
+// Names and valid ranges of the knobs, in enum order.
+constexpr KnobInfo kKnobs[KNOB_COUNT] = {
+    {"kchunk", 1, 4096}, {"vector", 0, 1}, {"form", 0, 1},     {"rows", 1, 4096}, {"depth", 1, 8},
+    {"order", 0, 3},     {"tile", 0, 12},  {"lanes", 64, 1024}, {"grid", 1, 1 << 30},
+};
+
+int knob_index(const char *name) {
+  for (int k = 0; k < KNOB_COUNT; ++k)
+    if (std::strcmp(kKnobs[k].name, name) == 0) return k;
+  return -1;
+}
+
+// The table's value of knob k for this kernel, element size and domain, most
+// specific entry first; -1 when no entry sets it.  Caller holds g_tune_mu.
+int tune_lookup_locked(const char *kname, size_t esize, const oeb_domain *d, int k) {
+  if (g_tune.empty()) return -1;
+  const int64_t nx = d ? d->nx : 0, ny = d ? d->ny : 0, nz = d ? d->nz : 0;
+  const TuneKey keys[4] = {{kname, esize, nx, ny, nz}, {kname, esize, 0, 0, 0}, {kname, 0, nx, ny, nz},
+                           {kname, 0, 0, 0, 0}};
+  for (const TuneKey &key : keys) {
+    const auto it = g_tune.find(key);
+    if (it != g_tune.end() && it->second.v[k] >= 0) return it->second.v[k];
+  }
+  return -1;
 }
 }  // namespace
 
-int kchunk_for(const char *kname, size_t esize, const oeb_domain *d, int dflt) {
-  const int env = env_tunable("OEB_KCHUNK_", kname, 0, 1, 4096);
-  if (env > 0) return env;
-  TuneVal v;
-  if (tune_lookup(kname, esize, d, &v) && v.kchunk > 0) return v.kchunk;
-  return dflt;
+int knob(const char *kname, size_t esize, const oeb_domain *d, Knob k, int dflt) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  const int v = tune_lookup_locked(kname, esize, d, int(k));
+  return v >= 0 ? v : dflt;
 }
-
-bool pairs_for(const char *kname, size_t esize, const oeb_domain *d, bool dflt) {
-  const int env = env_tunable("OEB_PAIRS_", kname, -1, 0, 1);
-  if (env >= 0) return env != 0;
-  TuneVal v;
-  if (tune_lookup(kname, esize, d, &v) && v.vec >= 0) return v.vec != 0;
-  return dflt;
-}
-
-int strip_rows_for(const char *kname, int dflt) { return env_tunable("OEB_ROWS_", kname, dflt, 1, 4096); }
-
-int env_choice(const char *prefix, const char *kname, int dflt, int lo, int hi) {
-  return env_tunable(prefix, kname, dflt, lo, hi);
-}
-
-bool xcd_for(const char *kname, bool dflt) { return env_tunable("OEB_XCD_", kname, dflt ? 1 : 0, 0, 1) != 0; }
-
-int xcd_mode_for(const char *kname, int dflt) { return env_tunable("OEB_XCD_", kname, dflt, 0, 2); }
-
-int batch_grid() { return env_tunable("OEB_BATCH_", "grid", kBatchGrid, 1, long(1) << 30); }
-
-int strip_depth_for(const char *kname, int dflt) { return env_tunable("OEB_DEPTH_", kname, dflt, 1, 8); }
-
-bool strip_for(const char *kname, bool dflt) { return env_tunable("OEB_STRIP_", kname, dflt ? 1 : 0, 0, 1) != 0; }
 
 int launch_status(const char *kname) {
   const hipError_t e = hipGetLastError();
@@ -237,6 +227,24 @@ int launch_status(const char *kname) {
   return OEB_OK;
 }
 
+}  // namespace oeb
+
+namespace oeb {
+namespace {
+int tune_args(const char *fn, const char *kernel, int element_bytes, const oeb_domain *d) {
+  using namespace oeb;
+  if (kernel == nullptr) return fail(OEB_ERR_NULL, std::string(fn) + ": kernel is NULL");
+  if (element_bytes != 0 && element_bytes != 4 && element_bytes != 8)
+    return fail(OEB_ERR_ARG, std::string(fn) + ": element_bytes must be 0, 4 or 8");
+  if (d != nullptr && (d->nx < 0 || d->ny < 0 || d->nz < 0))
+    return fail(OEB_ERR_DOMAIN, std::string(fn) + ": domain extents must be >= 0");
+  return OEB_OK;
+}
+
+TuneKey tune_key(const char *kernel, int element_bytes, const oeb_domain *d) {
+  return TuneKey{kernel, size_t(element_bytes), d ? d->nx : 0, d ? d->ny : 0, d ? d->nz : 0};
+}
+}  // namespace
 }  // namespace oeb
 
 extern "C" {
@@ -258,30 +266,60 @@ const char *oeb_status_string(int status) {
 
 const char *oeb_last_error(void) { return oeb::g_last_error.c_str(); }
 
+
+int oeb_tune_knob_set(const char *kernel, int element_bytes, const oeb_domain *d, const char *knob, int value) {
+  using namespace oeb;
+  OEB_TRY(tune_args("tune_knob_set", kernel, element_bytes, d));
+  if (knob == nullptr) return fail(OEB_ERR_NULL, "tune_knob_set: knob is NULL");
+  const int k = knob_index(knob);
+  if (k < 0) return fail(OEB_ERR_ARG, std::string("tune_knob_set: unknown knob '") + knob + "'");
+  if (value != -1 && (value < kKnobs[k].lo || value > kKnobs[k].hi)) {
+    std::ostringstream m;
+    m << "tune_knob_set: " << knob << " must lie in [" << kKnobs[k].lo << ", " << kKnobs[k].hi << "] or be -1";
+    return fail(OEB_ERR_ARG, m.str());
+  }
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  const TuneKey key = tune_key(kernel, element_bytes, d);
+  auto it = g_tune.find(key);
+  if (it == g_tune.end()) {
+    if (value == -1) return OEB_OK;
+    it = g_tune.emplace(key, TuneVal()).first;
+  }
+  it->second.v[k] = value;
+  if (it->second.empty()) g_tune.erase(it);
+  return OEB_OK;
+}
+
+int oeb_tune_knob_get(const char *kernel, int element_bytes, const oeb_domain *d, const char *knob, int *value) {
+  using namespace oeb;
+  OEB_TRY(tune_args("tune_knob_get", kernel, element_bytes, d));
+  if (knob == nullptr || value == nullptr) return fail(OEB_ERR_NULL, "tune_knob_get: knob or value is NULL");
+  const int k = knob_index(knob);
+  if (k < 0) return fail(OEB_ERR_ARG, std::string("tune_knob_get: unknown knob '") + knob + "'");
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  const auto it = g_tune.find(tune_key(kernel, element_bytes, d));
+  *value = it == g_tune.end() ? -1 : it->second.v[k];
+  return OEB_OK;
+}
+
 int oeb_tune_set(const char *kernel, int element_bytes, const oeb_domain *d, int kchunk, int vector) {
   using namespace oeb;
   if (kernel == nullptr || d == nullptr) return fail(OEB_ERR_NULL, "tune_set: kernel or domain is NULL");
   if (element_bytes != 4 && element_bytes != 8) return fail(OEB_ERR_ARG, "tune_set: element_bytes must be 4 or 8");
   if (kchunk < 0 || kchunk > 4096 || vector < -1 || vector > 1)
     return fail(OEB_ERR_ARG, "tune_set: kchunk must lie in [0, 4096] and vector in [-1, 1]");
-  std::lock_guard<std::mutex> lock(g_tune_mu);
-  const TuneKey key{kernel, size_t(element_bytes), d->nx, d->ny, d->nz};
-  if (kchunk == 0 && vector == -1) {
-    g_tune.erase(key);
-  } else {
-    g_tune[key] = TuneVal{kchunk, vector};
-  }
-  return OEB_OK;
+  OEB_TRY(oeb_tune_knob_set(kernel, element_bytes, d, "kchunk", kchunk == 0 ? -1 : kchunk));
+  return oeb_tune_knob_set(kernel, element_bytes, d, "vector", vector);
 }
 
 int oeb_tune_get(const char *kernel, int element_bytes, const oeb_domain *d, int *kchunk, int *vector) {
   using namespace oeb;
   if (kernel == nullptr || d == nullptr || kchunk == nullptr || vector == nullptr)
     return fail(OEB_ERR_NULL, "tune_get: a pointer argument is NULL");
-  TuneVal v{0, -1};
-  tune_lookup(kernel, size_t(element_bytes), d, &v);
-  *kchunk = v.kchunk;
-  *vector = v.vec;
+  int kc = -1;
+  OEB_TRY(oeb_tune_knob_get(kernel, element_bytes, d, "kchunk", &kc));
+  OEB_TRY(oeb_tune_knob_get(kernel, element_bytes, d, "vector", vector));
+  *kchunk = kc < 0 ? 0 : kc;
   return OEB_OK;
 }
 

@@ -237,20 +237,20 @@ int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S
   a.dt = dt;
   // Levels per workgroup: 8, and 4 in the FP32 form, which measured 56 / 203 us against 59 / 204
   // with 8 at 256x256x80 / 512x512x80 (the ten-stencil step); FP64 measured the same.
-  const int kc = kchunk_for(K, sizeof(T), d, sizeof(T) == 4 ? 4 : kChunk);
+  const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, sizeof(T) == 4 ? 4 : kChunk);
   // The tiles of a row on one XCD (Tiles::irow): FP32 204 / 58 us against 209 / 60
-  // at 512x512x80 / 256x256x80, FP64 the same (OEB_XCD_FASTWAVESUV=0 for row order).
-  const int xo = xcd_mode_for(K, 2);
-  if (pairs_for(K, sizeof(T), d, true) && nx % 2 == 0 &&
+  // at 512x512x80 / 256x256x80, FP64 the same (order knob 0 for row order).
+  const int xo = knob(K, sizeof(T), d, KNOB_ORDER, 2);
+  if ((knob(K, sizeof(T), d, KNOB_VECTOR, true) != 0) && nx % 2 == 0 &&
       pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.utens, a.vtens, a.wgtfac, a.ppuv, a.hhl, a.rho})) {
-    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc, xo == 1, xo == 2);
+    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc, xo);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
     if (batch_record(kFastwavesuvPairBatch<T>, a, g, t, t.n)) return OEB_OK;
     hipLaunchKernelGGL(fastwavesuv_pair_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g,
                        t);
     return launch_status(K);
   }
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo == 1, xo == 2);
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
   if (batch_record(kFastwavesuvBatch<T>, a, g, t, t.n)) return OEB_OK;
   hipLaunchKernelGGL(fastwavesuv_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);

@@ -368,9 +368,9 @@ int fvtp2d_launch(const char *K, const FvIn<T> &a, const FvOut<T> &o, T p1, T p2
                   const oeb_domain *d, void *stream) {
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   // The tiles of a row on one XCD (Tiles::irow): 546 / 157 us against 554 /
-  // 159 in FP64 at 512x512x80 / 256x256x80 (OEB_XCD_FVTP2D=0 for row order).
-  const int order = xcd_mode_for(K, 2);
-  const Tiles t = make_tiles(nx, ny, int(nz), TI, TJ, kchunk_for(K, sizeof(T), d, kChunk), order == 1, order == 2);
+  // 159 in FP64 at 512x512x80 / 256x256x80 (order knob 0 for row order).
+  const int order = knob(K, sizeof(T), d, KNOB_ORDER, 2);
+  const Tiles t = make_tiles(nx, ny, int(nz), TI, TJ, knob(K, sizeof(T), d, KNOB_KCHUNK, kChunk), order);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), TI, TJ));
   hipLaunchKernelGGL((fvtp2d_kernel<TI, TJ, NT, T>), dim3(t.n), dim3(NT), 0, hipStream_t(stream), a, o, p1, p2, g, t);
   return launch_status(K);
@@ -430,7 +430,7 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   o.fy1 = const_cast<T *>(pfy1);
   o.fy2 = const_cast<T *>(pfy2);
 
-  // Strips by default in FP32, tiles in FP64; OEB_STRIP_FVTP2D forces either.
+  // Strips by default in FP32, tiles in FP64; the form knob forces either.
   // Measured in the ten-stencil step at 256x256x80 and 512x512x80: FP32
   // strips of 320 lanes take 93 / 318 us against 118 / 396 us for the best
   // tiles (1.25x).  FP64 strips read 1.5x fewer bytes than the tiles
@@ -443,17 +443,17 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   // registers (no j-halo re-fetched or recomputed), fetched 599 MB and took
   // 164 / 570 us: the tiles' extra bytes are mostly the i-halo (a 64-wide FP64
   // row with 3 halo points each side spans 6 cache lines instead of 4).
-  if (strip_for(K, sizeof(T) == 4)) {
-    // Strips of at most 512 lanes (OEB_LANES_FVTP2D: 64 .. 1024).
-    const int max_lanes = env_choice("OEB_LANES_", K, 512, 64, 1024);
+  if (knob_on(K, sizeof(T), d, KNOB_FORM, sizeof(T) == 4)) {
+    // Strips of at most 512 lanes (the lanes knob: 64 .. 1024).
+    const int max_lanes = knob(K, sizeof(T), d, KNOB_LANES, 512);
     // At most 12 rows per chunk: FP32 308-310 us against 321-322 for the 32
     // rows of default_strip_rows at 512x512x80, 94-95 us against 95-96 for
     // its 20 at 256x256x80 (the ten-stencil step, repeated).
     Strips st = make_strips(nx, ny, int(nz), 3, 1, 1, max_lanes);
-    st = make_strips(nx, ny, int(nz), 3, strip_rows_for(K, std::min(12, default_strip_rows(ny, st.lanes))), 1,
+    st = make_strips(nx, ny, int(nz), 3, knob(K, sizeof(T), d, KNOB_ROWS, std::min(12, default_strip_rows(ny, st.lanes))), 1,
                      max_lanes);
     OEB_TRY(check_strips(K, st));
-    if (strip_depth_for(K, 1) >= 2) return fvtp2d_strip_launch<T, 2>(K, a, o, p1, p2, g, st, stream);
+    if (knob(K, sizeof(T), d, KNOB_DEPTH, 1) >= 2) return fvtp2d_strip_launch<T, 2>(K, a, o, p1, p2, g, st, stream);
     return fvtp2d_strip_launch<T, 1>(K, a, o, p1, p2, g, st, stream);
   }
 
@@ -465,9 +465,9 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   // of their operands, and more waves hide that.  1024 lanes hold fewer
   // workgroups and ran 1.2-1.3x slower.  FP32: 128x16 tiles of 512 lanes
   // (half the bytes per tile, so less halo for the same LDS) are fastest.
-  // OEB_TILE_FVTP2D = 1..12 forces one of the measured shapes, 0 the 64x8
+  // the tile knob = 1..12 forces one of the measured shapes, 0 the 64x8
   // tiles of 256 lanes.
-  switch (env_choice("OEB_TILE_", K, sizeof(T) == 4 ? 3 : 8, 0, 12)) {
+  switch (knob(K, sizeof(T), d, KNOB_TILE, sizeof(T) == 4 ? 3 : 8)) {
     case 10: return fvtp2d_launch<64, 16, 1024, T>(K, a, o, p1, p2, g, d, stream);
     case 11: return fvtp2d_launch<128, 16, 1024, T>(K, a, o, p1, p2, g, d, stream);
     case 12: return fvtp2d_launch<128, 8, 1024, T>(K, a, o, p1, p2, g, d, stream);

@@ -226,14 +226,14 @@ int hadv_run(const char *K, const oeb_domain *d, const S3 *uout, const S3 *vout,
   const int kc_dflt = sizeof(T) == 4 ? (ORDER == 3 ? 5 : kChunk) : (level_tiles >= 256 ? 8 : kChunk);
   // The tiles of a row on one XCD (Tiles::irow): FP64 hadvuv 141 us against
   // 145 at 512x512x80, hadvuv5th 1% faster, the same or better at 256x256x80.
-  const int order = xcd_mode_for(K, 2);
-  const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJ, kchunk_for(K, sizeof(T), d, kc_dflt), order == 1, order == 2);
+  const int order = knob(K, sizeof(T), d, KNOB_ORDER, 2);
+  const Tiles t = make_tiles(nx, ny, int(nz), kTI, kTJ, knob(K, sizeof(T), d, KNOB_KCHUNK, kc_dflt), order);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, kTJ));
   const HadvArgs<T> a{const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_a0, p_a1, p_t0, p_t1,
                       s0, s1, s2, s3, eddlat, eddlon};
   if (batch_record(kHadvuvBatch<ORDER, T>, a, g, t, t.n)) return OEB_OK;
   // Lanes per 64x16 tile: 256 (4 rows each), 512 (2) or 1024 (1).
-  const int nt = env_choice("OEB_LANES_", K, kHadvLanes<T>, 256, 1024);
+  const int nt = knob(K, sizeof(T), d, KNOB_LANES, kHadvLanes<T>);
   if (nt >= 1024)
     hipLaunchKernelGGL((hadvuv_kernel<ORDER, T, 1024>), dim3(t.n), dim3(kTI, 16), 0, hipStream_t(stream), a, g, t);
   else if (nt >= 512)

@@ -176,23 +176,23 @@ int hdiffsa_run(const oeb_domain *d, const S3 *out, const S3 *in, const S3 *mask
   as.add1("crlato", p_lo, -1, ny + 1, s_lo);
   as.add1("crlatu", p_lu, -1, ny + 1, s_lu);
   OEB_TRY(as.check(K));
-  if (strip_for(K, true)) {
+  if (knob_on(K, sizeof(T), d, KNOB_FORM, true)) {
     // At most 16 rows per chunk: measured in the ten-stencil step against
     // the 20 (256x256x80) and 32 (512x512x80) rows of default_strip_rows,
     // FP64 34 / 106 us against 37 / 109, FP32 23 / 64 against 26 / 62.
     Strips st = make_strips(nx, ny, int(nz), 2, 1);
-    st = make_strips(nx, ny, int(nz), 2, strip_rows_for(K, std::min(16, default_strip_rows(ny, st.lanes))));
+    st = make_strips(nx, ny, int(nz), 2, knob(K, sizeof(T), d, KNOB_ROWS, std::min(16, default_strip_rows(ny, st.lanes))));
     OEB_TRY(check_strips(K, st));
     // Measured at 256x256x80 and 512x512x80: prefetching 2 rows ahead is
     // fastest in FP64 (4 and 8 rows: 1.2x and 1.8x slower), 4 in FP32.
-    const int depth = strip_depth_for(K, sizeof(T) == 8 ? 2 : 4);
+    const int depth = knob(K, sizeof(T), d, KNOB_DEPTH, sizeof(T) == 8 ? 2 : 4);
     auto kern = depth <= 2 ? hdiffsa_strip_kernel<T, 2> : depth <= 4 ? hdiffsa_strip_kernel<T, 4>
                                                                      : hdiffsa_strip_kernel<T, 8>;
     hipLaunchKernelGGL(kern, dim3(unsigned(st.n)), dim3(unsigned(st.lanes)), size_t(5) * st.lanes * sizeof(T),
                        hipStream_t(stream), const_cast<T *>(p_out), p_in, p_mask, p_lo, s_lo, p_lu, s_lu, g, st);
     return launch_status(K);
   }
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kchunk_for(K, sizeof(T), d, kChunk));
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, knob(K, sizeof(T), d, KNOB_KCHUNK, kChunk));
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
   hipLaunchKernelGGL(hdiffsa_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream),
                      const_cast<T *>(p_out), p_in, p_mask, p_lo, s_lo, p_lu, s_lu, g, t);

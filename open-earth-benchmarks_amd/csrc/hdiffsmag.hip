@@ -245,11 +245,11 @@ int hdiffsmag_seg_run(const HsArgs<T> &a, const Geo &g, const oeb_domain *d, hip
   // measured 184 us against 194 for the formula's 32 in FP64 at 512x512x80
   // (the ten-stencil step; the other cases already take 16 rows or fewer).
   const int row_lanes = make_segments(d->nx, d->ny, int(d->nz), V, 1).ti * kSegLanes;
-  const int rows = strip_rows_for(K, std::min(16, default_strip_rows(d->ny, row_lanes)));
+  const int rows = knob(K, sizeof(T), d, KNOB_ROWS, std::min(16, default_strip_rows(d->ny, row_lanes)));
   const Tiles t = make_segments(d->nx, d->ny, int(d->nz), V, rows);
   if (t.n < 1 || int64_t(t.ti) * t.tj * t.nchunk > (int64_t(1) << 30))
     return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
-  if (strip_depth_for(K, 1) >= 2) {
+  if (knob(K, sizeof(T), d, KNOB_DEPTH, 1) >= 2) {
     if (batch_record(kHdiffsmagSegBatch<T, V, 2>, a, g, t, t.n)) return OEB_OK;
     hipLaunchKernelGGL((hdiffsmag_seg_kernel<T, V, 2>), dim3(t.n), dim3(kSegLanes * kSegWaves), 0, stream, a, g, t);
   } else {
@@ -293,13 +293,13 @@ int hdiffsmag_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 
   as.add1("crlatu", p_tu1, 0, ny, s_tu);
   as.add1("acrlat0", p_ac, -1, ny + 1, s_ac);
   OEB_TRY(as.check(K));
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kchunk_for(K, sizeof(T), d, kChunk));
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, knob(K, sizeof(T), d, KNOB_KCHUNK, kChunk));
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
   const HsArgs<T> a{const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_m, p_vo1, p_vu1, p_to1, p_tu1, p_ac,
                     s_vo, s_vu, s_to, s_tu, s_ac, eddlat, eddlon, tau_smag, weight_smag};
-  if (strip_for(K, true)) {
+  if (knob_on(K, sizeof(T), d, KNOB_FORM, true)) {
     constexpr int W = 16 / int(sizeof(T));
-    if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_uo, p_vo, p_ui, p_vi, p_m}))
+    if ((knob(K, sizeof(T), d, KNOB_VECTOR, true) != 0) && vec_ok(g, W, sizeof(T), {p_uo, p_vo, p_ui, p_vi, p_m}))
       return hdiffsmag_seg_run<T, W>(a, g, d, hipStream_t(stream));
     return hdiffsmag_seg_run<T, 1>(a, g, d, hipStream_t(stream));
   }

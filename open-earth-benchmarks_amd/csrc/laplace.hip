@@ -131,9 +131,9 @@ template <class T, int V>
 int laplace_strip_launch(const T *p_in, T *p_out, const Geo &g, const oeb_domain *d, hipStream_t stream) {
   const char *K = "laplace";
   Strips st = make_strips(d->nx, d->ny, int(d->nz), 0, 1, V);
-  st = make_strips(d->nx, d->ny, int(d->nz), 0, strip_rows_for(K, default_strip_rows(d->ny, st.lanes)), V);
+  st = make_strips(d->nx, d->ny, int(d->nz), 0, knob(K, sizeof(T), d, KNOB_ROWS, default_strip_rows(d->ny, st.lanes)), V);
   OEB_TRY(check_strips(K, st));
-  const int depth = strip_depth_for(K, 2);
+  const int depth = knob(K, sizeof(T), d, KNOB_DEPTH, 2);
   auto kern = depth <= 1 ? laplace_strip_kernel<T, V, 1>
                          : (depth <= 2 ? laplace_strip_kernel<T, V, 2> : laplace_strip_kernel<T, V, 4>);
   hipLaunchKernelGGL(kern, dim3(unsigned(st.n)), dim3(unsigned(st.lanes)), 0, stream, p_in, p_out, g, st);
@@ -158,29 +158,28 @@ int laplace_run(const oeb_domain *d, const S *in, const S *out, void *stream) {
   // The row-march form is off by default: in the ten-stencil step (its input
   // not in the MALL) the tile form measured 1.15x faster at 256x256x80 and
   // 512x512x80; run back to back on one field (input in the MALL) the strip
-  // form measured 1.1-1.2x faster.  OEB_STRIP_LAPLACE=1 selects it.
-  if (strip_for(K, false)) {
-    if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_in, p_out}))
+  // form measured 1.1-1.2x faster.  form knob 1 selects it.
+  if (knob_on(K, sizeof(T), d, KNOB_FORM, false)) {
+    if ((knob(K, sizeof(T), d, KNOB_VECTOR, true) != 0) && vec_ok(g, W, sizeof(T), {p_in, p_out}))
       return laplace_strip_launch<T, W>(p_in, const_cast<T *>(p_out), g, d, hipStream_t(stream));
     return laplace_strip_launch<T, 1>(p_in, const_cast<T *>(p_out), g, d, hipStream_t(stream));
   }
-  const int kc = kchunk_for(K, sizeof(T), d, kChunk);
+  const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, kChunk);
   // Level chunks interleaved so that a tile and the tile below it run on one
   // XCD (Tiles::xcd): measured in the ten-stencil step, 1.05-1.26x faster
   // at 256x256x80 (from run to run) and 1.11x at 512x512x80 in FP64,
   // 1.05-1.16x in FP32.  The other tiled stencils measured no gain from it.
-  // OEB_XCD_LAPLACE=0 turns it off.
-  // Tiles::irow (OEB_XCD_LAPLACE=2) measured 1.04x slower.
-  const int xo = xcd_mode_for(K, 1);
-  const bool xcd = xo == 1;
-  if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_in, p_out})) {
-    const Tiles t = make_tiles(nx, ny, int(nz), W * kTileI, kTileJ, kc, xcd, xo == 2);
+  // order knob 0 turns it off.
+  // Tiles::irow (order knob 2) measured 1.04x slower.
+  const int xo = knob(K, sizeof(T), d, KNOB_ORDER, 1);
+  if ((knob(K, sizeof(T), d, KNOB_VECTOR, true) != 0) && vec_ok(g, W, sizeof(T), {p_in, p_out})) {
+    const Tiles t = make_tiles(nx, ny, int(nz), W * kTileI, kTileJ, kc, xo);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), W * kTileI, kTileJ));
     hipLaunchKernelGGL(laplace_vec_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), p_in,
                        const_cast<T *>(p_out), g, t);
     return launch_status(K);
   }
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xcd, xo == 2);
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
   hipLaunchKernelGGL(laplace_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), p_in,
                      const_cast<T *>(p_out), g, t);

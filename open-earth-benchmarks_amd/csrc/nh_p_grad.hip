@@ -161,22 +161,22 @@ int nh_p_grad_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 
   a.vout = const_cast<T *>(p_vo);
   // Levels per workgroup: 8, and 4 in the FP32 form, which measured 60 / 219 us against 62 / 219
   // with 8 at 256x256x80 / 512x512x80 (the ten-stencil step); FP64 measured the same.
-  const int kc = kchunk_for(K, sizeof(T), d, sizeof(T) == 4 ? 4 : kChunk);
+  const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, sizeof(T) == 4 ? 4 : kChunk);
   // The tiles of a row on one XCD (Tiles::irow): FP32 219 / 62 us against 223 / 63
-  // at 512x512x80 / 256x256x80, FP64 the same (OEB_XCD_NH_P_GRAD=0 for row order).
-  const int xo = xcd_mode_for(K, 2);
+  // at 512x512x80 / 256x256x80, FP64 the same (order knob 0 for row order).
+  const int xo = knob(K, sizeof(T), d, KNOB_ORDER, 2);
   // Column pairs by default in the FP32 form only.  Measured in the
   // ten-stencil step at 256x256x80 and 512x512x80: FP32 pairs take
   // 63 / 223 us against 68 / 241 us for single columns; FP64 pairs were slower.
-  if (pairs_for(K, sizeof(T), d, sizeof(T) == 4) && nx % 2 == 0 &&
+  if ((knob(K, sizeof(T), d, KNOB_VECTOR, sizeof(T) != 0) == 4) && nx % 2 == 0 &&
       pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdx, a.rdy, a.gz, a.pp, a.pk3, a.wk1})) {
-    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc, xo == 1, xo == 2);
+    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc, xo);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
     hipLaunchKernelGGL(nh_p_grad_pair_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt, g,
                        t);
     return launch_status(K);
   }
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo == 1, xo == 2);
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
   hipLaunchKernelGGL(nh_p_grad_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt, g, t);
   return launch_status(K);

@@ -49,39 +49,79 @@ struct Box {
   int64_t ilo, ihi, jlo, jhi, klo, khi;
 };
 
+// ------------------------------------------------------------------ knobs
+// Launch parameters that the autotuner may set per kernel, element size and
+// domain (oeb_tune_knob_set).  Each entry point asks for the knobs it has with
+// knob(), giving its built-in default; the tuning table's most specific entry
+// (this domain, then every domain; this element size, then both) wins over it.
+//   KNOB_KCHUNK  levels per workgroup of a tile launch
+//   KNOB_VECTOR  1: the 16-byte vector / column-pair body where it applies
+//   KNOB_FORM    1: the strip (row march) form, 0: the tile form
+//   KNOB_ROWS    rows per strip chunk
+//   KNOB_DEPTH   rows a strip loads ahead
+//   KNOB_ORDER   tile order: 0 rows in order, 1 Tiles::xcd, 2 Tiles::irow,
+//                3 Tiles::klev
+//   KNOB_TILE    tile shape (kernel-specific index)
+//   KNOB_LANES   workgroup size of a strip or tile
+//   KNOB_GRID    workgroups of a batched launch
+enum Knob {
+  KNOB_KCHUNK,
+  KNOB_VECTOR,
+  KNOB_FORM,
+  KNOB_ROWS,
+  KNOB_DEPTH,
+  KNOB_ORDER,
+  KNOB_TILE,
+  KNOB_LANES,
+  KNOB_GRID,
+  KNOB_COUNT
+};
+int knob(const char *kname, size_t esize, const oeb_domain *d, Knob k, int dflt);
+inline bool knob_on(const char *kname, size_t esize, const oeb_domain *d, Knob k, bool dflt) {
+  return knob(kname, esize, d, k, dflt ? 1 : 0) != 0;
+}
+
 // Tile layout of a column launch: logical id -> (tile_i, tile_j, chunk),
-// tile_i fastest.  With xcd = g > 1, g consecutive level chunks are
-// interleaved between the rows of tiles: id = ci + ti (sub + g (cj + tj kk)),
-// chunk = g kk + sub.  Workgroup ids go round-robin to the 8 XCDs, so with
-// ti g = 8 a tile and the tile below it (id + 8) run on the same XCD, and the
-// halo rows they share are fetched into that XCD's L2 once.
-// With irow set instead, the ti tiles of one row of one level chunk go to one
-// XCD, in consecutive slots of its dispatch order: id = x + 8 (ci + ti q) for
-// row r = x + 8 q (r = cj + tj chunk), so that tiles side by side fetch the
-// cache lines they share into one L2.
+// tile_i fastest.  Workgroup ids go round-robin to the 8 XCDs (id mod 8), each
+// with its own 4 MB L2; the orders below place tiles that share halo lines on
+// one XCD.
+//   ORDER_ROWS  rows of tiles in order, level chunks last.
+//   ORDER_XCD   (xcd = g > 1) g consecutive level chunks interleaved between
+//               the rows of tiles: id = ci + ti (sub + g (cj + tj kk)),
+//               chunk = g kk + sub.  With ti g = 8 a tile and the tile below
+//               it (id + 8) run on one XCD and fetch their shared halo rows
+//               into its L2 once.
+//   ORDER_IROW  (irow) the ti tiles of one row of one level chunk go to one
+//               XCD, in consecutive slots of its dispatch order:
+//               id = x + 8 (ci + ti q) for row r = x + 8 q (r = cj + tj chunk),
+//               so that tiles side by side share their i-halo lines.
+//   ORDER_KLEV  (klev) XCD x takes the level chunks x, x + 8, x + 16, ...,
+//               each one whole, tiles in row order: id = x + 8 s with
+//               s = ci + ti (cj + tj m), chunk = x + 8 m.  Tiles side by side
+//               and one above the other then run on one XCD a few slots
+//               apart, so both their i- and j-halo lines are shared in its L2.
+//               The last nchunk mod 8 chunks follow in row order.
+enum TileOrder { ORDER_ROWS = 0, ORDER_XCD = 1, ORDER_IROW = 2, ORDER_KLEV = 3 };
 struct Tiles {
   int ti, tj, nchunk, kchunk, n;
   int xcd = 1;
   int irow = 0;
+  int klev = 0;
 };
 
-// Whether kernel `kname` interleaves level chunks (Tiles::xcd): OEB_XCD_<KNAME>
-// when it holds 0 or 1, else `dflt`.
-bool xcd_for(const char *kname, bool dflt);
-// The tile order of kernel `kname`: OEB_XCD_<KNAME> when it holds 0 (rows of
-// tiles in order), 1 (Tiles::xcd) or 2 (Tiles::irow), else `dflt`.
-int xcd_mode_for(const char *kname, int dflt);
-
-inline Tiles make_tiles(int64_t ni, int64_t nj, int nz, int tile_i, int tile_j, int kchunk,
-                        bool interleave = false, bool irow = false) {
+// Tiles of TILE_I x TILE_J columns and `kchunk` levels in tile order `order`
+// (TileOrder).  An order whose condition the shape does not meet falls back
+// to ORDER_ROWS.
+inline Tiles make_tiles(int64_t ni, int64_t nj, int nz, int tile_i, int tile_j, int kchunk, int order = ORDER_ROWS) {
   Tiles t;
   t.ti = int((ni + tile_i - 1) / tile_i);
   t.tj = int((nj + tile_j - 1) / tile_j);
   t.kchunk = kchunk < 1 ? 1 : kchunk;
   t.nchunk = (nz + t.kchunk - 1) / t.kchunk;
   t.n = t.ti * t.tj * t.nchunk;
-  if (interleave && t.ti < 8 && 8 % t.ti == 0 && t.nchunk % (8 / t.ti) == 0) t.xcd = 8 / t.ti;
-  if (irow && (int64_t(t.tj) * t.nchunk) % 8 == 0) t.irow = 1;
+  if (order == ORDER_XCD && t.ti < 8 && 8 % t.ti == 0 && t.nchunk % (8 / t.ti) == 0) t.xcd = 8 / t.ti;
+  if (order == ORDER_IROW && (int64_t(t.tj) * t.nchunk) % 8 == 0) t.irow = 1;
+  if (order == ORDER_KLEV && t.nchunk >= 8) t.klev = 1;
   return t;
 }
 
@@ -179,7 +219,20 @@ struct TileCoord {
 // Tile coordinates of logical block `id` of a launch (tile_i fastest).
 __device__ __forceinline__ TileCoord tile_coord_of(const Tiles &t, int id, int tile_i, int tile_j, int nz) {
   int ci, cj, ck;
-  if (t.irow) {
+  if (t.klev) {
+    const int per = t.ti * t.tj, full = 8 * per * (t.nchunk / 8);
+    if (id < full) {
+      const int s = id >> 3;
+      ci = s % t.ti;
+      cj = (s / t.ti) % t.tj;
+      ck = (id & 7) + 8 * (s / per);
+    } else {  // the last nchunk mod 8 chunks, in row order
+      const int r = id - full;
+      ci = r % t.ti;
+      cj = (r / t.ti) % t.tj;
+      ck = 8 * (t.nchunk / 8) + r / per;
+    }
+  } else if (t.irow) {
     const int s = id >> 3, r = (id & 7) + 8 * (s / t.ti);
     ci = s % t.ti;
     cj = r % t.tj;
@@ -288,28 +341,13 @@ __device__ __forceinline__ StripCoord strip_coord(const Strips &s, int id, int n
 // Fails with OEB_ERR_DOMAIN when a strip launch does not fit one grid.
 int check_strips(const char *kname, const Strips &s);
 
-// Rows per strip chunk for kernel `kname`: OEB_ROWS_<KNAME> when it holds an
-// integer in [1, 4096], else `dflt`.
-int strip_rows_for(const char *kname, int dflt);
-
-// Rows a strip kernel prefetches ahead (2, 4 or 8): OEB_DEPTH_<KNAME> when
-// it holds an integer in [1, 8], else `dflt`.
-int strip_depth_for(const char *kname, int dflt);
-
-// The environment variable <prefix><KNAME> when it holds an integer in
-// [lo, hi], else `dflt`.
-int env_choice(const char *prefix, const char *kname, int dflt, int lo, int hi);
-
-// Whether kernel `kname` takes its strip path: OEB_STRIP_<KNAME> when it
-// holds 0 or 1, else `dflt`.
-bool strip_for(const char *kname, bool dflt);
 
 // ------------------------------------------------------------------ batches
 // A batch (oeb_batch_*) records the launches the entry points would make,
 // for many members of an ensemble, and replays each run of consecutive
 // records of one kernel as ONE launch.  The run's blocks are numbered member
 // after member (member 0's blocks, then member 1's, ...): that list is the
-// tile queue.  A grid of at most batch_grid() workgroups walks it with stride
+// tile queue.  A grid of at most KNOB_GRID workgroups walks it with stride
 // gridDim.x, each workgroup taking every gridDim.x-th tile, so no atomic
 // counter has to be reset between replays; with the default cap the grid
 // covers the whole list and each workgroup takes one tile.  Each record keeps the
@@ -397,13 +435,10 @@ __global__ void __launch_bounds__(NT) batch_kernel(const BatchMember *__restrict
   }
 }
 
-// Workgroups of a persistent launch: OEB_BATCH_GRID when it holds an integer
-// in [1, 2^30], else kBatchGrid.
-int batch_grid();
 
 template <class A, void (*Body)(const A &, const Geo &, const Tiles &, int), int NT = 256>
 hipError_t launch_batch(const BatchMember *table, int n, int total, hipStream_t stream, dim3 block) {
-  const int cap = batch_grid();
+  const int cap = knob("batch", 0, nullptr, KNOB_GRID, kBatchGrid);
   const int grid = total < cap ? total : cap;
   hipLaunchKernelGGL((batch_kernel<A, Body, NT>), dim3(grid), block, 0, stream, table, n, total);
   return hipGetLastError();
@@ -462,17 +497,7 @@ int launch_status(const char *kname);
 // grid dimension holds.
 int check_tiles(const char *kname, const Tiles &t, int64_t ni, int64_t nj, int nz, int tile_i, int tile_j);
 
-// Levels per workgroup for kernel `kname` on element size `esize` and domain
-// `d`: the environment variable OEB_KCHUNK_<KNAME> (upper case, e.g.
-// OEB_KCHUNK_P_GRAD_C) when it holds an integer in [1, 4096]; else the
-// tuning table's entry for (kname, esize, nx, ny, nz) (oeb_tune_set); else
-// `dflt`.
-int kchunk_for(const char *kname, size_t esize, const oeb_domain *d, int dflt);
 
-// Whether kernel `kname` may take its vector (16-byte) path: OEB_PAIRS_<KNAME>
-// when it holds 0 or 1; else the tuning table; else `dflt`.  The vector path
-// still needs its alignment conditions (pairs_ok / vec_ok).
-bool pairs_for(const char *kname, size_t esize, const oeb_domain *d, bool dflt);
 
 #define OEB_TRY(expr)              \
   do {                             \

@@ -134,22 +134,22 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
   OEB_TRY(as.check(K));
   a.uout = const_cast<T *>(p_uo);
   a.vout = const_cast<T *>(p_vo);
-  const int kc = kchunk_for(K, sizeof(T), d, kChunk);
+  const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, kChunk);
   // The tiles of a row on one XCD (Tiles::irow): FP32 184 / 52 us against 189 / 53
-  // at 512x512x80 / 256x256x80, FP64 the same (OEB_XCD_P_GRAD_C=0 for row order).
-  const int xo = xcd_mode_for(K, 2);
+  // at 512x512x80 / 256x256x80, FP64 the same (order knob 0 for row order).
+  const int xo = knob(K, sizeof(T), d, KNOB_ORDER, 2);
   // Column pairs by default in the FP32 form only.  Measured in the
   // ten-stencil step at 256x256x80 and 512x512x80: FP32 pairs take
   // 52 / 189 us against 55 / 203 us for single columns; FP64 pairs were slower.
-  if (pairs_for(K, sizeof(T), d, sizeof(T) == 4) && nx % 2 == 0 &&
+  if ((knob(K, sizeof(T), d, KNOB_VECTOR, sizeof(T) != 0) == 4) && nx % 2 == 0 &&
       pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdxc, a.rdyc, a.delpc, a.gz, a.pkc})) {
-    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc, xo == 1, xo == 2);
+    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc, xo);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
     hipLaunchKernelGGL(p_grad_c_pair_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g,
                        t);
     return launch_status(K);
   }
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo == 1, xo == 2);
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
   hipLaunchKernelGGL(p_grad_c_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g, t);
   return launch_status(K);

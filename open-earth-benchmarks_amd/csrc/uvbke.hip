@@ -119,19 +119,19 @@ int uvbke_run(const oeb_domain *d, const S *ub, const S *vb, const S *uc, const 
   as.add("cosa", p_ca, r0, g, false);
   as.add("rsina", p_rs, r0, g, false);
   OEB_TRY(as.check(K));
-  const int kc = kchunk_for(K, sizeof(T), d, kChunk);
-  // Tile order (OEB_XCD_UVBKE): rows of tiles in order; Tiles::irow measured the same.
-  const int xo = xcd_mode_for(K, 0);
+  const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, kChunk);
+  // Tile order (the order knob): rows of tiles in order; Tiles::irow measured the same.
+  const int xo = knob(K, sizeof(T), d, KNOB_ORDER, 0);
   const UvArgs<T> a{const_cast<T *>(p_ub), const_cast<T *>(p_vb), p_uc, p_vc, p_ca, p_rs, dt5};
   constexpr int W = kVecW<T>;
-  if (pairs_for(K, sizeof(T), d, true) && vec_ok(g, W, sizeof(T), {p_ub, p_vb, p_uc, p_vc, p_ca, p_rs})) {
-    const Tiles t = make_tiles(nx, ny, int(nz), W * kTileI, kTileJ, kc, xo == 1, xo == 2);
+  if ((knob(K, sizeof(T), d, KNOB_VECTOR, true) != 0) && vec_ok(g, W, sizeof(T), {p_ub, p_vb, p_uc, p_vc, p_ca, p_rs})) {
+    const Tiles t = make_tiles(nx, ny, int(nz), W * kTileI, kTileJ, kc, xo);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), W * kTileI, kTileJ));
     if (batch_record(kUvbkeVecBatch<T>, a, g, t, t.n)) return OEB_OK;
     hipLaunchKernelGGL(uvbke_vec_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
     return launch_status(K);
   }
-  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo == 1, xo == 2);
+  const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
   if (batch_record(kUvbkeBatch<T>, a, g, t, t.n)) return OEB_OK;
   hipLaunchKernelGGL(uvbke_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);

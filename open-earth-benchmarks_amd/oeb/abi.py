@@ -126,7 +126,8 @@ EXPORTS = ["oeb_version", "oeb_status_string", "oeb_last_error"] + [
     f"oeb_{k}_{p}" for p in PRECISIONS for k in SIGNATURES
 ] + ["oeb_halo_fill_f64", "oeb_halo_fill_f32", "oeb_halo_fill_n_f64", "oeb_halo_fill_n_f32",
      "oeb_graph_begin", "oeb_graph_end", "oeb_graph_launch", "oeb_graph_destroy", "oeb_batch_create",
-     "oeb_batch_record", "oeb_batch_stop", "oeb_batch_launch", "oeb_batch_info", "oeb_batch_destroy", "oeb_tune_set", "oeb_tune_get", "oeb_tune_clear"] + [
+     "oeb_batch_record", "oeb_batch_stop", "oeb_batch_launch", "oeb_batch_info", "oeb_batch_destroy", "oeb_tune_set", "oeb_tune_get", "oeb_tune_clear",
+     "oeb_tune_knob_set", "oeb_tune_knob_get"] + [
     f"oeb_{k}_adj_{p}" for p in PRECISIONS for k in ADJOINTS
 ]
 
@@ -202,6 +203,10 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.oeb_tune_get.restype = C.c_int
     lib.oeb_tune_get.argtypes = [C.c_char_p, C.c_int, PD, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     lib.oeb_tune_clear.restype = C.c_int
+    lib.oeb_tune_knob_set.restype = C.c_int
+    lib.oeb_tune_knob_set.argtypes = [C.c_char_p, C.c_int, PD, C.c_char_p, C.c_int]
+    lib.oeb_tune_knob_get.restype = C.c_int
+    lib.oeb_tune_knob_get.argtypes = [C.c_char_p, C.c_int, PD, C.c_char_p, C.POINTER(C.c_int)]
     lib.oeb_tune_clear.argtypes = []
     for prec in PRECISIONS:
         for name, (_, kinds) in ADJOINTS.items():

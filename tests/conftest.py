@@ -31,3 +31,23 @@ def pytest_collection_modifyitems(config, items):
     for item in items:
         if "gpu" in item.keywords:
             item.add_marker(skip)
+
+
+class _Knobs:
+    """Sets launch knobs of the library's tuning table for every domain and
+    both precisions (oeb_tune_knob_set); the fixture clears the table after
+    the test."""
+
+    def set(self, kernel: str, knob: str, value) -> None:
+        from oeb import tune
+
+        tune.set_knob(kernel, None, None, knob, int(value))
+
+
+@pytest.fixture
+def knobs():
+    from oeb import tune
+
+    tune.clear()
+    yield _Knobs()
+    tune.clear()

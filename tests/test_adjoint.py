@@ -221,9 +221,9 @@ def test_adjoint_central_differences(name):
 
 @pytest.mark.parametrize("rows", [1, 3, 64])
 @pytest.mark.parametrize("shape", [(40, 36, 5), (67, 9, 3)], ids=lambda s: "x".join(map(str, s)))
-def test_hdiffsa_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
+def test_hdiffsa_adjoint_strip_matches_two_pass(shape, rows, knobs):
     """The fused row-march hdiffsa adjoint and the two-pass one (lap_bar
-    through memory), forced through OEB_STRIP_HDIFFSA_ADJ, give the same
+    through memory), forced through the form knob, give the same
     doubles, at row chunks of one row, of a number that does not divide
     ny + 4, and of more than ny + 4."""
     import torch
@@ -238,8 +238,8 @@ def test_hdiffsa_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
     print(f"[gpu] hdiffsa adjoint strip vs two-pass {shape} rows={rows}", flush=True)
     out = []
     for strip in (0, 1):
-        monkeypatch.setenv("OEB_STRIP_HDIFFSA_ADJ", str(strip))
-        monkeypatch.setenv("OEB_ROWS_HDIFFSA_ADJ", str(rows))
+        knobs.set("hdiffsa_adj", "form", strip)
+        knobs.set("hdiffsa_adj", "rows", rows)
         out.append(_grads(lambda v: (AG.Hdiffsa.apply(v, c, L, dom),), [x], [ybar])[0])
     assert torch.equal(out[0], out[1])
 
@@ -247,11 +247,11 @@ def test_hdiffsa_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
 @pytest.mark.parametrize("depth", [1, 2])
 @pytest.mark.parametrize("rows", [1, 4, 64])
 @pytest.mark.parametrize("shape", [(40, 36, 5), (67, 9, 3)], ids=lambda s: "x".join(map(str, s)))
-def test_hadvuv5th_adjoint_strip_matches_two_pass(shape, rows, depth, monkeypatch):
+def test_hadvuv5th_adjoint_strip_matches_two_pass(shape, rows, depth, knobs):
     """The fused row-march hadvuv5th adjoint (x contributions gathered from
     LDS, y contributions pushed down each column), with rows loaded one and
     two ahead, and the two-pass one (eight scratch arrays), forced through
-    OEB_STRIP_HADVUV5TH_ADJ, agree to 1e-13 of the largest cotangent: the same
+    the form knob, agree to 1e-13 of the largest cotangent: the same
     terms, summed in another order."""
     import torch
 
@@ -264,11 +264,11 @@ def test_hadvuv5th_adjoint_strip_matches_two_pass(shape, rows, depth, monkeypatc
     ybars = [torch.randn(x.shape, generator=g, device="cuda", dtype=x.dtype) for x in xs]
     print(f"[gpu] hadvuv5th adjoint strip vs two-pass {shape} rows={rows} depth={depth}", flush=True)
     out = []
-    monkeypatch.setenv("OEB_TILE_HADVUV5TH_ADJ", "0")
-    monkeypatch.setenv("OEB_DEPTH_HADVUV5TH_ADJ", str(depth))
+    knobs.set("hadvuv5th_adj", "tile", 0)
+    knobs.set("hadvuv5th_adj", "depth", depth)
     for strip in (0, 1):
-        monkeypatch.setenv("OEB_STRIP_HADVUV5TH_ADJ", str(strip))
-        monkeypatch.setenv("OEB_ROWS_HADVUV5TH_ADJ", str(rows))
+        knobs.set("hadvuv5th_adj", "form", strip)
+        knobs.set("hadvuv5th_adj", "rows", rows)
         out.append(_grads(lambda u, v: AG.Hadvuv5th.apply(u, v, c, L, dom), xs, ybars))
     for a, b in zip(*out):
         assert _relmax(b, a) <= 1e-13
@@ -279,10 +279,10 @@ def test_hadvuv5th_adjoint_strip_matches_two_pass(shape, rows, depth, monkeypatc
 @pytest.mark.parametrize("tile", [1, 2])
 @pytest.mark.parametrize("shape", [(40, 36, 5), (67, 9, 3), (130, 21, 2), (64, 32, 4), (3, 2, 2), (200, 40, 2)],
                          ids=lambda s: "x".join(map(str, s)))
-def test_hadvuv5th_adjoint_tile_matches_two_pass(shape, tile, kchunk, precision, monkeypatch):
+def test_hadvuv5th_adjoint_tile_matches_two_pass(shape, tile, kchunk, precision, knobs):
     """The tiled hadvuv5th adjoint (stage 1 of a tile and its 3-point halo
     in LDS, then the two-pass form's gathers), forced through
-    OEB_TILE_HADVUV5TH_ADJ (FP64: 64x16 tiles; FP32: 64x8 and 64x16) with
+    the tile knob (FP64: 64x16 tiles; FP32: 64x8 and 64x16) with
     1-3 levels per workgroup, and the two-pass form give the same bits: the
     same expressions, the same terms in the same order.  Shapes with edge
     tiles narrower than a tile, several tiles each way, a 3x2 domain, and
@@ -301,10 +301,10 @@ def test_hadvuv5th_adjoint_tile_matches_two_pass(shape, tile, kchunk, precision,
     ybars = [torch.randn(x.shape, generator=g, device="cuda", dtype=x.dtype) for x in xs]
     print(f"[gpu] hadvuv5th adjoint tile {tile} vs two-pass {shape} kchunk={kchunk} {precision}", flush=True)
     out = []
-    monkeypatch.setenv("OEB_STRIP_HADVUV5TH_ADJ", "0")
-    monkeypatch.setenv("OEB_KCHUNK_HADVUV5TH_ADJ", str(kchunk))
+    knobs.set("hadvuv5th_adj", "form", 0)
+    knobs.set("hadvuv5th_adj", "kchunk", kchunk)
     for t in (0, tile):
-        monkeypatch.setenv("OEB_TILE_HADVUV5TH_ADJ", str(t))
+        knobs.set("hadvuv5th_adj", "tile", t)
         out.append(_grads(lambda u, v: AG.Hadvuv5th.apply(u, v, c, L, dom), xs, ybars))
     for a, b in zip(*out):
         assert torch.equal(a, b)
@@ -315,9 +315,9 @@ def test_hadvuv5th_adjoint_tile_matches_two_pass(shape, tile, kchunk, precision,
 @pytest.mark.parametrize("tile", [1, 2])
 @pytest.mark.parametrize("shape", [(40, 36, 5), (67, 9, 3), (130, 21, 2), (64, 32, 4), (3, 2, 2), (200, 40, 2)],
                          ids=lambda s: "x".join(map(str, s)))
-def test_hdiffsmag_adjoint_tile_matches_two_pass(shape, tile, kchunk, precision, monkeypatch):
+def test_hdiffsmag_adjoint_tile_matches_two_pass(shape, tile, kchunk, precision, knobs):
     """The tiled hdiffsmag adjoint (stage 1, then T_bar and S_bar, in LDS,
-    then the two-pass form's gathers), forced through OEB_TILE_HDIFFSMAG_ADJ
+    then the two-pass form's gathers), forced through the tile knob
     with 64x8 and 64x16 tiles and 1 or 3 levels per workgroup, and the
     two-pass form give the same bits: the same expressions, terms and order.
     Shapes with edge tiles narrower than a tile, several tiles each way and
@@ -335,10 +335,10 @@ def test_hdiffsmag_adjoint_tile_matches_two_pass(shape, tile, kchunk, precision,
     ybars = [torch.randn(x.shape, generator=g, device="cuda", dtype=x.dtype) for x in xs]
     print(f"[gpu] hdiffsmag adjoint tile {tile} vs two-pass {shape} kchunk={kchunk} {precision}", flush=True)
     out = []
-    monkeypatch.setenv("OEB_STRIP_HDIFFSMAG_ADJ", "0")
-    monkeypatch.setenv("OEB_KCHUNK_HDIFFSMAG_ADJ", str(kchunk))
+    knobs.set("hdiffsmag_adj", "form", 0)
+    knobs.set("hdiffsmag_adj", "kchunk", kchunk)
     for t in (0, tile):
-        monkeypatch.setenv("OEB_TILE_HDIFFSMAG_ADJ", str(t))
+        knobs.set("hdiffsmag_adj", "tile", t)
         out.append(_grads(lambda u, v: AG.Hdiffsmag.apply(u, v, c, L, dom), xs, ybars))
     for a, b in zip(*out):
         assert torch.equal(a, b)
@@ -346,9 +346,9 @@ def test_hdiffsmag_adjoint_tile_matches_two_pass(shape, tile, kchunk, precision,
 
 @pytest.mark.parametrize("rows", [1, 3, 64])
 @pytest.mark.parametrize("shape", [(40, 36, 5), (67, 9, 3)], ids=lambda s: "x".join(map(str, s)))
-def test_hdiffsmag_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
+def test_hdiffsmag_adjoint_strip_matches_two_pass(shape, rows, knobs):
     """The fused row-march hdiffsmag adjoint and the two-pass one (gu, gv,
-    wu, wv through memory), forced through OEB_STRIP_HDIFFSMAG_ADJ, give the
+    wu, wv through memory), forced through the form knob, give the
     same doubles, at row chunks of one row, of a number that does not divide
     ny + 2, and of more than ny + 2."""
     import torch
@@ -362,20 +362,20 @@ def test_hdiffsmag_adjoint_strip_matches_two_pass(shape, rows, monkeypatch):
     ybars = [torch.randn(x.shape, generator=g, device="cuda", dtype=x.dtype) for x in xs]
     print(f"[gpu] hdiffsmag adjoint strip vs two-pass {shape} rows={rows}", flush=True)
     out = []
-    monkeypatch.setenv("OEB_TILE_HDIFFSMAG_ADJ", "0")
+    knobs.set("hdiffsmag_adj", "tile", 0)
     for strip in (0, 1):
-        monkeypatch.setenv("OEB_STRIP_HDIFFSMAG_ADJ", str(strip))
-        monkeypatch.setenv("OEB_ROWS_HDIFFSMAG_ADJ", str(rows))
+        knobs.set("hdiffsmag_adj", "form", strip)
+        knobs.set("hdiffsmag_adj", "rows", rows)
         out.append(_grads(lambda u, v: AG.Hdiffsmag.apply(u, v, c, L, dom), xs, ybars))
     for a, b in zip(*out):
         assert torch.equal(a, b)
 
 
 @pytest.mark.parametrize("shape", [(40, 36, 5), (67, 9, 3), (130, 21, 2)], ids=lambda s: "x".join(map(str, s)))
-def test_fvtp2d_adjoint_tiled_matches_per_point(shape, monkeypatch):
+def test_fvtp2d_adjoint_tiled_matches_per_point(shape, knobs):
     """The tiled transposed PPM gathers of the fvtp2d adjoint (the products
     of each face formed once in LDS) and the per-point ones, forced through
-    OEB_TILE_FVTP2D_ADJ, agree to 1e-14 of the largest cotangent: the same
+    the tile knob, agree to 1e-14 of the largest cotangent: the same
     products summed in the same order (a skipped face adds an exact zero)."""
     import torch
 
@@ -390,6 +390,6 @@ def test_fvtp2d_adjoint_tiled_matches_per_point(shape, monkeypatch):
     print(f"[gpu] fvtp2d adjoint tiled vs per point {shape}", flush=True)
     out = []
     for tiled in (0, 1):
-        monkeypatch.setenv("OEB_TILE_FVTP2D_ADJ", str(tiled))
+        knobs.set("fvtp2d_adj", "tile", tiled)
         out.append(_grads(lambda v: AG.Fvtp2d.apply(v, c, L, dom), [q], ybars)[0])
     assert _relmax(out[1], out[0]) <= 1e-14

@@ -84,12 +84,12 @@ def test_f32_256_against_f64(name):
 @pytest.mark.parametrize("nx", [64, 66, 67])
 @pytest.mark.parametrize("pairs", [0, 1])
 @pytest.mark.parametrize("name", ["laplace", "uvbke", "fastwavesuv", "p_grad_c", "nh_p_grad"])
-def test_f32_vector_path_switch(name, pairs, nx, monkeypatch):
+def test_f32_vector_path_switch(name, pairs, nx, knobs):
     """The vector paths of the FP32 form (4-wide for laplace and uvbke,
     column pairs with FP64 island pairs for the others), forced on or off
-    with OEB_PAIRS_<K>, and nx where they do not apply (66 for the 4-wide
+    with the vector knob, and nx where they do not apply (66 for the 4-wide
     paths, 67 for all), agree bit for bit with the FP32 oracle."""
-    monkeypatch.setenv(f"OEB_PAIRS_{name.upper()}", str(pairs))
+    knobs.set(name, "vector", pairs)
     L, host = inputs(name, nx, 36, 7, "baseline", seed=9)
     print(f"[gpu] f32 {name} pairs={pairs} nx={nx}", flush=True)
     _check_exact(name, L, cast(name, host, "f32"))
@@ -100,12 +100,12 @@ def test_f32_vector_path_switch(name, pairs, nx, monkeypatch):
 @pytest.mark.parametrize("strip", [0, 1])
 @pytest.mark.parametrize("shape", [(64, 36, 3), (1030, 9, 2)], ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("name", ["hdiffsa", "hdiffsmag", "fvtp2d"])
-def test_f32_strip_switches(name, shape, strip, rows, monkeypatch):
+def test_f32_strip_switches(name, shape, strip, rows, knobs):
     """The FP32 row-march (strip) and tile paths, forced through
-    OEB_STRIP_<K>, with row chunks of one row, of a number that does not
+    the form knob, with row chunks of one row, of a number that does not
     divide ny, and of more than ny, agree bit for bit with the FP32 oracle."""
-    monkeypatch.setenv(f"OEB_STRIP_{name.upper()}", str(strip))
-    monkeypatch.setenv(f"OEB_ROWS_{name.upper()}", str(rows))
+    knobs.set(name, "form", strip)
+    knobs.set(name, "rows", rows)
     L, host = inputs(name, *shape, "baseline", seed=14)
     print(f"[gpu] f32 {name} strip={strip} rows={rows} {shape}", flush=True)
     _check_exact(name, L, cast(name, host, "f32"))
@@ -114,13 +114,13 @@ def test_f32_strip_switches(name, shape, strip, rows, monkeypatch):
 @pytest.mark.gpu
 @pytest.mark.parametrize("depth", [1, 2])
 @pytest.mark.parametrize("lanes", [64, 128, 512, 1024])
-def test_f32_fvtp2d_strip_shapes(lanes, depth, monkeypatch):
+def test_f32_fvtp2d_strip_shapes(lanes, depth, knobs):
     """fvtp2d's FP32 strips of at most 64 .. 1024 lanes (one to five strips
     across nx = 300), with one and two rows loaded ahead, agree bit for bit
     with the FP32 oracle."""
-    monkeypatch.setenv("OEB_STRIP_FVTP2D", "1")
-    monkeypatch.setenv("OEB_LANES_FVTP2D", str(lanes))
-    monkeypatch.setenv("OEB_DEPTH_FVTP2D", str(depth))
+    knobs.set("fvtp2d", "form", 1)
+    knobs.set("fvtp2d", "lanes", lanes)
+    knobs.set("fvtp2d", "depth", depth)
     L, host = inputs("fvtp2d", 300, 23, 2, "stress", seed=15)
     print(f"[gpu] f32 fvtp2d strip lanes={lanes} depth={depth}", flush=True)
     _check_exact("fvtp2d", L, cast("fvtp2d", host, "f32"))
@@ -129,10 +129,10 @@ def test_f32_fvtp2d_strip_shapes(lanes, depth, monkeypatch):
 @pytest.mark.gpu
 @pytest.mark.parametrize("xcd", [0, 1])
 @pytest.mark.parametrize("shape", [(64, 36, 8), (200, 9, 4), (256, 20, 6)], ids=lambda s: "x".join(map(str, s)))
-def test_f32_laplace_xcd_interleave(shape, xcd, monkeypatch):
+def test_f32_laplace_xcd_interleave(shape, xcd, knobs):
     """FP32 laplace with and without the level-chunk interleave agrees bit
     for bit with the FP32 oracle."""
-    monkeypatch.setenv("OEB_XCD_LAPLACE", str(xcd))
+    knobs.set("laplace", "order", xcd)
     L, host = inputs("laplace", *shape, "baseline", seed=23)
     print(f"[gpu] f32 laplace xcd={xcd} {shape}", flush=True)
     _check_exact("laplace", L, cast("laplace", host, "f32"))

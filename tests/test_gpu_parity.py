@@ -89,12 +89,12 @@ PAIRED = ["laplace", "uvbke", "fastwavesuv", "p_grad_c", "nh_p_grad"]
 @pytest.mark.parametrize("kchunk", [1, 3, 80])
 @pytest.mark.parametrize("pairs", [0, 1])
 @pytest.mark.parametrize("name", PAIRED)
-def test_path_switches(name, pairs, kchunk, monkeypatch):
+def test_path_switches(name, pairs, kchunk, knobs):
     """Both the scalar and the paired (16-byte) path, forced through
-    OEB_PAIRS_<K>, at k-chunks that divide nz, do not divide it, and exceed
+    the vector knob, at k-chunks that divide nz, do not divide it, and exceed
     it, agree bit for bit with the oracle."""
-    monkeypatch.setenv(f"OEB_PAIRS_{name.upper()}", str(pairs))
-    monkeypatch.setenv(f"OEB_KCHUNK_{name.upper()}", str(kchunk))
+    knobs.set(name, "vector", pairs)
+    knobs.set(name, "kchunk", kchunk)
     L, host = inputs(name, 64, 36, 7, "baseline", seed=9)
     print(f"[gpu] {name} pairs={pairs} kchunk={kchunk}", flush=True)
     _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
@@ -108,13 +108,13 @@ STRIPPED = ["laplace", "hdiffsa", "hdiffsmag", "fvtp2d"]
 @pytest.mark.parametrize("strip", [0, 1])
 @pytest.mark.parametrize("shape", [(64, 36, 3), (1030, 9, 2), (2100, 5, 2)], ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("name", STRIPPED)
-def test_strip_switches(name, shape, strip, rows, monkeypatch):
+def test_strip_switches(name, shape, strip, rows, knobs):
     """The row-march (strip) path and the tile path, forced through
-    OEB_STRIP_<K>, with row chunks of one row, of a number that does not
+    the form knob, with row chunks of one row, of a number that does not
     divide ny, and of more than ny, and with nx wide enough for two or three
     strips, agree bit for bit with the oracle."""
-    monkeypatch.setenv(f"OEB_STRIP_{name.upper()}", str(strip))
-    monkeypatch.setenv(f"OEB_ROWS_{name.upper()}", str(rows))
+    knobs.set(name, "form", strip)
+    knobs.set(name, "rows", rows)
     L, host = inputs(name, *shape, "baseline", seed=13)
     print(f"[gpu] {name} strip={strip} rows={rows} {shape}", flush=True)
     _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
@@ -124,12 +124,12 @@ def test_strip_switches(name, shape, strip, rows, monkeypatch):
 @pytest.mark.parametrize("xcd", [0, 1])
 @pytest.mark.parametrize("shape", [(64, 36, 8), (200, 9, 4), (256, 20, 6), (66, 17, 2)],
                          ids=lambda s: "x".join(map(str, s)))
-def test_laplace_xcd_interleave(shape, xcd, monkeypatch):
-    """laplace with level chunks interleaved between tile rows (OEB_XCD_LAPLACE)
+def test_laplace_xcd_interleave(shape, xcd, knobs):
+    """laplace with level chunks interleaved between tile rows (the order knob)
     and without, on shapes where the interleave applies (8 or 4 chunks per
     group) and where it does not (chunks not a multiple of the group, or the
     scalar path at nx = 66), agrees bit for bit with the oracle."""
-    monkeypatch.setenv("OEB_XCD_LAPLACE", str(xcd))
+    knobs.set("laplace", "order", xcd)
     L, host = inputs("laplace", *shape, "baseline", seed=22)
     print(f"[gpu] laplace xcd={xcd} {shape}", flush=True)
     _check("laplace", L, host, run_gpu("laplace", copy(host), L), require_exact=True)
@@ -137,27 +137,31 @@ def test_laplace_xcd_interleave(shape, xcd, monkeypatch):
 @pytest.mark.gpu
 @pytest.mark.parametrize("kchunk", [1, 3, 8, 80])
 @pytest.mark.parametrize("name", ["hadvuv", "hadvuv5th"])
-def test_hadv_level_chunks(name, kchunk, monkeypatch):
+def test_hadv_level_chunks(name, kchunk, knobs):
     """hadvuv and hadvuv5th at level chunks that divide nz, do not divide
     it, and exceed it (the next level's rows are loaded during the current
     one) agree bit for bit with the oracle."""
-    monkeypatch.setenv(f"OEB_KCHUNK_{name.upper()}", str(kchunk))
+    knobs.set(name, "kchunk", kchunk)
     L, host = inputs(name, 130, 36, 11, "baseline", seed=25)
     print(f"[gpu] {name} kchunk={kchunk}", flush=True)
     _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("order", [0, 2])
-@pytest.mark.parametrize("name", ["fvtp2d", "hadvuv", "hadvuv5th", "fastwavesuv", "p_grad_c", "nh_p_grad"])
-@pytest.mark.parametrize("shape", [(200, 64, 8), (130, 32, 16), (67, 9, 3)], ids=lambda s: "x".join(map(str, s)))
-def test_tile_row_order(shape, name, order, monkeypatch):
-    """The tiled stencils with the tiles of a row on one XCD (OEB_XCD_<K>=2,
-    Tiles::irow) and in row order, on shapes where that order applies (4 and
-    3 tiles per row, 8 or 16 rows of tiles and level chunks) and where it
-    does not, agree bit for bit with the oracle."""
-    monkeypatch.setenv(f"OEB_XCD_{name.upper()}", str(order))
-    monkeypatch.setenv("OEB_STRIP_FVTP2D", "0")
+@pytest.mark.parametrize("order", [0, 1, 2, 3])
+@pytest.mark.parametrize("name", ["fvtp2d", "hadvuv", "hadvuv5th", "fastwavesuv", "p_grad_c", "nh_p_grad", "uvbke",
+                                  "laplace"])
+@pytest.mark.parametrize("shape", [(200, 64, 8), (130, 32, 16), (67, 9, 3), (64, 40, 64)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_tile_row_order(shape, name, order, knobs):
+    """The tiled stencils in every tile order of the order knob: rows in order
+    (0), level chunks interleaved between tile rows (1, Tiles::xcd), the tiles
+    of a row on one XCD (2, Tiles::irow) and whole level chunks per XCD (3,
+    Tiles::klev), on shapes where each order applies (8, 16 and 64 level
+    chunks; 1 to 4 tiles per row) and where it does not, agree bit for bit
+    with the oracle."""
+    knobs.set(name, "order", order)
+    knobs.set("fvtp2d", "form", 0)
     L, host = inputs(name, *shape, "baseline", seed=24)
     print(f"[gpu] {name} tile order {order} {shape}", flush=True)
     _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
@@ -167,13 +171,13 @@ def test_tile_row_order(shape, name, order, monkeypatch):
 @pytest.mark.parametrize("depth", [1, 2])
 @pytest.mark.parametrize("lanes", [64, 128, 512, 1024])
 @pytest.mark.parametrize("family", ["baseline", "stress"])
-def test_fvtp2d_strip_shapes(family, lanes, depth, monkeypatch):
+def test_fvtp2d_strip_shapes(family, lanes, depth, knobs):
     """fvtp2d's strips of at most 64 .. 1024 lanes (one to five strips across
     nx = 300), with one and two rows loaded ahead, agree bit for bit with the
     oracle."""
-    monkeypatch.setenv("OEB_STRIP_FVTP2D", "1")
-    monkeypatch.setenv("OEB_LANES_FVTP2D", str(lanes))
-    monkeypatch.setenv("OEB_DEPTH_FVTP2D", str(depth))
+    knobs.set("fvtp2d", "form", 1)
+    knobs.set("fvtp2d", "lanes", lanes)
+    knobs.set("fvtp2d", "depth", depth)
     L, host = inputs("fvtp2d", 300, 23, 2, family, seed=15)
     print(f"[gpu] fvtp2d strip {family} lanes={lanes} depth={depth}", flush=True)
     _check("fvtp2d", L, host, run_gpu("fvtp2d", copy(host), L), require_exact=True)
