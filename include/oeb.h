@@ -26,7 +26,9 @@
  *     region the stencil touches, halo reach included, must lie inside each
  *     array's allocated box.  All 3-D arrays of one call must share
  *     strides[1] and strides[0], with strides[2] == 1.  Outputs may not overlap
- *     inputs.  A failed check returns a non-zero status and launches nothing.
+ *     inputs.  Every array must lie in device, managed or pinned host memory
+ *     (checked with hipPointerGetAttributes).  A failed check returns a
+ *     non-zero status and launches nothing.
  *     oeb_last_error() then gives a message naming the array and the reason.
  *   - Launches are asynchronous on `stream`.  A return of OEB_OK means the
  *     kernels were enqueued.  It says nothing about completion.
@@ -55,7 +57,8 @@ extern "C" {
 #define OEB_ERR_LAYOUT 3 /* a descriptor does not cover the region the stencil touches, or strides differ */
 #define OEB_ERR_ALIAS 4  /* an output array overlaps an input array */
 #define OEB_ERR_HIP 5    /* a HIP runtime call or kernel launch failed */
-#define OEB_ERR_ARG 6    /* a non-array argument is out of its range */
+#define OEB_ERR_ARG 6    /* a non-array argument is out of its range, or a call made while recording a batch */
+#define OEB_ERR_MEMORY 7 /* an array lies in pageable host memory, which a kernel cannot access */
 
 typedef struct oeb_domain {
   int64_t nx; /* interior extent along i (the reference's domain_size) */
@@ -440,17 +443,20 @@ int oeb_tune_clear(void);
  *                        fastwavesuv, uvbke and oeb_halo_fill(_n), both
  *                        precisions) validate their arguments as usual and
  *                        record their launch into b instead of launching;
- *                        the stream argument is ignored.  Other entry points
- *                        still launch at once.
+ *                        the stream argument is ignored.  The other stencil
+ *                        and adjoint entry points fail with OEB_ERR_ARG and
+ *                        launch nothing while a batch records.
  *   oeb_batch_stop(b)    ends recording and uploads the member tables (the
  *                        only allocation: device memory, freed by destroy).
- *   oeb_batch_launch     enqueues one persistent launch per run of
- *                        consecutive records of one kernel, on `stream`.
- *                        Kernels only: it can be recorded in an oeb_graph.
- * Record member after member within each stencil (all members' halo fills,
- * then all members' first stencil, ...): records of one run execute
- * concurrently, runs in order.  oeb_batch_info reports the number of records
- * and of launches. */
+ *   oeb_batch_launch     enqueues one launch per run of consecutive records
+ *                        of one kernel, on `stream`.  Kernels only: it can
+ *                        be recorded in an oeb_graph.
+ * The records of a run execute concurrently.  A record that reads what an
+ * earlier record of its run writes, or writes what one reads or writes,
+ * starts a new run, so the launches keep the recorded order of dependent
+ * calls.  Record member after member within each stencil (all members' halo
+ * fills, then all members' first stencil, ...) for the fewest launches.
+ * oeb_batch_info reports the number of records and of launches. */
 typedef struct oeb_batch oeb_batch;
 int oeb_batch_create(oeb_batch **batch);
 int oeb_batch_record(oeb_batch *batch);

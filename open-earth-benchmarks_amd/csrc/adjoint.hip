@@ -16,6 +16,11 @@
 // cotangents in a caller-provided scratch array (the reference's own scratch
 // arguments: lap for hdiffsa, T_sqr_s / S_sqr_uv for hdiffsmag, al / almq /
 // br / b0 / smt5 for fvtp2d), on the same layout as the inputs.
+#include <algorithm>
+#include <mutex>
+#include <sstream>
+#include <vector>
+
 #include "oeb_common.hpp"
 
 namespace oeb {
@@ -284,6 +289,7 @@ int hdiffsa_adj_run(const oeb_domain *d, const S3 *in, const S3 *mask, const S1 
                     const S3 *out_bar, const S3 *in_bar, const S3 *lap, void *stream) {
   const char *K = "hdiffsa_adj";
   OEB_TRY(check_domain(K, d));
+  OEB_TRY(refuse_while_recording(K));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz}, r_in{-2, nx + 2, -2, ny + 2, 0, nz}, r_l{-1, nx + 1, -1, ny + 1, 0, nz};
@@ -430,6 +436,7 @@ int fastwavesuv_adj_run(const oeb_domain *d, const S3 *wgtfac, const oeb_s3 *ppu
                         const oeb_s3 *ppuv_bar, T edadlat, T dt, void *stream) {
   const char *K = "fastwavesuv_adj";
   OEB_TRY(check_domain(K, d));
+  OEB_TRY(refuse_while_recording(K));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz}, r_h{0, nx + 1, 0, ny + 1, 0, nz};
@@ -1084,6 +1091,7 @@ int hadvuv5th_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
                       void *stream) {
   const char *K = "hadvuv5th_adj";
   OEB_TRY(check_domain(K, d));
+  OEB_TRY(refuse_while_recording(K));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz}, r_in{-3, nx + 3, -3, ny + 3, 0, nz};
@@ -1700,6 +1708,7 @@ int hdiffsmag_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
                       T eddlon, T tau_smag, T weight_smag, void *stream) {
   const char *K = "hdiffsmag_adj";
   OEB_TRY(check_domain(K, d));
+  OEB_TRY(refuse_while_recording(K));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz}, r_uv{-1, nx + 1, -1, ny + 1, 0, nz};
@@ -2134,22 +2143,23 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile12(FvAdj<T> a, Geo g, Bo
 
 inline dim3 ptile_blocks(const BoxGrid &b, int nz) { return dim3(unsigned(ptile_count(b)), unsigned(nz)); }
 
+// Validates the arrays of an fvtp2d adjoint call other than its work arrays:
+// fills a (inputs and q_bar), g and the alias set.
 template <class T, class S>
-int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, const S *ra_x, const S *ra_y,
-                   const S *xfx, const S *yfx, const S *area, const S *q_i, const S *q_j, const S *q_i_bar,
-                   const S *q_j_bar, const S *fx1_bar, const S *fx2_bar, const S *fy1_bar, const S *fy2_bar,
-                   const S *q_bar, const S *const work[4], T p1, T p2, void *stream) {
-  const char *K = "fvtp2d_adj";
+int fvtp2d_adj_args(const char *K, const oeb_domain *d, const S *q, const S *crx, const S *cry, const S *ra_x,
+                    const S *ra_y, const S *xfx, const S *yfx, const S *area, const S *q_i, const S *q_j,
+                    const S *q_i_bar, const S *q_j_bar, const S *fx1_bar, const S *fx2_bar, const S *fy1_bar,
+                    const S *fy2_bar, const S *q_bar, T p1, T p2, FvAdj<T> &a, Geo &g, AliasSet &as) {
   OEB_TRY(check_domain(K, d));
+  OEB_TRY(refuse_while_recording(K));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
-  Geo g{int(nx), int(ny), int(nz), 0, 0};
+  g = Geo{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz};
   const Box r_fy2{-3, nx + 2, 0, ny + 1, 0, nz}, r_fx2{0, nx + 1, -3, ny + 2, 0, nz};
   const Box r_qi{-3, nx + 2, 0, ny, 0, nz}, r_qj{0, nx, -3, ny + 2, 0, nz};
   const Box r_q{-3, nx + 3, -3, ny + 3, 0, nz}, r_area{-3, nx + 2, -3, ny + 2, 0, nz};
-  FvAdj<T> a;
   a.p1 = p1, a.p2 = p2;
-  const T *p_qb, *p_w[4];
+  const T *p_qb;
   OEB_TRY(check3(K, "q", q, r_q, g, true, &a.q));
   OEB_TRY(check3(K, "crx", crx, r_fx2, g, false, &a.crx));
   OEB_TRY(check3(K, "cry", cry, r_fy2, g, false, &a.cry));
@@ -2167,12 +2177,7 @@ int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, 
   OEB_TRY(check3(K, "fy1_bar", fy1_bar, r0, g, false, &a.fy1b));
   OEB_TRY(check3(K, "fy2_bar", fy2_bar, r_fy2, g, false, &a.fy2b));
   OEB_TRY(check3(K, "q_bar", q_bar, r_q, g, false, &p_qb));
-  const Box wr[4] = {r_qj, r_qi, r_fx2, r_fy2};
-  static const char *const kWork[4] = {"work0", "work1", "work2", "work3"};
-  for (int t = 0; t < 4; ++t) OEB_TRY(check3(K, kWork[t], work[t], wr[t], g, false, &p_w[t]));
-  AliasSet as;
   as.add("q_bar", p_qb, r_q, g, true);
-  for (int t = 0; t < 4; ++t) as.add(kWork[t], p_w[t], wr[t], g, true);
   as.add("q", a.q, r_q, g, false);
   as.add("q_i", a.q_i, r_qi, g, false);
   as.add("q_j", a.q_j, r_qj, g, false);
@@ -2182,10 +2187,52 @@ int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, 
   as.add("fx2_bar", a.fx2b, r_fx2, g, false);
   as.add("fy1_bar", a.fy1b, r0, g, false);
   as.add("fy2_bar", a.fy2b, r_fy2, g, false);
+  a.qbar = const_cast<T *>(p_qb);
+  return OEB_OK;
+}
+
+// The two launches of the tiled adjoint (K1 and K2, then K5) on levels
+// [0, g.nz) of the arrays in a.
+template <class T>
+int fvtp2d_adj_tiled_launch(const char *K, const oeb_domain *d, const FvAdj<T> &a, const Geo &g,
+                            hipStream_t st) {
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const BoxGrid bqj{0, -3, nx, ny + 5}, bqi{-3, 0, nx + 5, ny};
+  const BoxGrid bq{-3, -3, nx + 6, ny + 6};
+  const int n1 = ptile_count(bqj), n2 = ptile_count(bqi);
+  hipLaunchKernelGGL(fvtp2d_adj_ptile12<T>, dim3(unsigned(n1 + n2), unsigned(nz)), dim3(kPNT), 0, st, a, g, bqj,
+                     bqi, n1);
+  OEB_TRY(launch_status(K));
+  // K5 with the tiles of a row on one XCD (order knob 2, the default;
+  // 0 for row order): 408 us against 411 for the FP64 adjoint with 64 x 8 tiles.
+  const int nty = (bq.nj + kPY - 1) / kPY;
+  const int irow = (knob(K, sizeof(T), d, KNOB_ORDER, 2) == 2 && (int64_t(nty) * nz) % 8 == 0) ? 1 : 0;
+  hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 5>), ptile_blocks(bq, nz), dim3(kPNT), 0, st, a, g, bq, irow);
+  return launch_status(K);
+}
+
+template <class T, class S>
+int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, const S *ra_x, const S *ra_y,
+                   const S *xfx, const S *yfx, const S *area, const S *q_i, const S *q_j, const S *q_i_bar,
+                   const S *q_j_bar, const S *fx1_bar, const S *fx2_bar, const S *fy1_bar, const S *fy2_bar,
+                   const S *q_bar, const S *const work[4], T p1, T p2, void *stream) {
+  const char *K = "fvtp2d_adj";
+  FvAdj<T> a;
+  Geo g;
+  AliasSet as;
+  OEB_TRY(fvtp2d_adj_args<T>(K, d, q, crx, cry, ra_x, ra_y, xfx, yfx, area, q_i, q_j, q_i_bar, q_j_bar, fx1_bar,
+                             fx2_bar, fy1_bar, fy2_bar, q_bar, p1, p2, a, g, as));
+  const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
+  const Box r_fy2{-3, nx + 2, 0, ny + 1, 0, nz}, r_fx2{0, nx + 1, -3, ny + 2, 0, nz};
+  const Box r_qi{-3, nx + 2, 0, ny, 0, nz}, r_qj{0, nx, -3, ny + 2, 0, nz};
+  const Box wr[4] = {r_qj, r_qi, r_fx2, r_fy2};
+  static const char *const kWork[4] = {"work0", "work1", "work2", "work3"};
+  const T *p_w[4];
+  for (int t = 0; t < 4; ++t) OEB_TRY(check3(K, kWork[t], work[t], wr[t], g, false, &p_w[t]));
+  for (int t = 0; t < 4; ++t) as.add(kWork[t], p_w[t], wr[t], g, true);
   OEB_TRY(as.check(K));
   a.w0 = const_cast<T *>(p_w[0]), a.w1 = const_cast<T *>(p_w[1]);
   a.w2 = const_cast<T *>(p_w[2]), a.w3 = const_cast<T *>(p_w[3]);
-  a.qbar = const_cast<T *>(p_qb);
   const BoxGrid bqj{0, -3, int(nx), int(ny + 5)}, bqi{-3, 0, int(nx + 5), int(ny)};
   const BoxGrid bfx2{0, -3, int(nx + 1), int(ny + 5)}, bfy2{-3, 0, int(nx + 5), int(ny + 1)};
   const BoxGrid bq{-3, -3, int(nx + 6), int(ny + 6)};
@@ -2196,18 +2243,7 @@ int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, 
   // forms K3 and K4 inside) by default; tile knob 0 selects the five per-point passes,
   // with W2 and W3 through work2 and work3.
   const bool tiled = knob(K, sizeof(T), d, KNOB_TILE, 1) != 0;
-  if (tiled) {
-    const int n1 = ptile_count(bqj), n2 = ptile_count(bqi);
-    hipLaunchKernelGGL(fvtp2d_adj_ptile12<T>, dim3(unsigned(n1 + n2), unsigned(nz)), dim3(kPNT), 0, st, a, g, bqj,
-                       bqi, n1);
-    OEB_TRY(launch_status(K));
-    // K5 with the tiles of a row on one XCD (order knob 2, the default;
-    // 0 for row order): 408 us against 411 for the FP64 adjoint with 64 x 8 tiles.
-    const int nty = (bq.nj + kPY - 1) / kPY;
-    const int irow = (knob(K, sizeof(T), d, KNOB_ORDER, 2) == 2 && (int64_t(nty) * nz) % 8 == 0) ? 1 : 0;
-    hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 5>), ptile_blocks(bq, int(nz)), dim3(kPNT), 0, st, a, g, bq, irow);
-    return launch_status(K);
-  }
+  if (tiled) return fvtp2d_adj_tiled_launch<T>(K, d, a, g, st);
   hipLaunchKernelGGL(fvtp2d_adj_k1<T>, box_blocks(bqj, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqj);
   OEB_TRY(launch_status(K));
   hipLaunchKernelGGL(fvtp2d_adj_k2<T>, box_blocks(bqi, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqi);

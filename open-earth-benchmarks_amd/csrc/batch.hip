@@ -6,7 +6,10 @@
 // BatchMember (kernel arguments, geometry, tiles) to the batch.
 // oeb_batch_stop splits the records into runs of consecutive records of one
 // kernel, numbers each run's blocks member after member, and uploads each
-// run's member table once.  oeb_batch_launch then enqueues one persistent
+// run's member table once.  The records of a run execute concurrently, so a
+// record that reads what an earlier record of its run writes, or writes what
+// an earlier record reads or writes, starts a new run: the launches then
+// keep the order in which they were recorded.  oeb_batch_launch then enqueues one persistent
 // launch per run (batch_kernel in oeb_common.hpp): kernels only, so a launch
 // can be captured in a graph and replayed.
 #include <cstring>
@@ -15,6 +18,10 @@
 #include "oeb_common.hpp"
 
 struct oeb_batch {
+  struct Range {
+    uintptr_t lo, hi;
+    bool out;
+  };
   struct Run {
     const oeb::BatchKind *kind;
     oeb::BatchMember *table;  // device memory, n entries
@@ -22,6 +29,7 @@ struct oeb_batch {
   };
   std::vector<oeb::BatchMember> records;
   std::vector<const oeb::BatchKind *> kinds;
+  std::vector<std::vector<Range>> ranges;  // bytes each record reads / writes
   std::vector<Run> runs;
   bool recording = false, stopped = false;
 };
@@ -31,9 +39,13 @@ namespace {
 thread_local oeb_batch *g_recording = nullptr;
 }
 
-bool batch_record(const BatchKind &kind, const void *args, size_t bytes, const Geo &g, const Tiles &t, int blocks) {
+bool batch_record(const BatchKind &kind, const void *args, size_t bytes, const Geo &g, const Tiles &t, int blocks,
+                  const AliasSet &as) {
   oeb_batch *b = g_recording;
   if (b == nullptr) return false;
+  std::vector<oeb_batch::Range> rs;
+  for (int i = 0; i < as.n; ++i) rs.push_back({as.lo[i], as.hi[i], as.out[i]});
+  b->ranges.push_back(std::move(rs));
   BatchMember m{};
   m.g = g;
   m.t = t;
@@ -44,6 +56,24 @@ bool batch_record(const BatchKind &kind, const void *args, size_t bytes, const G
   b->kinds.push_back(&kind);
   return true;
 }
+
+int refuse_while_recording(const char *kname) {
+  if (g_recording == nullptr) return OEB_OK;
+  return fail(OEB_ERR_ARG, std::string(kname) +
+                               ": has no batched form and cannot be called while this thread records a batch "
+                               "(call it before oeb_batch_record or after oeb_batch_stop)");
+}
+
+namespace {
+// True when record b depends on record a: b writes what a reads or writes,
+// or reads what a writes.
+bool depends(const std::vector<oeb_batch::Range> &a, const std::vector<oeb_batch::Range> &b) {
+  for (const auto &x : a)
+    for (const auto &y : b)
+      if ((x.out || y.out) && x.lo < y.hi && y.lo < x.hi) return true;
+  return false;
+}
+}  // namespace
 
 }  // namespace oeb
 
@@ -76,6 +106,9 @@ extern "C" int oeb_batch_stop(oeb_batch *batch) {
     size_t b = a;
     int64_t total = 0;
     while (b < n && batch->kinds[b] == batch->kinds[a]) {
+      bool dep = false;
+      for (size_t c = a; c < b && !dep; ++c) dep = depends(batch->ranges[c], batch->ranges[b]);
+      if (dep) break;
       batch->records[b].first = int(total);
       total += batch->records[b].blocks;
       ++b;

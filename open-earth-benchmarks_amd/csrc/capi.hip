@@ -114,7 +114,10 @@ int check1(const char *kname, const char *name, const oeb_s1f *s, int64_t lo, in
 }
 
 void AliasSet::add_range(const char *nm, uintptr_t lo_b, uintptr_t hi_b, bool is_out) {
-  if (n >= kMax) return;
+  if (n >= kMax) {
+    overflow = true;
+    return;
+  }
   name[n] = nm;
   lo[n] = lo_b;
   hi[n] = hi_b;
@@ -123,6 +126,7 @@ void AliasSet::add_range(const char *nm, uintptr_t lo_b, uintptr_t hi_b, bool is
 }
 
 int AliasSet::check(const char *kname) const {
+  if (overflow) return fail(OEB_ERR_ARG, std::string(kname) + ": too many arrays for the overlap check");
   for (int a = 0; a < n; ++a) {
     if (!out[a]) continue;
     for (int b = 0; b < n; ++b) {
@@ -133,6 +137,21 @@ int AliasSet::check(const char *kname) const {
           << "'";
         return fail(OEB_ERR_ALIAS, m.str());
       }
+    }
+  }
+  // Every array must be memory a kernel may access: device memory, managed
+  // memory or pinned host memory.  Pageable host memory would fault the GPU.
+  for (int a = 0; a < n; ++a) {
+    hipPointerAttribute_t attr;
+    const hipError_t e = hipPointerGetAttributes(&attr, reinterpret_cast<const void *>(lo[a]));
+    const bool ok = e == hipSuccess && (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+                                        attr.type == hipMemoryTypeHost || attr.type == hipMemoryTypeUnified);
+    if (!ok) {
+      (void)hipGetLastError();  // a failed query leaves an error behind; clear it
+      std::ostringstream m;
+      m << kname << ": array '" << name[a] << "' is not in device memory (nor managed or pinned host memory)";
+      if (e != hipSuccess) m << ": " << hipGetErrorString(e);
+      return fail(OEB_ERR_MEMORY, m.str());
     }
   }
   return OEB_OK;
@@ -260,6 +279,7 @@ const char *oeb_status_string(int status) {
     case OEB_ERR_ALIAS: return "an output overlaps another array";
     case OEB_ERR_HIP: return "HIP runtime error";
     case OEB_ERR_ARG: return "an argument is out of its range";
+    case OEB_ERR_MEMORY: return "an array is not in memory the GPU may access";
   }
   return "unknown status";
 }

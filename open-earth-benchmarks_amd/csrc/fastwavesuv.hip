@@ -245,14 +245,14 @@ int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S
       pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.utens, a.vtens, a.wgtfac, a.ppuv, a.hhl, a.rho})) {
     const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc, xo);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-    if (batch_record(kFastwavesuvPairBatch<T>, a, g, t, t.n)) return OEB_OK;
+    if (batch_record(kFastwavesuvPairBatch<T>, a, g, t, t.n, as)) return OEB_OK;
     hipLaunchKernelGGL(fastwavesuv_pair_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g,
                        t);
     return launch_status(K);
   }
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-  if (batch_record(kFastwavesuvBatch<T>, a, g, t, t.n)) return OEB_OK;
+  if (batch_record(kFastwavesuvBatch<T>, a, g, t, t.n, as)) return OEB_OK;
   hipLaunchKernelGGL(fastwavesuv_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }

@@ -382,6 +382,7 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
                const S *yfx, const S *area, T p1, T p2, void *stream) {
   const char *K = "fvtp2d";
   OEB_TRY(check_domain(K, d));
+  OEB_TRY(refuse_while_recording(K));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz};

@@ -231,7 +231,7 @@ int hadv_run(const char *K, const oeb_domain *d, const S3 *uout, const S3 *vout,
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, kTJ));
   const HadvArgs<T> a{const_cast<T *>(p_uo), const_cast<T *>(p_vo), p_ui, p_vi, p_a0, p_a1, p_t0, p_t1,
                       s0, s1, s2, s3, eddlat, eddlon};
-  if (batch_record(kHadvuvBatch<ORDER, T>, a, g, t, t.n)) return OEB_OK;
+  if (batch_record(kHadvuvBatch<ORDER, T>, a, g, t, t.n, as)) return OEB_OK;
   // Lanes per 64x16 tile: 256 (4 rows each), 512 (2) or 1024 (1).
   const int nt = knob(K, sizeof(T), d, KNOB_LANES, kHadvLanes<T>);
   if (nt >= 1024)

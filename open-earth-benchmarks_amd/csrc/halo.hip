@@ -138,7 +138,7 @@ int halo_run(const char *K, const oeb_domain *d, const S *const *fields, int nfi
   a.periodic = mode == OEB_HALO_PERIODIC ? 1 : 0;
   a.bpf = int(bpf), a.sk = g.sk, a.ring = ring, a.nring = ring * nz, a.npts = npts;
   const int blocks = int(bpf * nfields);
-  if (!batch_record(kHaloBatch<T>, a, g, Tiles{}, blocks)) {
+  if (!batch_record(kHaloBatch<T>, a, g, Tiles{}, blocks, as)) {
     hipLaunchKernelGGL((halo_kernel<HaloArgs<T>, halo_body<T>>), dim3(grid_for(blocks)), dim3(kThreads), 0,
                        hipStream_t(stream), a, blocks);
     OEB_TRY(launch_status(K));

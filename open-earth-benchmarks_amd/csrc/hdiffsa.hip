@@ -158,6 +158,7 @@ int hdiffsa_run(const oeb_domain *d, const S3 *out, const S3 *in, const S3 *mask
                 const S1 *crlatu, void *stream) {
   const char *K = "hdiffsa";
   OEB_TRY(check_domain(K, d));
+  OEB_TRY(refuse_while_recording(K));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz};

@@ -238,7 +238,8 @@ const BatchKind kHdiffsmagSegBatch{"hdiffsmag", hdiffsmag_seg_batch<T, V, D>};
 
 // Plain launch or batch record of the segment form with V columns per lane.
 template <class T, int V>
-int hdiffsmag_seg_run(const HsArgs<T> &a, const Geo &g, const oeb_domain *d, hipStream_t stream) {
+int hdiffsmag_seg_run(const HsArgs<T> &a, const Geo &g, const oeb_domain *d, const AliasSet &as,
+                      hipStream_t stream) {
   const char *K = "hdiffsmag";
   // Rows per chunk by the lanes that cover one row (default_strip_rows), so
   // that narrow rows get short chunks and enough waves; at most 16, which
@@ -250,10 +251,10 @@ int hdiffsmag_seg_run(const HsArgs<T> &a, const Geo &g, const oeb_domain *d, hip
   if (t.n < 1 || int64_t(t.ti) * t.tj * t.nchunk > (int64_t(1) << 30))
     return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
   if (knob(K, sizeof(T), d, KNOB_DEPTH, 1) >= 2) {
-    if (batch_record(kHdiffsmagSegBatch<T, V, 2>, a, g, t, t.n)) return OEB_OK;
+    if (batch_record(kHdiffsmagSegBatch<T, V, 2>, a, g, t, t.n, as)) return OEB_OK;
     hipLaunchKernelGGL((hdiffsmag_seg_kernel<T, V, 2>), dim3(t.n), dim3(kSegLanes * kSegWaves), 0, stream, a, g, t);
   } else {
-    if (batch_record(kHdiffsmagSegBatch<T, V, 1>, a, g, t, t.n)) return OEB_OK;
+    if (batch_record(kHdiffsmagSegBatch<T, V, 1>, a, g, t, t.n, as)) return OEB_OK;
     hipLaunchKernelGGL((hdiffsmag_seg_kernel<T, V, 1>), dim3(t.n), dim3(kSegLanes * kSegWaves), 0, stream, a, g, t);
   }
   return launch_status(K);
@@ -300,10 +301,10 @@ int hdiffsmag_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 
   if (knob_on(K, sizeof(T), d, KNOB_FORM, true)) {
     constexpr int W = 16 / int(sizeof(T));
     if ((knob(K, sizeof(T), d, KNOB_VECTOR, true) != 0) && vec_ok(g, W, sizeof(T), {p_uo, p_vo, p_ui, p_vi, p_m}))
-      return hdiffsmag_seg_run<T, W>(a, g, d, hipStream_t(stream));
-    return hdiffsmag_seg_run<T, 1>(a, g, d, hipStream_t(stream));
+      return hdiffsmag_seg_run<T, W>(a, g, d, as, hipStream_t(stream));
+    return hdiffsmag_seg_run<T, 1>(a, g, d, as, hipStream_t(stream));
   }
-  if (batch_record(kHdiffsmagBatch<T>, a, g, t, t.n)) return OEB_OK;
+  if (batch_record(kHdiffsmagBatch<T>, a, g, t, t.n, as)) return OEB_OK;
   hipLaunchKernelGGL(hdiffsmag_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }

@@ -144,6 +144,7 @@ template <class T, class S>
 int laplace_run(const oeb_domain *d, const S *in, const S *out, void *stream) {
   const char *K = "laplace";
   OEB_TRY(check_domain(K, d));
+  OEB_TRY(refuse_while_recording(K));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r_in{-1, nx + 1, -1, ny + 1, 0, nz}, r_out{0, nx, 0, ny, 0, nz};

@@ -129,6 +129,7 @@ int nh_p_grad_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 
                   void *stream) {
   const char *K = "nh_p_grad";
   OEB_TRY(check_domain(K, d));
+  OEB_TRY(refuse_while_recording(K));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz};

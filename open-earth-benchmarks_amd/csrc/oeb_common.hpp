@@ -376,16 +376,26 @@ struct BatchKind {
   BatchLauncher launch;
 };
 
-// When this host thread is recording a batch, appends the launch and returns
-// true; the caller then launches nothing.
-bool batch_record(const BatchKind &kind, const void *args, size_t bytes, const Geo &g, const Tiles &t, int blocks);
+struct AliasSet;
+
+// When this host thread is recording a batch, appends the launch, with the
+// byte ranges the call reads and writes (`as`), and returns true; the caller
+// then launches nothing.
+bool batch_record(const BatchKind &kind, const void *args, size_t bytes, const Geo &g, const Tiles &t, int blocks,
+                  const AliasSet &as);
 
 template <class A>
-bool batch_record(const BatchKind &kind, const A &args, const Geo &g, const Tiles &t, int blocks) {
+bool batch_record(const BatchKind &kind, const A &args, const Geo &g, const Tiles &t, int blocks,
+                  const AliasSet &as) {
   static_assert(sizeof(A) <= kBatchArgBytes, "batch arguments too large");
   static_assert(std::is_trivially_copyable<A>::value, "batch arguments must be trivially copyable");
-  return batch_record(kind, &args, sizeof(A), g, t, blocks);
+  return batch_record(kind, &args, sizeof(A), g, t, blocks, as);
 }
+
+// Fails with OEB_ERR_ARG when this host thread is recording a batch: for the
+// entry points that have no batched form, which would otherwise launch at
+// once while the batch's earlier records have not run.
+int refuse_while_recording(const char *kname);
 
 // The member whose blocks include flattened block b (members sorted by first).
 __device__ __forceinline__ int batch_find(const BatchMember *tab, int n, int b) {
@@ -486,6 +496,7 @@ struct AliasSet {
     add_range(nm, reinterpret_cast<uintptr_t>(base + lo_i * stride),
               reinterpret_cast<uintptr_t>(base + (hi_i - 1) * stride + 1), false);
   }
+  bool overflow = false;  // more than kMax arrays were added
   void add_range(const char *nm, uintptr_t lo_b, uintptr_t hi_b, bool is_out);
   int check(const char *kname) const;
 };

@@ -106,6 +106,7 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
                  const S3 *rdyc, const S3 *delpc, const oeb_s3 *gz, const oeb_s3 *pkc, T dt2, void *stream) {
   const char *K = "p_grad_c";
   OEB_TRY(check_domain(K, d));
+  OEB_TRY(refuse_while_recording(K));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
   const Box r0{0, nx, 0, ny, 0, nz};

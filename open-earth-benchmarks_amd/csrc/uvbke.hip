@@ -127,13 +127,13 @@ int uvbke_run(const oeb_domain *d, const S *ub, const S *vb, const S *uc, const 
   if ((knob(K, sizeof(T), d, KNOB_VECTOR, true) != 0) && vec_ok(g, W, sizeof(T), {p_ub, p_vb, p_uc, p_vc, p_ca, p_rs})) {
     const Tiles t = make_tiles(nx, ny, int(nz), W * kTileI, kTileJ, kc, xo);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), W * kTileI, kTileJ));
-    if (batch_record(kUvbkeVecBatch<T>, a, g, t, t.n)) return OEB_OK;
+    if (batch_record(kUvbkeVecBatch<T>, a, g, t, t.n, as)) return OEB_OK;
     hipLaunchKernelGGL(uvbke_vec_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
     return launch_status(K);
   }
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-  if (batch_record(kUvbkeBatch<T>, a, g, t, t.n)) return OEB_OK;
+  if (batch_record(kUvbkeBatch<T>, a, g, t, t.n, as)) return OEB_OK;
   hipLaunchKernelGGL(uvbke_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }

@@ -1,6 +1,7 @@
-// oeb_driver.h -- shared pieces of the benchmark drivers in src/: the
+// oeb_driver.h -- pieces of the benchmark driver (src/oeb_bench.cpp): the
 // reference's command line (src/<name>.cpp:16-29), the timed loop, and a
-// result line with GPU event timing, bandwidth and output checksums.
+// result line with GPU event timing, bandwidth, output checksums and the
+// verification result.
 //
 // Include after util.h and the kernel header.
 #ifndef OEB_DRIVER_H
@@ -14,6 +15,8 @@
 #include <initializer_list>
 #include <iostream>
 #include <string>
+#include <utility>
+#include <vector>
 
 namespace oeb_driver {
 
@@ -33,7 +36,6 @@ inline int parse(int argc, char **argv) {
   std::cout << "ALGO: " << algo << std::endl;
   std::cout << "domain_size: " << domain_size << std::endl;
   std::cout << "domain_height: " << domain_height << std::endl;
-  if (const char *c = std::getenv("OEB_COUNT")) count = atoll(c);
   return algo;
 }
 
@@ -76,20 +78,12 @@ double timed_loop(const char *label, F fn) {
   return n ? total / n : 0.0;
 }
 
-// Sum of an output over [i0, n+i1) x [j0, n+j1) x [0, nz) (download first).
-inline double checksum(Storage3D &s, int i0 = 0, int i1 = 0, int j0 = 0, int j1 = 0) {
-  toHost(s);
-  double acc = 0.0;
-  for (int64_t k = 0; k < domain_height; ++k)
-    for (int64_t j = j0; j < domain_size + j1; ++j)
-      for (int64_t i = i0; i < domain_size + i1; ++i) acc += s(i, j, k);
-  return acc;
-}
-
 // One JSON result line: mean kernel time from HIP events, Gpts/s and the
-// fraction of the 8 TB/s HBM peak at the stencil's algorithmic bytes per point.
+// fraction of the 8 TB/s HBM peak at the stencil's algorithmic bytes per point
+// (of the build's precision), the region
+// sums of the outputs and whether every output passed verification.
 inline void report(const char *name, int algo, double ms, int bytes_per_point,
-                   std::initializer_list<std::pair<const char *, double>> sums) {
+                   const std::vector<std::pair<const char *, std::pair<double, double>>> &sums, bool verified) {
   const double pts = double(domain_size) * double(domain_size) * double(domain_height);
   const double gbs = bytes_per_point * pts / (ms * 1e-3) / 1e9;
   std::printf("{\"kernel\": \"%s\", \"algo\": %d, \"domain_size\": %lld, \"domain_height\": %lld, \"count\": %lld, "
@@ -98,10 +92,16 @@ inline void report(const char *name, int algo, double ms, int bytes_per_point,
               pts / (ms * 1e-3) / 1e9, gbs, gbs / 8000.0);
   bool first = true;
   for (const auto &kv : sums) {
-    std::printf("%s\"%s\": %.17g", first ? "" : ", ", kv.first, kv.second);
+    std::printf("%s\"%s\": %.17g", first ? "" : ", ", kv.first, kv.second.first);
     first = false;
   }
-  std::printf("}}\n");
+  std::printf("}, \"abs_checksums\": {");
+  first = true;
+  for (const auto &kv : sums) {
+    std::printf("%s\"%s\": %.17g", first ? "" : ", ", kv.first, kv.second.second);
+    first = false;
+  }
+  std::printf("}, \"verified\": %s}\n", verified ? "true" : "false");
 }
 
 }  // namespace oeb_driver

@@ -1,6 +1,6 @@
 // fastwavesuv.h -- drop-in for the reference's kernels/fastwavesuv.h (kernels/fastwavesuv.h:4-191).
 // fastwavesuv, fastwavesuv_fullfusion, fastwavesuv_partialfusion, fastwavesuv_openmp keep their names and parameter lists; each runs liboeb's fused
-// HIP kernel (oeb_fastwavesuv_f64, include/oeb.h).  The reference variants are CPU
+// HIP kernel (oeb_fastwavesuv_f64, or oeb_fastwavesuv_f32 when ElementType is float; include/oeb.h).  The reference variants are CPU
 // loop-fusion studies with one result; on the GPU there is one implementation.
 // Scratch arguments are accepted and left untouched.
 #ifndef FASTWAVESUV_H
@@ -26,11 +26,11 @@ inline void fastwavesuv(Storage3D& uout, Storage3D& vout, const Storage3D& uin, 
   const auto a_utens = oeb_dropin::dev(utens);
   const auto a_vtens = oeb_dropin::dev(vtens);
   const auto a_wgtfac = oeb_dropin::dev(wgtfac);
-  const auto a_ppuv = oeb_dropin::dev(ppuv);
-  const auto a_hhl = oeb_dropin::dev(hhl);
+  const auto a_ppuv = oeb_dropin::island(ppuv);
+  const auto a_hhl = oeb_dropin::island(hhl);
   const auto a_rho = oeb_dropin::dev(rho);
   const auto a_fx = oeb_dropin::dev(fx);
-  oeb_dropin::check(oeb_fastwavesuv_f64(&d, &a_uout, &a_vout, &a_uin, &a_vin, &a_utens, &a_vtens, &a_wgtfac, &a_ppuv, &a_hhl,
+  oeb_dropin::check(OEB_DROPIN_CALL(fastwavesuv, &d, &a_uout, &a_vout, &a_uin, &a_vin, &a_utens, &a_vtens, &a_wgtfac, &a_ppuv, &a_hhl,
                             &a_rho, &a_fx, nullptr, nullptr, nullptr, nullptr, edadlat, dt, oeb_dropin::stream()),
                     "fastwavesuv");
   oeb_dropin::finish(uout, vout);

@@ -1,6 +1,6 @@
 // fvtp2d.h -- drop-in for the reference's kernels/fvtp2d.h (kernels/fvtp2d.h:4-336; p1/p2 are the driver macros, src/fvtp2d.cpp:11-12).
 // fvtp2d, fvtp2d_fullfusion keep their names and parameter lists; each runs liboeb's fused
-// HIP kernel (oeb_fvtp2d_f64, include/oeb.h).  The reference variants are CPU
+// HIP kernel (oeb_fvtp2d_f64, or oeb_fvtp2d_f32 when ElementType is float; include/oeb.h).  The reference variants are CPU
 // loop-fusion studies with one result; on the GPU there is one implementation.
 // Scratch arguments are accepted and left untouched.
 #ifndef FVTP2D_H
@@ -36,7 +36,7 @@ inline void fvtp2d(Storage3D& q_i, Storage3D& q_j, Storage3D& fx1, Storage3D& fx
   const auto a_xfx = oeb_dropin::dev(xfx);
   const auto a_yfx = oeb_dropin::dev(yfx);
   const auto a_area = oeb_dropin::dev(area);
-  oeb_dropin::check(oeb_fvtp2d_f64(&d, &a_q_i, &a_q_j, &a_fx1, &a_fx2, &a_fy1, &a_fy2, &a_q, &a_crx, &a_cry, &a_ra_x, &a_ra_y,
+  oeb_dropin::check(OEB_DROPIN_CALL(fvtp2d, &d, &a_q_i, &a_q_j, &a_fx1, &a_fx2, &a_fy1, &a_fy2, &a_q, &a_crx, &a_cry, &a_ra_x, &a_ra_y,
                        &a_xfx, &a_yfx, &a_area, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, p1, p2,
                        oeb_dropin::stream()),
                     "fvtp2d");

@@ -1,6 +1,6 @@
 // hadvuv5th.h -- drop-in for the reference's kernels/hadvuv5th.h (kernels/hadvuv5th.h:41-201).
 // hadvuv5th, hadvuv5th_fullfusion, hadvuv5th_partialfusion, hadvuv5th_openmp keep their names and parameter lists; each runs liboeb's fused
-// HIP kernel (oeb_hadvuv5th_f64, include/oeb.h).  The reference variants are CPU
+// HIP kernel (oeb_hadvuv5th_f64, or oeb_hadvuv5th_f32 when ElementType is float; include/oeb.h).  The reference variants are CPU
 // loop-fusion studies with one result; on the GPU there is one implementation.
 // Scratch arguments are accepted and left untouched.
 #ifndef HADVUV5TH_H
@@ -31,7 +31,7 @@ inline void hadvuv5th(Storage3D& uout, Storage3D& vout, const Storage3D& uin, co
   const auto a_acrlat1 = oeb_dropin::dev(acrlat1);
   const auto a_tgrlatda0 = oeb_dropin::dev(tgrlatda0);
   const auto a_tgrlatda1 = oeb_dropin::dev(tgrlatda1);
-  oeb_dropin::check(oeb_hadvuv5th_f64(&d, &a_uout, &a_vout, &a_uin, &a_vin, &a_acrlat0, &a_acrlat1, &a_tgrlatda0, &a_tgrlatda1,
+  oeb_dropin::check(OEB_DROPIN_CALL(hadvuv5th, &d, &a_uout, &a_vout, &a_uin, &a_vin, &a_acrlat0, &a_acrlat1, &a_tgrlatda0, &a_tgrlatda1,
                            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, eddlat, eddlon,
                            oeb_dropin::stream()),
                     "hadvuv5th");

@@ -1,6 +1,6 @@
 // hdiffsmag.h -- drop-in for the reference's kernels/hdiffsmag.h (kernels/hdiffsmag.h:4-263).
 // hdiffsmag, hdiffsmag_fullfusion, hdiffsmag_partialfusion, hdiffsmag_openmp keep their names and parameter lists; each runs liboeb's fused
-// HIP kernel (oeb_hdiffsmag_f64, include/oeb.h).  The reference variants are CPU
+// HIP kernel (oeb_hdiffsmag_f64, or oeb_hdiffsmag_f32 when ElementType is float; include/oeb.h).  The reference variants are CPU
 // loop-fusion studies with one result; on the GPU there is one implementation.
 // Scratch arguments are accepted and left untouched.
 #ifndef HDIFFSMAG_H
@@ -27,7 +27,7 @@ inline void hdiffsmag(Storage3D& uout, Storage3D& vout, const Storage3D& uin, co
   const auto a_crlato = oeb_dropin::dev(crlato);
   const auto a_crlatu = oeb_dropin::dev(crlatu);
   const auto a_acrlat0 = oeb_dropin::dev(acrlat0);
-  oeb_dropin::check(oeb_hdiffsmag_f64(&d, &a_uout, &a_vout, &a_uin, &a_vin, &a_mask, &a_crlavo, &a_crlavu, &a_crlato, &a_crlatu,
+  oeb_dropin::check(OEB_DROPIN_CALL(hdiffsmag, &d, &a_uout, &a_vout, &a_uin, &a_vin, &a_mask, &a_crlavo, &a_crlavu, &a_crlato, &a_crlatu,
                           &a_acrlat0, nullptr, nullptr, eddlat, eddlon, tau_smag, weight_smag, oeb_dropin::stream()),
                     "hdiffsmag");
   oeb_dropin::finish(uout, vout);

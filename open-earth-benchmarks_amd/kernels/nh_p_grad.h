@@ -1,6 +1,6 @@
 // nh_p_grad.h -- drop-in for the reference's kernels/nh_p_grad.h (kernels/nh_p_grad.h:4-146).
 // nh_p_grad, nh_p_grad_fullfusion, nh_p_grad_partialfusion, nh_p_grad_openmp keep their names and parameter lists; each runs liboeb's fused
-// HIP kernel (oeb_nh_p_grad_f64, include/oeb.h).  The reference variants are CPU
+// HIP kernel (oeb_nh_p_grad_f64, or oeb_nh_p_grad_f32 when ElementType is float; include/oeb.h).  The reference variants are CPU
 // loop-fusion studies with one result; on the GPU there is one implementation.
 // Scratch arguments are accepted and left untouched.
 #ifndef NH_P_GRAD_H
@@ -23,11 +23,11 @@ inline void nh_p_grad(Storage3D& uout, Storage3D& vout, const Storage3D& uin, co
   const auto a_vin = oeb_dropin::dev(vin);
   const auto a_rdx = oeb_dropin::dev(rdx);
   const auto a_rdy = oeb_dropin::dev(rdy);
-  const auto a_gz = oeb_dropin::dev(gz);
-  const auto a_pp = oeb_dropin::dev(pp);
-  const auto a_pk3 = oeb_dropin::dev(pk3);
+  const auto a_gz = oeb_dropin::island(gz);
+  const auto a_pp = oeb_dropin::island(pp);
+  const auto a_pk3 = oeb_dropin::island(pk3);
   const auto a_wk1 = oeb_dropin::dev(wk1);
-  oeb_dropin::check(oeb_nh_p_grad_f64(&d, &a_uout, &a_vout, &a_uin, &a_vin, &a_rdx, &a_rdy, &a_gz, &a_pp, &a_pk3, &a_wk1, nullptr,
+  oeb_dropin::check(OEB_DROPIN_CALL(nh_p_grad, &d, &a_uout, &a_vout, &a_uin, &a_vin, &a_rdx, &a_rdy, &a_gz, &a_pp, &a_pk3, &a_wk1, nullptr,
                           nullptr, nullptr, dt, oeb_dropin::stream()),
                     "nh_p_grad");
   oeb_dropin::finish(uout, vout);

@@ -1,6 +1,6 @@
 // uvbke.h -- drop-in for the reference's kernels/uvbke.h (kernels/uvbke.h:4-75; dt5 is the driver global, src/uvbke.cpp:11-12).
 // uvbke, uvbke_fullfusion, uvbke_partialfusion, uvbke_openmp keep their names and parameter lists; each runs liboeb's fused
-// HIP kernel (oeb_uvbke_f64, include/oeb.h).  The reference variants are CPU
+// HIP kernel (oeb_uvbke_f64, or oeb_uvbke_f32 when ElementType is float; include/oeb.h).  The reference variants are CPU
 // loop-fusion studies with one result; on the GPU there is one implementation.
 // Scratch arguments are accepted and left untouched.
 #ifndef UVBKE_H
@@ -18,7 +18,7 @@ inline void uvbke(Storage3D& ub, Storage3D& vb, const Storage3D& uc, const Stora
   const auto a_vc = oeb_dropin::dev(vc);
   const auto a_cosa = oeb_dropin::dev(cosa);
   const auto a_rsina = oeb_dropin::dev(rsina);
-  oeb_dropin::check(oeb_uvbke_f64(&d, &a_ub, &a_vb, &a_uc, &a_vc, &a_cosa, &a_rsina, dt5, oeb_dropin::stream()),
+  oeb_dropin::check(OEB_DROPIN_CALL(uvbke, &d, &a_ub, &a_vb, &a_uc, &a_vc, &a_cosa, &a_rsina, dt5, oeb_dropin::stream()),
                     "uvbke");
   oeb_dropin::finish(ub, vb);
 }

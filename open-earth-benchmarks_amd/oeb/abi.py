@@ -20,6 +20,7 @@ OEB_ERR_LAYOUT = 3
 OEB_ERR_ALIAS = 4
 OEB_ERR_HIP = 5
 OEB_ERR_ARG = 6
+OEB_ERR_MEMORY = 7
 
 OEB_HALO_PERIODIC = 0
 OEB_HALO_ZERO_GRADIENT = 1

@@ -17,13 +17,14 @@ import torch
 from . import abi, stencils
 from .layout import Layout
 from .spec import KERNELS
-from .storage import Storage3D
+from .storage import Storage3D, require_device
 
 
 def _adjoint(name: str, fields: dict, scalars: dict, domain, precision: str, stream=None) -> None:
     """Calls oeb_<name>_adj_<precision> with arrays looked up by argument name
     (abi.ADJOINTS) on `stream` (default: torch's current stream)."""
     names, kinds = abi.ADJOINTS[name]
+    require_device(f"{name} adjoint", {n: fields[n] for n, ch in zip(names, kinds) if ch != "s"})
     lib = abi.load()
     nx, ny, nz = domain
     dom = abi.OebDomain(nx, ny, nz)

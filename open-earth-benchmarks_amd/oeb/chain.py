@@ -21,7 +21,7 @@ from . import abi, stencils
 from .generators import cast, generate
 from .layout import Layout, Layout1D
 from .spec import KERNELS
-from .storage import Storage1D, Storage3D
+from .storage import Storage1D, Storage3D, require_device
 
 
 @dataclass(frozen=True)
@@ -44,7 +44,13 @@ class Call:
     scalars: dict = field(default_factory=dict)
 
 
-def halo_fill(f: Storage3D, domain, h: int, hk: int = 0, mode: int = abi.OEB_HALO_PERIODIC, stream=None) -> None:
+def halo_fill(f: Storage3D, domain, h: int, hk: int = 0, mode: int = abi.OEB_HALO_PERIODIC, stream=None,
+              check_device: bool = True) -> None:
+    """Refills the halo of one field (oeb_halo_fill_*).  Host buffers raise
+    TypeError (check_device=False hands them to the library, which rejects
+    them: the validation tests use that)."""
+    if check_device:
+        require_device("halo_fill", {"field": f})
     lib = abi.load()
     nx, ny, nz = domain
     dom = abi.OebDomain(nx, ny, nz)
@@ -52,9 +58,12 @@ def halo_fill(f: Storage3D, domain, h: int, hk: int = 0, mode: int = abi.OEB_HAL
     abi.check(fn(C.byref(dom), f.ref(), int(h), int(hk), int(mode), stencils._stream(stream)))
 
 
-def halo_fill_n(fs: list, domain, h: int, hk: int = 0, mode: int = abi.OEB_HALO_PERIODIC, stream=None) -> None:
+def halo_fill_n(fs: list, domain, h: int, hk: int = 0, mode: int = abi.OEB_HALO_PERIODIC, stream=None,
+                check_device: bool = True) -> None:
     """Refills the halos of 1..8 fields of one layout and precision in one
     launch (oeb_halo_fill_n_*); the same result as halo_fill on each."""
+    if check_device:
+        require_device("halo_fill_n", {f"fields[{i}]": f for i, f in enumerate(fs)})
     lib = abi.load()
     nx, ny, nz = domain
     dom = abi.OebDomain(nx, ny, nz)

@@ -24,6 +24,10 @@ from .layout import Layout, Layout1D
 from .spec import KERNELS
 from .storage import Storage1D, Storage3D
 
+# Stencils with a batched form (oeb_batch_record); the others return
+# OEB_ERR_ARG while a batch records.  Halo refills are batched too.
+BATCHABLE = frozenset({"hadvuv", "hadvuv5th", "hdiffsmag", "fastwavesuv", "uvbke"})
+
 # Member extents of the ensemble workload.
 SIZES_IJ = (96, 128, 192, 256, 384, 512)
 SIZES_K = (60, 80, 137)
@@ -94,6 +98,10 @@ class Ensemble:
     def record(self, steps: int = 1) -> Batch:
         """Records `steps` steps; the returned batch runs them with one
         launch per op (per step)."""
+        bad = sorted({op.stencil for op in self.program if isinstance(op, Call) and op.stencil not in BATCHABLE})
+        if bad:
+            raise ValueError(f"stencils without a batched form cannot be recorded: {', '.join(bad)} "
+                             f"(batched: {', '.join(sorted(BATCHABLE))})")
         b = Batch()
         with b:
             for _ in range(steps):

@@ -18,6 +18,7 @@ import ctypes as C
 
 from . import abi
 from .spec import KERNELS
+from .storage import require_device
 
 domain_size = 64  # src/<name>.cpp:4
 domain_height = 60  # src/<name>.cpp:5
@@ -48,13 +49,18 @@ def _domain(domain) -> abi.OebDomain:
     return abi.OebDomain(nx, ny, nz)
 
 
-def call(name: str, arrays: list, scalars: list, domain=None, stream=None, precision=None) -> None:
+def call(name: str, arrays: list, scalars: list, domain=None, stream=None, precision=None,
+         check_device: bool = True) -> None:
     """Calls oeb_<name>_<precision> with arrays in reference order (None ->
     NULL).  The precision is that of the first array (an output) unless
     given.  In the FP32 form the FP64 island arrays of fastwavesuv, p_grad_c
     and nh_p_grad must be float64, every other array float32; the element
-    types are checked here, because the C-ABI cannot see them."""
+    types are checked here, because the C-ABI cannot see them.  Host buffers
+    raise TypeError here (check_device=False hands them to the library,
+    whose own check rejects them: the C-ABI validation tests use that)."""
     lib = abi.load()
+    if check_device:
+        require_device(name, {a.name: x for a, x in zip(KERNELS[name].args, arrays)})
     dom = _domain(domain)
     if precision is None:
         precision = arrays[0].precision if arrays and arrays[0] is not None else "f64"
@@ -73,7 +79,8 @@ def call(name: str, arrays: list, scalars: list, domain=None, stream=None, preci
     abi.check(getattr(lib, f"oeb_{name}_{precision}")(C.byref(dom), *cargs, _stream(stream)))
 
 
-def run(name: str, fields: dict, scalars: dict | None = None, domain=None, stream=None, precision=None) -> None:
+def run(name: str, fields: dict, scalars: dict | None = None, domain=None, stream=None, precision=None,
+        check_device: bool = True) -> None:
     """Calls stencil `name` with arrays looked up by reference argument name
     (missing scratch -> NULL) and scalars defaulting to the driver values."""
     spec = KERNELS[name]
@@ -81,7 +88,7 @@ def run(name: str, fields: dict, scalars: dict | None = None, domain=None, strea
     sc = dict(spec.scalars)
     if scalars:
         sc.update(scalars)
-    call(name, arrays, [sc[n] for n, _ in spec.scalars], domain, stream, precision)
+    call(name, arrays, [sc[n] for n, _ in spec.scalars], domain, stream, precision, check_device)
 
 
 # ------------------------------------------------- reference signatures

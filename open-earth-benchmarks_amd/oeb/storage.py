@@ -46,6 +46,20 @@ def _ptr(buf) -> int:
     return int(buf.data_ptr())
 
 
+def on_device(buf) -> bool:
+    """True for a torch tensor in GPU memory (numpy arrays and CPU tensors
+    are host memory, which a kernel cannot read)."""
+    return not isinstance(buf, np.ndarray) and bool(getattr(buf, "is_cuda", False))
+
+
+def require_device(what: str, arrays: dict) -> None:
+    """Raises TypeError naming the first array of `arrays` (name -> Storage or
+    None) whose buffer is not in GPU memory."""
+    for name, a in arrays.items():
+        if a is not None and not on_device(a.buf):
+            raise TypeError(f"{what}: array '{name}' is a host buffer; pass a GPU tensor (e.g. Storage3D.to_device)")
+
+
 def _precision(buf) -> str:
     size = buf.itemsize if isinstance(buf, np.ndarray) else buf.element_size()
     return "f64" if size == 8 else "f32"

@@ -1,6 +1,10 @@
 """The C-ABI library on the CPU: it loads, exports every symbol include/oeb.h
 declares, and rejects malformed calls before launching anything.  No GPU is
-needed: every case here fails validation, so no kernel is launched."""
+needed: every case here fails validation, so no kernel is launched.  The
+buffers are host (numpy) arrays, handed to the library past the Python
+layer's own host-buffer check (check_device=False); the library's last check
+rejects host memory, so even a call that passed every other check would
+launch nothing (test_host_memory_rejected)."""
 import ctypes as C
 import re
 from pathlib import Path
@@ -58,9 +62,13 @@ def test_abi_table_matches_header():
 
 def test_status_strings():
     lib = abi.load()
-    for code in range(6):
+    for code in range(8):
         assert lib.oeb_status_string(code)
     assert lib.oeb_status_string(99) == b"unknown status"
+
+
+def _run(name, f, dom):
+    stencils.run(name, f, domain=dom, check_device=False)
 
 
 def _host(name, nx=8, ny=8, nz=2, **kw):
@@ -81,7 +89,7 @@ def test_null_array_rejected(name):
     first = KERNELS[name].inputs3[0]
     f[first] = None
     with pytest.raises(abi.OebError) as e:
-        stencils.run(name, f, domain=(8, 8, 2))
+        _run(name, f, (8, 8, 2))
     assert e.value.status == abi.OEB_ERR_NULL and first in e.value.message
 
 
@@ -90,7 +98,7 @@ def test_output_aliasing_input_rejected(name):
     L, f = _host(name)
     f[KERNELS[name].outputs[0]] = f[KERNELS[name].inputs3[0]]
     with pytest.raises(abi.OebError) as e:
-        stencils.run(name, f, domain=(8, 8, 2))
+        _run(name, f, (8, 8, 2))
     assert e.value.status == abi.OEB_ERR_ALIAS
 
 
@@ -104,7 +112,7 @@ def test_too_small_halo_rejected(name):
     else:
         L, f = _host(name, h=spec.h - 1, align=1)
     with pytest.raises(abi.OebError) as e:
-        stencils.run(name, f, domain=(8, 8, 2))
+        _run(name, f, (8, 8, 2))
     assert e.value.status == abi.OEB_ERR_LAYOUT
 
 
@@ -112,10 +120,10 @@ def test_domain_rejected():
     L, f = _host("laplace")
     for dom in [(0, 8, 2), (8, -1, 2), (8, 8, 0)]:
         with pytest.raises(abi.OebError) as e:
-            stencils.run("laplace", f, domain=dom)
+            _run("laplace", f, dom)
         assert e.value.status == abi.OEB_ERR_DOMAIN
     with pytest.raises(abi.OebError) as e:  # more rows than the allocation holds
-        stencils.run("laplace", f, domain=(8, 12, 2))
+        _run("laplace", f, (8, 12, 2))
     assert e.value.status == abi.OEB_ERR_LAYOUT
 
 
@@ -124,7 +132,7 @@ def test_mismatched_strides_rejected():
     L2 = Layout(8, 8, 2, align=1)
     f["cosa"] = Storage3D(L2, np.zeros(L2.size))
     with pytest.raises(abi.OebError) as e:
-        stencils.run("uvbke", f, domain=(8, 8, 2))
+        _run("uvbke", f, (8, 8, 2))
     assert e.value.status == abi.OEB_ERR_LAYOUT and "strides" in e.value.message
 
 
@@ -133,12 +141,12 @@ def test_bad_descriptor_fields_rejected():
     d = f["in"].desc
     d.strides[2] = 2
     with pytest.raises(abi.OebError) as e:
-        stencils.run("laplace", f, domain=(8, 8, 2))
+        _run("laplace", f, (8, 8, 2))
     assert e.value.status == abi.OEB_ERR_LAYOUT
     d.strides[2] = 1
     d.offset = -1
     with pytest.raises(abi.OebError):
-        stencils.run("laplace", f, domain=(8, 8, 2))
+        _run("laplace", f, (8, 8, 2))
 
 
 def test_one_dimensional_too_short_rejected():
@@ -146,8 +154,21 @@ def test_one_dimensional_too_short_rejected():
     short = Layout1D(8, 0)
     f["crlato"] = Storage1D(short, np.zeros(short.size))
     with pytest.raises(abi.OebError) as e:
-        stencils.run("hdiffsa", f, domain=(8, 8, 2))
+        _run("hdiffsa", f, (8, 8, 2))
     assert e.value.status == abi.OEB_ERR_LAYOUT and "crlato" in e.value.message
+
+
+@pytest.mark.parametrize("name", list(KERNELS))
+def test_host_memory_rejected(name):
+    """Well-formed descriptors of host (pageable) memory: the library rejects
+    the call with OEB_ERR_MEMORY instead of launching a kernel that would
+    fault on them; the Python layer rejects them earlier with TypeError."""
+    L, f = _host(name)
+    with pytest.raises(abi.OebError) as e:
+        _run(name, f, (8, 8, 2))
+    assert e.value.status == abi.OEB_ERR_MEMORY
+    with pytest.raises(TypeError, match="host buffer"):
+        stencils.run(name, f, domain=(8, 8, 2))
 
 
 def test_last_error_is_per_call():
@@ -155,7 +176,7 @@ def test_last_error_is_per_call():
     L, f = _host("laplace")
     f["out"] = f["in"]
     with pytest.raises(abi.OebError):
-        stencils.run("laplace", f, domain=(8, 8, 2))
+        _run("laplace", f, (8, 8, 2))
     assert b"overlaps" in lib.oeb_last_error()
 
 

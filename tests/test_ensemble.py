@@ -111,3 +111,90 @@ def test_batch_graph_replays():
     finally:
         g.close()
         b.close()
+
+
+@pytest.mark.parametrize("name", ["laplace", "hdiffsa", "p_grad_c", "nh_p_grad", "fvtp2d"])
+def test_unbatchable_stencil_refused_while_recording(name):
+    """A stencil without a batched form returns OEB_ERR_ARG while this thread
+    records a batch (it would otherwise launch at once, before the batch's
+    earlier records run), and launches nothing; the refusal comes before any
+    descriptor is read, so host descriptors suffice here."""
+    from oeb import stencils
+    from oeb.layout import Layout1D
+    from oeb.spec import KERNELS
+    from oeb.storage import Storage1D, Storage3D
+
+    L = Layout(8, 8, 2)
+    L1 = Layout1D(8, 3)
+    f = {a.name: (Storage1D(L1, np.zeros(L1.size)) if a.kind == "in1" else Storage3D(L, np.zeros(L.size)))
+         for a in KERNELS[name].args if a.kind != "scratch"}
+    with Batch() as b:
+        with pytest.raises(abi.OebError) as e:
+            stencils.run(name, f, domain=(8, 8, 2), check_device=False)
+        assert e.value.status == abi.OEB_ERR_ARG and "batch" in e.value.message
+    assert b.info() == (0, 0)
+    b.close()
+
+
+def test_ensemble_refuses_unbatchable_program():
+    from oeb.chain import Call
+
+    ens = Ensemble.__new__(Ensemble)
+    ens.program = [Call("hdiffsa", {}), Call("uvbke", {})]
+    with pytest.raises(ValueError, match="hdiffsa"):
+        ens.record()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_batch_splits_dependent_records(precision):
+    """Records of one kernel run concurrently in one launch unless one reads
+    or writes what an earlier one writes.  uvbke(u, v -> a, b) then
+    uvbke(a, b -> c, d) on one member must run in order (two launches) and
+    equal the two calls made one after the other; the same calls on two
+    members' disjoint fields merge into one launch."""
+    import torch
+
+    from gpu_util import to_device
+    from oeb import stencils
+    from oeb.generators import cast
+    from oeb.storage import Storage3D
+
+    L, host = _inputs_uvbke(precision)
+    print(f"[gpu] batch dependency split {precision}", flush=True)
+
+    def fields():
+        d = to_device("uvbke", copy(host), L)
+        d["c"] = Storage3D(L, torch.full_like(d["ub"].buf, float("nan")))
+        d["d"] = Storage3D(L, torch.full_like(d["ub"].buf, float("nan")))
+        return d
+
+    def calls(d):
+        stencils.uvbke(d["ub"], d["vb"], d["uc"], d["vc"], d["cosa"], d["rsina"], domain=(L.nx, L.ny, L.nz))
+        stencils.uvbke(d["c"], d["d"], d["ub"], d["vb"], d["cosa"], d["rsina"], domain=(L.nx, L.ny, L.nz))
+
+    ref, dep = fields(), fields()
+    calls(ref)
+    with Batch() as b:
+        calls(dep)
+    assert b.info() == (2, 2)
+    b.launch()
+    torch.cuda.synchronize()
+    b.close()
+    for n in ("ub", "vb", "c", "d"):
+        x, y = L.interior(dep[n].to_host()), L.interior(ref[n].to_host())
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), n
+    m1, m2 = fields(), fields()
+    with Batch() as b2:
+        for m in (m1, m2):
+            stencils.uvbke(m["ub"], m["vb"], m["uc"], m["vc"], m["cosa"], m["rsina"], domain=(L.nx, L.ny, L.nz))
+    assert b2.info() == (2, 1)
+    b2.close()
+
+
+def _inputs_uvbke(precision):
+    from common import inputs
+    from oeb.generators import cast
+
+    L, host = inputs("uvbke", 40, 24, 5, "baseline", seed=31)
+    return L, cast("uvbke", host, precision)
