@@ -403,6 +403,31 @@ int oeb_fvtp2d_adj_f32(const oeb_domain *d, const oeb_s3f *q, const oeb_s3f *crx
 #pragma pop_macro("p2")
 #pragma pop_macro("p1")
 
+/* oeb_fvtp2d_adj_f64_noalloc -- no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): oeb_fvtp2d_adj_f64 without
+ * the four work arrays, same arguments otherwise, same q_bar bit for bit.  It
+ * calls no allocating routine.  Its two work planes (the quotients the tiled
+ * adjoint forms on the regions of q_j and q_i) live in a 32 MiB device array
+ * inside the library, one per GPU, held until the process ends; levels are
+ * processed in chunks that fit it, one after another on `stream`.  A domain
+ * whose single level needs more than 16 MiB per plane (about 2.1 million
+ * elements of (ny + 5) rows at the j stride) fails with OEB_ERR_DOMAIN.  Every
+ * call on one GPU shares that array: calls that overlap on one GPU (on two
+ * streams, or from two host threads) race and give wrong results. */
+#pragma push_macro("p1")
+#pragma push_macro("p2")
+#undef p1
+#undef p2
+int oeb_fvtp2d_adj_f64_noalloc(const oeb_domain *d, const oeb_s3 *q, const oeb_s3 *crx, const oeb_s3 *cry,
+                               const oeb_s3 *ra_x, const oeb_s3 *ra_y, const oeb_s3 *xfx, const oeb_s3 *yfx,
+                               const oeb_s3 *area, const oeb_s3 *q_i, const oeb_s3 *q_j, const oeb_s3 *q_i_bar,
+                               const oeb_s3 *q_j_bar, const oeb_s3 *fx1_bar, const oeb_s3 *fx2_bar,
+                               const oeb_s3 *fy1_bar, const oeb_s3 *fy2_bar, const oeb_s3 *q_bar, double p1,
+                               double p2, void *stream);
+#pragma pop_macro("p2")
+#pragma pop_macro("p1")
+
 /* ---------------------------------------------------------------- tuning
  * No counterpart in the reference: its drivers fix every loop order at
  * compile time.  Each entry point has launch parameters ("knobs") with

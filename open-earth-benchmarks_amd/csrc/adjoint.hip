@@ -2259,6 +2259,96 @@ int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, 
 }  // namespace
 }  // namespace oeb
 
+// ------------------------------------------------- allocation-free fvtp2d adjoint
+// oeb_fvtp2d_adj_f64_noalloc keeps the two work planes of the tiled fvtp2d
+// adjoint (W0 / ra_x on the region of q_j and W1 / ra_y on the region of q_i,
+// written by K1 / K2 and read by K5 in the same call) in this file-scope
+// device array instead of caller-passed work arrays.  Each GPU the library
+// runs on holds its own copy until the process ends.  Host code reaches it
+// only through the device address hipGetSymbolAddress returns for the current
+// device, never through its name.
+static constexpr size_t kNoAllocScratchDoubles = size_t(4) << 20;  // 32 MiB
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace oeb {
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_ptr;  // per device ordinal: address of `scratch` on it
+
+// The device address of `scratch` on the current device, looked up on the
+// first call on each device.
+int scratch_on_current_device(const char *K, double **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(OEB_ERR_HIP, std::string(K) + ": hipGetDevice: " + hipGetErrorString(e));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (size_t(dev) >= g_scratch_ptr.size()) g_scratch_ptr.resize(size_t(dev) + 1, nullptr);
+  if (g_scratch_ptr[size_t(dev)] == nullptr) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(OEB_ERR_HIP, std::string(K) + ": hipGetSymbolAddress(scratch): " + hipGetErrorString(e));
+    g_scratch_ptr[size_t(dev)] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[size_t(dev)];
+  return OEB_OK;
+}
+
+}  // namespace
+}  // namespace oeb
+
+extern "C" int oeb_fvtp2d_adj_f64_noalloc(const oeb_domain *d, const oeb_s3 *q, const oeb_s3 *crx,
+                                          const oeb_s3 *cry, const oeb_s3 *ra_x, const oeb_s3 *ra_y,
+                                          const oeb_s3 *xfx, const oeb_s3 *yfx, const oeb_s3 *area,
+                                          const oeb_s3 *q_i, const oeb_s3 *q_j, const oeb_s3 *q_i_bar,
+                                          const oeb_s3 *q_j_bar, const oeb_s3 *fx1_bar, const oeb_s3 *fx2_bar,
+                                          const oeb_s3 *fy1_bar, const oeb_s3 *fy2_bar, const oeb_s3 *q_bar,
+                                          double p1, double p2, void *stream) {
+  using namespace oeb;
+  const char *K = "fvtp2d_adj_noalloc";
+  FvAdj<double> a;
+  Geo g;
+  AliasSet as;
+  OEB_TRY(fvtp2d_adj_args<double>(K, d, q, crx, cry, ra_x, ra_y, xfx, yfx, area, q_i, q_j, q_i_bar, q_j_bar,
+                                  fx1_bar, fx2_bar, fy1_bar, fy2_bar, q_bar, p1, p2, a, g, as));
+  OEB_TRY(as.check(K));
+  double *scr = nullptr;
+  OEB_TRY(scratch_on_current_device(K, &scr));
+  // Element offsets, from (0, 0) of a level, that the work planes span: W0 on
+  // i [0, nx) x j [-3, ny+2), W1 on i [-3, nx+2) x j [0, ny).  A chunk of kc
+  // levels needs (kc - 1) sk + (hi - lo) elements per plane, two planes.
+  const int64_t sj = g.sj, sk = g.sk;
+  const int64_t lo = -3 * sj - 3, hi = (int64_t(g.ny) + 1) * sj + g.nx + 2;
+  const int64_t cap = int64_t(kNoAllocScratchDoubles) / 2;
+  if (hi - lo > cap) {
+    std::ostringstream m;
+    m << K << ": one level's two work planes need " << 2 * (hi - lo) * 8 << " bytes (j stride " << sj
+      << ", ny " << g.ny << "); the scratch holds " << kNoAllocScratchDoubles * 8
+      << " bytes (at most about 2.1 million elements per plane); use oeb_fvtp2d_adj_f64 with work arrays";
+    return fail(OEB_ERR_DOMAIN, m.str());
+  }
+  const int kc = int(std::min<int64_t>(g.nz, (cap - (hi - lo)) / sk + 1));
+  const int64_t span = int64_t(kc - 1) * sk + (hi - lo);
+  // Level chunks one after another on `stream`: stream order keeps a chunk's
+  // K5 ahead of the next chunk's K1 / K2, which rewrite the planes.
+  for (int k0 = 0; k0 < g.nz; k0 += kc) {
+    FvAdj<double> c = a;
+    const int64_t sh = int64_t(k0) * sk;
+    for (const double **f : {&c.q, &c.crx, &c.cry, &c.ra_x, &c.ra_y, &c.xfx, &c.yfx, &c.area, &c.q_i, &c.q_j,
+                             &c.qib, &c.qjb, &c.fx1b, &c.fx2b, &c.fy1b, &c.fy2b})
+      *f += sh;
+    c.qbar += sh;
+    c.w0 = scr - lo;
+    c.w1 = scr + span - lo;
+    c.w2 = c.w3 = nullptr;
+    Geo gc = g;
+    gc.nz = std::min(kc, g.nz - k0);
+    OEB_TRY(fvtp2d_adj_tiled_launch<double>(K, d, c, gc, hipStream_t(stream)));
+  }
+  return OEB_OK;
+}
+
 extern "C" int oeb_hdiffsa_adj_f64(const oeb_domain *d, const oeb_s3 *in, const oeb_s3 *mask, const oeb_s1 *crlato,
                                    const oeb_s1 *crlatu, const oeb_s3 *out_bar, const oeb_s3 *in_bar,
                                    const oeb_s3 *lap, void *stream) {

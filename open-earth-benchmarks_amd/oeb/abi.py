@@ -128,7 +128,7 @@ EXPORTS = ["oeb_version", "oeb_status_string", "oeb_last_error"] + [
 ] + ["oeb_halo_fill_f64", "oeb_halo_fill_f32", "oeb_halo_fill_n_f64", "oeb_halo_fill_n_f32",
      "oeb_graph_begin", "oeb_graph_end", "oeb_graph_launch", "oeb_graph_destroy", "oeb_batch_create",
      "oeb_batch_record", "oeb_batch_stop", "oeb_batch_launch", "oeb_batch_info", "oeb_batch_destroy", "oeb_tune_set", "oeb_tune_get", "oeb_tune_clear",
-     "oeb_tune_knob_set", "oeb_tune_knob_get"] + [
+     "oeb_tune_knob_set", "oeb_tune_knob_get", "oeb_fvtp2d_adj_f64_noalloc"] + [
     f"oeb_{k}_adj_{p}" for p in PRECISIONS for k in ADJOINTS
 ]
 
@@ -215,6 +215,10 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
             fn.restype = C.c_int
             sc = C.c_double if prec == "f64" else C.c_float
             fn.argtypes = [PD] + [sc if ch == "s" else (P1 if ch == "1" else P3) for ch in kinds] + [C.c_void_p]
+    # The allocation-free fvtp2d adjoint: the FP64 adjoint's arguments without
+    # the four work arrays.
+    lib.oeb_fvtp2d_adj_f64_noalloc.restype = C.c_int
+    lib.oeb_fvtp2d_adj_f64_noalloc.argtypes = [PD] + [P3] * 17 + [C.c_double, C.c_double, C.c_void_p]
     if path is None:
         _lib = lib
     return lib
