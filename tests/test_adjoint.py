@@ -17,7 +17,7 @@ import pytest
 import aten_ref as A
 from common import inputs
 from oeb import autograd as AG
-from oeb.generators import generate
+from oeb.generators import cast, generate
 from oeb.layout import Layout, Layout1D
 from oeb.spec import KERNELS
 from oeb.storage import Storage1D, Storage3D
@@ -56,35 +56,50 @@ def _relmax(x, y):
 
 
 # ------------------------------------------------------------ the problems
-# Each returns (forward(*diff) -> tuple of outputs, ATen forward, diff inputs,
-# read boxes of the diff inputs as (i0, i1, j0, j1, k0, k1), reach).
-def _hdiffsa(seed):
-    L = Layout(*SHAPE, h=3, hk=1)
-    host = generate("hdiffsa", L, "baseline", seed)
-    c = _dev("hdiffsa", L, host, skip=("in",))
+# Each builder takes (seed, precision, shape, rounded) and returns (L, forward
+# (*diff) -> tuple of outputs, ATen forward, diff inputs, read boxes of the
+# diff inputs as (i0, i1, j0, j1, k0, k1), reach).  precision "f32" runs the
+# FP32 form (FP64 islands).  With precision "f32" or rounded=True every input
+# is first rounded to what the FP32 form stores, so the FP64 and FP32 problems
+# of one seed have the same inputs; the ATen forward always takes those
+# inputs in FP64.
+def _host(name, L, seed, precision, rounded):
+    # hdiffsmag takes the stress inputs (acrlat0 ~ 10^U(0, 4)): with the
+    # baseline ones its Smagorinsky coefficient clamps to 0 everywhere, so the
+    # adjoint would be the identity.
+    host = generate(name, L, "stress" if name == "hdiffsmag" else "baseline", seed)
+    if precision == "f32" or rounded:
+        host = {k: v.astype(np.float64) for k, v in cast(name, host, "f32").items()}
+    return host, cast(name, host, precision)
+
+
+def _hdiffsa(seed, precision="f64", shape=SHAPE, rounded=False):
+    L = Layout(*shape, h=3, hk=1)
+    host, hp = _host("hdiffsa", L, seed, precision, rounded)
+    c = _dev("hdiffsa", L, hp, skip=("in",))
     dom = (L.nx, L.ny, L.nz)
     fwd = lambda x: (AG.Hdiffsa.apply(x, c, L, dom),)  # noqa: E731
     ref = lambda x: (A.hdiffsa(x, _t(host["mask"]), _t(host["crlato"]), _t(host["crlatu"]), L),)  # noqa: E731
-    return L, fwd, ref, [_t(host["in"])], [(-2, L.nx + 2, -2, L.ny + 2, 0, L.nz)], 2
+    return L, fwd, ref, [_t(hp["in"])], [(-2, L.nx + 2, -2, L.ny + 2, 0, L.nz)], 2
 
 
-def _hadvuv5th(seed):
-    L = Layout(*SHAPE, h=3, hk=1)
-    host = generate("hadvuv5th", L, "baseline", seed)
-    c = _dev("hadvuv5th", L, host, skip=("uin", "vin"))
+def _hadvuv5th(seed, precision="f64", shape=SHAPE, rounded=False):
+    L = Layout(*shape, h=3, hk=1)
+    host, hp = _host("hadvuv5th", L, seed, precision, rounded)
+    c = _dev("hadvuv5th", L, hp, skip=("uin", "vin"))
     dom = (L.nx, L.ny, L.nz)
     sc = dict(KERNELS["hadvuv5th"].scalars)
     fwd = lambda u, v: AG.Hadvuv5th.apply(u, v, c, L, dom)  # noqa: E731
     ref = lambda u, v: A.hadvuv5th(u, v, _t(host["acrlat0"]), _t(host["acrlat1"]), _t(host["tgrlatda0"]),  # noqa
                                    _t(host["tgrlatda1"]), L, sc["eddlat"], sc["eddlon"])
     box = (-3, L.nx + 3, -3, L.ny + 3, 0, L.nz)
-    return L, fwd, ref, [_t(host["uin"]), _t(host["vin"])], [box, box], 3
+    return L, fwd, ref, [_t(hp["uin"]), _t(hp["vin"])], [box, box], 3
 
 
-def _fastwavesuv(seed):
-    L = Layout(*SHAPE, h=3, hk=1)
-    host = generate("fastwavesuv", L, "baseline", seed)
-    c = _dev("fastwavesuv", L, host, skip=("uin", "vin", "ppuv"))
+def _fastwavesuv(seed, precision="f64", shape=SHAPE, rounded=False):
+    L = Layout(*shape, h=3, hk=1)
+    host, hp = _host("fastwavesuv", L, seed, precision, rounded)
+    c = _dev("fastwavesuv", L, hp, skip=("uin", "vin", "ppuv"))
     dom = (L.nx, L.ny, L.nz)
     sc = dict(KERNELS["fastwavesuv"].scalars)
     fwd = lambda u, v, p: AG.Fastwavesuv.apply(u, v, p, c, L, dom)  # noqa: E731
@@ -92,14 +107,14 @@ def _fastwavesuv(seed):
                                         _t(host["hhl"]), _t(host["rho"]), _t(host["fx"]), L, sc["edadlat"],
                                         sc["dt"])
     inner = (0, L.nx, 0, L.ny, 0, L.nz)
-    return (L, fwd, ref, [_t(host["uin"]), _t(host["vin"]), _t(host["ppuv"])],
+    return (L, fwd, ref, [_t(hp["uin"]), _t(hp["vin"]), _t(hp["ppuv"])],
             [inner, inner, (0, L.nx + 1, 0, L.ny + 1, -1, L.nz + 1)], 1)
 
 
-def _hdiffsmag(seed):
-    L = Layout(*SHAPE, h=3, hk=1)
-    host = generate("hdiffsmag", L, "baseline", seed)
-    c = _dev("hdiffsmag", L, host, skip=("uin", "vin"))
+def _hdiffsmag(seed, precision="f64", shape=SHAPE, rounded=False):
+    L = Layout(*shape, h=3, hk=1)
+    host, hp = _host("hdiffsmag", L, seed, precision, rounded)
+    c = _dev("hdiffsmag", L, hp, skip=("uin", "vin"))
     dom = (L.nx, L.ny, L.nz)
     sc = dict(KERNELS["hdiffsmag"].scalars)
     fwd = lambda u, v: AG.Hdiffsmag.apply(u, v, c, L, dom)  # noqa: E731
@@ -107,19 +122,19 @@ def _hdiffsmag(seed):
                                    _t(host["crlato"]), _t(host["crlatu"]), _t(host["acrlat0"]), L, sc["eddlat"],
                                    sc["eddlon"], sc["tau_smag"], sc["weight_smag"])
     box = (-1, L.nx + 1, -1, L.ny + 1, 0, L.nz)
-    return L, fwd, ref, [_t(host["uin"]), _t(host["vin"])], [box, box], 2
+    return L, fwd, ref, [_t(hp["uin"]), _t(hp["vin"])], [box, box], 2
 
 
-def _fvtp2d(seed):
-    L = Layout(*SHAPE, h=3, hk=1)
-    host = generate("fvtp2d", L, "baseline", seed)
-    c = _dev("fvtp2d", L, host, skip=("q",))
+def _fvtp2d(seed, precision="f64", shape=SHAPE, rounded=False):
+    L = Layout(*shape, h=3, hk=1)
+    host, hp = _host("fvtp2d", L, seed, precision, rounded)
+    c = _dev("fvtp2d", L, hp, skip=("q",))
     dom = (L.nx, L.ny, L.nz)
     sc = dict(KERNELS["fvtp2d"].scalars)
     names = ("crx", "cry", "ra_x", "ra_y", "xfx", "yfx", "area")
     fwd = lambda q: AG.Fvtp2d.apply(q, c, L, dom)  # noqa: E731
     ref = lambda q: A.fvtp2d(q, *[_t(host[n]) for n in names], L, sc["p1"], sc["p2"])  # noqa: E731
-    return L, fwd, ref, [_t(host["q"])], [(-3, L.nx + 3, -3, L.ny + 3, 0, L.nz)], 6
+    return L, fwd, ref, [_t(hp["q"])], [(-3, L.nx + 3, -3, L.ny + 3, 0, L.nz)], 6
 
 
 PROBLEMS = {"hdiffsa": _hdiffsa, "hadvuv5th": _hadvuv5th, "fastwavesuv": _fastwavesuv, "hdiffsmag": _hdiffsmag,
@@ -154,17 +169,14 @@ def test_adjoint_matches_aten_autograd(name):
         assert err <= ATEN_TOL, f"{name}: adjoint vs autograd rel max {err:.2e}"
 
 
-@pytest.mark.parametrize("name", list(PROBLEMS))
-def test_adjoint_central_differences(name):
+def _central_differences(name, L, fwd, xs, boxes, reach, ybars, grads, tol):
+    """Checks grads (cotangents of xs for the output cotangents ybars)
+    against central differences of fwd at FD_POINTS seeded points, to tol
+    relative to the largest gradient; returns (points, branch flips)."""
     import torch
 
-    L, fwd, _, xs, boxes, reach = PROBLEMS[name](seed=32)
-    print(f"[gpu] adjoint vs central differences {name}", flush=True)
     with torch.no_grad():
         ys = fwd(*xs)
-    g = torch.Generator(device="cuda").manual_seed(6)
-    ybars = [torch.randn(y.shape, generator=g, device="cuda", dtype=y.dtype) for y in ys]
-    grads = _grads(fwd, xs, ybars)
     gmax = max(float(gr.abs().max()) for gr in grads)
     rng = np.random.default_rng(7)
     spacing = 2 * reach + 1
@@ -205,7 +217,7 @@ def test_adjoint_central_differences(name):
                 fd /= 2.0 * float(eps[n])
                 adj = float(grads[f][q])
                 checked += 1
-                if abs(fd - adj) > FD_TOL * gmax:
+                if abs(fd - adj) > tol * gmax:
                     # A frozen branch flipped under the perturbation: the
                     # outputs jump, and the second difference shows it.
                     if curv > 1e-3 * abs(fd) * float(eps[n]):
@@ -216,7 +228,89 @@ def test_adjoint_central_differences(name):
     print(f"  {name}: {checked} points, {flips} branch flips excluded, {len(bad)} failures", flush=True)
     assert checked >= FD_POINTS
     assert flips <= 0.01 * checked
-    assert not bad, f"{name}: {len(bad)} points off by more than {FD_TOL:.0e} x max|grad|: {bad[:3]}"
+    assert not bad, f"{name}: {len(bad)} points off by more than {tol:.0e} x max|grad|: {bad[:3]}"
+    return checked, flips
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_adjoint_central_differences(name):
+    import torch
+
+    L, fwd, _, xs, boxes, reach = PROBLEMS[name](seed=32)
+    print(f"[gpu] adjoint vs central differences {name}", flush=True)
+    with torch.no_grad():
+        ys = fwd(*xs)
+    g = torch.Generator(device="cuda").manual_seed(6)
+    ybars = [torch.randn(y.shape, generator=g, device="cuda", dtype=y.dtype) for y in ys]
+    grads = _grads(fwd, xs, ybars)
+    _central_differences(name, L, fwd, xs, boxes, reach, ybars, grads, FD_TOL)
+
+
+# ----------------------------------------------------------- FP32 adjoints
+# The FP32 adjoints (FP32 storage and arithmetic, FP64 islands) differentiate
+# the FP32 forward, whose branches (limiters, upwind directions, clamps, PPM
+# monotonicity) they freeze.  Independent checks, on inputs rounded to what the
+# FP32 form stores:
+#   - against torch.autograd through the ATen compositions in FP64 on those
+#     inputs.  The values differ by FP32 rounding (bound F32_ATEN_TOL of the
+#     largest gradient) everywhere except where the FP32 and FP64 forwards
+#     take different branches (products within rounding of zero): there a
+#     point's gradient jumps, and the fraction of such points is bounded
+#     (F32_FLIP_FRAC; hdiffsa's flux limiter flips at ~1e-4 of the points of
+#     256x256x8, the others at none).
+#   - against central differences of the FP64 HIP forward (eps 1e-6) at 1000
+#     seeded points, to F32_FD_TOL of the largest gradient, branch flips
+#     excluded as in FP64.
+F32_ATEN_TOL = 1e-5
+F32_FLIP_FRAC = 1e-3
+F32_FD_TOL = 1e-5
+
+
+def _aten_vs_f32(name, seed, shape):
+    import torch
+
+    L, fwd, ref, xs, boxes, _ = PROBLEMS[name](seed=seed, precision="f32", shape=shape)
+    ys = fwd(*xs)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    ybars = [torch.randn(y.shape, generator=g, device="cuda", dtype=y.dtype) for y in ys]
+    ours = _grads(fwd, xs, ybars)
+    xs64 = [x.double() for x in xs]
+    theirs = _grads(ref, xs64, [yb.double() for yb in ybars])
+    for a, b in zip(ours, theirs):
+        a = a.double()
+        gmax = float(b.abs().max())
+        diff = (a - b).abs()
+        frac = float((diff > F32_ATEN_TOL * gmax).double().mean())
+        rms = float(diff.pow(2).mean().sqrt() / b.pow(2).mean().sqrt())
+        print(f"  {name} {shape}: FP32 adjoint vs FP64 autograd rel max {float(diff.max()) / gmax:.2e} "
+              f"rms {rms:.2e} frac>{F32_ATEN_TOL:.0e} {frac:.1e}", flush=True)
+        # Points where the two forwards take different branches are off by
+        # O(1); they set the max and the RMS, so only their fraction is bounded.
+        assert frac <= F32_FLIP_FRAC, f"{name}: {frac:.1e} of the points off by more than {F32_ATEN_TOL:.0e}"
+
+
+@pytest.mark.parametrize("shape", [SHAPE, (256, 256, 8)], ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_f32_adjoint_matches_aten_autograd(name, shape):
+    print(f"[gpu] FP32 adjoint vs autograd {name} {shape}", flush=True)
+    _aten_vs_f32(name, 33, shape)
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_f32_adjoint_central_differences(name):
+    import torch
+
+    L, fwd32, _, xs32, boxes, reach = PROBLEMS[name](seed=34, precision="f32")
+    _, fwd64, _, xs64, _, _ = PROBLEMS[name](seed=34, rounded=True)
+    for a, b in zip(xs32, xs64):
+        assert torch.equal(a.double(), b)
+    print(f"[gpu] FP32 adjoint vs central differences {name}", flush=True)
+    with torch.no_grad():
+        ys = fwd32(*xs32)
+    g = torch.Generator(device="cuda").manual_seed(6)
+    ybars = [torch.randn(y.shape, generator=g, device="cuda", dtype=y.dtype) for y in ys]
+    grads = [gr.double() for gr in _grads(fwd32, xs32, ybars)]
+    _central_differences(name, L, fwd64, xs64, boxes, reach, [yb.double() for yb in ybars], grads, F32_FD_TOL)
 
 
 @pytest.mark.parametrize("rows", [1, 3, 64])

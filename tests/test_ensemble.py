@@ -198,3 +198,46 @@ def _inputs_uvbke(precision):
 
     L, host = inputs("uvbke", 40, 24, 5, "baseline", seed=31)
     return L, cast("uvbke", host, precision)
+
+
+@pytest.mark.gpu
+def test_ensemble_48_member_draw():
+    """The ensemble workload itself: 48 members drawn from {96..512}^2 x
+    {60, 80, 137} (draw_shapes(48, 0)), FP32 with FP64 islands.  Two steps of
+    the batched launches equal two steps of per-member launches bit for bit on
+    every member, every state is finite, and the two smallest members equal
+    the host chain (the oracle) bit for bit."""
+    import torch
+
+    from oeb.ensemble import forward_ensemble
+
+    shapes = draw_shapes(48, 0)
+    print(f"[gpu] ensemble 48 members, {sum(x * y * z for x, y, z in shapes)} points", flush=True)
+    plain, batched = forward_ensemble(shapes, "f32", seed=0), forward_ensemble(shapes, "f32", seed=0)
+    small = sorted(range(48), key=lambda m: shapes[m][0] * shapes[m][1] * shapes[m][2])[:2]
+    hosts = {m: {n: f.buf.cpu().numpy().copy() for n, f in plain.members[m].fields.items()} for m in small}
+    b = batched.record()
+    try:
+        records, launches = b.info()
+        assert records == 48 * 7 and launches == 7
+        for _ in range(2):
+            plain.step_plain()
+            b.launch()
+        torch.cuda.synchronize()
+    finally:
+        b.close()
+    for m, (mp, mb) in enumerate(zip(plain.members, batched.members)):
+        for n in STATE:
+            x, y = mb.fields[n].buf, mp.fields[n].buf
+            assert torch.equal(x.view(torch.int32), y.view(torch.int32)), f"member {m} {shapes[m]}: {n}"
+        L = mp.fields["u"].layout
+        for n in ("u", "v"):
+            assert bool(torch.isfinite(L.interior(mp.fields[n].buf)).all()), f"member {m}: {n} not finite"
+    for m in small:
+        L = plain.members[m].fields["u"].layout
+        ref = hosts[m]
+        run_host(forward_program(), ref, L, "f32", steps=2)
+        for n in STATE:
+            got = plain.members[m].fields[n].buf.cpu().numpy()
+            assert np.array_equal(L.interior(got).view(np.uint8), L.interior(ref[n]).view(np.uint8)), \
+                f"member {m} {shapes[m]}: {n} differs from the host chain"

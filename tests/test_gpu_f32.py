@@ -4,13 +4,17 @@ Bar: bit for bit equal to the oracle's FP32 form (oracle_<name>_f32) on the
 same float32 inputs (FP64 islands float64), with no contraction on either
 side.  Points outside each output's region keep their NaN fill.
 
-At 256x256x80 the error against the FP64 oracle on the unrounded inputs is
-also bounded, per output, with a margin over what was measured
-(F32_BOUNDS): relative max-norm and RMS <= 1e-6 for most outputs.  Two
-effects of the inputs, not of the arithmetic, need wider bounds:
+At 256x256x80 and at the FP32 workload's own size, 512x512x80, the error
+against the FP64 oracle on the unrounded inputs is also bounded, per output,
+with a margin over what was measured (F32_BOUNDS): relative max-norm and RMS
+<= 1e-6 for most outputs.  Two effects of the inputs, not of the
+arithmetic, need wider bounds, so the 1e-5 target is not met by these
+outputs at these sizes:
   laplace.out        rounding `in` to float32 (u * max|in| ~ 6e-8) against
-                     an output ~100x smaller than its input (the Laplacian
-                     of a smooth field): 4.7e-5 max measured, 1.9e-5 RMS.
+                     an output ~100x (256^2) to ~400x (512^2) smaller than its
+                     input (the Laplacian of a smooth field scales with the
+                     grid spacing squared): 4.7e-5 max measured at 256^2,
+                     1.2e-4 at 512^2.
   hdiffsa, fvtp2d    rounding flips a limiter decision (hdiffsa's flux
                      limiter; fvtp2d's almq*br < 0) at isolated points, where
                      the output jumps: up to 4e-2 max at ~1e-4 of the points.
@@ -32,6 +36,7 @@ DEFAULT_BOUND = (1e-6, 1e-6, 0.0)
 LIMITER_BOUND = (1e-1, 1e-3, 1e-3)
 F32_BOUNDS = {("laplace", "out"): (2e-4, 1e-4, 1.0), ("hdiffsa", "out"): LIMITER_BOUND}
 F32_BOUNDS.update({("fvtp2d", o): LIMITER_BOUND for o in KERNELS["fvtp2d"].outputs})
+F32_BOUNDS_512 = {**F32_BOUNDS, ("laplace", "out"): (5e-4, 3e-4, 1.0)}
 SMALL = [(12, 12, 5, 16), (20, 20, 3, 1), (24, 17, 4, 16), (65, 3, 2, 1), (1, 1, 1, 16), (64, 64, 60, 16)]
 
 
@@ -62,20 +67,23 @@ def test_f32_parity_small(name, shape, family):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n", [256, 512])
 @pytest.mark.parametrize("name", ORDER)
-def test_f32_256_against_f64(name):
-    """256x256x80: exact against the FP32 oracle, and within F32_BOUNDS of FP64."""
-    L, host = inputs(name, 256, 256, 80, "baseline", seed=2)
-    print(f"[gpu] f32 {name} 256x256x80", flush=True)
+def test_f32_against_f64(name, n):
+    """n x n x 80: exact against the FP32 oracle, and within the bounds of
+    F32_BOUNDS (256) or F32_BOUNDS_512 of the FP64 oracle."""
+    L, host = inputs(name, n, n, 80, "baseline", seed=2)
+    print(f"[gpu] f32 {name} {n}x{n}x80", flush=True)
     gpu = _check_exact(name, L, cast(name, host, "f32"))
     exact = copy(host)
     B.run_oracle(name, exact, L)
+    bounds = F32_BOUNDS if n == 256 else F32_BOUNDS_512
     for o in KERNELS[name].outputs:
         x, y = out_region(name, o, L, gpu[o]), out_region(name, o, L, exact[o])
         err, rms = rel_maxnorm(x, y), rel_rms(x, y)
         frac = float(np.mean(np.abs(x.astype(np.float64) - y) > 1e-5 * np.max(np.abs(y))))
-        print(f"  {name}.{o}: FP32 vs FP64 rel max {err:.2e} rms {rms:.2e} frac>1e-5 {frac:.1e}", flush=True)
-        bmax, brms, bfrac = F32_BOUNDS.get((name, o), DEFAULT_BOUND)
+        print(f"  {name}.{o} {n}^2: FP32 vs FP64 rel max {err:.2e} rms {rms:.2e} frac>1e-5 {frac:.1e}", flush=True)
+        bmax, brms, bfrac = bounds.get((name, o), DEFAULT_BOUND)
         assert err <= bmax and rms <= brms and frac <= bfrac, (
             f"{name}.{o}: max {err:.2e} (bound {bmax:.0e}), rms {rms:.2e} ({brms:.0e}), frac {frac:.1e} ({bfrac:.0e})")
 
