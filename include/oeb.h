@@ -490,6 +490,15 @@ int oeb_batch_launch(const oeb_batch *batch, void *stream);
 int oeb_batch_info(const oeb_batch *batch, int *records, int *launches);
 int oeb_batch_destroy(oeb_batch *batch);
 
+/* ---------------------------------------------------------- measurement
+ * No counterpart in the reference.  oeb_stream_copy copies `bytes` from src
+ * to dst (device memory, non-overlapping; pointers and size multiples of 16)
+ * with 16-byte non-temporal loads and stores, asynchronously on `stream`:
+ * the practical ceiling of HBM bandwidth that the benchmark reports beside
+ * the stencils. */
+#include <stddef.h>
+int oeb_stream_copy(void *dst, const void *src, size_t bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

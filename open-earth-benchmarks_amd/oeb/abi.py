@@ -128,7 +128,7 @@ EXPORTS = ["oeb_version", "oeb_status_string", "oeb_last_error"] + [
 ] + ["oeb_halo_fill_f64", "oeb_halo_fill_f32", "oeb_halo_fill_n_f64", "oeb_halo_fill_n_f32",
      "oeb_graph_begin", "oeb_graph_end", "oeb_graph_launch", "oeb_graph_destroy", "oeb_batch_create",
      "oeb_batch_record", "oeb_batch_stop", "oeb_batch_launch", "oeb_batch_info", "oeb_batch_destroy", "oeb_tune_set", "oeb_tune_get", "oeb_tune_clear",
-     "oeb_tune_knob_set", "oeb_tune_knob_get", "oeb_fvtp2d_adj_f64_noalloc"] + [
+     "oeb_tune_knob_set", "oeb_tune_knob_get", "oeb_fvtp2d_adj_f64_noalloc", "oeb_stream_copy"] + [
     f"oeb_{k}_adj_{p}" for p in PRECISIONS for k in ADJOINTS
 ]
 
@@ -159,6 +159,15 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
     if _lib is not None and path is None:
         return _lib
     p = Path(path) if path is not None else LIB_PATH
+    # One HIP runtime per process: torch ships its own libamdhip64, and
+    # liboeb binds to whichever copy of that soname is loaded first.  Loading
+    # liboeb before torch maps /opt/rocm's copy as well, and the two runtimes
+    # then fail each other's calls (no device found, out of memory).  So torch
+    # is imported first, and liboeb binds to torch's runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:  # pragma: no cover - torch is a requirement of the GPU path
+        pass
     if not p.exists():
         raise OebLibraryError(
             f"{p} not found: build it first (make -C open-earth-benchmarks_amd lib, or __graft_entry__.build())"
@@ -215,6 +224,8 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
             fn.restype = C.c_int
             sc = C.c_double if prec == "f64" else C.c_float
             fn.argtypes = [PD] + [sc if ch == "s" else (P1 if ch == "1" else P3) for ch in kinds] + [C.c_void_p]
+    lib.oeb_stream_copy.restype = C.c_int
+    lib.oeb_stream_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
     # The allocation-free fvtp2d adjoint: the FP64 adjoint's arguments without
     # the four work arrays.
     lib.oeb_fvtp2d_adj_f64_noalloc.restype = C.c_int

@@ -189,3 +189,26 @@ def test_reference_signature_mirror():
                   if p not in ("domain", "stream")]
         want = [a.name if a.name != "in" else "in_" for a in spec.args] + [s for s, _ in spec.scalars]
         assert params == want, name
+
+
+def test_stream_copy_validation():
+    """oeb_stream_copy rejects NULL pointers and sizes or addresses that are
+    not multiples of 16 before touching memory; a zero-byte copy is a no-op."""
+    lib = abi.load()
+    a = np.zeros(64, dtype=np.float64)
+    b = np.zeros(64, dtype=np.float64)
+    pa, pb = a.ctypes.data, b.ctypes.data
+    assert lib.oeb_stream_copy(None, pb, 64, None) == abi.OEB_ERR_NULL
+    assert lib.oeb_stream_copy(pa, None, 64, None) == abi.OEB_ERR_NULL
+    assert lib.oeb_stream_copy(pa, pb, 24, None) == abi.OEB_ERR_ARG
+    assert lib.oeb_stream_copy(pa + 8, pb, 64, None) == abi.OEB_ERR_ARG
+    assert lib.oeb_stream_copy(pa, pb, 0, None) == abi.OEB_OK
+
+
+def test_stream_copy_rejects_overlap_and_host_memory():
+    lib = abi.load()
+    a = np.zeros(64, dtype=np.float64)
+    pa = a.ctypes.data
+    assert lib.oeb_stream_copy(pa, pa + 128, 256, None) == abi.OEB_ERR_ALIAS
+    b = np.zeros(64, dtype=np.float64)
+    assert lib.oeb_stream_copy(pa, b.ctypes.data, 256, None) == abi.OEB_ERR_MEMORY

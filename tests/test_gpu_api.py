@@ -102,3 +102,19 @@ def test_reference_layout_descriptor_from_util_h():
     B.run_oracle(name, ref, L)
     for o in KERNELS[name].outputs:
         assert np.array_equal(out_region(name, o, L, dev[o].to_host()), out_region(name, o, L, ref[o]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nbytes", [16, 4096 + 48, 37 * 4096, (1 << 24) + 16])
+def test_stream_copy(nbytes):
+    """oeb_stream_copy moves exactly `nbytes` (partial last workgroup
+    included) and leaves the byte after them untouched."""
+    import torch
+
+    src = torch.randint(0, 256, (nbytes + 64,), dtype=torch.uint8, device="cuda")
+    dst = torch.zeros(nbytes + 64, dtype=torch.uint8, device="cuda")
+    print(f"[gpu] stream copy {nbytes} bytes", flush=True)
+    abi.check(abi.load().oeb_stream_copy(dst.data_ptr(), src.data_ptr(), nbytes, None))
+    torch.cuda.synchronize()
+    assert torch.equal(dst[:nbytes], src[:nbytes])
+    assert int(dst[nbytes:].count_nonzero()) == 0
