@@ -19,6 +19,18 @@ output against the FP64 path's output (which equals the FP64 oracle's bit for
 bit), with the fraction of points whose error exceeds 1e-5 of the output's
 max-norm.
 
+target_512x512x80: the stated target workload, the same ten-stencil step at
+512x512x80 in plain FP64 and in the FP32 form (inputs generated on the
+device with the baseline recipes): per stencil, µs, Gpts/s and fraction of
+HBM peak (with whether it meets 0.70), the FP32 speed-up and its error
+against the FP64 outputs per output field.
+
+hbm_ceiling: a streaming device copy (oeb_stream_copy, 16-byte non-temporal
+loads and stores) of 336 MB, 1 GiB and 2 GiB, the practical HBM ceiling.
+
+config1: hdiffsa at the reference's default shape 64x64x60 (k-halo 0), GPU
+and the reference's OpenMP variant on the host, µs per call.
+
 chain: the forward chain hadvuv5th -> hdiffsmag -> fastwavesuv -> uvbke with
 periodic halo refills (oeb.chain: one launch refills both winds), 10 steps
 at the same size, enqueued eagerly and replayed as one recorded HIP graph;
@@ -39,7 +51,9 @@ time (the backward recomputes each segment), their ratio, and peak memory.
 
 cpu_baseline: the reference's own kernels (OpenMP variants, serial DEFAULT for
 fvtp2d, which has no OpenMP variant), compiled by oracle/Makefile, timed on
-this host over a bounded sample of the same workload.
+this host over a bounded sample of the same workload, with one OpenMP thread
+per physical core of the process's CPUs (at most 16, the CPU share of one
+GPU), OMP_PROC_BIND=close and OMP_PLACES=cores.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 256] [--nz 80]
                   [--precision both|f64|f32]
@@ -75,6 +89,9 @@ def parse_args(argv=None):
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--precision", choices=("both", "f64", "f32"), default="both",
                    help="forms to time: the FP64 line plus its fp32 block (both), or one form alone")
+    p.add_argument("--target-size", type=int, default=512,
+                   help="nx = ny of the target block (the stated 512x512x80 workload, FP64 and FP32); 0 skips it")
+    p.add_argument("--no-target", action="store_true", help="skip the target block")
     p.add_argument("--no-chain", action="store_true", help="skip the chain block (eager vs HIP-graph steps)")
     p.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble block")
     p.add_argument("--members", type=int, default=48, help="members of the ensemble block")
@@ -100,25 +117,58 @@ def _traffic(size: int, nz: int, kernel: str):
 
 
 def host_info() -> dict:
-    """The host the CPU baseline ran on: CPU model, logical CPUs, physical
-    cores and sockets (from /proc/cpuinfo, where present)."""
+    """The host the CPU baseline ran on: CPU model, sockets and physical cores
+    of the machine (from /proc/cpuinfo), logical CPUs, and the CPUs and
+    physical cores this process may run on (its affinity mask)."""
     sys.path.insert(0, str(ROOT))
     from oracle import binding as B
 
-    model, sockets = None, set()
+    model, sockets, cores = None, set(), set()
     try:
         with open("/proc/cpuinfo") as f:
+            phys = None
             for line in f:
                 key, _, val = line.partition(":")
                 key, val = key.strip(), val.strip()
                 if key == "model name" and model is None:
                     model = val
                 elif key == "physical id":
+                    phys = val
                     sockets.add(val)
+                elif key == "core id":
+                    cores.add((phys, val))
     except OSError:
         pass
-    return {"model": model, "logical_cpus": os.cpu_count(), "physical_cores": B.cpu_count_physical(),
-            "sockets": len(sockets) or None}
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        affinity = os.cpu_count()
+    return {"model": model, "sockets": len(sockets) or None, "physical_cores": len(cores) or None,
+            "logical_cpus": os.cpu_count(), "affinity_cpus": affinity,
+            "affinity_physical_cores": B.cpu_count_physical()}
+
+
+# The CPU share of one GPU on the benchmark machines: at most 16 threads.
+CPU_SHARE = 16
+
+
+def _omp_env() -> dict:
+    """OpenMP placement of the CPU baseline (threads on cores,
+    packed).  Set before the reference's library, and with it its OpenMP
+    runtime, is loaded."""
+    env = {"OMP_PROC_BIND": "close", "OMP_PLACES": "cores"}
+    for k, v in env.items():
+        os.environ.setdefault(k, v)
+    return {k: os.environ[k] for k in env}
+
+
+def cpu_threads() -> int:
+    """Threads of the CPU baseline: every physical core this process may run
+    on, up to the CPU share of one GPU (CPU_SHARE)."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import binding as B
+
+    return max(1, min(CPU_SHARE, B.cpu_count_physical()))
 
 
 def cpu_baseline(names, hosts, L, budget_s: float):
@@ -127,8 +177,8 @@ def cpu_baseline(names, hosts, L, budget_s: float):
     sys.path.insert(0, str(ROOT))
     from oracle import binding as B
 
-    avail = B.cpu_count_physical()
-    threads = max(1, min(16, avail))
+    omp = _omp_env()
+    threads = cpu_threads()
     if B.ref_available():
         kind = "reference"
         B.ref_lib().ref_set_threads(threads)
@@ -158,9 +208,66 @@ def cpu_baseline(names, hosts, L, budget_s: float):
         "cores": threads,
         "kind": kind,
         "sample": f"{steps} step(s) of {len(names)} stencils at {L.nx}x{L.ny}x{L.nz}, {what}, {threads} thread(s)",
+        "threads_rule": f"physical cores of the affinity mask, at most {CPU_SHARE} (the CPU share of one GPU)",
+        "omp": omp,
         "per_stencil_ms": {n: round(1e3 * statistics.median(v), 3) for n, v in per.items()},
         "host": host_info(),
     }
+
+
+def config1_block(seed: int, budget_s: float, with_cpu: bool = True, reps: int = 200):
+    """hdiffsa at the reference's default shape, 64x64x60 (halo 3, k-halo 0):
+    the GPU per call (HIP events around `reps` back-to-back calls) and the
+    reference's OpenMP variant on this host (median of timed calls)."""
+    import torch
+
+    from oeb import stencils
+    from oeb.generators import generate
+    from oeb.layout import Layout, Layout1D
+    from oeb.spec import KERNELS
+    from oeb.storage import Storage1D, Storage3D
+
+    L = Layout(64, 64, 60, h=3, hk=0, align=16)
+    L1 = Layout1D(64, 3)
+    host = generate("hdiffsa", L, "baseline", seed)
+    dev = {a.name: (Storage1D.to_device(L1, host[a.name]) if a.kind == "in1" else Storage3D.to_device(L, host[a.name]))
+           for a in KERNELS["hdiffsa"].args if a.kind != "scratch"}
+    run = lambda: stencils.run("hdiffsa", dev, domain=(64, 64, 60))  # noqa: E731
+    for _ in range(10):
+        run()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    gpu_us = e0.elapsed_time(e1) * 1e3 / reps
+    pts = 64 * 64 * 60
+    out = {"workload": "hdiffsa, plain FP64, 64x64x60 interior, halo 3, k-halo 0 (the reference's default shape)",
+           "gpu_us_per_call": round(gpu_us, 2), "gpu_gpts_s": round(pts / (gpu_us * 1e-6) / 1e9, 3),
+           "gpu_timing": f"HIP events around {reps} back-to-back calls (inputs resident in the MALL)"}
+    if with_cpu:
+        sys.path.insert(0, str(ROOT))
+        from oracle import binding as B
+
+        _omp_env()
+        threads = cpu_threads()
+        if B.ref_available():
+            B.ref_lib().ref_set_threads(threads)
+            fn, kind = (lambda: B.run_ref("hdiffsa", host, L)), "reference"
+        else:
+            threads = 1
+            fn, kind = (lambda: B.run_oracle("hdiffsa", host, L)), "port"
+        ts, t_start = [], time.perf_counter()
+        while len(ts) < 5 or (time.perf_counter() - t_start < budget_s and len(ts) < 200):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        cpu_us = statistics.median(ts) * 1e6
+        out["cpu"] = {"us_per_call": round(cpu_us, 1), "gpts_s": round(pts / (cpu_us * 1e-6) / 1e9, 4),
+                      "kind": kind, "threads": threads, "calls": len(ts)}
+        out["gpu_over_cpu"] = round(cpu_us / gpu_us, 1)
+    return out
 
 
 def time_steps(names, dev, run, steps: int, warmup: int):
@@ -214,6 +321,127 @@ def per_stencil(names, durs, pts, bytes_per_pt):
             "gbs": round(gbs, 1),
             "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
         }
+    return out
+
+
+def fp32_errors(names, dev64, dev32, L) -> dict:
+    """Per output: relative max and RMS error of the FP32 form's output against
+    the FP64 form's (on the same inputs), and the fraction of points off by
+    more than 1e-5 of the FP64 output's max-norm."""
+    from oeb.spec import KERNELS
+
+    out = {}
+    for name in names:
+        errs = {}
+        for o in KERNELS[name].outputs:
+            i0, i1, j0, j1 = KERNELS[name].region(o, L.nx, L.ny)
+            x = L.region(dev32[name][o].buf, i0, i1, j0, j1).double()
+            y = L.region(dev64[name][o].buf, i0, i1, j0, j1)
+            diff = (x - y).abs()
+            ymax = y.abs().max()
+            errs[o] = {"rel_max": float(f"{float(diff.max() / ymax):.3e}"),
+                       "rel_rms": float(f"{float(diff.pow(2).mean().sqrt() / y.pow(2).mean().sqrt()):.3e}"),
+                       "frac_above_1e-5": float(f"{float((diff > 1e-5 * ymax).double().mean()):.3e}")}
+        out[name] = errs
+    return out
+
+
+def target_block(n: int, nz: int, seed: int, steps: int, warmup: int) -> dict:
+    """The stated target workload, n x n x nz (512 x 512 x 80): the ten-stencil
+    step in plain FP64 and in the FP32 form, timed as the headline step is
+    (per-stencil HIP events on the launch stream), with each FP64 stencil's
+    fraction of the 8 TB/s HBM peak against the 0.70 target, and each FP32
+    stencil's speed-up and error against the FP64 outputs.  Inputs are
+    generated on the device (oeb.generators.generate_device: the baseline
+    recipes), since host generation of 67 fields of 185 MB dominates."""
+    import torch
+
+    from oeb import stencils
+    from oeb.generators import generate_device
+    from oeb.layout import Layout, Layout1D
+    from oeb.spec import KERNELS, ORDER
+    from oeb.storage import Storage1D, Storage3D
+
+    L = Layout(n, n, nz, h=3, hk=1, align=16)
+    L1 = Layout1D(n, 3)
+    pts = n * n * nz
+    dev = {"f64": {}, "f32": {}}
+    for name in ORDER:
+        h = generate_device(name, L, seed)
+        for prec in ("f64", "f32"):
+            d = {}
+            for a in KERNELS[name].args:
+                if a.kind == "scratch":
+                    continue
+                t = h[a.name]
+                if prec == "f32" and KERNELS[name].arg_precision(a.name, "f32") == "f32":
+                    t = t.float()
+                d[a.name] = Storage1D(L1, t) if a.kind == "in1" else Storage3D(L, t)
+            dev[prec][name] = d
+    torch.cuda.synchronize()
+    run = lambda name, fields: stencils.run(name, fields, domain=(n, n, nz))  # noqa: E731
+    res = {}
+    for prec in ("f64", "f32"):
+        step_ms, durs = time_steps(ORDER, dev[prec], run, steps, warmup)
+        res[prec] = (step_ms, per_stencil(ORDER, durs, pts, {nm: KERNELS[nm].bytes_per_point(prec) for nm in ORDER}))
+    (s64, p64), (s32, p32) = res["f64"], res["f32"]
+    errs = fp32_errors(ORDER, dev["f64"], dev["f32"], L)
+    for name in ORDER:
+        p64[name]["meets_0.70"] = p64[name]["hbm_frac"] >= 0.70
+        p32[name]["speedup"] = round(p64[name]["us"] / p32[name]["us"], 3)
+        p32[name]["error_vs_f64"] = errs[name]
+    del dev
+    torch.cuda.empty_cache()
+    v64, v32 = len(ORDER) * pts / (s64 * 1e-3) / 1e9, len(ORDER) * pts / (s32 * 1e-3) / 1e9
+    return {
+        "workload": f"10 stencils, forward, {n}x{n}x{nz} interior, halo 3, k-halo 1; FP64 and FP32 (FP64 islands)",
+        "data": "synthetic, generated on the device (baseline recipes)",
+        "steps": steps,
+        "f64": {"value": round(v64, 3), "unit": "Gpts/s", "ms_per_step": round(s64, 4),
+                "stencils_at_0.70": sum(1 for nm in ORDER if p64[nm]["meets_0.70"]), "per_stencil": p64},
+        "f32": {"value": round(v32, 3), "unit": "Gpts/s", "ms_per_step": round(s32, 4),
+                "speedup_vs_f64": round(v32 / v64, 3),
+                "error_reference": "FP64 outputs of the same inputs (equal to the FP64 oracle's bit for bit)",
+                "per_stencil": p32},
+    }
+
+
+def ceiling_block(sizes=(336 << 20, 1 << 30, 2 << 30), reps: int = 10) -> dict:
+    """The practical HBM ceiling: oeb_stream_copy (16-byte non-temporal loads
+    and stores, a grid striding through the buffer) of `sizes` bytes, HIP
+    events around `reps` back-to-back copies; GB/s counts the bytes read plus
+    the bytes written."""
+    import ctypes as C
+
+    import torch
+
+    from oeb import abi, stencils
+
+    lib = abi.load()
+    out = {}
+    for nbytes in sizes:
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+
+        def copy():
+            abi.check(lib.oeb_stream_copy(C.c_void_p(dst.data_ptr()), C.c_void_p(src.data_ptr()), C.c_size_t(nbytes),
+                                          stencils._stream(None)))
+
+        for _ in range(3):
+            copy()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            copy()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        assert bool(torch.equal(dst[:: max(1, dst.numel() // 4096)], src[:: max(1, src.numel() // 4096)]))
+        gbs = 2 * nbytes / (ms * 1e-3) / 1e9
+        out[f"{nbytes / 1e6:.0f}MB"] = {"us": round(ms * 1e3, 1), "gbs": round(gbs, 1),
+                                        "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+        del src, dst
+    torch.cuda.empty_cache()
     return out
 
 
@@ -612,19 +840,10 @@ def main(argv=None):
     if main_prec == "f64" and "f32" in forms:
         # FP32 outputs of the last timed step against the FP64 outputs of the same inputs.
         step32, per32 = timed["f32"]
+        errs = fp32_errors(names, dev["f64"], dev["f32"], L)
         for name in names:
-            errs = {}
-            for o in KERNELS[name].outputs:
-                i0, i1, j0, j1 = KERNELS[name].region(o, n, n)
-                x = L.region(dev["f32"][name][o].buf, i0, i1, j0, j1).double()
-                y = L.region(dev["f64"][name][o].buf, i0, i1, j0, j1)
-                diff = (x - y).abs()
-                ymax = y.abs().max()
-                errs[o] = {"rel_max": float(diff.max() / ymax),
-                           "rel_rms": float(diff.pow(2).mean().sqrt() / y.pow(2).mean().sqrt()),
-                           "frac_above_1e-5": float((diff > 1e-5 * ymax).double().mean())}
             per32[name]["speedup"] = round(per[name]["us"] / per32[name]["us"], 3)
-            per32[name]["error_vs_f64"] = {o: {k: float(f"{v:.3e}") for k, v in e.items()} for o, e in errs.items()}
+            per32[name]["error_vs_f64"] = errs[name]
         value32 = len(names) * pts / (step32 * 1e-3) / 1e9
         result["fp32"] = {
             "value": round(value32, 3),
@@ -634,6 +853,13 @@ def main(argv=None):
             "error_reference": "FP64 outputs of the same step (equal to the FP64 oracle's bit for bit)",
             "per_stencil": per32,
         }
+    if not args.no_target and args.target_size > 0:
+        del dev
+        torch.cuda.empty_cache()
+        result[f"target_{args.target_size}x{args.target_size}x{args.nz}"] = target_block(
+            args.target_size, args.nz, args.seed, args.steps, args.warmup)
+    result["config1"] = config1_block(args.seed, 5.0, with_cpu=not args.no_cpu_baseline)
+    result["hbm_ceiling"] = ceiling_block()
     if not args.no_chain:
         result["chain"] = chain_block(L, forms, args.seed)
     if not args.no_ensemble and args.members > 0:

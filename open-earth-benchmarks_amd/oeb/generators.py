@@ -249,3 +249,79 @@ def cast(kernel: str, host: dict, precision: str, islands: bool = True) -> dict[
         keep = precision == "f64" or (islands and spec.arg_precision(name, precision) == "f64")
         out[name] = arr.astype(np.float64 if keep else np.float32)
     return out
+
+
+# ------------------------------------------------------- on-device generation
+def generate_device(kernel: str, L: Layout, seed: int = 0, device="cuda") -> dict:
+    """The baseline family's fields of `kernel` generated on the device with
+    torch, as flat float64 tensors of L.size (3-D) or Layout1D size (1-D)
+    elements; outputs NaN.  Same recipes and distributions as make3 / make1,
+    from torch's generator instead of numpy's, so the values differ from
+    generate(); used where host generation of large domains would dominate
+    (the 512x512x80 benchmark block)."""
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed * 1000 + ORDER.index(kernel))
+    f64 = torch.float64
+    shape = (L.levels, L.rows, L.sj)
+    i = torch.arange(L.sj, device=device, dtype=f64) - L.lead
+    j = torch.arange(L.rows, device=device, dtype=f64) - L.h
+    J, I = torch.meshgrid(j, i, indexing="ij")
+    k = torch.arange(L.levels, device=device, dtype=f64) - L.hk
+    z = 250.0 * k
+
+    def uni(lo, hi, shp=shape):
+        return torch.rand(shp, generator=g, device=device, dtype=f64) * (hi - lo) + lo
+
+    def normal(std, shp=shape):
+        return torch.randn(shp, generator=g, device=device, dtype=f64) * std
+
+    def tracer():
+        out = torch.zeros(shape, device=device, dtype=f64)
+        for _ in range(3):
+            kx, ky = (torch.randint(1, 5, (2,), generator=g, device=device)).tolist()
+            phase = uni(0.0, 2 * math.pi, (L.levels,))
+            arg = 2 * math.pi * (kx * I / L.nx + ky * J / L.ny)
+            out += torch.sin(arg[None] + phase[:, None, None])
+        return out + uni(-1e-3, 1e-3)
+
+    def wind():
+        n = normal(10.0)
+        return (n + n.roll(1, 2) + n.roll(-1, 2) + n.roll(1, 1) + n.roll(-1, 1)) / 5.0
+
+    def pressure():
+        return (1e5 * torch.exp(-z / 8000.0))[:, None, None] * (1.0 + normal(0.01, shape[1:]))[None]
+
+    def thickness():
+        prof = 1e5 * (torch.exp(-z / 8000.0) - torch.exp(-(z + 250.0) / 8000.0))
+        return prof[:, None, None] * (1.0 + normal(0.01, shape[1:]))[None]
+
+    def geopot():
+        return (9.81 * z)[:, None, None] + 9.81 * normal(1.0)
+
+    def hhl():
+        p1, p2 = uni(0.0, 2 * math.pi, (2,)).tolist()
+        t = 0.5 + 0.25 * (torch.sin(2 * math.pi * I / L.nx + p1) + torch.sin(2 * math.pi * J / L.ny + p2))
+        return 250.0 * k[:, None, None] * (1.0 + 0.1 * t)[None] + 500.0 * t[None]
+
+    L1 = Layout1D(L.ny, L.h)
+    j1 = torch.arange(L1.size, device=device, dtype=f64) - L1.offset
+
+    def coslat():
+        step = 160.0 / (L.ny - 1) if L.ny > 1 else 0.0
+        return torch.cos(torch.deg2rad(-80.0 + step * j1))
+
+    makers3 = {"tracer": tracer, "coef": lambda: uni(0.01, 0.1), "wind": wind, "pressure": pressure,
+               "thickness": thickness, "geopot": geopot, "hhl": hhl, "courant": lambda: uni(-0.4, 0.4)}
+    makers1 = {"coslat": coslat, "coef1": lambda: uni(0.01, 0.1, (L1.size,))}
+    out = {}
+    for arg in KERNELS[kernel].args:
+        if arg.kind == "scratch":
+            continue
+        if arg.kind == "out":
+            out[arg.name] = torch.full((L.size,), float("nan"), device=device, dtype=f64)
+            continue
+        recipe = BASELINE[kernel][arg.name][0]
+        out[arg.name] = (makers3 if arg.kind == "in" else makers1)[recipe]().reshape(-1).contiguous()
+    return out

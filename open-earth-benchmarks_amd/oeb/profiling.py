@@ -1,8 +1,8 @@
 """Summaries of rocprofv3 output databases (the SQLite ``*_results.db`` that
 rocprofv3 writes) as short markdown for profiles/.
 
-  python -m oeb.profiling stats  <results.db> --command "<cmd>" [--out file.md]
-      per-kernel dispatch count, average / min / max duration and resources
+  python -m oeb.profiling stats  <results.db> --command "<cmd>" [--out file.md] [--top N] [--skip-first W]
+      per (kernel, grid size) dispatch count, average / min / max duration and resources
   python -m oeb.profiling pmc    --fetch <db> --write <db> --copy-bytes N --shape 256x256x80
       --command-fetch "<cmd>" --command-write "<cmd>" [--out-md f.md] [--out-json f.json]
       HBM bytes per launch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
@@ -54,18 +54,25 @@ def _rows(db: Path, sql: str):
 
 
 def kernel_stats(db: Path) -> dict:
+    """Dispatches per (kernel, grid size): durations (us, in dispatch order)
+    and resources.  One kernel launched on two domain sizes gives two rows."""
     rows = _rows(db, "select name, duration, vgpr_count, accum_vgpr_count, sgpr_count, lds_size, scratch_size, "
                      "grid_x, grid_y, grid_z, workgroup_x, workgroup_y, workgroup_z from kernels")
     out = defaultdict(lambda: {"durations": []})
     for (name, dur, vgpr, agpr, sgpr, lds, scratch, gx, gy, gz, wx, wy, wz) in rows:
-        k = out[short_name(name)]
+        k = out[(short_name(name), int(gx) * int(gy) * int(gz))]
         k["durations"].append(float(dur) / 1e3)  # ns -> us
         k.update(vgpr=vgpr, agpr=agpr, sgpr=sgpr, lds=lds, scratch=scratch, grid=(gx, gy, gz), wg=(wx, wy, wz))
     return out
 
 
-def stats_markdown(db: Path, command: str) -> str:
+def stats_markdown(db: Path, command: str, top: int = 0, skip_first: int = 0, first: int = 0,
+                   match: str = "") -> str:
+    import re
+
     st = kernel_stats(db)
+    if match:
+        st = {k: v for k, v in st.items() if re.search(match, k[0])}
     lines = [
         f"# rocprofv3 kernel trace: `{db.name}`",
         "",
@@ -75,14 +82,26 @@ def stats_markdown(db: Path, command: str) -> str:
         command,
         "```",
         "",
-        "| kernel | calls | avg us | min us | max us | VGPR | SGPR | LDS B | scratch B | grid (threads) | workgroup |",
-        "|---|---|---|---|---|---|---|---|---|---|---|",
+        "One row per (kernel, grid size).  avg / min / max over all dispatches; "
+        + (f"'window avg' is the mean of each row's dispatches {skip_first + 1}..{first or 'last'} "
+           f"(in dispatch order)." if skip_first or first else ""),
+        "",
+        "| kernel | grid (threads) | calls | avg us | "
+        + ("window avg us | " if skip_first or first else "")
+        + "min us | max us | VGPR | SGPR | LDS B | scratch B | workgroup |",
+        "|---|---|---|---|" + ("---|" if skip_first or first else "") + "---|---|---|---|---|---|---|",
     ]
-    for name, k in sorted(st.items(), key=lambda kv: -statistics.mean(kv[1]["durations"])):
+    rows = sorted(st.items(), key=lambda kv: -statistics.mean(kv[1]["durations"]))
+    if top:
+        rows = rows[:top]
+    for (name, grid), k in rows:
         d = k["durations"]
+        after = d[skip_first:first or None] or d
         lines.append(
-            f"| {name} | {len(d)} | {statistics.mean(d):.2f} | {min(d):.2f} | {max(d):.2f} | {k['vgpr']} | "
-            f"{k['sgpr']} | {k['lds']} | {k['scratch']} | {k['grid'][0]} | {k['wg'][0]}x{k['wg'][1]} |")
+            f"| {name} | {grid} | {len(d)} | {statistics.mean(d):.2f} | "
+            + (f"{statistics.mean(after):.2f} | " if skip_first or first else "")
+            + f"{min(d):.2f} | {max(d):.2f} | {k['vgpr']} | {k['sgpr']} | {k['lds']} | {k['scratch']} | "
+            f"{k['wg'][0]}x{k['wg'][1]} |")
     return "\n".join(lines) + "\n"
 
 
@@ -94,6 +113,32 @@ def pmc_per_kernel(db: Path, counter: str) -> dict:
     for name, grid, val in rows:
         acc[(short_name(name), int(grid))].append(float(val))
     return {k: statistics.mean(v) for k, v in acc.items()}
+
+
+def counters_markdown(dbs: list, command: str, match: str = "") -> str:
+    """Per (kernel, grid size): the mean per dispatch of every counter in the
+    rocprofv3 --pmc databases `dbs` (one pass each)."""
+    import re
+
+    vals: dict = defaultdict(dict)
+    names: list = []
+    for db in dbs:
+        rows = _rows(Path(db), "select kernel_name, grid_size, counter_name, value from counters_collection")
+        acc = defaultdict(list)
+        for kname, grid, cname, val in rows:
+            acc[(short_name(kname), int(grid), cname)].append(float(val))
+        for (k, g, c), v in acc.items():
+            if match and not re.search(match, k):
+                continue
+            vals[(k, g)][c] = statistics.mean(v)
+            if c not in names:
+                names.append(c)
+    lines = [f"# rocprofv3 counters per dispatch", "", "Commands (from the repository root on the GPU box):", "",
+             "```", command, "```", "",
+             "| kernel | grid (threads) | " + " | ".join(names) + " |", "|---|---|" + "---|" * len(names)]
+    for (k, g), cv in sorted(vals.items()):
+        lines.append(f"| {k} | {g} | " + " | ".join(f"{cv[c]:.4g}" if c in cv else "" for c in names) + " |")
+    return "\n".join(lines) + "\n"
 
 
 def stencil_of(kernel: str):
@@ -116,6 +161,15 @@ def main(argv=None) -> int:
     s.add_argument("db")
     s.add_argument("--command", required=True)
     s.add_argument("--out")
+    s.add_argument("--top", type=int, default=0, help="only the slowest N rows")
+    s.add_argument("--skip-first", type=int, default=0, help="window: leave out each row's first N dispatches")
+    s.add_argument("--first", type=int, default=0, help="window: end after each row's Nth dispatch")
+    s.add_argument("--match", default="", help="only kernels whose short name matches this regex")
+    c = sub.add_parser("counters")
+    c.add_argument("dbs", nargs="+")
+    c.add_argument("--command", required=True)
+    c.add_argument("--match", default="")
+    c.add_argument("--out")
     m = sub.add_parser("pmc")
     m.add_argument("--fetch", required=True)
     m.add_argument("--write", required=True)
@@ -127,8 +181,14 @@ def main(argv=None) -> int:
     m.add_argument("--out-md")
     m.add_argument("--out-json")
     a = p.parse_args(argv)
+    if a.cmd == "counters":
+        md = counters_markdown(a.dbs, a.command, a.match)
+        if a.out:
+            Path(a.out).write_text(md)
+        print(md)
+        return 0
     if a.cmd == "stats":
-        md = stats_markdown(Path(a.db), a.command)
+        md = stats_markdown(Path(a.db), a.command, a.top, a.skip_first, a.first, a.match)
         if a.out:
             Path(a.out).write_text(md)
         print(md)

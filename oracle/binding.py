@@ -170,7 +170,22 @@ def run_ref(name: str, host: dict, L: Layout, variant: int | None = None, scalar
 
 
 def cpu_count_physical() -> int:
+    """Physical cores among the CPUs this process may run on: distinct
+    (socket, core) pairs of the affinity mask, from sysfs topology; the
+    affinity count where sysfs has no topology."""
     try:
-        return len(os.sched_getaffinity(0))
+        cpus = sorted(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
         return os.cpu_count() or 1
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology"
+        try:
+            with open(f"{base}/physical_package_id") as f:
+                pkg = f.read().strip()
+            with open(f"{base}/core_id") as f:
+                core = f.read().strip()
+        except OSError:
+            return len(cpus)
+        cores.add((pkg, core))
+    return len(cores) or len(cpus)

@@ -86,3 +86,46 @@ def test_refdriver_matches_fillmath_formula():
     x, y = i / 18.0, j / 18.0
     want = k * 10e-3 + a * (b + np.cos(np.pi * (x + c * y)) + np.sin(d * np.pi * (x + e * y))) / f
     assert abs(g[L.index(i, j, k)] - want) < 1e-15
+
+
+@pytest.mark.parametrize("name", ORDER)
+def test_device_generator_matches_host_distributions(name):
+    """generate_device (torch, here on the CPU) follows the host generator's
+    recipes: same fields and sizes, NaN outputs, finite inputs everywhere,
+    the same distributions (3-D fields: 1st, 50th and 99th percentiles and
+    the standard deviation within 10% of the host field's 1-99 percentile
+    spread; 1-D fields: the host field's range), and the
+    structural properties the stencils rely on."""
+    import torch
+
+    from oeb.generators import generate_device
+
+    L = Layout(64, 48, 12, h=3, hk=1, align=16)
+    host = generate(name, L, "baseline", seed=2)
+    dev = generate_device(name, L, seed=2, device="cpu")
+    assert set(dev) == set(host)
+    for arg in KERNELS[name].args:
+        if arg.kind == "scratch":
+            continue
+        a = dev[arg.name].numpy()
+        assert a.dtype == np.float64 and a.shape == host[arg.name].shape, arg.name
+        if arg.kind == "out":
+            assert np.isnan(a).all()
+            continue
+        h = host[arg.name]
+        assert np.isfinite(a).all(), arg.name
+        if arg.kind == "in1":  # a few dozen samples: the range only
+            span = h.max() - h.min()
+            assert h.min() - 0.1 * span <= a.min() and a.max() <= h.max() + 0.1 * span, arg.name
+            continue
+        qa, qh = np.percentile(a, [1, 50, 99]), np.percentile(h, [1, 50, 99])
+        spread = max(qh[2] - qh[0], 1e-12)
+        assert np.abs(qa - qh).max() <= 0.1 * spread, (arg.name, qa, qh)
+        assert abs(a.std() - h.std()) <= 0.1 * spread, (arg.name, a.std(), h.std())
+    if name == "nh_p_grad":
+        assert (np.diff(L.view(dev["pk3"].numpy()), axis=0) < 0).all()
+        assert (dev["wk1"] > 0).all()
+    if name == "fastwavesuv":
+        assert (np.diff(L.view(dev["hhl"].numpy()), axis=0) > 0).all()
+    again = generate_device(name, L, seed=2, device="cpu")
+    assert all(torch.equal(dev[k], again[k]) or dev[k].isnan().all() for k in dev)
