@@ -902,7 +902,9 @@ struct HadvTileLds {
   static constexpr int kW2 = kTI + 12, kH2 = TJ + 12;  // u, v: [i0-6, i0+70) x [j0-6, j0+TJ+6)
   static constexpr int kW1 = kTI + 6, kH1 = TJ + 6;    // stage 1: [i0-3, i0+67) x [j0-3, j0+TJ+3)
   static constexpr int kN2 = kW2 * kH2, kN1 = kW1 * kH1;
-  static constexpr size_t kBytes = (2 * size_t(kN2) + 9 * size_t(kN1)) * sizeof(T) + size_t(kN1) * sizeof(int);
+  // 1-D tables of acrlat0/1 and tgrlatda0/1 over the stage-1 rows.
+  static constexpr size_t kBytes =
+      (2 * size_t(kN2) + 9 * size_t(kN1) + 4 * size_t(kH1)) * sizeof(T) + size_t(kN1) * sizeof(int);
 };
 
 // At least 4 waves per SIMD: at most 128 registers.
@@ -916,7 +918,8 @@ __global__ void __launch_bounds__(NT, 4) hadvuv5th_adj_tile(HadvAdj<T> a, Geo g,
   T *s_wux = s_v + kN2, *s_wvx = s_wux + kN1, *s_wuy = s_wvx + kN1, *s_wvy = s_wuy + kN1;
   T *s_gu = s_wvy + kN1, *s_gv = s_gu + kN1, *s_gu2 = s_gv + kN1, *s_gv2 = s_gu2 + kN1;
   T *s_tgt = s_gv2 + kN1;
-  int *s_br = reinterpret_cast<int *>(s_tgt + kN1);
+  T *s_a0 = s_tgt + kN1, *s_a1 = s_a0 + Lds::kH1, *s_t0 = s_a1 + Lds::kH1, *s_t1 = s_t0 + Lds::kH1;
+  int *s_br = reinterpret_cast<int *>(s_t1 + Lds::kH1);
   constexpr T rr = kEarthRadiusRecipAdj<T>;
   const TileCoord tc = tile_coord(t, kTI, TJ, g.nz);
   const int i0 = tc.i0, j0 = tc.j0;
@@ -928,6 +931,15 @@ __global__ void __launch_bounds__(NT, 4) hadvuv5th_adj_tile(HadvAdj<T> a, Geo g,
   const int ow = oi1 - oi0, on = ow * (oj1 - oj0);
   const bool far = i0 >= 3 && i0 + kTI + 3 <= nx && j0 >= 3 && j0 + TJ + 3 <= ny;
   auto inside = [&](int i, int j) { return in_range(i, 0, nx) && in_range(j, 0, ny); };
+  // The 1-D tables: stage-1 row pj at pj - j0 + 3, clamped into [0, ny) (a
+  // row outside it belongs to no interior point and is not read).
+  if (tid < Lds::kH1) {
+    const int64_t jr = min(max(j0 - 3 + tid, 0), ny - 1);
+    s_a0[tid] = a.a0[jr * a.s0];
+    s_a1[tid] = a.a1[jr * a.s1];
+    s_t0[tid] = a.t0[jr * a.s2];
+    s_t1[tid] = a.t1[jr * a.s3];
+  }
   // u, v of level k with a 6-point halo, into registers.
   T lu[kLoad], lv[kLoad];
   auto load_uv = [&](int k) {
@@ -945,7 +957,9 @@ __global__ void __launch_bounds__(NT, 4) hadvuv5th_adj_tile(HadvAdj<T> a, Geo g,
 
   for (int k = tc.k0; k < tc.k1; ++k) {
     const int64_t plane = int64_t(k) * g.sk;
-    // 1. u, v to LDS.
+    // 1. u, v to LDS.  (Loading stage 1's uout_bar and vout_bar here, before
+    // the barrier, spilled 48 bytes per lane and measured 208 us against
+    // 170 at 256x256x80.)
 #pragma unroll
     for (int n = 0; n < kLoad; ++n) {
       const int e = tid + n * NT;
@@ -965,8 +979,8 @@ __global__ void __launch_bounds__(NT, 4) hadvuv5th_adj_tile(HadvAdj<T> a, Geo g,
       if (inside(pi, pj)) {
         const int64_t at = plane + pi + int64_t(pj) * sj;
         const T ob = a.ubar[at], ob2 = a.vbar[at];
-        const T a0 = a.a0[int64_t(pj) * a.s0], a1 = a.a1[int64_t(pj) * a.s1];
-        const T tg0 = a.t0[int64_t(pj) * a.s2], tg1 = a.t1[int64_t(pj) * a.s3];
+        const T a0 = s_a0[r1], a1 = s_a1[r1];
+        const T tg0 = s_t0[r1], tg1 = s_t1[r1];
         const int e2 = (r1 + 3) * kW2 + c1 + 3;
         const T *u = s_u + e2, *v = s_v + e2;
         const T U = (T(1.0) / T(3.0)) * (u[-1] + u[0] + u[1]);
@@ -1532,14 +1546,21 @@ __global__ void __launch_bounds__(1024) hdiffsmag_adj_strip(HsmAdj<T> a, Geo g, 
 // first three:
 //  1. u and v on the tile plus 3 points each side to LDS (addresses clamped
 //     into the box [-1, nx+1) x [-1, ny+1)); the next level's are loaded
-//     into registers during step 4 of this one.
+//     into registers during step 4 of this one.  mask, uout_bar and vout_bar
+//     of the lane's stage-1 points are loaded here too, before the barrier,
+//     so that their latency overlaps the wait.
 //  2. Stage 1 of hdiffsmag_adj_stage1 (gu, gv, wu, wv) at the interior points
 //     of the tile plus 2 each side, 0 elsewhere.
 //  3. T_bar and S_bar of hdiffsmag_adj_stage2 at the tile plus 1 each side
 //     (the points whose weight can be non-zero for the points the tile
-//     owns; outside that region they are 0).
+//     owns; outside that region they are 0).  uout_bar and vout_bar of the
+//     lane's owned points are loaded before the barrier that follows.
 //  4. The gathers of hdiffsmag_adj_stage2 at the points the tile owns: its
 //     own, plus the halo beyond the domain where it touches an edge.
+// The rows of the five 1-D arrays that the tile reads (the same for every
+// level) are copied to LDS tables once per workgroup.  With one 1024-lane
+// workgroup per CU, every global load issued after a barrier waited in full:
+// stage 1 measured 37 of 142 us at 256x256x80 before these loads moved.
 // The same expressions, terms and order as the two-pass form, so both forms
 // give the same bits.  Only uin_bar and vin_bar go to memory.
 template <class T, int TJ>
@@ -1547,20 +1568,23 @@ struct HsmTileLds {
   static constexpr int kW2 = kTI + 6, kH2 = TJ + 6;  // u, v: [i0-3, i0+67) x [j0-3, j0+TJ+3)
   static constexpr int kW1 = kTI + 4, kH1 = TJ + 4;  // stage 1: [i0-2, i0+66) x [j0-2, j0+TJ+2)
   static constexpr int kW0 = kTI + 2, kH0 = TJ + 2;  // T_bar, S_bar: [i0-1, i0+65) x [j0-1, j0+TJ+1)
+  static constexpr int kRows = TJ + 6;                // 1-D tables: rows [j0-3, j0+TJ+3)
   static constexpr int kN2 = kW2 * kH2, kN1 = kW1 * kH1, kN0 = kW0 * kH0;
-  static constexpr size_t kBytes = (2 * size_t(kN2) + 4 * size_t(kN1) + 2 * size_t(kN0)) * sizeof(T);
+  static constexpr size_t kBytes =
+      (2 * size_t(kN2) + 4 * size_t(kN1) + 2 * size_t(kN0) + 5 * size_t(kRows)) * sizeof(T);
 };
 
 template <class T, int TJ, int NT>
 __global__ void __launch_bounds__(NT) hdiffsmag_adj_tile(HsmAdj<T> a, Geo g, Tiles t) {
   using Lds = HsmTileLds<T, TJ>;
   constexpr int kW2 = Lds::kW2, kN2 = Lds::kN2, kW1 = Lds::kW1, kN1 = Lds::kN1, kW0 = Lds::kW0, kH0 = Lds::kH0,
-                kN0 = Lds::kN0;
-  constexpr int kLoad = (kN2 + NT - 1) / NT;
+                kN0 = Lds::kN0, kRows = Lds::kRows;
+  constexpr int kLoad = (kN2 + NT - 1) / NT, kSt1 = (kN1 + NT - 1) / NT, kOwn = (kTI * TJ + NT - 1) / NT;
   __shared__ __align__(16) unsigned char smem[Lds::kBytes];
   T *s_u = reinterpret_cast<T *>(smem), *s_v = s_u + kN2;
   T *s_gu = s_v + kN2, *s_gv = s_gu + kN1, *s_wu = s_gv + kN1, *s_wv = s_wu + kN1;
   T *s_tb = s_wv + kN1, *s_sb = s_tb + kN0;
+  T *s_ac = s_sb + kN0, *s_lo = s_ac + kRows, *s_lu = s_lo + kRows, *s_vo = s_lu + kRows, *s_vu = s_vo + kRows;
   constexpr T rr = kEarthRadiusRecipAdj<T>;
   const T eddlat = a.eddlat, eddlon = a.eddlon;
   const TileCoord tc = tile_coord(t, kTI, TJ, g.nz);
@@ -1570,9 +1594,26 @@ __global__ void __launch_bounds__(NT) hdiffsmag_adj_tile(HsmAdj<T> a, Geo g, Til
   const int oi0 = i0 == 0 ? -1 : i0, oi1 = i0 + kTI >= nx ? nx + 1 : i0 + kTI;
   const int oj0 = j0 == 0 ? -1 : j0, oj1 = j0 + TJ >= ny ? ny + 1 : j0 + TJ;
   const int ow = oi1 - oi0, on = ow * (oj1 - oj0);
+  const bool far = i0 >= 1 && i0 + kTI + 1 <= nx && j0 >= 1 && j0 + TJ + 1 <= ny;
   auto inside = [&](int i, int j) { return in_range(i, 0, nx) && in_range(j, 0, ny); };
-  auto acr = [&](int jj) { return a.ac[int64_t(jj) * a.sac]; };
-  T lu[kLoad], lv[kLoad];
+  // The 1-D tables: row jj at jj - j0 + 3.  Rows outside an array's range
+  // (acrlat0: [-1, ny + 1), the others [0, ny)) hold a clamped copy that no
+  // point uses: only interior points read them.
+  if (tid < kRows) {
+    const int jj = j0 - 3 + tid;
+    const int64_t ja = min(max(jj, -1), ny), jr = min(max(jj, 0), ny - 1);
+    s_ac[tid] = a.ac[ja * a.sac];
+    s_lo[tid] = a.lo[jr * a.slo];
+    s_lu[tid] = a.lu[jr * a.slu];
+    s_vo[tid] = a.vo[jr * a.svo];
+    s_vu[tid] = a.vu[jr * a.svu];
+  }
+  auto acr = [&](int jj) { return s_ac[jj - j0 + 3]; };
+  auto lo = [&](int jj) { return s_lo[jj - j0 + 3]; };
+  auto lu = [&](int jj) { return s_lu[jj - j0 + 3]; };
+  auto vo = [&](int jj) { return s_vo[jj - j0 + 3]; };
+  auto vu = [&](int jj) { return s_vu[jj - j0 + 3]; };
+  T lu_[kLoad], lv_[kLoad];
   auto load_uv = [&](int k) {
     const int64_t plane = int64_t(k) * g.sk;
 #pragma unroll
@@ -1580,32 +1621,47 @@ __global__ void __launch_bounds__(NT) hdiffsmag_adj_tile(HsmAdj<T> a, Geo g, Til
       const int e = tid + n * NT;
       const int gi = min(max(i0 - 3 + e % kW2, -1), nx), gj = min(max(j0 - 3 + e / kW2, -1), ny);
       const int64_t at = plane + gi + int64_t(gj) * sj;
-      lu[n] = a.uin[at];
-      lv[n] = a.vin[at];
+      lu_[n] = a.uin[at];
+      lv_[n] = a.vin[at];
     }
   };
   load_uv(tc.k0);
 
   for (int k = tc.k0; k < tc.k1; ++k) {
     const int64_t plane = int64_t(k) * g.sk;
-    // 1. u, v to LDS.
+    // 1. u, v to LDS; stage 1's loads of mask, uout_bar and vout_bar.
 #pragma unroll
     for (int n = 0; n < kLoad; ++n) {
       const int e = tid + n * NT;
       if (e < kN2) {
-        s_u[e] = lu[n];
-        s_v[e] = lv[n];
+        s_u[e] = lu_[n];
+        s_v[e] = lv_[n];
+      }
+    }
+    T pm[kSt1], pub[kSt1], pvb[kSt1];
+#pragma unroll
+    for (int n = 0; n < kSt1; ++n) {
+      const int e = tid + n * NT;
+      const int i = i0 - 2 + e % kW1, j = j0 - 2 + e / kW1;
+      pm[n] = pub[n] = pvb[n] = T(0);
+      if (e < kN1 && inside(i, j)) {
+        const int64_t at = plane + i + int64_t(j) * sj;
+        pm[n] = a.mask[at];
+        pub[n] = a.ubar[at];
+        pvb[n] = a.vbar[at];
       }
     }
     lds_barrier();
 
     // 2. Stage 1 (kernels/hdiffsmag.h:213-259), as hdiffsmag_adj_stage1.
-    for (int e = tid; e < kN1; e += NT) {
+#pragma unroll
+    for (int n = 0; n < kSt1; ++n) {
+      const int e = tid + n * NT;
+      if (e >= kN1) break;
       const int c1 = e % kW1, r1 = e / kW1;
       const int i = i0 - 2 + c1, j = j0 - 2 + r1;
       T gu = T(0), gv = T(0), wu = T(0), wv = T(0);
       if (inside(i, j)) {
-        const int64_t at = plane + i + int64_t(j) * sj;
         const T *u = s_u + (r1 + 1) * kW2 + c1 + 1, *v = s_v + (r1 + 1) * kW2 + c1 + 1;
         const T ac_s = acr(j - 1), ac_c = acr(j), ac_n = acr(j + 1);
         const T t0 = (v[-kW2] - v[0]) * eddlat * rr - (u[-1] - u[0]) * ac_c * eddlon;
@@ -1616,18 +1672,16 @@ __global__ void __launch_bounds__(NT) hdiffsmag_adj_tile(HsmAdj<T> a, Geo g, Til
         const T s_jm = (u[0] - u[-kW2]) * eddlat * rr + (v[1 - kW2] - v[-kW2]) * ac_s * eddlon;
         const T tt0 = t0 * t0, tt_ip = t_ip * t_ip, tt_jp = t_jp * t_jp;
         const T ss0 = s0 * s0, ss_im = s_im * s_im, ss_jm = s_jm * s_jm;
-        const T hdweight = a.weight * a.mask[at];
+        const T hdweight = a.weight * pm[n];
         const T qu = T(0.5) * (tt_ip + tt0) + T(0.5) * (ss_jm + ss0);
         const T qv = T(0.5) * (tt_jp + tt0) + T(0.5) * (ss_im + ss0);
         const T su = a.tau * sqrt(qu) - hdweight, sv = a.tau * sqrt(qv) - hdweight;
         const bool pass_u = (T(0) < su) && (su < T(0.5)), pass_v = (T(0) < sv) && (sv < T(0.5));
         const T smag_u = pass_u ? su : ((T(0) < su) ? T(0.5) : T(0));
         const T smag_v = pass_v ? sv : ((T(0) < sv) ? T(0.5) : T(0));
-        const T lapu = u[1] + u[-1] - T(2.0) * u[0] + a.lo[int64_t(j) * a.slo] * (u[kW2] - u[0]) +
-                       a.lu[int64_t(j) * a.slu] * (u[-kW2] - u[0]);
-        const T lapv = v[1] + v[-1] - T(2.0) * v[0] + a.vo[int64_t(j) * a.svo] * (v[kW2] - v[0]) +
-                       a.vu[int64_t(j) * a.svu] * (v[-kW2] - v[0]);
-        const T ub = a.ubar[at], vb = a.vbar[at];
+        const T lapu = u[1] + u[-1] - T(2.0) * u[0] + lo(j) * (u[kW2] - u[0]) + lu(j) * (u[-kW2] - u[0]);
+        const T lapv = v[1] + v[-1] - T(2.0) * v[0] + vo(j) * (v[kW2] - v[0]) + vu(j) * (v[-kW2] - v[0]);
+        const T ub = pub[n], vb = pvb[n];
         gu = pass_u ? ub * lapu * a.tau * T(0.5) / sqrt(qu) : T(0);
         gv = pass_v ? vb * lapv * a.tau * T(0.5) / sqrt(qv) : T(0);
         wu = smag_u * ub;
@@ -1640,7 +1694,8 @@ __global__ void __launch_bounds__(NT) hdiffsmag_adj_tile(HsmAdj<T> a, Geo g, Til
     }
     lds_barrier();
 
-    // 3. T_bar, S_bar (hdiffsmag_adj_stage2's Tbar and Sbar).
+    // 3. T_bar, S_bar (hdiffsmag_adj_stage2's Tbar and Sbar); the owned
+    // points' uout_bar and vout_bar.
     for (int e = tid; e < kN0; e += NT) {
       const int c0 = e % kW0, r0 = e / kW0;
       const int x1 = (r0 + 1) * kW1 + c0 + 1;  // in the stage-1 region
@@ -1656,11 +1711,57 @@ __global__ void __launch_bounds__(NT) hdiffsmag_adj_tile(HsmAdj<T> a, Geo g, Til
       s_tb[e] = tb;
       s_sb[e] = sb;
     }
+    T oub[kOwn], ovb[kOwn];
+    if (far) {
+#pragma unroll
+      for (int n = 0; n < kOwn; ++n) {
+        const int e = tid + n * NT;
+        oub[n] = ovb[n] = T(0);
+        if (e < kTI * TJ) {
+          const int64_t at = plane + i0 + e % kTI + int64_t(j0 + e / kTI) * sj;
+          oub[n] = a.ubar[at];
+          ovb[n] = a.vbar[at];
+        }
+      }
+    }
     lds_barrier();
 
     // 4. The gathers at the owned points, with the next level's u and v in
     // flight.  T_bar and S_bar are 0 outside their region (no weight there).
     if (k + 1 < tc.k1) load_uv(k + 1);
+    if (far) {
+      // A tile 1 or more points from every edge owns only its own points,
+      // all of whose neighbours are interior: the same terms in the same
+      // order without the tests.
+#pragma unroll
+      for (int n = 0; n < kOwn; ++n) {
+        const int e = tid + n * NT;
+        if (e >= kTI * TJ) break;
+        const int ci = e % kTI, cj = e / kTI;
+        const int i = i0 + ci, j = j0 + cj;
+        const int x0 = (ci + 1) + (cj + 1) * kW0, c1 = (ci + 2) + (cj + 2) * kW1;
+        const int64_t at = plane + i + int64_t(j) * sj;
+        const T ac_j = acr(j);
+        T ru = oub[n], rv = ovb[n];
+        const T tq = s_tb[x0];
+        ru += tq * ac_j * eddlon - s_tb[x0 + 1] * ac_j * eddlon;
+        rv += s_tb[x0 + kW0] * eddlat * rr - tq * eddlat * rr;
+        const T sq = s_sb[x0];
+        ru += s_sb[x0 - kW0] * eddlat * rr - sq * eddlat * rr;
+        rv += s_sb[x0 - 1] * ac_j * eddlon - sq * ac_j * eddlon;
+        ru += s_wu[c1] * (T(-2.0) - lo(j) - lu(j));
+        rv += s_wv[c1] * (T(-2.0) - vo(j) - vu(j));
+        ru += s_wu[c1 - 1] + s_wu[c1 + 1];
+        rv += s_wv[c1 - 1] + s_wv[c1 + 1];
+        ru += s_wu[c1 - kW1] * lo(j - 1);
+        rv += s_wv[c1 - kW1] * vo(j - 1);
+        ru += s_wu[c1 + kW1] * lu(j + 1);
+        rv += s_wv[c1 + kW1] * vu(j + 1);
+        a.uinbar[at] = ru;
+        a.vinbar[at] = rv;
+      }
+      continue;
+    }
     for (int e = tid; e < on; e += NT) {
       const int i = oi0 + e % ow, j = oj0 + e / ow;
       const int ci = i - i0, cj = j - j0;
@@ -1680,18 +1781,18 @@ __global__ void __launch_bounds__(NT) hdiffsmag_adj_tile(HsmAdj<T> a, Geo g, Til
       ru += bar(s_sb, 0, -1) * eddlat * rr - sq * eddlat * rr;
       rv += bar(s_sb, -1, 0) * acr(j) * eddlon - sq * acr(j) * eddlon;
       if (inside(i, j)) {
-        ru += w1(s_wu, 0, 0) * (T(-2.0) - a.lo[int64_t(j) * a.slo] - a.lu[int64_t(j) * a.slu]);
-        rv += w1(s_wv, 0, 0) * (T(-2.0) - a.vo[int64_t(j) * a.svo] - a.vu[int64_t(j) * a.svu]);
+        ru += w1(s_wu, 0, 0) * (T(-2.0) - lo(j) - lu(j));
+        rv += w1(s_wv, 0, 0) * (T(-2.0) - vo(j) - vu(j));
       }
       ru += w1(s_wu, -1, 0) + w1(s_wu, 1, 0);
       rv += w1(s_wv, -1, 0) + w1(s_wv, 1, 0);
       if (inside(i, j - 1)) {
-        ru += w1(s_wu, 0, -1) * a.lo[int64_t(j - 1) * a.slo];
-        rv += w1(s_wv, 0, -1) * a.vo[int64_t(j - 1) * a.svo];
+        ru += w1(s_wu, 0, -1) * lo(j - 1);
+        rv += w1(s_wv, 0, -1) * vo(j - 1);
       }
       if (inside(i, j + 1)) {
-        ru += w1(s_wu, 0, 1) * a.lu[int64_t(j + 1) * a.slu];
-        rv += w1(s_wv, 0, 1) * a.vu[int64_t(j + 1) * a.svu];
+        ru += w1(s_wu, 0, 1) * lu(j + 1);
+        rv += w1(s_wv, 0, 1) * vu(j + 1);
       }
       a.uinbar[at] = ru;
       a.vinbar[at] = rv;
