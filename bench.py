@@ -346,6 +346,44 @@ def fp32_errors(names, dev64, dev32, L) -> dict:
     return out
 
 
+def deviation_fields(dev32: dict, dev64: dict, L) -> dict:
+    """The FP32 step's fields with the islands of p_grad_c and nh_p_grad as
+    FP32 deviations from FP64 per-level means (oeb_<name>_f32d), split on the
+    device from the FP64 form's island arrays."""
+    from oeb.islands import split_device
+    from oeb.layout import Layout1D
+    from oeb.spec import DEVIATION_STENCILS
+    from oeb.storage import DeviationIsland, Storage1D, Storage3D
+
+    out = {name: dict(fields) for name, fields in dev32.items()}
+    for name, islands in DEVIATION_STENCILS.items():
+        if name not in out:
+            continue
+        for a in islands:
+            d, base = split_device(L, dev64[name][a].buf)
+            out[name][a] = DeviationIsland(Storage3D(L, d), Storage1D(Layout1D(L.nz, L.hk), base))
+    return out
+
+
+def deviation_summary(names, step64, per64, step32d, per32d, errs, pts) -> dict:
+    """The FP32 step with deviation islands: its rate and speed-up over FP64,
+    and the two stencils that take the deviation form."""
+    from oeb.spec import DEVIATION_STENCILS
+
+    v64 = len(names) * pts / (step64 * 1e-3) / 1e9
+    v32d = len(names) * pts / (step32d * 1e-3) / 1e9
+    per = {}
+    for name in names:
+        if name in DEVIATION_STENCILS:
+            per[name] = dict(per32d[name], speedup=round(per64[name]["us"] / per32d[name]["us"], 3),
+                             error_vs_f64=errs[name])
+    return {"value": round(v32d, 3), "unit": "Gpts/s", "ms_per_step": round(step32d, 4),
+            "speedup_vs_f64": round(v32d / v64, 3),
+            "islands": "p_grad_c (gz, pkc) and nh_p_grad (gz, pp, pk3) as FP32 deviations from FP64 level means; "
+                       "fastwavesuv keeps FP64 islands",
+            "per_stencil": per}
+
+
 def target_block(n: int, nz: int, seed: int, steps: int, warmup: int) -> dict:
     """The stated target workload, n x n x nz (512 x 512 x 80): the ten-stencil
     step in plain FP64 and in the FP32 form, timed as the headline step is
@@ -386,6 +424,12 @@ def target_block(n: int, nz: int, seed: int, steps: int, warmup: int) -> dict:
         res[prec] = (step_ms, per_stencil(ORDER, durs, pts, {nm: KERNELS[nm].bytes_per_point(prec) for nm in ORDER}))
     (s64, p64), (s32, p32) = res["f64"], res["f32"]
     errs = fp32_errors(ORDER, dev["f64"], dev["f32"], L)
+    # The deviation-island step writes the FP32 step's output arrays, so it
+    # runs after their errors are taken.
+    dev["f32d"] = deviation_fields(dev["f32"], dev["f64"], L)
+    step_ms, durs = time_steps(ORDER, dev["f32d"], run, steps, warmup)
+    s32d, p32d = step_ms, per_stencil(ORDER, durs, pts, {nm: KERNELS[nm].bytes_per_point("f32d") for nm in ORDER})
+    dsum = deviation_summary(ORDER, s64, p64, s32d, p32d, fp32_errors(ORDER, dev["f64"], dev["f32d"], L), pts)
     for name in ORDER:
         p64[name]["meets_0.70"] = p64[name]["hbm_frac"] >= 0.70
         p32[name]["speedup"] = round(p64[name]["us"] / p32[name]["us"], 3)
@@ -403,6 +447,7 @@ def target_block(n: int, nz: int, seed: int, steps: int, warmup: int) -> dict:
                 "speedup_vs_f64": round(v32 / v64, 3),
                 "error_reference": "FP64 outputs of the same inputs (equal to the FP64 oracle's bit for bit)",
                 "per_stencil": p32},
+        "f32_deviation_islands": dsum,
     }
 
 
@@ -845,6 +890,10 @@ def main(argv=None):
             per32[name]["speedup"] = round(per[name]["us"] / per32[name]["us"], 3)
             per32[name]["error_vs_f64"] = errs[name]
         value32 = len(names) * pts / (step32 * 1e-3) / 1e9
+        # After the FP32 errors: this step writes the FP32 step's output arrays.
+        dev32d = deviation_fields(dev["f32"], dev["f64"], L)
+        step32d, durs32d = time_steps(names, dev32d, run, args.steps, args.warmup)
+        per32d = per_stencil(names, durs32d, pts, {nm: KERNELS[nm].bytes_per_point("f32d") for nm in names})
         result["fp32"] = {
             "value": round(value32, 3),
             "unit": "Gpts/s",
@@ -852,7 +901,10 @@ def main(argv=None):
             "speedup_vs_f64": round(value32 / value, 3),
             "error_reference": "FP64 outputs of the same step (equal to the FP64 oracle's bit for bit)",
             "per_stencil": per32,
+            "deviation_islands": deviation_summary(names, step_ms, per, step32d, per32d,
+                                                   fp32_errors(names, dev["f64"], dev32d, L), pts),
         }
+        del dev32d
     if not args.no_target and args.target_size > 0:
         del dev
         torch.cuda.empty_cache()

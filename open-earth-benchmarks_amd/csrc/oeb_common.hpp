@@ -158,6 +158,44 @@ __device__ __forceinline__ void st2_nt(double *p, double x, double y) {
   __builtin_nontemporal_store(v, reinterpret_cast<dvec2 *>(p));
 }
 
+// The FP64 fields of the FP32 forms ("islands": pressure, geopotential and
+// heights, whose small differences between neighbouring points and levels
+// cancel most of their digits), in one of two storages:
+//   IslandF64  an FP64 array (oeb_<name>_f32);
+//   IslandDev  FP32 deviations from an FP64 per-level base profile,
+//              v(i,j,k) = base(k) + dev(i,j,k), added in FP64
+//              (oeb_<name>_f32d): 4 bytes per point, and the FP32 rounding
+//              applies to the deviation only.
+// level(k) reads what a level contributes (IslandDev: the base value; read
+// once per level, so that the deviation loads of the level issue together);
+// at(p, lv) is then the value at 3-D index p of that level, and pair(p, lv)
+// the values at p and p + 1 (p even: 16-byte aligned for doubles, 8-byte for
+// floats, as pairs_ok() guarantees).
+struct IslandF64 {
+  const double *p;
+  struct Level {};
+  __device__ __forceinline__ Level level(int) const { return Level{}; }
+  __device__ __forceinline__ double at(int64_t i, Level) const { return p[i]; }
+  __device__ __forceinline__ d2 pair(int64_t i, Level) const { return ld2(p + i); }
+  const void *ptr() const { return p; }
+};
+struct IslandDev {
+  const float *p;
+  const double *base;  // base(k) = base[k * bs]
+  int bs;
+  struct Level {
+    double b;
+  };
+  __device__ __forceinline__ Level level(int k) const { return Level{base[int64_t(k) * bs]}; }
+  __device__ __forceinline__ double at(int64_t i, Level lv) const { return lv.b + double(p[i]); }
+  __device__ __forceinline__ d2 pair(int64_t i, Level lv) const {
+    typedef float fvec2 __attribute__((ext_vector_type(2)));
+    const fvec2 v = *reinterpret_cast<const fvec2 *>(p + i);
+    return d2{lv.b + double(v.x), lv.b + double(v.y)};
+  }
+  const void *ptr() const { return p; }
+};
+
 // True when the paired (16-byte) kernels may be used for these arrays.
 inline bool pairs_ok(const Geo &g, std::initializer_list<const void *> bases) {
   if ((g.sj & 1) || (g.sk & 1)) return false;

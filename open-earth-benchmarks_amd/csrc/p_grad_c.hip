@@ -5,10 +5,12 @@
 // Reach: i, j in [-1, 0], k in [0, +1].  Each lane marches up its column and
 // carries the gz / pkc values of level k+1 over as level k of the next step,
 // so every plane of gz and pkc is loaded once per chunk.
-// FP32 form: gz and pkc stay FP64 arrays, and the bracket
-// (dgz*dpk + dgz'*dpk') is formed in FP64 and rounded once to float; the rest
-// is float.  Algorithmic traffic: 7 inputs + 2 outputs = 72 B per point
-// (FP64); 44 B in the FP32 form (gz and pkc 8 B, the rest 4 B).
+// FP32 forms: gz and pkc are FP64 values (FP64 arrays in oeb_p_grad_c_f32,
+// FP32 deviations from FP64 per-level profiles in oeb_p_grad_c_f32d), the
+// bracket (dgz*dpk + dgz'*dpk') is formed in FP64 and rounded once to float;
+// the rest is float.  Algorithmic traffic: 7 inputs + 2 outputs = 72 B per
+// point (FP64); 44 B in the FP32 form (gz and pkc 8 B, the rest 4 B), 36 B
+// with deviations.
 #include "oeb_common.hpp"
 
 namespace oeb {
@@ -29,27 +31,31 @@ __device__ __forceinline__ void pgc_point(double gzC, double gzW, double gzS, do
   vo = (vin + (((dt2 * rdyc) / (wkS + wk0)) * T(bv)));
 }
 
-template <class T>
+template <class T, class IS>
 struct PgcArgs {
   T *uout, *vout;
   const T *uin, *vin, *rdxc, *rdyc, *delpc;
-  const double *gz, *pkc;
+  IS gz, pkc;
 };
 
-template <class T>
-__global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_kernel(PgcArgs<T> a, T dt2, Geo g, Tiles t) {
+template <class T, class IS>
+__global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_kernel(PgcArgs<T, IS> a, T dt2, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
   const int64_t sk = g.sk;
   int64_t at = int64_t(c.k0) * sk + (i + j * sj);
-  double gzC = a.gz[at], gzW = a.gz[at - 1], gzS = a.gz[at - sj];
-  double pkC = a.pkc[at], pkW = a.pkc[at - 1], pkS = a.pkc[at - sj];
+  const auto z0 = a.gz.level(c.k0);
+  const auto p0 = a.pkc.level(c.k0);
+  double gzC = a.gz.at(at, z0), gzW = a.gz.at(at - 1, z0), gzS = a.gz.at(at - sj, z0);
+  double pkC = a.pkc.at(at, p0), pkW = a.pkc.at(at - 1, p0), pkS = a.pkc.at(at - sj, p0);
   for (int k = c.k0; k < c.k1; ++k, at += sk) {
     const int64_t up = at + sk;
-    const double gzC1 = a.gz[up], gzW1 = a.gz[up - 1], gzS1 = a.gz[up - sj];
-    const double pkC1 = a.pkc[up], pkW1 = a.pkc[up - 1], pkS1 = a.pkc[up - sj];
+    const auto z1 = a.gz.level(k + 1);
+    const auto p1 = a.pkc.level(k + 1);
+    const double gzC1 = a.gz.at(up, z1), gzW1 = a.gz.at(up - 1, z1), gzS1 = a.gz.at(up - sj, z1);
+    const double pkC1 = a.pkc.at(up, p1), pkW1 = a.pkc.at(up - 1, p1), pkS1 = a.pkc.at(up - sj, p1);
     T uo, vo;
     pgc_point(gzC, gzW, gzS, gzC1, gzW1, gzS1, pkC, pkW, pkS, pkC1, pkW1, pkS1, a.delpc[at], a.delpc[at - 1],
               a.delpc[at - sj], ld_nt(a.uin + at), ld_nt(a.vin + at), ld_nt(a.rdxc + at), ld_nt(a.rdyc + at), dt2,
@@ -62,26 +68,31 @@ __global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_kernel(PgcArgs<T> a, 
 }
 
 // Two adjacent columns (i, i+1) per lane, i even: the element-type fields
-// move as pairs (16 bytes in FP64, 8 in FP32), the FP64 gz and pkc as 16-byte
-// pairs.  For column i+1 the west neighbour is column i of the pair.
-template <class T>
-__global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_pair_kernel(PgcArgs<T> a, T dt2, Geo g, Tiles t) {
+// move as pairs (16 bytes in FP64, 8 in FP32), gz and pkc as pairs of their
+// storage (16-byte FP64 or 8-byte FP32 deviation pairs).  For column i+1 the
+// west neighbour is column i of the pair.
+template <class T, class IS>
+__global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_pair_kernel(PgcArgs<T, IS> a, T dt2, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
   const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
   const int64_t sk = g.sk;
   int64_t at = int64_t(c.k0) * sk + (i + j * sj);
-  d2 gzC = ld2(a.gz + at), gzS = ld2(a.gz + at - sj);
-  double gzW = a.gz[at - 1];
-  d2 pkC = ld2(a.pkc + at), pkS = ld2(a.pkc + at - sj);
-  double pkW = a.pkc[at - 1];
+  const auto z0 = a.gz.level(c.k0);
+  const auto p0 = a.pkc.level(c.k0);
+  d2 gzC = a.gz.pair(at, z0), gzS = a.gz.pair(at - sj, z0);
+  double gzW = a.gz.at(at - 1, z0);
+  d2 pkC = a.pkc.pair(at, p0), pkS = a.pkc.pair(at - sj, p0);
+  double pkW = a.pkc.at(at - 1, p0);
   for (int k = c.k0; k < c.k1; ++k, at += sk) {
     const int64_t up = at + sk;
-    const d2 gzC1 = ld2(a.gz + up), gzS1 = ld2(a.gz + up - sj);
-    const double gzW1 = a.gz[up - 1];
-    const d2 pkC1 = ld2(a.pkc + up), pkS1 = ld2(a.pkc + up - sj);
-    const double pkW1 = a.pkc[up - 1];
+    const auto z1 = a.gz.level(k + 1);
+    const auto p1 = a.pkc.level(k + 1);
+    const d2 gzC1 = a.gz.pair(up, z1), gzS1 = a.gz.pair(up - sj, z1);
+    const double gzW1 = a.gz.at(up - 1, z1);
+    const d2 pkC1 = a.pkc.pair(up, p1), pkS1 = a.pkc.pair(up - sj, p1);
+    const double pkW1 = a.pkc.at(up - 1, p1);
     const typename Vec<T, 2>::type dpC = ldv<T, 2>(a.delpc + at), dpS = ldv<T, 2>(a.delpc + at - sj);
     const T dpW = a.delpc[at - 1];
     const typename Vec<T, 2>::type ui = ldv_nt<T, 2>(a.uin + at), vi = ldv_nt<T, 2>(a.vin + at);
@@ -101,10 +112,16 @@ __global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_pair_kernel(PgcArgs<T
   }
 }
 
-template <class T, class S3>
+// Validates and launches.  gz and pkc: FP64 arrays (gz0 == pkc0 == NULL), or
+// FP32 deviation arrays with their FP64 base profiles gz0, pkc0 over the
+// levels [0, nz + 1).
+template <class T, class S3, class SI>
 int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *uin, const S3 *vin, const S3 *rdxc,
-                 const S3 *rdyc, const S3 *delpc, const oeb_s3 *gz, const oeb_s3 *pkc, T dt2, void *stream) {
-  const char *K = "p_grad_c";
+                 const S3 *rdyc, const S3 *delpc, const SI *gz, const SI *pkc, const oeb_s1 *gz0, const oeb_s1 *pkc0,
+                 T dt2, void *stream) {
+  constexpr bool kDev = std::is_same<SI, oeb_s3f>::value;
+  using IS = typename std::conditional<kDev, IslandDev, IslandF64>::type;
+  const char *K = "p_grad_c";  // one name for both storages: knobs and messages
   OEB_TRY(check_domain(K, d));
   OEB_TRY(refuse_while_recording(K));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
@@ -112,7 +129,7 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
   const Box r0{0, nx, 0, ny, 0, nz};
   const Box r_dp{-1, nx, -1, ny, 0, nz}, r_k{-1, nx, -1, ny, 0, nz + 1};
   const T *p_uo, *p_vo;
-  PgcArgs<T> a;
+  PgcArgs<T, IS> a;
   OEB_TRY(check3(K, "uout", uout, r0, g, true, &p_uo));
   OEB_TRY(check3(K, "vout", vout, r0, g, false, &p_vo));
   OEB_TRY(check3(K, "uin", uin, r0, g, false, &a.uin));
@@ -120,9 +137,17 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
   OEB_TRY(check3(K, "rdxc", rdxc, r0, g, false, &a.rdxc));
   OEB_TRY(check3(K, "rdyc", rdyc, r0, g, false, &a.rdyc));
   OEB_TRY(check3(K, "delpc", delpc, r_dp, g, false, &a.delpc));
-  OEB_TRY(check3(K, "gz", gz, r_k, g, false, &a.gz));
-  OEB_TRY(check3(K, "pkc", pkc, r_k, g, false, &a.pkc));
+  OEB_TRY(check3(K, "gz", gz, r_k, g, false, &a.gz.p));
+  OEB_TRY(check3(K, "pkc", pkc, r_k, g, false, &a.pkc.p));
   AliasSet as;
+  if constexpr (kDev) {
+    OEB_TRY(check1(K, "gz0", gz0, 0, nz + 1, &a.gz.base, &a.gz.bs));
+    OEB_TRY(check1(K, "pkc0", pkc0, 0, nz + 1, &a.pkc.base, &a.pkc.bs));
+    as.add1("gz0", a.gz.base, 0, nz + 1, a.gz.bs);
+    as.add1("pkc0", a.pkc.base, 0, nz + 1, a.pkc.bs);
+  } else {
+    (void)gz0, (void)pkc0;
+  }
   as.add("uout", p_uo, r0, g, true);
   as.add("vout", p_vo, r0, g, true);
   as.add("uin", a.uin, r0, g, false);
@@ -130,8 +155,8 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
   as.add("rdxc", a.rdxc, r0, g, false);
   as.add("rdyc", a.rdyc, r0, g, false);
   as.add("delpc", a.delpc, r_dp, g, false);
-  as.add("gz", a.gz, r_k, g, false);
-  as.add("pkc", a.pkc, r_k, g, false);
+  as.add("gz", a.gz.p, r_k, g, false);
+  as.add("pkc", a.pkc.p, r_k, g, false);
   OEB_TRY(as.check(K));
   a.uout = const_cast<T *>(p_uo);
   a.vout = const_cast<T *>(p_vo);
@@ -139,20 +164,21 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
   // The tiles of a row on one XCD (Tiles::irow): FP32 184 / 52 us against 189 / 53
   // at 512x512x80 / 256x256x80, FP64 the same (order knob 0 for row order).
   const int xo = knob(K, sizeof(T), d, KNOB_ORDER, 2);
-  // Column pairs by default in the FP32 form only.  Measured in the
+  // Column pairs by default in the FP32 forms only.  Measured in the
   // ten-stencil step at 256x256x80 and 512x512x80: FP32 pairs take
   // 52 / 189 us against 55 / 203 us for single columns; FP64 pairs were slower.
-  if ((knob(K, sizeof(T), d, KNOB_VECTOR, sizeof(T) != 0) == 4) && nx % 2 == 0 &&
-      pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdxc, a.rdyc, a.delpc, a.gz, a.pkc})) {
+  if (knob_on(K, sizeof(T), d, KNOB_VECTOR, sizeof(T) == 4) && nx % 2 == 0 &&
+      pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdxc, a.rdyc, a.delpc, a.gz.ptr(), a.pkc.ptr()})) {
     const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc, xo);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-    hipLaunchKernelGGL(p_grad_c_pair_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g,
-                       t);
+    hipLaunchKernelGGL((p_grad_c_pair_kernel<T, IS>), dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a,
+                       dt2, g, t);
     return launch_status(K);
   }
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-  hipLaunchKernelGGL(p_grad_c_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g, t);
+  hipLaunchKernelGGL((p_grad_c_kernel<T, IS>), dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, dt2, g,
+                     t);
   return launch_status(K);
 }
 
@@ -164,12 +190,22 @@ extern "C" int oeb_p_grad_c_f64(const oeb_domain *d, const oeb_s3 *uout, const o
                                 const oeb_s3 *vin, const oeb_s3 *rdxc, const oeb_s3 *rdyc, const oeb_s3 *delpc,
                                 const oeb_s3 *gz, const oeb_s3 *pkc, const oeb_s3 *wk, double dt2, void *stream) {
   (void)wk;
-  return oeb::p_grad_c_run<double>(d, uout, vout, uin, vin, rdxc, rdyc, delpc, gz, pkc, dt2, stream);
+  return oeb::p_grad_c_run<double>(d, uout, vout, uin, vin, rdxc, rdyc, delpc, gz, pkc, nullptr, nullptr, dt2,
+                                   stream);
 }
 
 extern "C" int oeb_p_grad_c_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
                                 const oeb_s3f *vin, const oeb_s3f *rdxc, const oeb_s3f *rdyc, const oeb_s3f *delpc,
                                 const oeb_s3 *gz, const oeb_s3 *pkc, const oeb_s3f *wk, float dt2, void *stream) {
   (void)wk;
-  return oeb::p_grad_c_run<float>(d, uout, vout, uin, vin, rdxc, rdyc, delpc, gz, pkc, dt2, stream);
+  return oeb::p_grad_c_run<float>(d, uout, vout, uin, vin, rdxc, rdyc, delpc, gz, pkc, nullptr, nullptr, dt2,
+                                  stream);
+}
+
+extern "C" int oeb_p_grad_c_f32d(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
+                                 const oeb_s3f *vin, const oeb_s3f *rdxc, const oeb_s3f *rdyc, const oeb_s3f *delpc,
+                                 const oeb_s3f *gz, const oeb_s1 *gz0, const oeb_s3f *pkc, const oeb_s1 *pkc0,
+                                 const oeb_s3f *wk, float dt2, void *stream) {
+  (void)wk;
+  return oeb::p_grad_c_run<float>(d, uout, vout, uin, vin, rdxc, rdyc, delpc, gz, pkc, gz0, pkc0, dt2, stream);
 }

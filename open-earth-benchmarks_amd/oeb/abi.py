@@ -97,6 +97,10 @@ SIGNATURES_F32 = {
     "fvtp2d": "33333333333333" "3333333" "ff",
 }
 assert SIGNATURES_F32.keys() == SIGNATURES.keys()
+
+# oeb_<name>_f32d: oeb_<name>_f32 with each FP64 island ('D') replaced by an
+# FP32 deviation array ('E') and its FP64 per-level base profile ('b').
+SIGNATURES_F32D = {k: SIGNATURES_F32[k].replace("D", "Eb") for k in ("p_grad_c", "nh_p_grad")}
 assert all(len(SIGNATURES[k]) == len(SIGNATURES_F32[k]) for k in SIGNATURES)
 
 PRECISIONS = ("f64", "f32")
@@ -129,6 +133,8 @@ EXPORTS = ["oeb_version", "oeb_status_string", "oeb_last_error"] + [
      "oeb_graph_begin", "oeb_graph_end", "oeb_graph_launch", "oeb_graph_destroy", "oeb_batch_create",
      "oeb_batch_record", "oeb_batch_stop", "oeb_batch_launch", "oeb_batch_info", "oeb_batch_destroy", "oeb_tune_set", "oeb_tune_get", "oeb_tune_clear",
      "oeb_tune_knob_set", "oeb_tune_knob_get", "oeb_fvtp2d_adj_f64_noalloc", "oeb_stream_copy"] + [
+    f"oeb_{k}_f32d" for k in SIGNATURES_F32D
+] + [
     f"oeb_{k}_adj_{p}" for p in PRECISIONS for k in ADJOINTS
 ]
 
@@ -150,7 +156,7 @@ _lib = None
 
 
 def _ctype(kind: str):
-    return {"3": P3, "D": P3, "1": P1, "d": C.c_double, "f": C.c_float}[kind]
+    return {"3": P3, "D": P3, "E": P3, "1": P1, "b": P1, "d": C.c_double, "f": C.c_float}[kind]
 
 
 def load(path: str | os.PathLike | None = None) -> C.CDLL:
@@ -224,6 +230,10 @@ def load(path: str | os.PathLike | None = None) -> C.CDLL:
             fn.restype = C.c_int
             sc = C.c_double if prec == "f64" else C.c_float
             fn.argtypes = [PD] + [sc if ch == "s" else (P1 if ch == "1" else P3) for ch in kinds] + [C.c_void_p]
+    for name, sig in SIGNATURES_F32D.items():
+        fn = getattr(lib, f"oeb_{name}_f32d")
+        fn.restype = C.c_int
+        fn.argtypes = [PD] + [_ctype(ch) for ch in sig] + [C.c_void_p]
     lib.oeb_stream_copy.restype = C.c_int
     lib.oeb_stream_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
     # The allocation-free fvtp2d adjoint: the FP64 adjoint's arguments without

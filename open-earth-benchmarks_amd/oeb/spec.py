@@ -60,11 +60,14 @@ class Kernel:
 
     def bytes_per_point(self, precision: str = "f64") -> int:
         """Algorithmic bytes per interior point in the given form: FP32 arrays
-        count 4 B, FP64 (and FP64 island) arrays 8 B."""
+        count 4 B, FP64 (and FP64 island) arrays 8 B; in "f32d" (the
+        deviation-island form, oeb_<name>_f32d) the islands count 4 B (their
+        per-level base profiles are negligible)."""
         if precision == "f64":
             return self.bytes_per_point_f64
         n3 = self.inputs3 + self.outputs
-        return sum(8 if a in self.islands else 4 for a in n3)
+        wide = self.islands if precision == "f32" else self.islands - set(DEVIATION_STENCILS.get(self.name, ()))
+        return sum(8 if a in wide else 4 for a in n3)
 
     def arg_precision(self, arg: str, precision: str) -> str:
         """Element type of array `arg` in the given form."""
@@ -82,6 +85,9 @@ ISLANDS = {
     "p_grad_c": frozenset({"gz", "pkc"}),
     "nh_p_grad": frozenset({"gz", "pp", "pk3"}),
 }
+# Stencils with an _f32d entry point, and the islands each takes as FP32
+# deviations from an FP64 per-level base profile (oeb.islands).
+DEVIATION_STENCILS = {"p_grad_c": ("gz", "pkc"), "nh_p_grad": ("gz", "pp", "pk3")}
 
 
 def _a(spec: str) -> tuple[Arg, ...]:

@@ -18,7 +18,7 @@ import ctypes as C
 
 from . import abi
 from .spec import KERNELS
-from .storage import require_device
+from .storage import DeviationIsland, require_device
 
 domain_size = 64  # src/<name>.cpp:4
 domain_height = 60  # src/<name>.cpp:5
@@ -55,7 +55,8 @@ def call(name: str, arrays: list, scalars: list, domain=None, stream=None, preci
     NULL).  The precision is that of the first array (an output) unless
     given.  In the FP32 form the FP64 island arrays of fastwavesuv, p_grad_c
     and nh_p_grad must be float64, every other array float32; the element
-    types are checked here, because the C-ABI cannot see them.  Host buffers
+    types are checked here, because the C-ABI cannot see them.  Islands
+    given as DeviationIsland (p_grad_c, nh_p_grad) call oeb_<name>_f32d.  Host buffers
     raise TypeError here (check_device=False hands them to the library,
     whose own check rejects them: the C-ABI validation tests use that)."""
     lib = abi.load()
@@ -66,9 +67,22 @@ def call(name: str, arrays: list, scalars: list, domain=None, stream=None, preci
         precision = arrays[0].precision if arrays and arrays[0] is not None else "f64"
     sig = abi.signature(name, precision)
     assert len(arrays) + len(scalars) == len(sig), name
+    # FP32 islands given as DeviationIsland select oeb_<name>_f32d: all or none.
+    devs = [isinstance(a, DeviationIsland) for a, kind in zip(arrays, sig) if kind == "D"]
+    entry = f"oeb_{name}_{precision}"
+    if any(devs):
+        if not all(devs) or name not in abi.SIGNATURES_F32D:
+            raise TypeError(f"{name}_{precision}: islands must be all FP64 arrays or all DeviationIsland"
+                            + ("" if name in abi.SIGNATURES_F32D else f" ({name} has no deviation form)"))
+        entry = f"oeb_{name}_f32d"
     cargs = []
     spec_args = KERNELS[name].args
     for a, kind, spec_arg in zip(arrays, sig, spec_args):
+        if isinstance(a, DeviationIsland):
+            if kind != "D":
+                raise TypeError(f"{name}_{precision}: array '{spec_arg.name}' is not an island")
+            cargs.extend(a.refs())
+            continue
         if a is not None:
             want = "f64" if (precision == "f64" or kind == "D") else "f32"
             if a.precision != want:
@@ -76,7 +90,7 @@ def call(name: str, arrays: list, scalars: list, domain=None, stream=None, preci
         cargs.append(None if a is None else a.ref())
     for s_, kind in zip(scalars, sig[len(arrays):]):
         cargs.append(C.c_double(float(s_)) if kind == "d" else C.c_float(float(s_)))
-    abi.check(getattr(lib, f"oeb_{name}_{precision}")(C.byref(dom), *cargs, _stream(stream)))
+    abi.check(getattr(lib, entry)(C.byref(dom), *cargs, _stream(stream)))
 
 
 def run(name: str, fields: dict, scalars: dict | None = None, domain=None, stream=None, precision=None,

@@ -56,8 +56,11 @@ def require_device(what: str, arrays: dict) -> None:
     """Raises TypeError naming the first array of `arrays` (name -> Storage or
     None) whose buffer is not in GPU memory."""
     for name, a in arrays.items():
-        if a is not None and not on_device(a.buf):
-            raise TypeError(f"{what}: array '{name}' is a host buffer; pass a GPU tensor (e.g. Storage3D.to_device)")
+        if a is None:
+            continue
+        for buf in getattr(a, "bufs", (a.buf,)):
+            if not on_device(buf):
+                raise TypeError(f"{what}: array '{name}' is a host buffer; pass a GPU tensor (e.g. Storage3D.to_device)")
 
 
 def _precision(buf) -> str:
@@ -122,3 +125,51 @@ class Storage1D:
         if isinstance(self.buf, np.ndarray):
             return self.buf
         return self.buf.detach().to("cpu").numpy()
+
+
+class DeviationIsland:
+    """An FP64 island of the FP32 forms (p_grad_c's gz and pkc, nh_p_grad's
+    gz, pp and pk3) stored as FP32 deviations from an FP64 per-level base
+    profile: value(i, j, k) = base(k) + dev(i, j, k), the sum formed in FP64
+    by the oeb_<name>_f32d entry points.  `dev` is a float32 Storage3D on the
+    island's layout, `base` a float64 Storage1D over the levels
+    (Layout1D(nz, hk): level k at index k + hk)."""
+
+    precision = "f32d"
+
+    def __init__(self, dev: Storage3D, base: Storage1D):
+        if dev.precision != "f32" or base.precision != "f64":
+            raise TypeError("DeviationIsland: dev must be float32 and base float64")
+        L = dev.layout
+        if base.layout.size != L.levels or base.layout.offset != L.hk:
+            raise ValueError(f"DeviationIsland: base must cover the {L.levels} levels of the layout "
+                             f"(Layout1D({L.nz}, {L.hk}))")
+        self.dev, self.base = dev, base
+        self.layout = L
+
+    @property
+    def buf(self):
+        return self.dev.buf
+
+    @property
+    def bufs(self):
+        return (self.dev.buf, self.base.buf)
+
+    def refs(self):
+        return [self.dev.ref(), self.base.ref()]
+
+    @classmethod
+    def split(cls, layout: Layout, values: np.ndarray, device="cuda") -> "DeviationIsland":
+        """Splits a host float64 island (layout.size values) into the mean of
+        each level (over every allocated point of the plane) and the FP32
+        deviations from it, on `device`."""
+        from .islands import split_host
+
+        dev, base = split_host(layout, values)
+        return cls(Storage3D.to_device(layout, dev, device), Storage1D.to_device(level_layout(layout), base, device))
+
+
+def level_layout(layout: Layout) -> Layout1D:
+    """The 1-D layout of a per-level profile of `layout`: the levels
+    [-hk, nz + hk), level k at index k + hk."""
+    return Layout1D(layout.nz, layout.hk)

@@ -212,3 +212,67 @@ def test_stream_copy_rejects_overlap_and_host_memory():
     assert lib.oeb_stream_copy(pa, pa + 128, 256, None) == abi.OEB_ERR_ALIAS
     b = np.zeros(64, dtype=np.float64)
     assert lib.oeb_stream_copy(pa, b.ctypes.data, 256, None) == abi.OEB_ERR_MEMORY
+
+
+def test_f32_and_f32d_tables_match_header():
+    """The FP32 and deviation-island entry points: each parameter's type in
+    the header matches the ctypes table ('3' oeb_s3f, 'D' oeb_s3, 'E' oeb_s3f,
+    'b' oeb_s1, '1' oeb_s1f, 'f' float, 'd' double)."""
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+
+    def kind(p):
+        for pat, k in (("oeb_s3f", "3"), ("oeb_s3 ", "D"), ("oeb_s1f", "1"), ("oeb_s1 ", "b")):
+            if pat in p + " ":
+                return k
+        return "f" if p.startswith("float") else "d" if p.startswith("double") else "?"
+
+    for suffix, table in (("f32", abi.SIGNATURES_F32), ("f32d", abi.SIGNATURES_F32D)):
+        for name, sig in table.items():
+            m = re.search(rf"int oeb_{name}_{suffix}\((.*?)\);", text, flags=re.S)
+            assert m, (name, suffix)
+            params = [p.strip() for p in m.group(1).split(",")][1:-1]
+            kinds = "".join(kind(p) for p in params)
+            want = sig.replace("E", "3")
+            assert kinds == want, (name, suffix, kinds, want)
+            if suffix == "f32d":
+                # each island is followed by its profile, named <island>0
+                pnames = [p.split("*")[-1].strip() for p in params if "*" in p]
+                islands = [n for n, k in zip(pnames, kinds) if k == "b"]
+                assert islands and all(n.endswith("0") and n[:-1] in pnames for n in islands), (name, islands)
+
+
+def test_f32d_validation():
+    """The deviation entry points reject a NULL or too-short base profile
+    before launching anything (host buffers: no GPU needed)."""
+    import torch  # noqa: F401  (the library binds to torch's HIP runtime)
+
+    from oeb.generators import cast, generate
+    from oeb.islands import split_host
+    from oeb.storage import DeviationIsland, level_layout
+
+    L = Layout(16, 12, 4, h=3, hk=1)
+    host = cast("p_grad_c", generate("p_grad_c", L, "baseline", 0), "f32")
+    arrays = {}
+    for a in KERNELS["p_grad_c"].args:
+        if a.kind in ("in", "out"):
+            if a.name in ("gz", "pkc"):
+                dev, base = split_host(L, host[a.name])
+                arrays[a.name] = DeviationIsland(Storage3D(L, dev), Storage1D(level_layout(L), base))
+            else:
+                arrays[a.name] = Storage3D(L, host[a.name])
+    with pytest.raises(abi.OebError) as e:
+        stencils.run("p_grad_c", arrays, domain=(16, 12, 4), check_device=False)
+    assert e.value.status == abi.OEB_ERR_MEMORY  # host memory, rejected last
+    short = Layout1D(3, 1)  # covers levels [-1, 4): level nz = 4 is missing
+    arrays["gz"] = DeviationIsland.__new__(DeviationIsland)
+    arrays["gz"].dev = Storage3D(L, split_host(L, host["gz"])[0])
+    arrays["gz"].base = Storage1D(short, np.zeros(short.size))
+    with pytest.raises(abi.OebError) as e:
+        stencils.run("p_grad_c", arrays, domain=(16, 12, 4), check_device=False)
+    assert e.value.status == abi.OEB_ERR_LAYOUT and "gz0" in str(e.value)
+    with pytest.raises(ValueError):
+        DeviationIsland(arrays["gz"].dev, Storage1D(short, np.zeros(short.size)))
+    mixed = dict(arrays)
+    mixed["gz"] = Storage3D(L, host["gz"])
+    with pytest.raises(TypeError, match="all FP64 arrays or all DeviationIsland"):
+        stencils.run("p_grad_c", mixed, domain=(16, 12, 4), check_device=False)
