@@ -347,9 +347,10 @@ def fp32_errors(names, dev64, dev32, L) -> dict:
 
 
 def deviation_fields(dev32: dict, dev64: dict, L) -> dict:
-    """The FP32 step's fields with the islands of p_grad_c and nh_p_grad as
-    FP32 deviations from FP64 per-level means (oeb_<name>_f32d), split on the
-    device from the FP64 form's island arrays."""
+    """The FP32 step's fields with the islands of fastwavesuv, p_grad_c and
+    nh_p_grad as FP32 deviations from FP64 per-level means
+    (oeb_<name>_f32d), split on the device from the FP64 form's island
+    arrays."""
     from oeb.islands import split_device
     from oeb.layout import Layout1D
     from oeb.spec import DEVIATION_STENCILS
@@ -367,7 +368,7 @@ def deviation_fields(dev32: dict, dev64: dict, L) -> dict:
 
 def deviation_summary(names, step64, per64, step32d, per32d, errs, pts) -> dict:
     """The FP32 step with deviation islands: its rate and speed-up over FP64,
-    and the two stencils that take the deviation form."""
+    and the three stencils that take the deviation form."""
     from oeb.spec import DEVIATION_STENCILS
 
     v64 = len(names) * pts / (step64 * 1e-3) / 1e9
@@ -379,8 +380,8 @@ def deviation_summary(names, step64, per64, step32d, per32d, errs, pts) -> dict:
                              error_vs_f64=errs[name])
     return {"value": round(v32d, 3), "unit": "Gpts/s", "ms_per_step": round(step32d, 4),
             "speedup_vs_f64": round(v32d / v64, 3),
-            "islands": "p_grad_c (gz, pkc) and nh_p_grad (gz, pp, pk3) as FP32 deviations from FP64 level means; "
-                       "fastwavesuv keeps FP64 islands",
+            "islands": "fastwavesuv (ppuv, hhl), p_grad_c (gz, pkc) and nh_p_grad (gz, pp, pk3) as FP32 "
+                       "deviations from FP64 level means",
             "per_stencil": per}
 
 

@@ -284,15 +284,21 @@ int oeb_fvtp2d_f32(const oeb_domain *d, const oeb_s3f *q_i, const oeb_s3f *q_j, 
  * each FP64 island array replaced by two arguments: an FP32 array of
  * deviations (oeb_storage3d_f32, same box as the island) followed by an FP64
  * per-level base profile (oeb_storage1d_f64 indexed by k, covering the
- * island's levels [0, nz + 1)).  The kernel reads the island value
+ * island's levels: [-1, nz + 1) for fastwavesuv's ppuv, [0, nz + 1) for the
+ * others).  The kernel reads the island value
  * base(k) + dev(i,j,k), the sum formed in FP64, and continues exactly as
  * oeb_<name>_f32 does on an FP64 island holding those values.  An island
  * moves 4 bytes per point instead of 8: p_grad_c 36 B per point instead of
- * 44, nh_p_grad 40 instead of 52.  Storage rounding now applies to the
+ * 44, nh_p_grad and fastwavesuv 40 instead of 52 and 48.  Storage rounding now applies to the
  * deviation only: for a base near the level's mean pressure or geopotential
  * the deviation is some 1e-2 of the value, so the stored field carries
  * about 1e-9 relative error instead of 6e-8.  The base profile may be any
  * per-level values (0 gives the FP32 array itself). */
+int oeb_fastwavesuv_f32d(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
+                         const oeb_s3f *vin, const oeb_s3f *utens, const oeb_s3f *vtens, const oeb_s3f *wgtfac,
+                         const oeb_s3f *ppuv, const oeb_s1 *ppuv0, const oeb_s3f *hhl, const oeb_s1 *hhl0,
+                         const oeb_s3f *rho, const oeb_s1f *fx, const oeb_s3f *ppgk, const oeb_s3f *ppgc,
+                         const oeb_s3f *ppgu, const oeb_s3f *ppgv, float edadlat, float dt, void *stream);
 int oeb_p_grad_c_f32d(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout, const oeb_s3f *uin,
                       const oeb_s3f *vin, const oeb_s3f *rdxc, const oeb_s3f *rdyc, const oeb_s3f *delpc,
                       const oeb_s3f *gz, const oeb_s1 *gz0, const oeb_s3f *pkc, const oeb_s1 *pkc0,

@@ -5,11 +5,13 @@
 // level k+1 to level k of the next step, and so are ppuv and hhl, so each plane
 // of wgtfac, ppuv and hhl is loaded once per chunk.
 // Reach: i, j in [0, +1]; k in [-1, +1] (the k-halo planes -1 and nz are read).
-// FP32 form: ppuv and hhl stay FP64 arrays (pressure ~1e5 Pa and heights lose
-// their small differences to cancellation in FP32), and ppgk, ppgc and
-// ppgu/ppgv are formed in FP64; ppgu/ppgv are rounded once to float.
+// FP32 forms: ppuv and hhl are FP64 values (FP64 arrays in
+// oeb_fastwavesuv_f32, FP32 deviations from FP64 per-level profiles in
+// oeb_fastwavesuv_f32d; pressure ~1e5 Pa and heights lose their small
+// differences to cancellation in FP32), and ppgk, ppgc and ppgu/ppgv are
+// formed in FP64; ppgu/ppgv are rounded once to float.
 // Algorithmic traffic: 8 inputs + 2 outputs = 80 B per point (FP64); 48 B in
-// the FP32 form (ppuv and hhl 8 B, the rest 4 B).
+// the FP32 form (ppuv and hhl 8 B, the rest 4 B), 40 B with deviations.
 #include "oeb_common.hpp"
 
 namespace oeb {
@@ -44,18 +46,18 @@ __device__ __forceinline__ void fw_point(double ppC, double ppE, double ppN, con
   vo = vin + dt * (vtens - T(ppgv) * (T(2.0) * edadlat / (rN + rC)));
 }
 
-template <class T>
+template <class T, class IS>
 struct FwArgs {
   T *uout, *vout;
   const T *uin, *vin, *utens, *vtens, *wgtfac, *rho, *fx;
-  const double *ppuv, *hhl;
+  IS ppuv, hhl;
   int fx_stride;
   T edadlat, dt;
 };
 
 // Logical block `id` of a launch: one 64 x 4 tile of columns over a k-chunk.
-template <class T>
-__device__ __forceinline__ void fastwavesuv_body(const FwArgs<T> &a, const Geo &g, const Tiles &t, int id) {
+template <class T, class IS>
+__device__ __forceinline__ void fastwavesuv_body(const FwArgs<T, IS> &a, const Geo &g, const Tiles &t, int id) {
   const T edadlat = a.edadlat, dt = a.dt;
   const TileCoord c = tile_coord_of(t, id, kTileI, kTileJ, g.nz);
   const int i = c.i0 + int(threadIdx.x), j = c.j0 + int(threadIdx.y);
@@ -65,19 +67,23 @@ __device__ __forceinline__ void fastwavesuv_body(const FwArgs<T> &a, const Geo &
   const T fxj = a.fx[int64_t(j) * a.fx_stride];
   int64_t at = int64_t(c.k0) * sk + (i + j * sj);
   const int64_t dn = at - sk;
-  double ppC = a.ppuv[at], ppE = a.ppuv[at + 1], ppN = a.ppuv[at + sj];
+  const auto p0 = a.ppuv.level(c.k0), pm = a.ppuv.level(c.k0 - 1);
+  const auto h0 = a.hhl.level(c.k0);
+  double ppC = a.ppuv.at(at, p0), ppE = a.ppuv.at(at + 1, p0), ppN = a.ppuv.at(at + sj, p0);
   CEN gk, h;
-  gk.C = ppgk_of(a.wgtfac[at], ppC, a.ppuv[dn]);
-  gk.E = ppgk_of(a.wgtfac[at + 1], ppE, a.ppuv[dn + 1]);
-  gk.N = ppgk_of(a.wgtfac[at + sj], ppN, a.ppuv[dn + sj]);
-  h.C = a.hhl[at], h.E = a.hhl[at + 1], h.N = a.hhl[at + sj];
+  gk.C = ppgk_of(a.wgtfac[at], ppC, a.ppuv.at(dn, pm));
+  gk.E = ppgk_of(a.wgtfac[at + 1], ppE, a.ppuv.at(dn + 1, pm));
+  gk.N = ppgk_of(a.wgtfac[at + sj], ppN, a.ppuv.at(dn + sj, pm));
+  h.C = a.hhl.at(at, h0), h.E = a.hhl.at(at + 1, h0), h.N = a.hhl.at(at + sj, h0);
   for (int k = c.k0; k < c.k1; ++k, at += sk) {
     const int64_t up = at + sk;
-    const double ppC1 = a.ppuv[up], ppE1 = a.ppuv[up + 1], ppN1 = a.ppuv[up + sj];
+    const auto p1 = a.ppuv.level(k + 1);
+    const auto h1 = a.hhl.level(k + 1);
+    const double ppC1 = a.ppuv.at(up, p1), ppE1 = a.ppuv.at(up + 1, p1), ppN1 = a.ppuv.at(up + sj, p1);
     gk.C1 = ppgk_of(a.wgtfac[up], ppC1, ppC);
     gk.E1 = ppgk_of(a.wgtfac[up + 1], ppE1, ppE);
     gk.N1 = ppgk_of(a.wgtfac[up + sj], ppN1, ppN);
-    h.C1 = a.hhl[up], h.E1 = a.hhl[up + 1], h.N1 = a.hhl[up + sj];
+    h.C1 = a.hhl.at(up, h1), h.E1 = a.hhl.at(up + 1, h1), h.N1 = a.hhl.at(up + sj, h1);
     T uo, vo;
     fw_point(ppC, ppE, ppN, gk, h, a.rho[at], a.rho[at + 1], a.rho[at + sj], ld_nt(a.uin + at), ld_nt(a.vin + at),
              ld_nt(a.utens + at), ld_nt(a.vtens + at), fxj, edadlat, dt, uo, vo);
@@ -89,21 +95,21 @@ __device__ __forceinline__ void fastwavesuv_body(const FwArgs<T> &a, const Geo &
   }
 }
 
-template <class T>
-__global__ void __launch_bounds__(kTileI *kTileJ) fastwavesuv_kernel(FwArgs<T> a, Geo g, Tiles t) {
+template <class T, class IS>
+__global__ void __launch_bounds__(kTileI *kTileJ) fastwavesuv_kernel(FwArgs<T, IS> a, Geo g, Tiles t) {
   fastwavesuv_body(a, g, t, int(blockIdx.x));
 }
 
-template <class T>
+template <class T, class IS>
 hipError_t fastwavesuv_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
-  return launch_batch<FwArgs<T>, fastwavesuv_body<T>>(tab, n, total, s, dim3(kTileI, kTileJ));
+  return launch_batch<FwArgs<T, IS>, fastwavesuv_body<T, IS>>(tab, n, total, s, dim3(kTileI, kTileJ));
 }
-template <class T>
-const BatchKind kFastwavesuvBatch{"fastwavesuv", fastwavesuv_batch<T>};
+template <class T, class IS>
+const BatchKind kFastwavesuvBatch{"fastwavesuv", fastwavesuv_batch<T, IS>};
 
 // Two adjacent columns (i, i+1) per lane, i even: the element-type fields
-// move as pairs (16 bytes in FP64, 8 in FP32), the FP64 ppuv and hhl as
-// 16-byte pairs.  The east neighbour of column i is column i+1 of the pair;
+// move as pairs (16 bytes in FP64, 8 in FP32), ppuv and hhl as pairs of
+// their storage (16-byte FP64 or 8-byte FP32 deviation pairs).  The east neighbour of column i is column i+1 of the pair;
 // that of column i+1 is a single load at i+2.  Suffix P: the pair at (i, i+1);
 // Q: the pair at row j+1; X: column i+2 at row j.
 template <class T>
@@ -128,8 +134,8 @@ __device__ __forceinline__ void stp_nt(T *p, T x, T y) {
   stv_nt<T, 2>(p, v);
 }
 
-template <class T>
-__device__ __forceinline__ void fastwavesuv_pair_body(const FwArgs<T> &a, const Geo &g, const Tiles &t, int id) {
+template <class T, class IS>
+__device__ __forceinline__ void fastwavesuv_pair_body(const FwArgs<T, IS> &a, const Geo &g, const Tiles &t, int id) {
   const T edadlat = a.edadlat, dt = a.dt;
   const TileCoord c = tile_coord_of(t, id, kPairI, kTileJ, g.nz);
   const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
@@ -139,30 +145,34 @@ __device__ __forceinline__ void fastwavesuv_pair_body(const FwArgs<T> &a, const 
   const T fxj = a.fx[int64_t(j) * a.fx_stride];
   int64_t at = int64_t(c.k0) * sk + (i + j * sj);
   const int64_t dn = at - sk;
-  d2 ppP = ld2(a.ppuv + at), ppQ = ld2(a.ppuv + at + sj);
-  double ppX = a.ppuv[at + 2];
+  const auto p0 = a.ppuv.level(c.k0), pm = a.ppuv.level(c.k0 - 1);
+  const auto h0 = a.hhl.level(c.k0);
+  d2 ppP = a.ppuv.pair(at, p0), ppQ = a.ppuv.pair(at + sj, p0);
+  double ppX = a.ppuv.at(at + 2, p0);
   d2 gkP, gkQ;
   double gkX;
   {
     const Pr<T> wP = ldp(a.wgtfac + at), wQ = ldp(a.wgtfac + at + sj);
-    const d2 mP = ld2(a.ppuv + dn), mQ = ld2(a.ppuv + dn + sj);
+    const d2 mP = a.ppuv.pair(dn, pm), mQ = a.ppuv.pair(dn + sj, pm);
     gkP = d2{ppgk_of(wP.x, ppP.x, mP.x), ppgk_of(wP.y, ppP.y, mP.y)};
     gkQ = d2{ppgk_of(wQ.x, ppQ.x, mQ.x), ppgk_of(wQ.y, ppQ.y, mQ.y)};
-    gkX = ppgk_of(a.wgtfac[at + 2], ppX, a.ppuv[dn + 2]);
+    gkX = ppgk_of(a.wgtfac[at + 2], ppX, a.ppuv.at(dn + 2, pm));
   }
-  d2 hP = ld2(a.hhl + at), hQ = ld2(a.hhl + at + sj);
-  double hX = a.hhl[at + 2];
+  d2 hP = a.hhl.pair(at, h0), hQ = a.hhl.pair(at + sj, h0);
+  double hX = a.hhl.at(at + 2, h0);
   for (int k = c.k0; k < c.k1; ++k, at += sk) {
     const int64_t up = at + sk;
-    const d2 ppP1 = ld2(a.ppuv + up), ppQ1 = ld2(a.ppuv + up + sj);
-    const double ppX1 = a.ppuv[up + 2];
+    const auto p1 = a.ppuv.level(k + 1);
+    const auto h1 = a.hhl.level(k + 1);
+    const d2 ppP1 = a.ppuv.pair(up, p1), ppQ1 = a.ppuv.pair(up + sj, p1);
+    const double ppX1 = a.ppuv.at(up + 2, p1);
     const Pr<T> wP = ldp(a.wgtfac + up), wQ = ldp(a.wgtfac + up + sj);
     const T wX = a.wgtfac[up + 2];
     const d2 gkP1{ppgk_of(wP.x, ppP1.x, ppP.x), ppgk_of(wP.y, ppP1.y, ppP.y)};
     const d2 gkQ1{ppgk_of(wQ.x, ppQ1.x, ppQ.x), ppgk_of(wQ.y, ppQ1.y, ppQ.y)};
     const double gkX1 = ppgk_of(wX, ppX1, ppX);
-    const d2 hP1 = ld2(a.hhl + up), hQ1 = ld2(a.hhl + up + sj);
-    const double hX1 = a.hhl[up + 2];
+    const d2 hP1 = a.hhl.pair(up, h1), hQ1 = a.hhl.pair(up + sj, h1);
+    const double hX1 = a.hhl.at(up + 2, h1);
     const Pr<T> rP = ldp(a.rho + at), rQ = ldp(a.rho + at + sj);
     const T rX = a.rho[at + 2];
     const Pr<T> ui = ldp_nt(a.uin + at), vi = ldp_nt(a.vin + at);
@@ -181,23 +191,28 @@ __device__ __forceinline__ void fastwavesuv_pair_body(const FwArgs<T> &a, const 
   }
 }
 
-template <class T>
-__global__ void __launch_bounds__(kTileI *kTileJ) fastwavesuv_pair_kernel(FwArgs<T> a, Geo g, Tiles t) {
+template <class T, class IS>
+__global__ void __launch_bounds__(kTileI *kTileJ) fastwavesuv_pair_kernel(FwArgs<T, IS> a, Geo g, Tiles t) {
   fastwavesuv_pair_body(a, g, t, int(blockIdx.x));
 }
 
-template <class T>
+template <class T, class IS>
 hipError_t fastwavesuv_pair_batch(const BatchMember *tab, int n, int total, hipStream_t s) {
-  return launch_batch<FwArgs<T>, fastwavesuv_pair_body<T>>(tab, n, total, s, dim3(kTileI, kTileJ));
+  return launch_batch<FwArgs<T, IS>, fastwavesuv_pair_body<T, IS>>(tab, n, total, s, dim3(kTileI, kTileJ));
 }
-template <class T>
-const BatchKind kFastwavesuvPairBatch{"fastwavesuv", fastwavesuv_pair_batch<T>};
+template <class T, class IS>
+const BatchKind kFastwavesuvPairBatch{"fastwavesuv", fastwavesuv_pair_batch<T, IS>};
 
-template <class T, class S3, class S1>
+// Validates and launches.  ppuv and hhl: FP64 arrays (ppuv0, hhl0 NULL), or
+// FP32 deviation arrays with FP64 base profiles over their levels, ppuv0
+// over [-1, nz + 1) and hhl0 over [0, nz + 1).
+template <class T, class S3, class S1, class SI>
 int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *uin, const S3 *vin,
-                    const S3 *utens, const S3 *vtens, const S3 *wgtfac, const oeb_s3 *ppuv, const oeb_s3 *hhl,
-                    const S3 *rho, const S1 *fx, T edadlat, T dt, void *stream) {
-  const char *K = "fastwavesuv";
+                    const S3 *utens, const S3 *vtens, const S3 *wgtfac, const SI *ppuv, const oeb_s1 *ppuv0,
+                    const SI *hhl, const oeb_s1 *hhl0, const S3 *rho, const S1 *fx, T edadlat, T dt, void *stream) {
+  constexpr bool kDev = std::is_same<SI, oeb_s3f>::value;
+  using IS = typename std::conditional<kDev, IslandDev, IslandF64>::type;
+  const char *K = "fastwavesuv";  // one name for both storages: knobs and messages
   OEB_TRY(check_domain(K, d));
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
   Geo g{int(nx), int(ny), int(nz), 0, 0};
@@ -205,7 +220,7 @@ int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S
   const Box r_h{0, nx + 1, 0, ny + 1, 0, nz};
   const Box r_w{0, nx + 1, 0, ny + 1, 0, nz + 1}, r_p{0, nx + 1, 0, ny + 1, -1, nz + 1};
   const T *p_uo, *p_vo;
-  FwArgs<T> a;
+  FwArgs<T, IS> a;
   a.fx_stride = 1;
   OEB_TRY(check3(K, "uout", uout, r0, g, true, &p_uo));
   OEB_TRY(check3(K, "vout", vout, r0, g, false, &p_vo));
@@ -214,11 +229,19 @@ int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S
   OEB_TRY(check3(K, "utens", utens, r0, g, false, &a.utens));
   OEB_TRY(check3(K, "vtens", vtens, r0, g, false, &a.vtens));
   OEB_TRY(check3(K, "wgtfac", wgtfac, r_w, g, false, &a.wgtfac));
-  OEB_TRY(check3(K, "ppuv", ppuv, r_p, g, false, &a.ppuv));
-  OEB_TRY(check3(K, "hhl", hhl, r_w, g, false, &a.hhl));
+  OEB_TRY(check3(K, "ppuv", ppuv, r_p, g, false, &a.ppuv.p));
+  OEB_TRY(check3(K, "hhl", hhl, r_w, g, false, &a.hhl.p));
   OEB_TRY(check3(K, "rho", rho, r_h, g, false, &a.rho));
   OEB_TRY(check1(K, "fx", fx, 0, ny, &a.fx, &a.fx_stride));
   AliasSet as;
+  if constexpr (kDev) {
+    OEB_TRY(check1(K, "ppuv0", ppuv0, -1, nz + 1, &a.ppuv.base, &a.ppuv.bs));
+    OEB_TRY(check1(K, "hhl0", hhl0, 0, nz + 1, &a.hhl.base, &a.hhl.bs));
+    as.add1("ppuv0", a.ppuv.base, -1, nz + 1, a.ppuv.bs);
+    as.add1("hhl0", a.hhl.base, 0, nz + 1, a.hhl.bs);
+  } else {
+    (void)ppuv0, (void)hhl0;
+  }
   as.add("uout", p_uo, r0, g, true);
   as.add("vout", p_vo, r0, g, true);
   as.add("uin", a.uin, r0, g, false);
@@ -226,8 +249,8 @@ int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S
   as.add("utens", a.utens, r0, g, false);
   as.add("vtens", a.vtens, r0, g, false);
   as.add("wgtfac", a.wgtfac, r_w, g, false);
-  as.add("ppuv", a.ppuv, r_p, g, false);
-  as.add("hhl", a.hhl, r_w, g, false);
+  as.add("ppuv", a.ppuv.p, r_p, g, false);
+  as.add("hhl", a.hhl.p, r_w, g, false);
   as.add("rho", a.rho, r_h, g, false);
   as.add1("fx", a.fx, 0, ny, a.fx_stride);
   OEB_TRY(as.check(K));
@@ -242,18 +265,18 @@ int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S
   // at 512x512x80 / 256x256x80, FP64 the same (order knob 0 for row order).
   const int xo = knob(K, sizeof(T), d, KNOB_ORDER, 2);
   if ((knob(K, sizeof(T), d, KNOB_VECTOR, true) != 0) && nx % 2 == 0 &&
-      pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.utens, a.vtens, a.wgtfac, a.ppuv, a.hhl, a.rho})) {
+      pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.utens, a.vtens, a.wgtfac, a.ppuv.ptr(), a.hhl.ptr(), a.rho})) {
     const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc, xo);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
-    if (batch_record(kFastwavesuvPairBatch<T>, a, g, t, t.n, as)) return OEB_OK;
-    hipLaunchKernelGGL(fastwavesuv_pair_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g,
-                       t);
+    if (batch_record(kFastwavesuvPairBatch<T, IS>, a, g, t, t.n, as)) return OEB_OK;
+    hipLaunchKernelGGL((fastwavesuv_pair_kernel<T, IS>), dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a,
+                       g, t);
     return launch_status(K);
   }
   const Tiles t = make_tiles(nx, ny, int(nz), kTileI, kTileJ, kc, xo);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTileI, kTileJ));
-  if (batch_record(kFastwavesuvBatch<T>, a, g, t, t.n, as)) return OEB_OK;
-  hipLaunchKernelGGL(fastwavesuv_kernel<T>, dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
+  if (batch_record(kFastwavesuvBatch<T, IS>, a, g, t, t.n, as)) return OEB_OK;
+  hipLaunchKernelGGL((fastwavesuv_kernel<T, IS>), dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a, g, t);
   return launch_status(K);
 }
 
@@ -267,8 +290,8 @@ extern "C" int oeb_fastwavesuv_f64(const oeb_domain *d, const oeb_s3 *uout, cons
                                    const oeb_s3 *ppgk, const oeb_s3 *ppgc, const oeb_s3 *ppgu, const oeb_s3 *ppgv,
                                    double edadlat, double dt, void *stream) {
   (void)ppgk, (void)ppgc, (void)ppgu, (void)ppgv;
-  return oeb::fastwavesuv_run<double>(d, uout, vout, uin, vin, utens, vtens, wgtfac, ppuv, hhl, rho, fx, edadlat, dt,
-                                      stream);
+  return oeb::fastwavesuv_run<double>(d, uout, vout, uin, vin, utens, vtens, wgtfac, ppuv, nullptr, hhl, nullptr, rho,
+                                      fx, edadlat, dt, stream);
 }
 
 extern "C" int oeb_fastwavesuv_f32(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout,
@@ -277,6 +300,17 @@ extern "C" int oeb_fastwavesuv_f32(const oeb_domain *d, const oeb_s3f *uout, con
                                    const oeb_s3f *rho, const oeb_s1f *fx, const oeb_s3f *ppgk, const oeb_s3f *ppgc,
                                    const oeb_s3f *ppgu, const oeb_s3f *ppgv, float edadlat, float dt, void *stream) {
   (void)ppgk, (void)ppgc, (void)ppgu, (void)ppgv;
-  return oeb::fastwavesuv_run<float>(d, uout, vout, uin, vin, utens, vtens, wgtfac, ppuv, hhl, rho, fx, edadlat, dt,
-                                     stream);
+  return oeb::fastwavesuv_run<float>(d, uout, vout, uin, vin, utens, vtens, wgtfac, ppuv, nullptr, hhl, nullptr, rho,
+                                     fx, edadlat, dt, stream);
+}
+
+extern "C" int oeb_fastwavesuv_f32d(const oeb_domain *d, const oeb_s3f *uout, const oeb_s3f *vout,
+                                    const oeb_s3f *uin, const oeb_s3f *vin, const oeb_s3f *utens,
+                                    const oeb_s3f *vtens, const oeb_s3f *wgtfac, const oeb_s3f *ppuv,
+                                    const oeb_s1 *ppuv0, const oeb_s3f *hhl, const oeb_s1 *hhl0, const oeb_s3f *rho,
+                                    const oeb_s1f *fx, const oeb_s3f *ppgk, const oeb_s3f *ppgc, const oeb_s3f *ppgu,
+                                    const oeb_s3f *ppgv, float edadlat, float dt, void *stream) {
+  (void)ppgk, (void)ppgc, (void)ppgu, (void)ppgv;
+  return oeb::fastwavesuv_run<float>(d, uout, vout, uin, vin, utens, vtens, wgtfac, ppuv, ppuv0, hhl, hhl0, rho, fx,
+                                     edadlat, dt, stream);
 }

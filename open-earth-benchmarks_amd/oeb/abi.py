@@ -100,7 +100,7 @@ assert SIGNATURES_F32.keys() == SIGNATURES.keys()
 
 # oeb_<name>_f32d: oeb_<name>_f32 with each FP64 island ('D') replaced by an
 # FP32 deviation array ('E') and its FP64 per-level base profile ('b').
-SIGNATURES_F32D = {k: SIGNATURES_F32[k].replace("D", "Eb") for k in ("p_grad_c", "nh_p_grad")}
+SIGNATURES_F32D = {k: SIGNATURES_F32[k].replace("D", "Eb") for k in ("fastwavesuv", "p_grad_c", "nh_p_grad")}
 assert all(len(SIGNATURES[k]) == len(SIGNATURES_F32[k]) for k in SIGNATURES)
 
 PRECISIONS = ("f64", "f32")

@@ -87,7 +87,7 @@ ISLANDS = {
 }
 # Stencils with an _f32d entry point, and the islands each takes as FP32
 # deviations from an FP64 per-level base profile (oeb.islands).
-DEVIATION_STENCILS = {"p_grad_c": ("gz", "pkc"), "nh_p_grad": ("gz", "pp", "pk3")}
+DEVIATION_STENCILS = {"fastwavesuv": ("ppuv", "hhl"), "p_grad_c": ("gz", "pkc"), "nh_p_grad": ("gz", "pp", "pk3")}
 
 
 def _a(spec: str) -> tuple[Arg, ...]:

@@ -56,7 +56,7 @@ def call(name: str, arrays: list, scalars: list, domain=None, stream=None, preci
     given.  In the FP32 form the FP64 island arrays of fastwavesuv, p_grad_c
     and nh_p_grad must be float64, every other array float32; the element
     types are checked here, because the C-ABI cannot see them.  Islands
-    given as DeviationIsland (p_grad_c, nh_p_grad) call oeb_<name>_f32d.  Host buffers
+    given as DeviationIsland call oeb_<name>_f32d.  Host buffers
     raise TypeError here (check_device=False hands them to the library,
     whose own check rejects them: the C-ABI validation tests use that)."""
     lib = abi.load()

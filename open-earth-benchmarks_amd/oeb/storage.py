@@ -128,12 +128,12 @@ class Storage1D:
 
 
 class DeviationIsland:
-    """An FP64 island of the FP32 forms (p_grad_c's gz and pkc, nh_p_grad's
-    gz, pp and pk3) stored as FP32 deviations from an FP64 per-level base
-    profile: value(i, j, k) = base(k) + dev(i, j, k), the sum formed in FP64
-    by the oeb_<name>_f32d entry points.  `dev` is a float32 Storage3D on the
-    island's layout, `base` a float64 Storage1D over the levels
-    (Layout1D(nz, hk): level k at index k + hk)."""
+    """An FP64 island of the FP32 forms (fastwavesuv's ppuv and hhl, p_grad_c's
+    gz and pkc, nh_p_grad's gz, pp and pk3) stored as FP32 deviations from an
+    FP64 per-level base profile: value(i, j, k) = base(k) + dev(i, j, k), the
+    sum formed in FP64 by the oeb_<name>_f32d entry points.  `dev` is a
+    float32 Storage3D on the island's layout, `base` a float64 Storage1D over
+    the levels (Layout1D(nz, hk): level k at index k + hk)."""
 
     precision = "f32d"
 

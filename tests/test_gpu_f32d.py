@@ -1,5 +1,5 @@
-"""GPU parity of the deviation-island FP32 form (oeb_p_grad_c_f32d,
-oeb_nh_p_grad_f32d) through the C-ABI on cuda:0.
+"""GPU parity of the deviation-island FP32 form (oeb_fastwavesuv_f32d,
+oeb_p_grad_c_f32d, oeb_nh_p_grad_f32d) through the C-ABI on cuda:0.
 
 Bar: bit for bit equal to the oracle's FP32 form (oracle_<name>_f32) run on
 the islands the entry point reads, base(k) + dev rebuilt in FP64 on the host
@@ -88,12 +88,14 @@ def test_f32d_against_f64(name, n):
 @pytest.mark.parametrize("shape", [(67, 9, 3), (130, 36, 4)], ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("name", sorted(DEVIATION_STENCILS))
 def test_f32d_poisoned_halo(name, shape, pairs, knobs):
-    """Every deviation and input point outside the stencil's read box, and the
-    base profiles' level -1 (outside [0, nz + 1)), set to NaN: the outputs
-    still equal the FP32 oracle on the unpoisoned inputs bit for bit."""
+    """Every deviation and input point outside the stencil's read box, and
+    each base profile's level -1 where the island's box starts at level 0,
+    set to NaN: the outputs still equal the FP32 oracle on the unpoisoned
+    inputs bit for bit."""
     import torch
 
-    from reach import BOXES, keep_mask
+    from oeb.layout import Layout1D
+    from reach import BOXES, BOXES_1D, keep_mask, keep_mask_1d
 
     knobs.set(name, "vector", pairs)
     L, host = inputs(name, *shape, "baseline", seed=43)
@@ -108,8 +110,13 @@ def test_f32d_poisoned_halo(name, shape, pairs, knobs):
             rebuilt[a.name] = reconstruct(L, d, base)
             L.view(d)[~keep_mask(L, BOXES[name][a.name])] = np.nan
             base = base.copy()
-            base[0] = np.nan  # level -1
+            if BOXES[name][a.name][4] >= 0:
+                base[0] = np.nan  # level -1, outside the island's levels
             arrays[a.name] = DeviationIsland(Storage3D.to_device(L, d), Storage1D.to_device(level_layout(L), base))
+        elif a.kind == "in1":
+            h = host32[a.name].copy()
+            h[~keep_mask_1d(L, BOXES_1D[name][a.name])] = np.nan
+            arrays[a.name] = Storage1D.to_device(Layout1D(L.ny, L.h), h)
         else:
             h = host32[a.name].copy()
             if a.kind == "in":

@@ -62,9 +62,9 @@ def _f32_error(name, L, host, islands):
 @pytest.mark.parametrize("name", sorted(DEVIATION_STENCILS))
 def test_deviation_islands_keep_the_fp64_island_accuracy(name):
     """With deviations about the level mean the FP32 form stays within 1e-6
-    of the FP64 oracle, as with FP64 islands (48x40x12, measured 1.3e-7 and
+    of the FP64 oracle, as with FP64 islands (48x40x12, measured 1.2e-7 to
     1.5e-7 for both), while FP32 storage of the island values themselves (a
-    zero base) is off by 1.8e-6 to 2.0e-6: 13-16x more, past the 1e-6 bound,
+    zero base) is off by 1.8e-6 to 2.6e-6: 13-20x more, past the 1e-6 bound,
     from the cancellation in the level and neighbour differences."""
     L, host = inputs(name, 48, 40, 12, "baseline", seed=4)
     e64 = _f32_error(name, L, host, "f64")
