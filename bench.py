@@ -538,7 +538,9 @@ def chain_block(L, forms, seed, steps: int = 10, reps: int = 5):
 
 def ensemble_block(n_members: int, seed: int, steps: int = 10, reps: int = 3):
     """ms per step of the forward chain over an uneven ensemble: per-member
-    launches against batched launches (eager and from a graph)."""
+    launches against batched launches (eager and from a graph), and the
+    batched graph after autotuning (the tuned knobs kept only if the step
+    gets faster with them)."""
     import torch
 
     from oeb.ensemble import draw_shapes, forward_ensemble
@@ -589,11 +591,13 @@ def ensemble_block(n_members: int, seed: int, steps: int = 10, reps: int = 3):
     untuned_graph_ms = batched_ms(batch)
     batch.close()
 
-    # Autotuned k-chunks per (stencil, member shape), then the batch again.
+    # Autotuned knobs per (stencil, member shape), accepted only if the
+    # batched graph step gets faster with them (Ensemble.autotune).
     t1 = time.perf_counter()
-    tuner = Tuner(path=Path(os.environ.get("TMPDIR", "/tmp")) / "oeb_bench_tuning.json")
+    tuner = Tuner(path=Path(os.environ.get("TMPDIR", "/tmp")) / "oeb_bench_tuning.json", reps=5)
     tuner.cache = {}
-    choices = ens.tune(tuner, "f32", s)
+    tuned = ens.autotune(tuner, "f32", s, steps=steps, reps=reps)
+    choices = tuned["choices"]
     tune_s = time.perf_counter() - t1
     batch = ens.record()
     graph_ms = batched_ms(batch)
@@ -614,7 +618,9 @@ def ensemble_block(n_members: int, seed: int, steps: int = 10, reps: int = 3):
         "batched_graph_ms_per_step": round(graph_ms, 3),
         "autotune": {"choices": len(choices), "seconds": round(tune_s, 2),
                      "non_default": sum(1 for v in choices.values() if v["knobs"]),
-                     "knob_histogram": _knob_histogram(choices)},
+                     "knob_histogram": _knob_histogram(choices),
+                     "step_default_ms": round(tuned["default_ms"], 3), "step_tuned_ms": round(tuned["tuned_ms"], 3),
+                     "accepted": tuned["accepted"]},
         "plain_gpts_s": gpts(plain_ms),
         "batched_graph_gpts_s": gpts(graph_ms),
         "speedup_graph_vs_plain": round(plain_ms / graph_ms, 3),

@@ -123,6 +123,53 @@ class Ensemble:
         return Graph(g.value)
 
 
+    def graph_step_ms(self, stream, steps: int = 10, reps: int = 3) -> float:
+        """Median ms per step of the batched step replayed from a graph of
+        `steps` launches (the members' state advances)."""
+        import statistics
+
+        import torch
+
+        batch = self.record()
+        try:
+            graph = self.capture(batch, stream, steps)
+            try:
+                s = torch.cuda.ExternalStream(stream)
+                graph.launch(stream)
+                s.synchronize()
+                out = []
+                for _ in range(reps):
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(s)
+                    graph.launch(stream)
+                    e1.record(s)
+                    s.synchronize()
+                    out.append(e0.elapsed_time(e1) / steps)
+                return statistics.median(out)
+            finally:
+                graph.close()
+        finally:
+            batch.close()
+
+    def autotune(self, tuner, precision: str, stream, steps: int = 10, reps: int = 3, margin: float = 0.02) -> dict:
+        """tune(), then a check in context: the batched step, replayed from a
+        graph, is timed with the built-in defaults and with the tuned table,
+        and the tuned choices stay only if the step is at least `margin`
+        faster; otherwise they are removed from the library's table.  Each
+        candidate is timed alone (with the MALL flushed), while the step runs
+        every member's blocks in one launch per op, where a choice that wins
+        alone can lose."""
+        from .tune import set_knobs
+
+        default_ms = self.graph_step_ms(stream, steps, reps)
+        choices = self.tune(tuner, precision, stream)
+        tuned_ms = self.graph_step_ms(stream, steps, reps)
+        accepted = tuned_ms < (1.0 - margin) * default_ms
+        if not accepted:
+            for stencil, domain in choices:
+                set_knobs(stencil, precision, domain, {})
+        return {"choices": choices, "default_ms": default_ms, "tuned_ms": tuned_ms, "accepted": accepted}
+
     def tune(self, tuner, precision: str, stream=None) -> dict:
         """Runs the autotuner (oeb.tune.Tuner) for every stencil of the
         program on every member shape (each (stencil, shape) once).  Outputs

@@ -241,3 +241,48 @@ def test_ensemble_48_member_draw():
             got = plain.members[m].fields[n].buf.cpu().numpy()
             assert np.array_equal(L.interior(got).view(np.uint8), L.interior(ref[n]).view(np.uint8)), \
                 f"member {m} {shapes[m]}: {n} differs from the host chain"
+
+
+@pytest.mark.gpu
+def test_autotune_accepts_only_a_faster_step(tmp_path):
+    """Ensemble.autotune: every (stencil, member shape) gets a choice; the
+    tuned knobs stay in the library's table only when the batched graph step
+    measured faster with them, and are all removed otherwise; the batched
+    step still equals the plain one either way."""
+    import torch
+
+    from oeb import tune
+
+    hosts = _host_members("f32", seed=5)
+    ens = _ensemble(hosts)
+    side = torch.cuda.Stream()
+    tuner = tune.Tuner(path=tmp_path / "tuning.json", reps=2)
+    print("[gpu] ensemble autotune", flush=True)
+    tune.clear()
+    try:
+        res = ens.autotune(tuner, "f32", side.cuda_stream, steps=2, reps=2)
+        assert set(res) == {"choices", "default_ms", "tuned_ms", "accepted"}
+        assert len(res["choices"]) == 4 * len(SHAPES)
+        assert res["default_ms"] > 0 and res["tuned_ms"] > 0
+        assert res["accepted"] == (res["tuned_ms"] < 0.98 * res["default_ms"])
+        for (stencil, dom), v in res["choices"].items():
+            for knob in tune.KNOBS:
+                got = tune.get_knob(stencil, "f32", dom, knob)
+                want = v["knobs"].get(knob, -1) if res["accepted"] else -1
+                assert got == want, (stencil, dom, knob, got, want)
+        # with the table autotune left, a batched step still equals a plain one
+        torch.cuda.synchronize()
+        ens2, plain2 = _ensemble(hosts), _ensemble(hosts)
+        b = ens2.record()
+        try:
+            for _ in range(2):
+                b.launch()
+                plain2.step_plain()
+            torch.cuda.synchronize()
+        finally:
+            b.close()
+        for sp, sb in zip(_states(plain2), _states(ens2)):
+            for n in STATE:
+                assert np.array_equal(sp[n].view(np.uint8), sb[n].view(np.uint8)), n
+    finally:
+        tune.clear()
