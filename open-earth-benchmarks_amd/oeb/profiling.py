@@ -1,8 +1,12 @@
 """Summaries of rocprofv3 output databases (the SQLite ``*_results.db`` that
 rocprofv3 writes) as short markdown for profiles/.
 
-  python -m oeb.profiling stats  <results.db> --command "<cmd>" [--out file.md] [--top N] [--skip-first W]
-      per (kernel, grid size) dispatch count, average / min / max duration and resources
+  python -m oeb.profiling stats  <results.db> --command "<cmd>" [--out file.md] [--top N] [--sort avg|total]
+          [--skip-first W] [--first F] [--match REGEX] [--dispatches A:B]
+      per (kernel, grid size) dispatch count, average / min / max duration, the
+      average over each row's dispatches W+1..F, and resources
+  python -m oeb.profiling counters <db> [<db> ...] --command "<cmd>" [--match REGEX]
+      per (kernel, grid size) mean of every counter of one or more --pmc passes
   python -m oeb.profiling pmc    --fetch <db> --write <db> --copy-bytes N --shape 256x256x80
       --command-fetch "<cmd>" --command-write "<cmd>" [--out-md f.md] [--out-json f.json]
       HBM bytes per launch from FETCH_SIZE / WRITE_SIZE (KiB per dispatch),
@@ -53,11 +57,19 @@ def _rows(db: Path, sql: str):
         con.close()
 
 
-def kernel_stats(db: Path) -> dict:
+def kernel_stats(db: Path, dispatches: tuple[int, int] | None = None) -> dict:
     """Dispatches per (kernel, grid size): durations (us, in dispatch order)
-    and resources.  One kernel launched on two domain sizes gives two rows."""
-    rows = _rows(db, "select name, duration, vgpr_count, accum_vgpr_count, sgpr_count, lds_size, scratch_size, "
-                     "grid_x, grid_y, grid_z, workgroup_x, workgroup_y, workgroup_z from kernels")
+    and resources.  One kernel launched on two domain sizes gives two rows.
+    `dispatches` = (a, b) keeps only the process's dispatches a..b-1 in start
+    order (all kernels counted)."""
+    cols = ("name, duration, vgpr_count, accum_vgpr_count, sgpr_count, lds_size, scratch_size, "
+            "grid_x, grid_y, grid_z, workgroup_x, workgroup_y, workgroup_z")
+    try:
+        rows = _rows(db, f"select {cols} from kernels order by start")
+    except sqlite3.OperationalError:  # no start column: table order
+        rows = _rows(db, f"select {cols} from kernels")
+    if dispatches is not None:
+        rows = rows[dispatches[0]:dispatches[1]]
     out = defaultdict(lambda: {"durations": []})
     for (name, dur, vgpr, agpr, sgpr, lds, scratch, gx, gy, gz, wx, wy, wz) in rows:
         k = out[(short_name(name), int(gx) * int(gy) * int(gz))]
@@ -67,10 +79,10 @@ def kernel_stats(db: Path) -> dict:
 
 
 def stats_markdown(db: Path, command: str, top: int = 0, skip_first: int = 0, first: int = 0,
-                   match: str = "") -> str:
+                   match: str = "", dispatches: tuple[int, int] | None = None, sort: str = "avg") -> str:
     import re
 
-    st = kernel_stats(db)
+    st = kernel_stats(db, dispatches)
     if match:
         st = {k: v for k, v in st.items() if re.search(match, k[0])}
     lines = [
@@ -82,7 +94,10 @@ def stats_markdown(db: Path, command: str, top: int = 0, skip_first: int = 0, fi
         command,
         "```",
         "",
-        "One row per (kernel, grid size).  avg / min / max over all dispatches; "
+        "One row per (kernel, grid size)"
+        + (f", of the process's dispatches {dispatches[0]}..{dispatches[1] - 1} in start order" if dispatches else "")
+        + (f", the {top} with the largest " + ("total" if sort == "total" else "average") + " time" if top else "")
+        + ".  avg / min / max over all dispatches; "
         + (f"'window avg' is the mean of each row's dispatches {skip_first + 1}..{first or 'last'} "
            f"(in dispatch order)." if skip_first or first else ""),
         "",
@@ -91,7 +106,8 @@ def stats_markdown(db: Path, command: str, top: int = 0, skip_first: int = 0, fi
         + "min us | max us | VGPR | SGPR | LDS B | scratch B | workgroup |",
         "|---|---|---|---|" + ("---|" if skip_first or first else "") + "---|---|---|---|---|---|---|",
     ]
-    rows = sorted(st.items(), key=lambda kv: -statistics.mean(kv[1]["durations"]))
+    key = (lambda kv: -sum(kv[1]["durations"])) if sort == "total" else (lambda kv: -statistics.mean(kv[1]["durations"]))
+    rows = sorted(st.items(), key=key)
     if top:
         rows = rows[:top]
     for (name, grid), k in rows:
@@ -165,6 +181,8 @@ def main(argv=None) -> int:
     s.add_argument("--skip-first", type=int, default=0, help="window: leave out each row's first N dispatches")
     s.add_argument("--first", type=int, default=0, help="window: end after each row's Nth dispatch")
     s.add_argument("--match", default="", help="only kernels whose short name matches this regex")
+    s.add_argument("--dispatches", default="", help="A:B, only the process's dispatches A..B-1 in start order")
+    s.add_argument("--sort", choices=("avg", "total"), default="avg", help="row order (and --top) by average or total time")
     c = sub.add_parser("counters")
     c.add_argument("dbs", nargs="+")
     c.add_argument("--command", required=True)
@@ -188,7 +206,8 @@ def main(argv=None) -> int:
         print(md)
         return 0
     if a.cmd == "stats":
-        md = stats_markdown(Path(a.db), a.command, a.top, a.skip_first, a.first, a.match)
+        disp = tuple(int(x) for x in a.dispatches.split(":")) if a.dispatches else None
+        md = stats_markdown(Path(a.db), a.command, a.top, a.skip_first, a.first, a.match, disp, a.sort)
         if a.out:
             Path(a.out).write_text(md)
         print(md)
