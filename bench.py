@@ -589,6 +589,13 @@ def ensemble_block(n_members: int, seed: int, steps: int = 10, reps: int = 3):
 
     batch_ms = timed(batched)
     untuned_graph_ms = batched_ms(batch)
+    # The persistent form: a grid of 4 workgroups per CU walks each run's block
+    # list with stride gridDim.x (the grid knob of kernel "batch").
+    from oeb.tune import set_knob
+
+    set_knob("batch", None, None, "grid", 4 * 256)
+    persistent_graph_ms = batched_ms(batch)
+    set_knob("batch", None, None, "grid", -1)
     batch.close()
 
     # Autotuned knobs per (stencil, member shape), accepted only if the
@@ -615,6 +622,8 @@ def ensemble_block(n_members: int, seed: int, steps: int = 10, reps: int = 3):
         "plain_ms_per_step": round(plain_ms, 3),
         "batched_ms_per_step": round(batch_ms, 3),
         "batched_graph_untuned_ms_per_step": round(untuned_graph_ms, 3),
+        "batched_graph_persistent_ms_per_step": round(persistent_graph_ms, 3),
+        "persistent_grid": 4 * 256,
         "batched_graph_ms_per_step": round(graph_ms, 3),
         "autotune": {"choices": len(choices), "seconds": round(tune_s, 2),
                      "non_default": sum(1 for v in choices.values() if v["knobs"]),

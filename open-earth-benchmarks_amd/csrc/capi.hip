@@ -1,5 +1,6 @@
 // capi.hip -- status strings, descriptor validation and error reporting
 // shared by every entry point of liboeb.
+#include <atomic>
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
@@ -244,6 +245,36 @@ int launch_status(const char *kname) {
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(OEB_ERR_HIP, std::string(kname) + ": launch failed: " + hipGetErrorString(e));
   return OEB_OK;
+}
+
+int cu_count() {
+  constexpr int kMaxDevices = 64;
+  static std::atomic<int> cached[kMaxDevices];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    (void)hipGetLastError();
+    return 256;
+  }
+  int n = cached[dev].load(std::memory_order_relaxed);
+  if (n > 0) return n;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) {
+    (void)hipGetLastError();
+    n = 256;
+  }
+  cached[dev].store(n, std::memory_order_relaxed);
+  return n;
+}
+
+int balanced_kchunk(int64_t tiles_per_level, int nz, int wg_per_cu, int lo, int hi, int dflt) {
+  const int64_t cap = int64_t(wg_per_cu) * cu_count();
+  int best = dflt;
+  double best_eff = 0.0;
+  for (int kc = hi; kc >= lo; --kc) {
+    const int64_t n = tiles_per_level * ((nz + kc - 1) / kc);
+    const double eff = double(n) / double(((n + cap - 1) / cap) * cap);
+    if (eff > best_eff + 1e-9) best = kc, best_eff = eff;
+  }
+  return best_eff >= 0.9 ? best : dflt;
 }
 
 }  // namespace oeb

@@ -259,8 +259,11 @@ int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S
   a.edadlat = edadlat;
   a.dt = dt;
   // Levels per workgroup: 8, and 4 in the FP32 form, which measured 56 / 203 us against 59 / 204
-  // with 8 at 256x256x80 / 512x512x80 (the ten-stencil step); FP64 measured the same.
-  const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, sizeof(T) == 4 ? 4 : kChunk);
+  // with 8 at 256x256x80 / 512x512x80 (the ten-stencil step).  FP64 takes 3 on levels of at
+  // most 256x256 points: 83.7 us against 85.1-85.9 with 8 at 256x256x80; at 512x512x80 the
+  // chunks 2..10 measured 303-313 us, 8 among the fastest.
+  const int kc_dflt = sizeof(T) == 4 ? 4 : (int64_t(nx) * ny <= 256 * 256 ? 3 : kChunk);
+  const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, kc_dflt);
   // The tiles of a row on one XCD (Tiles::irow): FP32 204 / 58 us against 209 / 60
   // at 512x512x80 / 256x256x80, FP64 the same (order knob 0 for row order).
   const int xo = knob(K, sizeof(T), d, KNOB_ORDER, 2);

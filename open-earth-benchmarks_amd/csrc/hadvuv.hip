@@ -218,12 +218,17 @@ int hadv_run(const char *K, const oeb_domain *d, const S3 *uout, const S3 *vout,
   as.add1("tgrlatda1", p_t1, 0, ny, s3);
   OEB_TRY(as.check(K));
   // Levels per workgroup: 4, and 5 for FP32 hadvuv, which measured 1.12x
-  // faster with 5 at 256x256x80 (the same at 512x512x80).  FP64 takes 8 when
-  // a level holds 256 tiles or more: at 512x512x80 (256 tiles) hadvuv measured
-  // 136 us against 141 and hadvuv5th 157 against 166; at 256x256x80 (64
-  // tiles) 8 levels measured slower than 4.
+  // faster with 5 at 256x256x80 (the same at 512x512x80).  FP64 (1024-lane
+  // workgroups, two per CU) takes the largest chunk of 4..10 levels whose
+  // workgroups fill whole rounds of the CUs (balanced_kchunk): 10 at
+  // 256x256x80 (64 tiles a level: 512 workgroups), where 4 left the third
+  // round half empty (1280 workgroups) and 5 or 10 measured 1.04-1.05x
+  // faster (hadvuv5th 45.7-45.9 us against 47.8-50.6, hadvuv 41.2-41.9 against
+  // 42.9-44.5); 10 at 512x512x80 (157 / 136 us against 159 / 137 for 8).
   const int64_t level_tiles = ((nx + kTI - 1) / kTI) * ((ny + kTJ - 1) / kTJ);
-  const int kc_dflt = sizeof(T) == 4 ? (ORDER == 3 ? 5 : kChunk) : (level_tiles >= 256 ? 8 : kChunk);
+  const int kc_dflt = sizeof(T) == 4 ? (ORDER == 3 ? 5 : kChunk)
+                                     : balanced_kchunk(level_tiles, int(nz), 2, 4, 10,
+                                                       level_tiles >= 256 ? 8 : kChunk);
   // The tiles of a row on one XCD (Tiles::irow): FP64 hadvuv 141 us against
   // 145 at 512x512x80, hadvuv5th 1% faster, the same or better at 256x256x80.
   const int order = knob(K, sizeof(T), d, KNOB_ORDER, 2);

@@ -542,6 +542,17 @@ struct AliasSet {
 // Converts the state of the last launch into a status.
 int launch_status(const char *kname);
 
+// Compute units of the current device (queried once per device).
+int cu_count();
+
+// Levels per workgroup in [lo, hi] for a tile launch of tiles_per_level
+// workgroups per level chunk, wg_per_cu of which fit one CU at a time: the
+// largest whose workgroup count fills the last round of the device's CUs
+// best (ties to the larger chunk); dflt when none fills 90% of it.  A launch
+// whose last round is half empty leaves half the CUs idle for a whole
+// workgroup's time.
+int balanced_kchunk(int64_t tiles_per_level, int nz, int wg_per_cu, int lo, int hi, int dflt);
+
 // Fails with OEB_ERR_DOMAIN when a launch would need more workgroups than one
 // grid dimension holds.
 int check_tiles(const char *kname, const Tiles &t, int64_t ni, int64_t nj, int nz, int tile_i, int tile_j);

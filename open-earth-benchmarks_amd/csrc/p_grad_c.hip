@@ -160,7 +160,9 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
   OEB_TRY(as.check(K));
   a.uout = const_cast<T *>(p_uo);
   a.vout = const_cast<T *>(p_vo);
-  const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, kChunk);
+  // Levels per workgroup: 4, and 3 in the FP64 form, which measured 69.6 / 262.0 us against
+  // 70.9-72.0 / 266.9-267.2 with 4 at 256x256x80 / 512x512x80 (the ten-stencil step).
+  const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, sizeof(T) == 8 ? 3 : kChunk);
   // The tiles of a row on one XCD (Tiles::irow): FP32 184 / 52 us against 189 / 53
   // at 512x512x80 / 256x256x80, FP64 the same (order knob 0 for row order).
   const int xo = knob(K, sizeof(T), d, KNOB_ORDER, 2);
