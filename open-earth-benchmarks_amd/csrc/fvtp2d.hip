@@ -38,21 +38,54 @@ constexpr int kNT = 256;  // threads per workgroup
 constexpr int kTJdefault = 8;
 constexpr int kChunk = 1;
 
-// PPM flux at face m between cells m-1 and m along one axis, given
-// f(m-3) .. f(m+2) and the Courant number c at m (kernels/fvtp2d.h:12, 20-34).
+// PPM along one axis (kernels/fvtp2d.h:12, 20-34), split into its edge, cell
+// and face parts so that a march along the axis can keep the parts it shares
+// with the next face.
+//   al(m)   edge value between cells m-1 and m, from f(m-2) .. f(m+1)
+//   cell m  almq = al(m) - f(m), br = al(m+1) - f(m), b0 = almq + br, and
+//           ext = almq * br < 0 (smt5 of the cell)
+//   face m  from cells m-1 and m and the Courant number c at m
+// Each value is the reference's expression, so it has the reference's bits.
+template <class T>
+__device__ __forceinline__ T ppm_al(T fm2, T fm1, T f0, T fp1, T p1, T p2) {
+  return ((p1 * (fm1 + f0)) + (p2 * (fm2 + fp1)));
+}
+
+template <class T>
+struct PpmCell {
+  T almq, br, b0;
+  bool ext;
+};
+
+template <class T>
+__device__ __forceinline__ PpmCell<T> ppm_cell(T f, T al_lo, T al_hi) {
+  PpmCell<T> c;
+  c.almq = (al_lo - f);
+  c.br = (al_hi - f);
+  c.b0 = (c.almq + c.br);
+  c.ext = (c.almq * c.br) < T(0);
+  return c;
+}
+
+// Face flux from cell m-1 (lo, value flo) and cell m (hi, value fhi).  The
+// reference's smt = smt5(m-1) + smt5(m) * (smt5(m-1) == 0 ? 1 : 0) of two
+// values in {0, 1} is 1 exactly when either cell is an extremum.
+template <class T>
+__device__ __forceinline__ T ppm_flux(const PpmCell<T> &lo, T flo, const PpmCell<T> &hi, T fhi, T c) {
+  const T smt = (lo.ext || hi.ext) ? T(1) : T(0);
+  const T up = (flo + (((T(1) - c) * (lo.br - (c * lo.b0))) * smt));
+  const T dn = (fhi + (((T(1) + c) * (hi.almq + (c * hi.b0))) * smt));
+  return (c > T(0)) ? up : dn;
+}
+
+// PPM flux at face m between cells m-1 and m, given f(m-3) .. f(m+2) and the
+// Courant number c at m.
 template <class T>
 __device__ __forceinline__ T ppm_face(T fm3, T fm2, T fm1, T f0, T fp1, T fp2, T c, T p1, T p2) {
-  const T al_m = ((p1 * (fm2 + fm1)) + (p2 * (fm3 + f0)));  // al(m-1)
-  const T al_0 = ((p1 * (fm1 + f0)) + (p2 * (fm2 + fp1)));  // al(m)
-  const T al_p = ((p1 * (f0 + fp1)) + (p2 * (fm1 + fp2)));  // al(m+1)
-  const T almq_m = (al_m - fm1), br_m = (al_0 - fm1), b0_m = (almq_m + br_m);
-  const T almq_0 = (al_0 - f0), br_0 = (al_p - f0), b0_0 = (almq_0 + br_0);
-  const T smt_m = ((almq_m * br_m) < T(0)) ? T(1) : T(0);
-  const T smt_0 = ((almq_0 * br_0) < T(0)) ? T(1) : T(0);
-  const T smt = (smt_m + (smt_0 * (smt_m == T(0) ? T(1) : T(0))));
-  const T up = (fm1 + (((T(1) - c) * (br_m - (c * b0_m))) * smt));
-  const T dn = (f0 + (((T(1) + c) * (almq_0 + (c * b0_0))) * smt));
-  return (c > T(0)) ? up : dn;
+  const T al_m = ppm_al(fm3, fm2, fm1, f0, p1, p2);  // al(m-1)
+  const T al_0 = ppm_al(fm2, fm1, f0, fp1, p1, p2);  // al(m)
+  const T al_p = ppm_al(fm1, f0, fp1, fp2, p1, p2);  // al(m+1)
+  return ppm_flux(ppm_cell(fm1, al_m, al_0), fm1, ppm_cell(f0, al_0, al_p), f0, c);
 }
 
 // Flux divergence ((q*area + xa*ya) - xb*yb) / ra: q_i with fyy = yfx*fy2 at
@@ -220,19 +253,23 @@ struct FvRow {
 //     q_j(L-1) from fx2(L-1) at faces i and i+1                 (:115-127)
 //     fy1(L-3) = PPM_j(q_j rows L-6 .. L-1, cry), and its flux  (:129-162, :173-176)
 //     fx1(L-3) = PPM_i(q_i row L-3, crx), and its flux          (:53-82, :166-170)
-// q and q_i rows take two LDS slots each; crx, xfx, area and fx2 take rings of
-// four, which also hold a lane's own values of rows L-3 .. L-1 until fx1,
-// q_i and q_j use them (so they need no registers).  q_j lags fx2 by a row so
-// that face i+1 of row L-1 is read from the ring after a barrier instead of
-// being computed a second time.  Each slot is rewritten only after the
-// barrier that follows its last read by another lane, so one barrier per row
-// suffices.  The expressions are those of fvtp2d_kernel (ppm_face,
+// The two PPM_j march with the rows: each row adds one edge value and one
+// cell, and the face takes the cell before it from the previous row
+// (ppm_al, ppm_cell, ppm_flux), where a face on its own forms three edges and
+// two cells.  q and q_i rows take two LDS slots each; crx, xfx, area and fx2
+// take rings of four, which also hold a lane's own values of rows L-3 .. L-1
+// until fx1, q_i and q_j use them (so they need no registers).  q_j lags fx2
+// by a row so that face i+1 of row L-1 is read from the ring after a barrier
+// instead of being computed a second time.  Each slot is rewritten only after
+// the barrier that follows its last read by another lane, so one barrier per
+// row suffices.  The expressions are those of fvtp2d_kernel (ppm_face,
 // flux_div), so the results are the same bits.  Rows are loaded kDepth
 // iterations ahead, a block at a time, with addresses clamped into the box
 // the strip reads (a masked load would make its select wait and drain the
-// prefetch).  A tile re-fetches the halo of every input for every tile it
-// borders; a strip re-fetches only its chunk's five halo rows and one line at
-// each row end.
+// prefetch); a row's address is a scalar row offset plus the lane's 32-bit
+// byte offset into the array's level (RowBuf).  A tile re-fetches the halo of
+// every input for every tile it borders; a strip re-fetches only its chunk's
+// five halo rows and one line at each row end.
 template <class T, int kDepth, int kMaxLanes>
 __global__ void __launch_bounds__(kMaxLanes)
     fvtp2d_strip_kernel(FvIn<T> a, FvOut<T> o, T p1, T p2, Geo g, Strips st) {
@@ -256,37 +293,49 @@ __global__ void __launch_bounds__(kMaxLanes)
   // lies beyond the domain on the side of a first or last strip / chunk.
   const bool own_col = (gi >= c.i0 || first_i) && (gi < c.i0 + w || last_i);
   auto own_row = [&](int r) { return (r >= j0 || first_j) && (r < j1 || last_j); };
-  const int64_t plane = int64_t(c.k) * g.sk;
-  const int64_t col = plane + gi;
-  const int64_t cq = plane + (c.i0 - 3 + min(l, w + 5));          // q lanes [0, w + 6)
-  const int64_t cy = plane + (c.i0 - 3 + min(l, w + 4));          // y lanes [0, w + 5)
-  const int64_t cx = plane + (c.i0 + min(max(l - 3, 0), w));      // faces [i0, i0 + w]
-  const int64_t cr = plane + (c.i0 + min(max(l - 3, 0), w - 1));  // x lanes [3, w + 3)
+  // Each array's level as a buffer based at (i0 - 3, -3): row r lies at
+  // byte (r + 3) sj E from there, and a lane's column at its byte offset.
+  constexpr int E = int(sizeof(T));
+  const int64_t base = int64_t(c.k) * g.sk + (c.i0 - 3) - 3 * int64_t(sj);
+  auto row = [&](int r) { return (r + 3) * sj * E; };
+  const RowBuf bq = row_buf(a.q + base), bcry = row_buf(a.cry + base), byfx = row_buf(a.yfx + base),
+               bray = row_buf(a.ra_y + base), bcrx = row_buf(a.crx + base), bxfx = row_buf(a.xfx + base),
+               barea = row_buf(a.area + base), brax = row_buf(a.ra_x + base);
+  const RowBuf bqi = row_buf(o.q_i + base), bqj = row_buf(o.q_j + base), bfx1 = row_buf(o.fx1 + base),
+               bfx2 = row_buf(o.fx2 + base), bfy1 = row_buf(o.fy1 + base), bfy2 = row_buf(o.fy2 + base);
+  const unsigned oq = unsigned(min(l, w + 5) * E);                   // q lanes [0, w + 6)
+  const unsigned oy = unsigned(min(l, w + 4) * E);                   // y lanes [0, w + 5)
+  const unsigned ox = unsigned((3 + min(max(l - 3, 0), w)) * E);     // faces [i0, i0 + w]
+  const unsigned orx = unsigned((3 + min(max(l - 3, 0), w - 1)) * E);  // x lanes [3, w + 3)
+  const unsigned ol = unsigned(l * E);                               // the lane's column
   auto ld = [&](int L) -> FvRow<T> {
     FvRow<T> r;
-    const int64_t rq = int64_t(min(L, j1 + 2)) * sj;               // q rows [j0 - 3, j1 + 2]
-    const int64_t r2 = int64_t(min(max(L - 2, j0), j1)) * sj;      // rows [j0, j1]
-    const int64_t r3 = int64_t(min(max(L - 3, j0), j1 - 1)) * sj;  // rows [j0, j1)
-    const int64_t rx = int64_t(min(L, j1 + 1)) * sj;               // rows [j0 - 3, j1 + 1]
-    const int64_t r1 = int64_t(min(max(L - 1, j0 - 3), j1 + 1)) * sj;
-    r.q = a.q[cq + rq];
-    r.cry = a.cry[cy + r2];
-    r.yfx = a.yfx[cy + r2];
-    r.ra_y = ld_nt(a.ra_y + cy + r3);
-    r.crx = a.crx[cx + rx];
-    r.xfx = a.xfx[cx + rx];
-    r.area = a.area[cy + rx];
-    r.ra_x = ld_nt(a.ra_x + cr + r1);
+    const int rq = min(L, j1 + 2);                    // q rows [j0 - 3, j1 + 2]
+    const int r2 = min(max(L - 2, j0), j1);           // rows [j0, j1]
+    const int r3 = min(max(L - 3, j0), j1 - 1);       // rows [j0, j1)
+    const int rx = min(L, j1 + 1);                    // rows [j0 - 3, j1 + 1]
+    const int r1 = min(max(L - 1, j0 - 3), j1 + 1);
+    r.q = buf_ld<T>(bq, oq, row(rq));
+    r.cry = buf_ld<T>(bcry, oy, row(r2));
+    r.yfx = buf_ld<T>(byfx, oy, row(r2));
+    r.ra_y = buf_ld<T, kBufNT>(bray, oy, row(r3));
+    r.crx = buf_ld<T>(bcrx, ox, row(rx));
+    r.xfx = buf_ld<T>(bxfx, ox, row(rx));
+    r.area = buf_ld<T>(barea, oy, row(rx));
+    r.ra_x = buf_ld<T, kBufNT>(brax, orx, row(r1));
     return r;
   };
   FvRow<T> cur[kDepth];
 #pragma unroll
   for (int u = 0; u < kDepth; ++u) cur[u] = ld(j0 - 3 + u);
-  // Column windows: q of rows L-5 .. L-1, q_j of rows L-6 .. L-2; fy2, yfx and
-  // cry of row L-3.
-  T q5 = T(0), q4 = T(0), q3 = T(0), q2 = T(0), q1 = T(0);
-  T j6 = T(0), j5 = T(0), j4 = T(0), j3 = T(0), j2 = T(0);
+  // Column windows: q of rows L-3 .. L-1, q_j of rows L-4 .. L-2; fy2, yfx
+  // and cry of row L-3.  The column PPM state: q's edge value al(L-2) and
+  // cell L-3, q_j's edge value al(L-3) and cell L-4.
+  T q3 = T(0), q2 = T(0), q1 = T(0);
+  T j4 = T(0), j3 = T(0), j2 = T(0);
   T fy2p = T(0), yfxp = T(0), cryp = T(0);
+  T yal = T(0), jal = T(0);
+  PpmCell<T> ycell{T(0), T(0), T(0), false}, jcell{T(0), T(0), T(0), false};
   const int nt = j1 - j0 + 6;
   for (int t0 = 0; t0 < nt; t0 += kDepth) {
     FvRow<T> nxt[kDepth];
@@ -303,14 +352,16 @@ __global__ void __launch_bounds__(kMaxLanes)
         const int s4 = (t & 3) * NL + l;        // this row's slot of the rings
         const int s4m1 = ((t + 3) & 3) * NL + l;  // row L-1's slot of the rings
         const int s4m3 = ((t + 1) & 3) * NL + l;  // row L-3's slot of the rings
-        // fy2(L-2) and q_i(L-3) from the column.
+        // fy2(L-2) from edge L-1 and cell L-2 (new) and cell L-3; q_i(L-3).
+        const T yal_n = ppm_al(q3, q2, q1, q0, p1, p2);
+        const PpmCell<T> ycell_n = ppm_cell(q2, yal, yal_n);
         T fy2n = T(0), qi = T(0);
         if (L - 2 >= j0 && y_lane) {
-          fy2n = ppm_face(q5, q4, q3, q2, q1, q0, r.cry, p1, p2);
-          if (own_col && own_row(L - 2)) st_nt(o.fy2 + col + int64_t(L - 2) * sj, fy2n);
+          fy2n = ppm_flux(ycell, q3, ycell_n, q2, r.cry);
+          if (own_col && own_row(L - 2)) buf_st_nt(bfy2, ol, row(L - 2), fy2n);
           if (L - 3 >= j0) {
             qi = flux_div(q3, s_ar[s4m3], yfxp, fy2p, r.yfx, fy2n, r.ra_y);
-            if (own_col) st_nt(o.q_i + col + int64_t(L - 3) * sj, qi);
+            if (own_col) buf_st_nt(bqi, ol, row(L - 3), qi);
           }
         }
         s_q[s2] = q0;
@@ -324,24 +375,28 @@ __global__ void __launch_bounds__(kMaxLanes)
           const T *sq = s_q + s2;
           const T fx2 = ppm_face(sq[-3], sq[-2], sq[-1], sq[0], sq[1], sq[2], r.crx, p1, p2);
           s_f2[s4] = fx2;
-          if (own_row(L) && (l < w + 3 || last_i)) st_nt(o.fx2 + col + int64_t(L) * sj, fx2);
+          if (own_row(L) && (l < w + 3 || last_i)) buf_st_nt(bfx2, ol, row(L), fx2);
         }
         T qj = T(0);
         if (L - 1 >= j0 - 3 && L - 1 <= j1 + 1 && x_lane) {
           qj = flux_div(q1, s_ar[s4m1], s_xf[s4m1], s_f2[s4m1], s_xf[s4m1 + 1], s_f2[s4m1 + 1], r.ra_x);
-          if (own_row(L - 1)) st_nt(o.q_j + col + int64_t(L - 1) * sj, qj);
+          if (own_row(L - 1)) buf_st_nt(bqj, ol, row(L - 1), qj);
         }
+        // fy1(L-3) from edge L-2 and cell L-3 (new) and cell L-4.
+        const T jal_n = ppm_al(j4, j3, j2, qj, p1, p2);
+        const PpmCell<T> jcell_n = ppm_cell(j3, jal, jal_n);
         if (L - 3 >= j0 && x_lane) {
-          const T fy1 = ppm_face(j6, j5, j4, j3, j2, qj, cryp, p1, p2);
-          st_nt(o.fy1 + col + int64_t(L - 3) * sj, ((T(0.5) * (fy1 + fy2p)) * yfxp));
+          const T fy1 = ppm_flux(jcell, j4, jcell_n, j3, cryp);
+          buf_st_nt(bfy1, ol, row(L - 3), ((T(0.5) * (fy1 + fy2p)) * yfxp));
         }
         if (L - 3 >= j0 && x_lane) {
           const T *si = s_qi + s2;
           const T fx1 = ppm_face(si[-3], si[-2], si[-1], si[0], si[1], si[2], s_cx[s4m3], p1, p2);
-          st_nt(o.fx1 + col + int64_t(L - 3) * sj, ((T(0.5) * (fx1 + s_f2[s4m3])) * s_xf[s4m3]));
+          buf_st_nt(bfx1, ol, row(L - 3), ((T(0.5) * (fx1 + s_f2[s4m3])) * s_xf[s4m3]));
         }
-        q5 = q4, q4 = q3, q3 = q2, q2 = q1, q1 = q0;
-        j6 = j5, j5 = j4, j4 = j3, j3 = j2, j2 = qj;
+        q3 = q2, q2 = q1, q1 = q0;
+        j4 = j3, j3 = j2, j2 = qj;
+        yal = yal_n, ycell = ycell_n, jal = jal_n, jcell = jcell_n;
         fy2p = fy2n, yfxp = r.yfx, cryp = r.cry;
       }
     }
@@ -444,12 +499,20 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   // registers (no j-halo re-fetched or recomputed), fetched 599 MB and took
   // 164 / 570 us: the tiles' extra bytes are mostly the i-halo (a 64-wide FP64
   // row with 3 halo points each side spans 6 cache lines instead of 4).
-  if (knob_on(K, sizeof(T), d, KNOB_FORM, sizeof(T) == 4)) {
+  // The strip form addresses a level as a buffer with byte offsets below 2^31
+  // (RowBuf); a larger level takes the tiles.
+  const bool strip_fits = (ny + 6) * int64_t(g.sj) * int64_t(sizeof(T)) + (nx + 6) * int64_t(sizeof(T)) <= INT32_MAX;
+  if (strip_fits && knob_on(K, sizeof(T), d, KNOB_FORM, sizeof(T) == 4)) {
     // Strips of at most 512 lanes (the lanes knob: 64 .. 1024).
     const int max_lanes = knob(K, sizeof(T), d, KNOB_LANES, 512);
     // At most 12 rows per chunk: FP32 308-310 us against 321-322 for the 32
     // rows of default_strip_rows at 512x512x80, 94-95 us against 95-96 for
-    // its 20 at 256x256x80 (the ten-stencil step, repeated).
+    // its 20 at 256x256x80 (the ten-stencil step, repeated).  Rows 8-24,
+    // 512 or 1024 lanes and prefetch depth 1 or 2 measured 304-419 us at
+    // 512x512x80, this default among the fastest.  One-wave strips (64
+    // lanes, which need no s_barrier) took 456-504 us and 128 lanes 391 us:
+    // a 58-column row spans three cache lines per array where 256 columns
+    // span little more than their own.
     Strips st = make_strips(nx, ny, int(nz), 3, 1, 1, max_lanes);
     st = make_strips(nx, ny, int(nz), 3, knob(K, sizeof(T), d, KNOB_ROWS, std::min(12, default_strip_rows(ny, st.lanes))), 1,
                      max_lanes);

@@ -135,6 +135,42 @@ __device__ __forceinline__ T ld_nt(const T *p) {
   return __builtin_nontemporal_load(p);
 }
 
+// An array read or written a row at a time as a raw buffer (stride 0, range
+// 2^31 - 1 bytes from its base).  An access takes the row's byte offset in a
+// scalar register (soffset) and the lane's byte offset in a vector register
+// (voffset), so a march down rows spends no vector instructions on addresses
+// and holds one 32-bit offset per lane instead of a 64-bit pointer per array.
+// The host keeps every offset below 2^31.  Word 3 of the descriptor is the
+// gfx9 family's raw-buffer format (32-bit data, no swizzle).
+struct RowBuf {
+  __amdgpu_buffer_rsrc_t r;
+};
+template <class T>
+__device__ __forceinline__ RowBuf row_buf(const T *base) {
+  return RowBuf{__builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(base), 0, 0x7fffffff, 0x00020000)};
+}
+// Cache policy bit of the load and store: non-temporal.
+constexpr int kBufNT = 2;
+template <class T, int kAux = 0>
+__device__ __forceinline__ T buf_ld(RowBuf b, unsigned voff, int soff) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(b.r, int(voff), soff, kAux));
+  } else {
+    static_assert(sizeof(T) == 8, "float or double");
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(b.r, int(voff), soff, kAux));
+  }
+}
+template <class T>
+__device__ __forceinline__ void buf_st_nt(RowBuf b, unsigned voff, int soff, T v) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), b.r, int(voff), soff, kBufNT);
+  } else {
+    static_assert(sizeof(T) == 8, "float or double");
+    using u2 = unsigned __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), b.r, int(voff), soff, kBufNT);
+  }
+}
+
 // Pairs of adjacent doubles along i, moved with one 16-byte access.  Used by
 // the paired kernels, which the host selects only when every 3-D array's
 // element (0,0,0) is 16-byte aligned and both strides are even, so that
