@@ -1143,17 +1143,21 @@ int hadvuv5th_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
   // workgroup per CU); FP32 64x8 tiles of 512 lanes (51 KB), or 64x16 of 1024
   // with tile knob 2.  Measured at 256x256x80: FP64 tiles 173 us
   // (4 levels per workgroup; 180 at 2) against 211 for the row march; FP32
-  // tiles 107 us against 86, so FP32 keeps the row march.
+  // tiles 107 us against 86, so FP32 keeps the row march.  FP64 tiles take 5
+  // levels per workgroup in the level-interleaved order (Tiles::klev):
+  // 147.7 / 535.6 us against 159.7 / 592.8 for 4 levels in row order at
+  // 256x256x80 / 512x512x80 (kernel time over HIP-graph replays; 4 and 6
+  // levels and orders 0, 2 and 3 measured 148.8-174.3 / 528.9-592.8).
   const int tile = knob(K, sizeof(T), d, KNOB_TILE, sizeof(T) == 8 ? 1 : 0);
   if (tile != 0) {
-    const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, 4);
+    const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, sizeof(T) == 8 ? 5 : 4);
     if (sizeof(T) == 4 && tile == 1) {
       const Tiles t = make_tiles(nx, ny, int(nz), kTI, 8, kc);
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 8));
       if constexpr (sizeof(T) == 4)
         hipLaunchKernelGGL((hadvuv5th_adj_tile<T, 8, 512>), dim3(t.n), dim3(512), 0, hipStream_t(stream), a, g, t);
     } else {
-      const int order = knob(K, sizeof(T), d, KNOB_ORDER, 0);
+      const int order = knob(K, sizeof(T), d, KNOB_ORDER, sizeof(T) == 8 ? 3 : 0);
       const Tiles t = make_tiles(nx, ny, int(nz), kTI, 16, kc, order);
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 16));
       hipLaunchKernelGGL((hadvuv5th_adj_tile<T, 16, 1024>), dim3(t.n), dim3(1024), 0, hipStream_t(stream), a, g, t);
@@ -1162,9 +1166,13 @@ int hadvuv5th_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
   }
   if (nx + 12 <= 640 && knob_on(K, sizeof(T), d, KNOB_FORM, true)) {
     const int lanes = int((nx + 12 + 63) / 64 * 64);
-    // Row chunks measured at 256x256x80 over 8-64: 32 rows fastest in both
-    // forms.  At most 256 rows, which bounds the 1-D tables in LDS.
-    const int rows = std::min(knob(K, sizeof(T), d, KNOB_ROWS, 32), 256);
+    // Row chunks measured at 256x256x80 over 8-64: 32 rows fastest in FP64.
+    // FP32 takes 22: 71.9 / 304.2 us against 78.8 / 309.8 for 32 at
+    // 256x256x80 / 512x512x80; 20-28 rows took 71.9-91.6 / 296.9-315.2.  At
+    // 256x256x80, 22 rows is the shortest chunk whose 960 workgroups fit one
+    // resident round of the CUs; 20 rows (1040) spill into a second.  At
+    // most 256 rows, which bounds the 1-D tables in LDS.
+    const int rows = std::min(knob(K, sizeof(T), d, KNOB_ROWS, sizeof(T) == 4 ? 22 : 32), 256);
     const int nchunk = int((ny + 6 + rows - 1) / rows);
     if (int64_t(nchunk) * nz > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
     // Rows loaded one ahead: two ahead spills registers and measured slower.
@@ -1858,7 +1866,11 @@ int hdiffsmag_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 8));
       hipLaunchKernelGGL((hdiffsmag_adj_tile<T, 8, 512>), dim3(t.n), dim3(512), 0, hipStream_t(stream), a, g, t);
     } else {
-      const int order = knob(K, sizeof(T), d, KNOB_ORDER, 2);  // Tiles::irow: 143 us against 145
+      // Level-interleaved order (Tiles::klev): 120.6 / 449.2 us against
+      // 125.5 / 458.3 for Tiles::irow at 256x256x80 / 512x512x80 (kernel time
+      // over HIP-graph replays); 3-5 levels and orders 1-3 took 118.8-134.5 /
+      // 449.2-477.5.
+      const int order = knob(K, sizeof(T), d, KNOB_ORDER, 3);
       const Tiles t = make_tiles(nx, ny, int(nz), kTI, 16, kc, order);
       OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kTI, 16));
       hipLaunchKernelGGL((hdiffsmag_adj_tile<T, 16, 1024>), dim3(t.n), dim3(1024), 0, hipStream_t(stream), a, g, t);
@@ -1868,13 +1880,16 @@ int hdiffsmag_adj_run(const oeb_domain *d, const S3 *uin, const S3 *vin, const S
   if (nx + 4 <= 1024 && knob_on(K, sizeof(T), d, KNOB_FORM, true)) {
     const int lanes = int((nx + 4 + 63) / 64 * 64);
     // Measured at 256x256x80: FP64 fastest with 16-row chunks and rows
-    // loaded one row ahead (two rows ahead spills registers), FP32 with
-    // 32-row chunks loaded two rows ahead.
+    // loaded one row ahead (two rows ahead spills registers).  FP32 takes
+    // 22-row chunks loaded one row ahead: 62.6 / 269.7 us against 67.7-70.8 /
+    // 376.1 for 32 rows loaded two ahead at 256x256x80 / 512x512x80; 20-32
+    // rows took 62.6-79.0 / 265.2-290.5.  At 256x256x80, 22 rows is the
+    // shortest chunk whose 960 workgroups fit one resident round of the CUs.
     // At most 256 rows, which bounds the 1-D tables in LDS.
-    const int rows = std::min(knob(K, sizeof(T), d, KNOB_ROWS, sizeof(T) == 8 ? 16 : 32), 256);
+    const int rows = std::min(knob(K, sizeof(T), d, KNOB_ROWS, sizeof(T) == 8 ? 16 : 22), 256);
     const int nchunk = int((ny + 2 + rows - 1) / rows);
     if (int64_t(nchunk) * nz > (int64_t(1) << 30)) return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
-    auto kern = knob(K, sizeof(T), d, KNOB_DEPTH, sizeof(T) == 8 ? 1 : 2) >= 2 ? hdiffsmag_adj_strip<T, 2> : hdiffsmag_adj_strip<T, 1>;
+    auto kern = knob(K, sizeof(T), d, KNOB_DEPTH, 1) >= 2 ? hdiffsmag_adj_strip<T, 2> : hdiffsmag_adj_strip<T, 1>;
     hipLaunchKernelGGL(kern, dim3(unsigned(nchunk * nz)), dim3(unsigned(lanes)),
                        (size_t(20) * lanes + 5 * size_t(rows + 9)) * sizeof(T), hipStream_t(stream), a, g, rows,
                        nchunk);
