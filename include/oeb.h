@@ -467,7 +467,8 @@ int oeb_fvtp2d_adj_f64_noalloc(const oeb_domain *d, const oeb_s3 *q, const oeb_s
  * table.  Knobs (a kernel ignores those it does not have):
  *   "kchunk" [1, 4096]  levels per workgroup of a tile launch
  *   "vector" [0, 1]     1: the 16-byte vector / column-pair body
- *   "form"   [0, 1]     1: the row-march (strip) form, 0: the tile form
+ *   "form"   [0, 2]     1: the row-march (strip) form, 0: the tile form,
+ *                       2: LDS-staged tiles (fvtp2d)
  *   "rows"   [1, 4096]  rows per strip chunk
  *   "depth"  [1, 8]     rows a strip loads ahead
  *   "order"  [0, 3]     tile order: 0 rows in order, 1 level chunks

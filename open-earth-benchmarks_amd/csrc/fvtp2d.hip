@@ -229,6 +229,187 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn<T> a, FvOut<T> o, T p1
   }
 }
 
+// ------------------------------------------------------------ LDS-staged tiles
+// The tile form with every input of a level brought into LDS up front: the
+// eight input regions of a tile level are loaded straight into LDS
+// (global_load_lds_dword: each active lane's dword lands at the instruction's
+// LDS base + 4 x lane) before phase B, so the phases read no global memory
+// and a level waits on its loads once instead of once per phase, with no
+// registers spent on loads in flight.  Not the default: measured in the
+// ten-stencil step at 256x256x80 / 512x512x80, FP64 64x8 tiles of 512 lanes
+// (72 KB of LDS, two workgroups per CU) took 157.6-160.6 / 606.6-622.1 us
+// against 144.6-144.9 / 544.8-545.1 for fvtp2d_kernel, 64x16 tiles of 1024
+// lanes (124 KB, one per CU) 162.0-170.3 / 645.5-648.1; FP32 93.3-99.3 /
+// 351.7-368.5 against the strip's 91.7-92.8 / 303.7-306.7.  A level's loads
+// are one batch that the workgroup waits for: at 7.9 us per tile level it
+// moved 3.2 TB/s, while the register-staged tile keeps more bytes in flight.
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// A lane's dwords in the rows of a W-element region: instruction q of a row
+// moves dwords [64 q, 64 q + 64) of the row's W E dwords (E dwords per
+// element).  Columns are clamped into the array's box [bi0, bi1); offsets are
+// bytes from column bi0 of the row.
+template <int W, class T>
+struct RegionCols {
+  static constexpr int E = int(sizeof(T)) / 4, D = W * E, NQ = (D + 63) / 64;
+  unsigned off[NQ];
+  bool on[NQ];
+  __device__ __forceinline__ RegionCols(int i_lo, int bi0, int bi1, int lane) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int d = 64 * q + lane;
+      on[q] = d < D;
+      const int c = d / E, part = d - c * E;
+      const int gi = min(max(i_lo + c, bi0), bi1 - 1);
+      off[q] = unsigned(((gi - bi0) * E + part) * 4);
+    }
+  }
+};
+
+// Issues the loads of rows j_lo .. j_lo + H - 1 of one level of an array
+// (rows clamped into [bj0, bj1)) into the row-major H x W LDS region dst.
+// Wave w takes rows w, w + nwaves, ...; a row's address is a scalar row base
+// plus the lane's 32-bit offset.
+template <int W, int H, class T>
+__device__ __forceinline__ void lds_load_region(T *dst, const T *level, const RegionCols<W, T> &cols, int j_lo,
+                                                int bj0, int bj1, int bi0, int sj, int wave, int nwaves) {
+  using C = RegionCols<W, T>;
+  for (int r = wave; r < H; r += nwaves) {
+    const int gj = min(max(j_lo + r, bj0), bj1 - 1);
+    const char *row = reinterpret_cast<const char *>(level + (int64_t(gj) * sj + bi0));
+    unsigned *d = reinterpret_cast<unsigned *>(dst + r * W);
+#pragma unroll
+    for (int q = 0; q < C::NQ; ++q)
+      if (cols.on[q])
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const unsigned *>(row + cols.off[q]),
+                                         (lds_void_t *)(d + 64 * q), 4, 0, 0);
+  }
+}
+
+// Tiles of 64 x TJ columns, one level at a time, NT lanes.  The regions and
+// phases are those of fvtp2d_kernel (same expressions, same bits); the inputs
+// come from LDS:
+//   q          [i0-3, i0+67) x [j0-3, j0+TJ+3)
+//   cry, yfx   [i0-3, i0+66) x [j0, j0+TJ+1)      (fy2 region)
+//   crx, xfx   [i0, i0+65)   x [j0-3, j0+TJ+2)    (fx2 region)
+//   area       [i0-3, i0+66) x [j0-3, j0+TJ+2)    (q_i and q_j regions)
+//   ra_y       [i0-3, i0+66) x [j0, j0+TJ)        (q_i region)
+//   ra_x       [i0, i0+64)   x [j0-3, j0+TJ+2)    (q_j region)
+// Each clamped into its array's box, so every load stays inside the array;
+// a clamped value feeds only points that are not stored.  Four barriers per
+// level: after the loads, after phase B, after phase C, and after phase D
+// (the next level's loads overwrite what phase D reads).
+template <int TJ, int NT, class T>
+__global__ void __launch_bounds__(NT) fvtp2d_lds_kernel(FvIn<T> a, FvOut<T> o, T p1, T p2, Geo g, Tiles t) {
+  constexpr int TI = 64;
+  constexpr int kQW = TI + 6, kQH = TJ + 6;
+  constexpr int kYW = TI + 5, kYH = TJ + 1;
+  constexpr int kXW = TI + 1, kXH = TJ + 5;
+  constexpr int kAH = TJ + 5;
+  constexpr int kIH = TJ;
+  constexpr int kJW = TI, kJH = TJ + 5;
+  constexpr int kWaves = NT / 64;
+  __shared__ T sq[kQH][kQW];
+  __shared__ T scry[kYH][kYW], syfx[kYH][kYW], sfy2[kYH][kYW];
+  __shared__ T scrx[kXH][kXW], sxfx[kXH][kXW], sfx2[kXH][kXW];
+  __shared__ T sarea[kAH][kYW];
+  __shared__ T sray[kIH][kYW], sqi[kIH][kYW];
+  __shared__ T srax[kJH][kJW], sqj[kJH][kJW];
+
+  const TileCoord tc = tile_coord(t, TI, TJ, g.nz);
+  const int i0 = tc.i0, j0 = tc.j0;
+  const int nx = g.nx, ny = g.ny, sj = g.sj;
+  const int tid = int(threadIdx.x), lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool first_i = i0 == 0, last_i = i0 + TI >= nx;
+  const bool first_j = j0 == 0, last_j = j0 + TJ >= ny;
+  auto own_i = [&](int gi) { return (gi >= i0 || first_i) && (gi < i0 + TI || last_i); };
+  auto own_j = [&](int gj) { return (gj >= j0 || first_j) && (gj < j0 + TJ || last_j); };
+  const RegionCols<kQW, T> cq(i0 - 3, -3, nx + 3, lane);
+  const RegionCols<kYW, T> cy(i0 - 3, -3, nx + 2, lane);   // cry, yfx, area, ra_y
+  const RegionCols<kXW, T> cx(i0, 0, nx + 1, lane);        // crx, xfx
+  const RegionCols<kJW, T> cr(i0, 0, nx, lane);            // ra_x
+
+  for (int k = tc.k0; k < tc.k1; ++k) {
+    const int64_t plane = int64_t(k) * g.sk;
+    lds_load_region<kQW, kQH>(&sq[0][0], a.q + plane, cq, j0 - 3, -3, ny + 3, -3, sj, wave, kWaves);
+    lds_load_region<kYW, kYH>(&scry[0][0], a.cry + plane, cy, j0, 0, ny + 1, -3, sj, wave, kWaves);
+    lds_load_region<kYW, kYH>(&syfx[0][0], a.yfx + plane, cy, j0, 0, ny + 1, -3, sj, wave, kWaves);
+    lds_load_region<kXW, kXH>(&scrx[0][0], a.crx + plane, cx, j0 - 3, -3, ny + 2, 0, sj, wave, kWaves);
+    lds_load_region<kXW, kXH>(&sxfx[0][0], a.xfx + plane, cx, j0 - 3, -3, ny + 2, 0, sj, wave, kWaves);
+    lds_load_region<kYW, kAH>(&sarea[0][0], a.area + plane, cy, j0 - 3, -3, ny + 2, -3, sj, wave, kWaves);
+    lds_load_region<kYW, kIH>(&sray[0][0], a.ra_y + plane, cy, j0, 0, ny, -3, sj, wave, kWaves);
+    lds_load_region<kJW, kJH>(&srax[0][0], a.ra_x + plane, cr, j0 - 3, -3, ny + 2, 0, sj, wave, kWaves);
+    // The loads write LDS and count in vmcnt: wait for all of them, then
+    // publish them to the other waves.
+    __builtin_amdgcn_s_waitcnt(0);
+    lds_barrier();
+
+    // B: fy2 on i [-3, nx+2) x j [0, ny+1) and fx2 on i [0, nx+1) x j [-3, ny+2).
+    for (int e = tid; e < kYH * kYW; e += NT) {
+      const int r = e / kYW, c = e % kYW;
+      const int gi = i0 - 3 + c, gj = j0 + r;
+      if (gi < nx + 2 && gj < ny + 1) {
+        const T v = ppm_face(sq[r][c], sq[r + 1][c], sq[r + 2][c], sq[r + 3][c], sq[r + 4][c], sq[r + 5][c],
+                             scry[r][c], p1, p2);
+        sfy2[r][c] = v;
+        if (own_i(gi) && own_j(gj)) st_nt(o.fy2 + plane + gi + int64_t(gj) * sj, v);
+      }
+    }
+    for (int e = tid; e < kXH * kXW; e += NT) {
+      const int r = e / kXW, c = e % kXW;
+      const int gi = i0 + c, gj = j0 - 3 + r;
+      if (gi < nx + 1 && gj < ny + 2) {
+        const T *row = &sq[r][c];
+        const T v = ppm_face(row[0], row[1], row[2], row[3], row[4], row[5], scrx[r][c], p1, p2);
+        sfx2[r][c] = v;
+        if (own_i(gi) && own_j(gj)) st_nt(o.fx2 + plane + gi + int64_t(gj) * sj, v);
+      }
+    }
+    lds_barrier();
+
+    // C: q_i on i [-3, nx+2) x j [0, ny), q_j on i [0, nx) x j [-3, ny+2).
+    for (int e = tid; e < kIH * kYW; e += NT) {
+      const int r = e / kYW, c = e % kYW;
+      const int gi = i0 - 3 + c, gj = j0 + r;
+      if (gi < nx + 2 && gj < ny) {
+        const T v = flux_div(sq[r + 3][c], sarea[r + 3][c], syfx[r][c], sfy2[r][c], syfx[r + 1][c], sfy2[r + 1][c],
+                             sray[r][c]);
+        sqi[r][c] = v;
+        if (own_i(gi) && own_j(gj)) st_nt(o.q_i + plane + gi + int64_t(gj) * sj, v);
+      }
+    }
+    for (int e = tid; e < kJH * kJW; e += NT) {
+      const int r = e / kJW, c = e % kJW;
+      const int gi = i0 + c, gj = j0 - 3 + r;
+      if (gi < nx && gj < ny + 2) {
+        const T v = flux_div(sq[r][c + 3], sarea[r][c + 3], sxfx[r][c], sfx2[r][c], sxfx[r][c + 1], sfx2[r][c + 1],
+                             srax[r][c]);
+        sqj[r][c] = v;
+        if (own_i(gi) && own_j(gj)) st_nt(o.q_j + plane + gi + int64_t(gj) * sj, v);
+      }
+    }
+    lds_barrier();
+
+    // D: fx1, fy1 on the tile's interior points.
+    for (int e = tid; e < TJ * TI; e += NT) {
+      const int r = e / TI, c = e % TI;
+      const int gi = i0 + c, gj = j0 + r;
+      if (gi < nx && gj < ny) {
+        const int64_t at = plane + gi + int64_t(gj) * sj;
+        const T *ri = &sqi[r][c];
+        const T fx1 = ppm_face(ri[0], ri[1], ri[2], ri[3], ri[4], ri[5], scrx[r + 3][c], p1, p2);
+        const T fy1 = ppm_face(sqj[r][c], sqj[r + 1][c], sqj[r + 2][c], sqj[r + 3][c], sqj[r + 4][c], sqj[r + 5][c],
+                               scry[r][c + 3], p1, p2);
+        st_nt(o.fx1 + at, ((T(0.5) * (fx1 + sfx2[r + 3][c])) * sxfx[r + 3][c]));
+        st_nt(o.fy1 + at, ((T(0.5) * (fy1 + sfy2[r][c + 3])) * syfx[r][c + 3]));
+      }
+    }
+    lds_barrier();
+  }
+}
+
 // One row's loads for the strip form, for lead row L of the march (rows
 // clamped into the ranges the strip reads).
 template <class T>
@@ -418,6 +599,17 @@ int fvtp2d_strip_launch(const char *K, const FvIn<T> &a, const FvOut<T> &o, T p1
   return launch_status(K);
 }
 
+template <int TJ, int NT, class T>
+int fvtp2d_lds_launch(const char *K, const FvIn<T> &a, const FvOut<T> &o, T p1, T p2, const Geo &g,
+                      const oeb_domain *d, void *stream) {
+  const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
+  const int order = knob(K, sizeof(T), d, KNOB_ORDER, 2);
+  const Tiles t = make_tiles(nx, ny, int(nz), 64, TJ, knob(K, sizeof(T), d, KNOB_KCHUNK, kChunk), order);
+  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), 64, TJ));
+  hipLaunchKernelGGL((fvtp2d_lds_kernel<TJ, NT, T>), dim3(t.n), dim3(NT), 0, hipStream_t(stream), a, o, p1, p2, g, t);
+  return launch_status(K);
+}
+
 template <int TI, int TJ, int NT, class T>
 int fvtp2d_launch(const char *K, const FvIn<T> &a, const FvOut<T> &o, T p1, T p2, const Geo &g,
                   const oeb_domain *d, void *stream) {
@@ -502,7 +694,17 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   // The strip form addresses a level as a buffer with byte offsets below 2^31
   // (RowBuf); a larger level takes the tiles.
   const bool strip_fits = (ny + 6) * int64_t(g.sj) * int64_t(sizeof(T)) + (nx + 6) * int64_t(sizeof(T)) <= INT32_MAX;
-  if (strip_fits && knob_on(K, sizeof(T), d, KNOB_FORM, sizeof(T) == 4)) {
+  // The form knob: 0 tiles, 1 strips, 2 LDS-staged tiles (fvtp2d_lds_kernel,
+  // with the tile knob: 0 64x8 of 512 lanes, 1 64x16 of 1024, 2 64x8 of 256).
+  const int form = knob(K, sizeof(T), d, KNOB_FORM, sizeof(T) == 4 ? 1 : 0);
+  if (form == 2) {
+    switch (knob(K, sizeof(T), d, KNOB_TILE, 0)) {
+      case 1: return fvtp2d_lds_launch<16, 1024, T>(K, a, o, p1, p2, g, d, stream);
+      case 2: return fvtp2d_lds_launch<8, 256, T>(K, a, o, p1, p2, g, d, stream);
+      default: return fvtp2d_lds_launch<8, 512, T>(K, a, o, p1, p2, g, d, stream);
+    }
+  }
+  if (strip_fits && form == 1) {
     // Strips of at most 512 lanes (the lanes knob: 64 .. 1024).
     const int max_lanes = knob(K, sizeof(T), d, KNOB_LANES, 512);
     // At most 12 rows per chunk: FP32 308-310 us against 321-322 for the 32

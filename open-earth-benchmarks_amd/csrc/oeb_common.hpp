@@ -56,7 +56,8 @@ struct Box {
 // (this domain, then every domain; this element size, then both) wins over it.
 //   KNOB_KCHUNK  levels per workgroup of a tile launch
 //   KNOB_VECTOR  1: the 16-byte vector / column-pair body where it applies
-//   KNOB_FORM    1: the strip (row march) form, 0: the tile form
+//   KNOB_FORM    1: the strip (row march) form, 0: the tile form, 2: fvtp2d's
+//                LDS-staged tiles
 //   KNOB_ROWS    rows per strip chunk
 //   KNOB_DEPTH   rows a strip loads ahead
 //   KNOB_ORDER   tile order: 0 rows in order, 1 Tiles::xcd, 2 Tiles::irow,

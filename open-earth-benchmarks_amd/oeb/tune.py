@@ -61,7 +61,8 @@ SPACES = {
     "p_grad_c": _tiled([("vector", (1, 0))]),
     "nh_p_grad": _tiled([("vector", (1, 0))]),
     "fvtp2d": [({"form": 0}, [("tile", (8, 7, 2, 3, 1)), ("order", (2, 3, 0)), ("kchunk", (1, 2))]),
-               ({"form": 1}, [("lanes", (512, 256, 1024)), ("rows", ROWS), ("depth", (1, 2))])],
+               ({"form": 1}, [("lanes", (512, 256, 1024)), ("rows", ROWS), ("depth", (1, 2))]),
+               ({"form": 2}, [("tile", (0, 1, 2)), ("order", (2, 3, 0)), ("kchunk", (1, 2))])],
 }
 # Kernels with a 16-byte vector path (kept for oeb_tune_set callers).
 VECTOR_KERNELS = {"laplace", "uvbke", "fastwavesuv", "p_grad_c", "nh_p_grad", "hdiffsmag"}

@@ -135,6 +135,19 @@ def test_f32_fvtp2d_strip_shapes(lanes, depth, knobs):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("shape", [(64, 36, 3), (67, 9, 2), (130, 17, 4)], ids=lambda s: "x".join(map(str, s)))
+def test_f32_fvtp2d_lds_tiles(shape, tile, knobs):
+    """fvtp2d's LDS-staged tiles (form knob 2) in FP32, each tile shape, on
+    whole and partial tiles, agree bit for bit with the FP32 oracle."""
+    knobs.set("fvtp2d", "form", 2)
+    knobs.set("fvtp2d", "tile", tile)
+    L, host = inputs("fvtp2d", *shape, "stress", seed=19)
+    print(f"[gpu] f32 fvtp2d lds tile={tile} {shape}", flush=True)
+    _check_exact("fvtp2d", L, cast("fvtp2d", host, "f32"))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("xcd", [0, 1])
 @pytest.mark.parametrize("shape", [(64, 36, 8), (200, 9, 4), (256, 20, 6)], ids=lambda s: "x".join(map(str, s)))
 def test_f32_laplace_xcd_interleave(shape, xcd, knobs):

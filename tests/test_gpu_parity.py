@@ -121,6 +121,24 @@ def test_strip_switches(name, shape, strip, rows, knobs):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kchunk", [1, 3])
+@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("shape", [(64, 36, 3), (67, 9, 2), (130, 17, 4), (1, 1, 1)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_fvtp2d_lds_tiles(shape, tile, kchunk, knobs):
+    """fvtp2d's LDS-staged tiles (form knob 2: every input region of a level
+    loaded straight into LDS) in each tile shape, on whole and partial tiles
+    and one-point domains, with level chunks that divide nz and that do not,
+    agree bit for bit with the oracle."""
+    knobs.set("fvtp2d", "form", 2)
+    knobs.set("fvtp2d", "tile", tile)
+    knobs.set("fvtp2d", "kchunk", kchunk)
+    L, host = inputs("fvtp2d", *shape, "baseline", seed=17)
+    print(f"[gpu] fvtp2d lds tile={tile} kchunk={kchunk} {shape}", flush=True)
+    _check("fvtp2d", L, host, run_gpu("fvtp2d", copy(host), L), require_exact=True)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("xcd", [0, 1])
 @pytest.mark.parametrize("shape", [(64, 36, 8), (200, 9, 4), (256, 20, 6), (66, 17, 2)],
                          ids=lambda s: "x".join(map(str, s)))
