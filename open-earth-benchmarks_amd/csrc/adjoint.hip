@@ -971,14 +971,14 @@ __global__ void __launch_bounds__(NT, 4) hadvuv5th_adj_tile(HadvAdj<T> a, Geo g,
     lds_barrier();
 
     // 2. Stage 1 (kernels/hadvuv5th.h:183-195).
-    for (int e = tid; e < kN1; e += NT) {
+    // Stage 1 at element e of the stage-1 region, given uout_bar and
+    // vout_bar there (read only for interior points).
+    auto stage1 = [&](int e, T ob, T ob2) {
       const int c1 = e % kW1, r1 = e / kW1;
       const int pi = i0 - 3 + c1, pj = j0 - 3 + r1;
       T f0 = T(0), f1 = T(0), f2 = T(0), f3 = T(0), f4 = T(0), f5 = T(0), f6 = T(0), f7 = T(0), f8 = T(0);
       int br = 0;
       if (inside(pi, pj)) {
-        const int64_t at = plane + pi + int64_t(pj) * sj;
-        const T ob = a.ubar[at], ob2 = a.vbar[at];
         const T a0 = s_a0[r1], a1 = s_a1[r1];
         const T tg0 = s_t0[r1], tg1 = s_t1[r1];
         const int e2 = (r1 + 3) * kW2 + c1 + 3;
@@ -1014,6 +1014,19 @@ __global__ void __launch_bounds__(NT, 4) hadvuv5th_adj_tile(HadvAdj<T> a, Geo g,
       s_gv2[e] = f7;
       s_tgt[e] = f8;
       s_br[e] = br;
+    };
+    // (Loading both stage-1 points' uout_bar and vout_bar first, at clamped
+    // addresses, spilled 40 bytes per lane and took 186 us against 152 at
+    // 256x256x80.)
+    for (int e = tid; e < kN1; e += NT) {
+      const int pi = i0 - 3 + e % kW1, pj = j0 - 3 + e / kW1;
+      T ob = T(0), ob2 = T(0);
+      if (inside(pi, pj)) {
+        const int64_t at = plane + pi + int64_t(pj) * sj;
+        ob = a.ubar[at];
+        ob2 = a.vbar[at];
+      }
+      stage1(e, ob, ob2);
     }
     lds_barrier();
 
@@ -2090,15 +2103,45 @@ __global__ void __launch_bounds__(kTI *kTJ) fvtp2d_adj_k5(FvAdj<T> a, Geo g, Box
 constexpr int kPX = 64, kPY = 16, kPNT = 512;
 constexpr int kPPts = kPX * kPY / kPNT;  // points per lane
 
-template <class T>
+// The field tile of the hoisted form: the field the products read (q, q_i
+// or q_j) on [i0-5, i0+69) x [j0-5, j0+21), the points that the faces of both
+// axes of a tile read.
+constexpr int kFW = kPX + 10, kFH = kPY + 10;
+
+template <class T, bool H>
 struct PpmLds {
   T p[6][kPX * (kPY + 5)];
+  T f[H ? kFH * kFW : 1];
 };
+
+// Rows [R0, R1) x columns [C0, C1) of the field tile, from the level at
+// `plane` of field f, each point clamped into the field's box [bi0, bi1) x
+// [bj0, bj1).  A clamped value is read only by faces outside their region,
+// which form no products.  Every lane's loads are issued before its first
+// LDS store.
+template <int R0, int R1, int C0, int C1, class T>
+__device__ __forceinline__ void field_tile(T *sf, const T *f, int64_t plane, int sj, int i0, int j0, int bi0,
+                                           int bi1, int bj0, int bj1) {
+  constexpr int W = C1 - C0, N = (R1 - R0) * W, NU = (N + kPNT - 1) / kPNT;
+  T v[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int e = min(int(threadIdx.x) + u * kPNT, N - 1);
+    const int gi = min(max(i0 - 5 + C0 + e % W, bi0), bi1 - 1);
+    const int gj = min(max(j0 - 5 + R0 + e / W, bj0), bj1 - 1);
+    v[u] = f[plane + gi + int64_t(gj) * sj];
+  }
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int e = int(threadIdx.x) + u * kPNT;
+    if (e < N) sf[(R0 + e / W) * kFW + C0 + e % W] = v[u];
+  }
+}
 
 // Products of the faces of the tile (i0, j0) along AXIS (0: i, 1: j).
 template <int AXIS, class T, class Face, class Gbar>
-__device__ __forceinline__ void ppm_products(PpmLds<T> &L, const T *f, const T *c, int64_t plane, int sj, int i0,
-                                             int j0, T p1, T p2, Face face, Gbar gbar) {
+__device__ __forceinline__ void ppm_products(PpmLds<T, false> &L, const T *f, const T *c, int64_t plane, int sj,
+                                             int i0, int j0, T p1, T p2, Face face, Gbar gbar) {
   constexpr int FW = AXIS == 0 ? kPX + 5 : kPX, FH = AXIS == 0 ? kPY : kPY + 5;
   const int s = AXIS == 0 ? 1 : sj;
   for (int e = int(threadIdx.x); e < FW * FH; e += kPNT) {
@@ -2124,8 +2167,8 @@ __device__ __forceinline__ void ppm_products(PpmLds<T> &L, const T *f, const T *
 
 // Sum over faces q-2 .. q+3 along AXIS of the products that name the tile
 // point (lx, ly).
-template <int AXIS, class T>
-__device__ __forceinline__ T ppm_gather(const PpmLds<T> &L, int lx, int ly) {
+template <int AXIS, class T, bool H>
+__device__ __forceinline__ T ppm_gather(const PpmLds<T, H> &L, int lx, int ly) {
   constexpr int FW = AXIS == 0 ? kPX + 5 : kPX;
   T r = T(0);
 #pragma unroll
@@ -2136,10 +2179,210 @@ __device__ __forceinline__ T ppm_gather(const PpmLds<T> &L, int lx, int ly) {
   return r;
 }
 
+// The two-pass face loop of the hoisted form.  Pass 1 issues, for every face
+// of the lane, the global loads of its Courant number and of the inputs of its
+// cotangent (FaceLd), at addresses clamped into the face's region or into the
+// box of the array read; pass 2 forms the products from them and from the
+// field tile in LDS.  A value loaded at a clamped address is used only where
+// the forward's region test holds, and there the address is the unclamped
+// one, so both passes give the bits of ppm_products.  The loop in
+// ppm_products waits for each face's loads in turn.
+template <class T>
+struct FaceLd {
+  T c, g0, g1, g2, g3, g4;
+};
+
+template <int AXIS>
+struct FaceGrid {
+  static constexpr int FW = AXIS == 0 ? kPX + 5 : kPX, FH = AXIS == 0 ? kPY : kPY + 5;
+  static constexpr int N = FW * FH, NU = (N + kPNT - 1) / kPNT;
+};
+
+// Pass 1: ld(fi, fj) for every face of the lane, (fi, fj) clamped into
+// [fi_lo, fi_hi) x [fj_lo, fj_hi).
+template <int AXIS, class T, class Load>
+__device__ __forceinline__ void face_loads(FaceLd<T> (&v)[FaceGrid<AXIS>::NU], int i0, int j0, int fi_lo, int fi_hi,
+                                           int fj_lo, int fj_hi, Load ld) {
+  using G = FaceGrid<AXIS>;
+#pragma unroll
+  for (int u = 0; u < G::NU; ++u) {
+    const int e = min(int(threadIdx.x) + u * kPNT, G::N - 1);
+    const int fi = min(max((AXIS == 0 ? i0 - 2 : i0) + e % G::FW, fi_lo), fi_hi - 1);
+    const int fj = min(max((AXIS == 0 ? j0 : j0 - 2) + e / G::FW, fj_lo), fj_hi - 1);
+    v[u] = ld(fi, fj);
+  }
+}
+
+// Pass 2: the products of ppm_products, with gbar(v, fi, fj) the face's
+// cotangent from its loaded values.
+template <int AXIS, class T, class Face, class Gbar>
+__device__ __forceinline__ void face_products(PpmLds<T, true> &L, const FaceLd<T> (&v)[FaceGrid<AXIS>::NU], int i0,
+                                              int j0, T p1, T p2, Face face, Gbar gbar) {
+  using G = FaceGrid<AXIS>;
+#pragma unroll
+  for (int u = 0; u < G::NU; ++u) {
+    const int e = int(threadIdx.x) + u * kPNT;
+    if (e < G::N) {
+      const int r = e / G::FW, cc = e % G::FW;
+      const int fi = (AXIS == 0 ? i0 - 2 : i0) + cc, fj = (AXIS == 0 ? j0 : j0 - 2) + r;
+      T pk[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+      if (face(fi, fj)) {
+        const T gb = gbar(v[u], fi, fj);
+        // Face (fi, fj) reads the field at fi - 3 .. fi + 2 (tile columns
+        // cc .. cc + 5 of row r + 5) or fj - 3 .. fj + 2 (tile rows r .. r + 5
+        // of column cc + 5).
+        T fv[6], w[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) fv[t] = AXIS == 0 ? L.f[(r + 5) * kFW + cc + t] : L.f[(r + t) * kFW + cc + 5];
+        ppm_weights(fv, v[u].c, p1, p2, w);
+#pragma unroll
+        for (int t = 0; t < 6; ++t) pk[t] = w[t] * gb;
+      }
+#pragma unroll
+      for (int t = 0; t < 6; ++t) L.p[t][e] = pk[t];
+    }
+  }
+}
+
+// Tile `blk` of stage MODE in the hoisted form: the field tile, the faces'
+// loads (face_loads) and the tile points' own loads are issued before the
+// first barrier; K5 issues its y faces' loads after forming the x products.
+// The same expressions in the same order as fvtp2d_adj_ptile_body.
+template <class T, int MODE>
+__device__ __forceinline__ void fvtp2d_adj_htile_body(PpmLds<T, true> &L, const FvAdj<T> &a, const Geo &g,
+                                                      const BoxGrid &b, int blk, int k) {
+  const int ntx = (b.ni + kPX - 1) / kPX;
+  const int i0 = b.ilo + (blk % ntx) * kPX, j0 = b.jlo + (blk / ntx) * kPY;
+  const int sj = g.sj, nx = g.nx, ny = g.ny;
+  const int64_t plane = int64_t(k) * g.sk;
+  const FvRegions R{nx, ny};
+  const T p1 = a.p1, p2 = a.p2;
+  auto at_of = [&](int i, int j) { return plane + i + int64_t(j) * sj; };
+  auto clampi = [](int x, int lo, int hi) { return min(max(x, lo), hi - 1); };
+  auto point = [&](int u, int &lx, int &ly) {
+    const int e = int(threadIdx.x) + u * kPNT;
+    lx = e % kPX;
+    ly = e / kPX;
+    return in_range(i0 + lx, b.ilo, b.ilo + b.ni) && in_range(j0 + ly, b.jlo, b.jlo + b.nj);
+  };
+  if (MODE == 1 || MODE == 2) {  // K1 (along j over q_j) or K2 (along i over q_i)
+    constexpr int AX = MODE == 1 ? 1 : 0;
+    FaceLd<T> v[FaceGrid<AX>::NU];
+    T tb[kPPts], tr[kPPts];  // q_j_bar, ra_x (K1) or q_i_bar, ra_y (K2) at the tile points
+    if (MODE == 1)
+      field_tile<0, kFH, 5, 5 + kPX>(L.f, a.q_j, plane, sj, i0, j0, 0, nx, -3, ny + 2);
+    else
+      field_tile<5, 5 + kPY, 0, kFW>(L.f, a.q_i, plane, sj, i0, j0, -3, nx + 2, 0, ny);
+    face_loads<AX, T>(v, i0, j0, 0, nx, 0, ny, [&](int fi, int fj) {
+      const int64_t m = at_of(fi, fj);
+      return MODE == 1 ? FaceLd<T>{a.cry[m], a.yfx[m], a.fy1b[m], T(0), T(0), T(0)}
+                       : FaceLd<T>{a.crx[m], a.xfx[m], a.fx1b[m], T(0), T(0), T(0)};
+    });
+#pragma unroll
+    for (int u = 0; u < kPPts; ++u) {
+      int lx, ly;
+      point(u, lx, ly);
+      const int64_t at = MODE == 1 ? at_of(clampi(i0 + lx, 0, nx), clampi(j0 + ly, -3, ny + 2))
+                                   : at_of(clampi(i0 + lx, -3, nx + 2), clampi(j0 + ly, 0, ny));
+      tb[u] = MODE == 1 ? a.qjb[at] : a.qib[at];
+      tr[u] = MODE == 1 ? a.ra_x[at] : a.ra_y[at];
+    }
+    lds_barrier();
+    face_products<AX, T>(L, v, i0, j0, p1, p2, [&](int i, int j) { return R.in(i, j); },
+                         [&](const FaceLd<T> &x, int, int) { return T(0.5) * x.g0 * x.g1; });
+    lds_barrier();
+#pragma unroll
+    for (int u = 0; u < kPPts; ++u) {
+      int lx, ly;
+      if (point(u, lx, ly)) {
+        const int64_t at = at_of(i0 + lx, j0 + ly);
+        if (MODE == 1)
+          a.w0[at] = (tb[u] + ppm_gather<1>(L, lx, ly)) / tr[u];
+        else
+          a.w1[at] = (tb[u] + ppm_gather<0>(L, lx, ly)) / tr[u];
+      }
+    }
+  } else {  // K5, with K3 and K4 formed per face
+    FaceLd<T> vx[FaceGrid<0>::NU];
+    field_tile<0, kFH, 0, kFW>(L.f, a.q, plane, sj, i0, j0, -3, nx + 3, -3, ny + 3);
+    // x faces on fx2's region [0, nx+1) x [-3, ny+2): W2 from fx2_bar, xfx,
+    // fx1_bar (read on [0, nx) x [0, ny)) and W0 at faces i and i-1 (read on
+    // q_j's region [0, nx) x [-3, ny+2)).
+    face_loads<0, T>(vx, i0, j0, 0, nx + 1, -3, ny + 2, [&](int fi, int fj) {
+      const int64_t m = at_of(fi, fj);
+      return FaceLd<T>{a.crx[m], a.fx2b[m], a.xfx[m], a.fx1b[at_of(min(fi, nx - 1), clampi(fj, 0, ny))],
+                       a.w0[at_of(min(fi, nx - 1), fj)], a.w0[at_of(max(fi - 1, 0), fj)]};
+    });
+    lds_barrier();
+    // W2 at face (i, j): fvtp2d_adj_k3's expression.
+    face_products<0, T>(L, vx, i0, j0, p1, p2, [&](int i, int j) { return R.fx2(i, j); },
+                        [&](const FaceLd<T> &x, int i, int j) {
+                          T r = x.g0;
+                          if (R.in(i, j)) r += T(0.5) * x.g1 * x.g2;
+                          T d = T(0);
+                          if (R.qj(i, j)) d += x.g3;
+                          if (R.qj(i - 1, j)) d -= x.g4;
+                          return r + x.g1 * d;
+                        });
+    // y faces on fy2's region [-3, nx+2) x [0, ny+1): W3 from fy2_bar, yfx,
+    // fy1_bar (read on [0, nx) x [0, ny)) and W1 at faces j and j-1 (read on
+    // q_i's region [-3, nx+2) x [0, ny)); and the tile points' area, W0, W1.
+    FaceLd<T> vy[FaceGrid<1>::NU];
+    face_loads<1, T>(vy, i0, j0, -3, nx + 2, 0, ny + 1, [&](int fi, int fj) {
+      const int64_t m = at_of(fi, fj);
+      return FaceLd<T>{a.cry[m], a.fy2b[m], a.yfx[m], a.fy1b[at_of(clampi(fi, 0, nx), min(fj, ny - 1))],
+                       a.w1[at_of(fi, min(fj, ny - 1))], a.w1[at_of(fi, max(fj - 1, 0))]};
+    });
+    T ta[kPPts], t0[kPPts], t1[kPPts];
+#pragma unroll
+    for (int u = 0; u < kPPts; ++u) {
+      int lx, ly;
+      point(u, lx, ly);
+      const int i = i0 + lx, j = j0 + ly;
+      ta[u] = a.area[at_of(clampi(i, -3, nx + 2), clampi(j, -3, ny + 2))];
+      t0[u] = a.w0[at_of(clampi(i, 0, nx), clampi(j, -3, ny + 2))];
+      t1[u] = a.w1[at_of(clampi(i, -3, nx + 2), clampi(j, 0, ny))];
+    }
+    lds_barrier();
+    T rx[kPPts];
+#pragma unroll
+    for (int u = 0; u < kPPts; ++u) {
+      int lx, ly;
+      point(u, lx, ly);
+      rx[u] = ppm_gather<0>(L, lx, ly);
+    }
+    lds_barrier();
+    // W3 at face (i, j): fvtp2d_adj_k4's expression.
+    face_products<1, T>(L, vy, i0, j0, p1, p2, [&](int i, int j) { return R.fy2(i, j); },
+                        [&](const FaceLd<T> &x, int i, int j) {
+                          T r = x.g0;
+                          if (R.in(i, j)) r += T(0.5) * x.g1 * x.g2;
+                          T d = T(0);
+                          if (R.qi(i, j)) d += x.g3;
+                          if (R.qi(i, j - 1)) d -= x.g4;
+                          return r + x.g1 * d;
+                        });
+    lds_barrier();
+#pragma unroll
+    for (int u = 0; u < kPPts; ++u) {
+      int lx, ly;
+      if (point(u, lx, ly)) {
+        const int i = i0 + lx, j = j0 + ly;
+        T r = T(0);
+        if (R.qj(i, j)) r += ta[u] * t0[u];
+        if (R.qi(i, j)) r += ta[u] * t1[u];
+        r += rx[u];
+        r += ppm_gather<1>(L, lx, ly);
+        a.qbar[at_of(i, j)] = r;
+      }
+    }
+  }
+}
+
 // Tile `blk` (of the box b) of one level of stage MODE, with the LDS of its
 // workgroup.
 template <class T, int MODE>
-__device__ __forceinline__ void fvtp2d_adj_ptile_body(PpmLds<T> &L, const FvAdj<T> &a, const Geo &g,
+__device__ __forceinline__ void fvtp2d_adj_ptile_body(PpmLds<T, false> &L, const FvAdj<T> &a, const Geo &g,
                                                       const BoxGrid &b, int blk, int k) {
   const int ntx = (b.ni + kPX - 1) / kPX;
   const int i0 = b.ilo + (blk % ntx) * kPX, j0 = b.jlo + (blk / ntx) * kPY;
@@ -2229,9 +2472,9 @@ __device__ __forceinline__ void fvtp2d_adj_ptile_body(PpmLds<T> &L, const FvAdj<
 // consecutive slots of its dispatch order (as Tiles::irow): linear block
 // x + 8 (ci + ntx q) takes row r = x + 8 q (r = tile row + nty level).  The
 // host sets it only when nty nz is a multiple of 8.
-template <class T, int MODE>
+template <class T, int MODE, bool H>
 __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxGrid b, int irow) {
-  __shared__ PpmLds<T> L;
+  __shared__ PpmLds<T, H> L;
   int blk = int(blockIdx.x), k = int(blockIdx.y);
   if (irow) {
     const int ntx = (b.ni + kPX - 1) / kPX, nty = (b.nj + kPY - 1) / kPY;
@@ -2240,21 +2483,31 @@ __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile(FvAdj<T> a, Geo g, BoxG
     blk = q % ntx + ntx * (r % nty);
     k = r / nty;
   }
-  fvtp2d_adj_ptile_body<T, MODE>(L, a, g, b, blk, k);
+  if constexpr (H)
+    fvtp2d_adj_htile_body<T, MODE>(L, a, g, b, blk, k);
+  else
+    fvtp2d_adj_ptile_body<T, MODE>(L, a, g, b, blk, k);
 }
 
 inline int ptile_count(const BoxGrid &b) { return ((b.ni + kPX - 1) / kPX) * ((b.nj + kPY - 1) / kPY); }
 
 // K1 and K2 in one launch: the first n1 tiles of each level run K1 on b1,
 // the others K2 on b2.  They read different fields and write work0 and work1.
-template <class T>
+template <class T, bool H>
 __global__ void __launch_bounds__(kPNT) fvtp2d_adj_ptile12(FvAdj<T> a, Geo g, BoxGrid b1, BoxGrid b2, int n1) {
-  __shared__ PpmLds<T> L;
+  __shared__ PpmLds<T, H> L;
   const int blk = int(blockIdx.x);
-  if (blk < n1)
-    fvtp2d_adj_ptile_body<T, 1>(L, a, g, b1, blk, int(blockIdx.y));
-  else
-    fvtp2d_adj_ptile_body<T, 2>(L, a, g, b2, blk - n1, int(blockIdx.y));
+  if constexpr (H) {
+    if (blk < n1)
+      fvtp2d_adj_htile_body<T, 1>(L, a, g, b1, blk, int(blockIdx.y));
+    else
+      fvtp2d_adj_htile_body<T, 2>(L, a, g, b2, blk - n1, int(blockIdx.y));
+  } else {
+    if (blk < n1)
+      fvtp2d_adj_ptile_body<T, 1>(L, a, g, b1, blk, int(blockIdx.y));
+    else
+      fvtp2d_adj_ptile_body<T, 2>(L, a, g, b2, blk - n1, int(blockIdx.y));
+  }
 }
 
 inline dim3 ptile_blocks(const BoxGrid &b, int nz) { return dim3(unsigned(ptile_count(b)), unsigned(nz)); }
@@ -2316,14 +2569,25 @@ int fvtp2d_adj_tiled_launch(const char *K, const oeb_domain *d, const FvAdj<T> &
   const BoxGrid bqj{0, -3, nx, ny + 5}, bqi{-3, 0, nx + 5, ny};
   const BoxGrid bq{-3, -3, nx + 6, ny + 6};
   const int n1 = ptile_count(bqj), n2 = ptile_count(bqi);
-  hipLaunchKernelGGL(fvtp2d_adj_ptile12<T>, dim3(unsigned(n1 + n2), unsigned(nz)), dim3(kPNT), 0, st, a, g, bqj,
-                     bqi, n1);
+  // Tile knob 2 (the default): the hoisted form, bit for bit the same as
+  // knob 1, the per-face loop of ppm_products.  At 256x256x80 FP64 / FP32:
+  // 295.6 / 150.5 us against 393.8 / 193.8 (1411 / 1057 us FP64 at
+  // 512x512x80); hoisted K1 and K2 alone gave 384 us, hoisted K5 alone 304.
+  const bool hoist = knob(K, sizeof(T), d, KNOB_TILE, 2) == 2;
+  const dim3 g12(unsigned(n1 + n2), unsigned(nz));
+  if (hoist)
+    hipLaunchKernelGGL((fvtp2d_adj_ptile12<T, true>), g12, dim3(kPNT), 0, st, a, g, bqj, bqi, n1);
+  else
+    hipLaunchKernelGGL((fvtp2d_adj_ptile12<T, false>), g12, dim3(kPNT), 0, st, a, g, bqj, bqi, n1);
   OEB_TRY(launch_status(K));
   // K5 with the tiles of a row on one XCD (order knob 2, the default;
   // 0 for row order): 408 us against 411 for the FP64 adjoint with 64 x 8 tiles.
   const int nty = (bq.nj + kPY - 1) / kPY;
   const int irow = (knob(K, sizeof(T), d, KNOB_ORDER, 2) == 2 && (int64_t(nty) * nz) % 8 == 0) ? 1 : 0;
-  hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 5>), ptile_blocks(bq, nz), dim3(kPNT), 0, st, a, g, bq, irow);
+  if (hoist)
+    hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 5, true>), ptile_blocks(bq, nz), dim3(kPNT), 0, st, a, g, bq, irow);
+  else
+    hipLaunchKernelGGL((fvtp2d_adj_ptile<T, 5, false>), ptile_blocks(bq, nz), dim3(kPNT), 0, st, a, g, bq, irow);
   return launch_status(K);
 }
 
@@ -2353,12 +2617,10 @@ int fvtp2d_adj_run(const oeb_domain *d, const S *q, const S *crx, const S *cry, 
   const BoxGrid bfx2{0, -3, int(nx + 1), int(ny + 5)}, bfy2{-3, 0, int(nx + 5), int(ny + 1)};
   const BoxGrid bq{-3, -3, int(nx + 6), int(ny + 6)};
   hipStream_t st = hipStream_t(stream);
-  // The tiled transposes (fvtp2d_adj_ptile) by default; tile knob 0
-  // selects the per-point gathers.
   // The tiled form (fvtp2d_adj_ptile: K1 and K2 in one launch, then K5, which
   // forms K3 and K4 inside) by default; tile knob 0 selects the five per-point passes,
   // with W2 and W3 through work2 and work3.
-  const bool tiled = knob(K, sizeof(T), d, KNOB_TILE, 1) != 0;
+  const bool tiled = knob(K, sizeof(T), d, KNOB_TILE, 2) != 0;
   if (tiled) return fvtp2d_adj_tiled_launch<T>(K, d, a, g, st);
   hipLaunchKernelGGL(fvtp2d_adj_k1<T>, box_blocks(bqj, int(nz)), dim3(kTI, kTJ), 0, st, a, g, bqj);
   OEB_TRY(launch_status(K));

@@ -229,6 +229,192 @@ __global__ void __launch_bounds__(kNT) fvtp2d_kernel(FvIn<T> a, FvOut<T> o, T p1
   }
 }
 
+// ------------------------------------------------------------- hoisted tiles
+// fvtp2d_kernel with the global loads of each phase issued together ahead of
+// its arithmetic.  In fvtp2d_kernel a lane's elements of a phase (three or
+// four of each region) each wait for their own loads in turn.  Here a lane
+// first loads the operands of all its elements of a region, at addresses
+// clamped into the region the phase computes (a clamped value belongs to an
+// element that is not computed, and is not used), and then forms them.  The
+// regions, expressions and stores are those of fvtp2d_kernel, so the results
+// are the same bits.  HV = 1: each region's loads at the start of its phase,
+// phase B's with phase A's.  HV = 2: one region ahead: the loads of q_i's
+// operands go out before phase B's arithmetic, q_j's before q_i's, and phase
+// D's before q_j's.  lds_barrier() orders LDS only, and the compiler moves
+// global loads across it: left alone it would issue every phase's loads at
+// the level's start (198 VGPRs, 2 waves per SIMD).  A compiler-only memory
+// fence before each later group keeps it where it is written.
+__device__ __forceinline__ void hold_loads() { __asm__ __volatile__("" ::: "memory"); }
+
+template <int kTI, int kTJ, int kNT, int HV, int WPE, class T>
+__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(WPE)))
+    fvtp2d_hkernel(FvIn<T> a, FvOut<T> o, T p1, T p2, Geo g, Tiles t) {
+  constexpr int kQW = kTI + 6, kQH = kTJ + 6;  // q:   i [i0-3, i0+TI+3) x j [j0-3, j0+TJ+3)
+  constexpr int kYW = kTI + 5, kYH = kTJ + 1;  // fy2: i [i0-3, i0+TI+2) x j [j0, j0+TJ+1)
+  constexpr int kXW = kTI + 1, kXH = kTJ + 5;  // fx2: i [i0, i0+TI+1)   x j [j0-3, j0+TJ+2)
+  constexpr int kIW = kTI + 5, kIH = kTJ;      // q_i: i [i0-3, i0+TI+2) x j [j0, j0+TJ)
+  constexpr int kJW = kTI, kJH = kTJ + 5;      // q_j: i [i0, i0+TI)     x j [j0-3, j0+TJ+2)
+  constexpr int nA = (kQH * kQW + kNT - 1) / kNT, nY = (kYH * kYW + kNT - 1) / kNT;
+  constexpr int nX = (kXH * kXW + kNT - 1) / kNT, nI = (kIH * kIW + kNT - 1) / kNT;
+  constexpr int nJ = (kJH * kJW + kNT - 1) / kNT, nD = (kTJ * kTI + kNT - 1) / kNT;
+  __shared__ T sq[kQH][kQW];
+  __shared__ T sfy2[kYH][kYW];
+  __shared__ T sfx2[kXH][kXW];
+  __shared__ T sqi[kIH][kIW];
+  __shared__ T sqj[kJH][kJW];
+
+  const TileCoord tc = tile_coord(t, kTI, kTJ, g.nz);
+  const int i0 = tc.i0, j0 = tc.j0;
+  const int nx = g.nx, ny = g.ny, sj = g.sj;
+  const int tid = int(threadIdx.x);
+  const bool first_i = i0 == 0, last_i = i0 + kTI >= nx;
+  const bool first_j = j0 == 0, last_j = j0 + kTJ >= ny;
+  auto own_i = [&](int gi) { return (gi >= i0 || first_i) && (gi < i0 + kTI || last_i); };
+  auto own_j = [&](int gj) { return (gj >= j0 || first_j) && (gj < j0 + kTJ || last_j); };
+  auto cl = [](int x, int lo, int hi) { return min(max(x, lo), hi - 1); };
+  // Element n of the lane in a region of W columns and N elements, clamped
+  // to the region's last element for the loads.
+  auto elem = [&](int n, int N) { return min(tid + n * kNT, N - 1); };
+
+  {  // one level per workgroup (the launch takes level chunks of 1)
+    const int64_t plane = int64_t(tc.k0) * g.sk;
+    const T *q = a.q + plane;
+    // Operands: q (A), cry (fy2), crx (fx2), area, yfx at j and j+1, ra_y
+    // (q_i), area, xfx at i and i+1, ra_x (q_j), and crx, cry, xfx, yfx (D).
+    T vq[nA], vcy[nY], vcx[nX];
+    T ia[nI], iy0[nI], iy1[nI], ir[nI];
+    T ja[nJ], jx0[nJ], jx1[nJ], jr[nJ];
+    T dcx[nD], dcy[nD], dxf[nD], dyf[nD];
+    auto load_qi = [&] {
+      hold_loads();
+#pragma unroll
+      for (int n = 0; n < nI; ++n) {
+        const int e = elem(n, kIH * kIW);
+        const int64_t at = plane + cl(i0 - 3 + e % kIW, -3, nx + 2) + int64_t(cl(j0 + e / kIW, 0, ny)) * sj;
+        ia[n] = a.area[at], iy0[n] = a.yfx[at], iy1[n] = a.yfx[at + sj], ir[n] = a.ra_y[at];
+      }
+    };
+    auto load_qj = [&] {
+      hold_loads();
+#pragma unroll
+      for (int n = 0; n < nJ; ++n) {
+        const int e = elem(n, kJH * kJW);
+        const int64_t at = plane + cl(i0 + e % kJW, 0, nx) + int64_t(cl(j0 - 3 + e / kJW, -3, ny + 2)) * sj;
+        ja[n] = a.area[at], jx0[n] = a.xfx[at], jx1[n] = a.xfx[at + 1], jr[n] = a.ra_x[at];
+      }
+    };
+    auto load_d = [&] {
+      hold_loads();
+#pragma unroll
+      for (int n = 0; n < nD; ++n) {
+        const int e = elem(n, kTJ * kTI);
+        const int64_t at = plane + cl(i0 + e % kTI, 0, nx) + int64_t(cl(j0 + e / kTI, 0, ny)) * sj;
+        dcx[n] = a.crx[at], dcy[n] = a.cry[at], dxf[n] = a.xfx[at], dyf[n] = a.yfx[at];
+      }
+    };
+
+    // A: q tile with halo (clamped into q's box [-3, nx+3) x [-3, ny+3); a
+    // point beyond it is never read by a computed element), with the
+    // operands of phase B.
+#pragma unroll
+    for (int n = 0; n < nA; ++n) {
+      const int e = elem(n, kQH * kQW);
+      vq[n] = q[cl(i0 - 3 + e % kQW, -3, nx + 3) + int64_t(cl(j0 - 3 + e / kQW, -3, ny + 3)) * sj];
+    }
+#pragma unroll
+    for (int n = 0; n < nY; ++n) {
+      const int e = elem(n, kYH * kYW);
+      vcy[n] = a.cry[plane + cl(i0 - 3 + e % kYW, -3, nx + 2) + int64_t(cl(j0 + e / kYW, 0, ny + 1)) * sj];
+    }
+#pragma unroll
+    for (int n = 0; n < nX; ++n) {
+      const int e = elem(n, kXH * kXW);
+      vcx[n] = a.crx[plane + cl(i0 + e % kXW, 0, nx + 1) + int64_t(cl(j0 - 3 + e / kXW, -3, ny + 2)) * sj];
+    }
+#pragma unroll
+    for (int n = 0; n < nA; ++n) {
+      const int e = tid + n * kNT;
+      if (e < kQH * kQW) sq[e / kQW][e % kQW] = vq[n];
+    }
+    lds_barrier();
+    if (HV == 2) load_qi();
+
+    // B: fy2 on i [-3, nx+2) x j [0, ny+1) and fx2 on i [0, nx+1) x j [-3, ny+2).
+#pragma unroll
+    for (int n = 0; n < nY; ++n) {
+      const int e = tid + n * kNT;
+      const int r = e / kYW, c = e % kYW;
+      const int gi = i0 - 3 + c, gj = j0 + r;
+      if (e < kYH * kYW && gi < nx + 2 && gj < ny + 1) {
+        const T v = ppm_face(sq[r][c], sq[r + 1][c], sq[r + 2][c], sq[r + 3][c], sq[r + 4][c], sq[r + 5][c], vcy[n],
+                             p1, p2);
+        sfy2[r][c] = v;
+        if (own_i(gi) && own_j(gj)) st_nt(o.fy2 + plane + gi + int64_t(gj) * sj, v);
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < nX; ++n) {
+      const int e = tid + n * kNT;
+      const int r = e / kXW, c = e % kXW;
+      const int gi = i0 + c, gj = j0 - 3 + r;
+      if (e < kXH * kXW && gi < nx + 1 && gj < ny + 2) {
+        const T *row = &sq[r][c];
+        const T v = ppm_face(row[0], row[1], row[2], row[3], row[4], row[5], vcx[n], p1, p2);
+        sfx2[r][c] = v;
+        if (own_i(gi) && own_j(gj)) st_nt(o.fx2 + plane + gi + int64_t(gj) * sj, v);
+      }
+    }
+    lds_barrier();
+
+    // C: q_i on i [-3, nx+2) x j [0, ny), q_j on i [0, nx) x j [-3, ny+2).
+    if (HV == 1) load_qi();
+    if (HV == 2) load_qj();
+#pragma unroll
+    for (int n = 0; n < nI; ++n) {
+      const int e = tid + n * kNT;
+      const int r = e / kIW, c = e % kIW;
+      const int gi = i0 - 3 + c, gj = j0 + r;
+      if (e < kIH * kIW && gi < nx + 2 && gj < ny) {
+        const T v = flux_div(sq[r + 3][c], ia[n], iy0[n], sfy2[r][c], iy1[n], sfy2[r + 1][c], ir[n]);
+        sqi[r][c] = v;
+        if (own_i(gi) && own_j(gj)) st_nt(o.q_i + plane + gi + int64_t(gj) * sj, v);
+      }
+    }
+    if (HV == 1) load_qj();
+    if (HV == 2) load_d();
+#pragma unroll
+    for (int n = 0; n < nJ; ++n) {
+      const int e = tid + n * kNT;
+      const int r = e / kJW, c = e % kJW;
+      const int gi = i0 + c, gj = j0 - 3 + r;
+      if (e < kJH * kJW && gi < nx && gj < ny + 2) {
+        const T v = flux_div(sq[r][c + 3], ja[n], jx0[n], sfx2[r][c], jx1[n], sfx2[r][c + 1], jr[n]);
+        sqj[r][c] = v;
+        if (own_i(gi) && own_j(gj)) st_nt(o.q_j + plane + gi + int64_t(gj) * sj, v);
+      }
+    }
+    lds_barrier();
+
+    // D: fx1, fy1 on the tile's interior points.
+    if (HV == 1) load_d();
+#pragma unroll
+    for (int n = 0; n < nD; ++n) {
+      const int e = tid + n * kNT;
+      const int r = e / kTI, c = e % kTI;
+      const int gi = i0 + c, gj = j0 + r;
+      if (e < kTJ * kTI && gi < nx && gj < ny) {
+        const int64_t at = plane + gi + gj * sj;
+        const T *ri = &sqi[r][c];
+        const T fx1 = ppm_face(ri[0], ri[1], ri[2], ri[3], ri[4], ri[5], dcx[n], p1, p2);
+        const T fy1 = ppm_face(sqj[r][c], sqj[r + 1][c], sqj[r + 2][c], sqj[r + 3][c], sqj[r + 4][c], sqj[r + 5][c],
+                               dcy[n], p1, p2);
+        st_nt(o.fx1 + at, ((T(0.5) * (fx1 + sfx2[r + 3][c])) * dxf[n]));
+        st_nt(o.fy1 + at, ((T(0.5) * (fy1 + sfy2[r][c + 3])) * dyf[n]));
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------ LDS-staged tiles
 // The tile form with every input of a level brought into LDS up front: the
 // eight input regions of a tile level are loaded straight into LDS
@@ -623,6 +809,18 @@ int fvtp2d_launch(const char *K, const FvIn<T> &a, const FvOut<T> &o, T p1, T p2
   return launch_status(K);
 }
 
+template <int TI, int TJ, int NT, int HV, int WPE, class T>
+int fvtp2d_hlaunch(const char *K, const FvIn<T> &a, const FvOut<T> &o, T p1, T p2, const Geo &g,
+                   const oeb_domain *d, void *stream) {
+  const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
+  const int order = knob(K, sizeof(T), d, KNOB_ORDER, 2);
+  const Tiles t = make_tiles(nx, ny, int(nz), TI, TJ, 1, order);
+  OEB_TRY(check_tiles(K, t, nx, ny, int(nz), TI, TJ));
+  hipLaunchKernelGGL((fvtp2d_hkernel<TI, TJ, NT, HV, WPE, T>), dim3(t.n), dim3(NT), 0, hipStream_t(stream), a, o, p1, p2,
+                     g, t);
+  return launch_status(K);
+}
+
 template <class T, class S>
 int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, const S *fx2, const S *fy1,
                const S *fy2, const S *q, const S *crx, const S *cry, const S *ra_x, const S *ra_y, const S *xfx,
@@ -732,8 +930,19 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   // workgroups and ran 1.2-1.3x slower.  FP32: 128x16 tiles of 512 lanes
   // (half the bytes per tile, so less halo for the same LDS) are fastest.
   // the tile knob = 1..12 forces one of the measured shapes, 0 the 64x8
-  // tiles of 256 lanes.
-  switch (knob(K, sizeof(T), d, KNOB_TILE, sizeof(T) == 4 ? 3 : 8)) {
+  // tiles of 256 lanes.  13-15 are the hoisted tiles (fvtp2d_hkernel): 64x16
+  // of 512 lanes with each region's loads at the start of its phase (13) or
+  // one region ahead (14, the FP64 default), and 64x8 of 256 lanes one region
+  // ahead (15).  FP64 at 256x256x80 / 512x512x80 (two interleaved series):
+  // 14 took 138.4-139.1 / 531.6-533.3 us against 147.2 / 556.0 for 8 (and
+  // 141-142.5 / 537-539 for 13, 142.5-143.3 / 545.8-546.6 for 15); the
+  // order knob 3 gave 138.7 / 529.4-529.9.  FP32 keeps the strip: hoisted
+  // tiles took 85-86 / 306-307 us against 91 / 311 for the strip and 82 / 393
+  // for its 128x16 tiles.
+  switch (knob(K, sizeof(T), d, KNOB_TILE, sizeof(T) == 4 ? 3 : 14)) {
+    case 13: return fvtp2d_hlaunch<64, 16, 512, 1, 6, T>(K, a, o, p1, p2, g, d, stream);
+    case 14: return fvtp2d_hlaunch<64, 16, 512, 2, 6, T>(K, a, o, p1, p2, g, d, stream);
+    case 15: return fvtp2d_hlaunch<64, 8, 256, 2, 5, T>(K, a, o, p1, p2, g, d, stream);
     case 10: return fvtp2d_launch<64, 16, 1024, T>(K, a, o, p1, p2, g, d, stream);
     case 11: return fvtp2d_launch<128, 16, 1024, T>(K, a, o, p1, p2, g, d, stream);
     case 12: return fvtp2d_launch<128, 8, 1024, T>(K, a, o, p1, p2, g, d, stream);

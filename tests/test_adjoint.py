@@ -483,7 +483,36 @@ def test_fvtp2d_adjoint_tiled_matches_per_point(shape, knobs):
     ybars = [torch.randn(y.shape, generator=g, device="cuda", dtype=y.dtype) for y in ys]
     print(f"[gpu] fvtp2d adjoint tiled vs per point {shape}", flush=True)
     out = []
-    for tiled in (0, 1):
+    for tiled in (0, 1, 2):
         knobs.set("fvtp2d_adj", "tile", tiled)
         out.append(_grads(lambda v: AG.Fvtp2d.apply(v, c, L, dom), [q], ybars)[0])
     assert _relmax(out[1], out[0]) <= 1e-14
+    # Tile 2 (the default) issues every face's loads first, at clamped
+    # addresses: the same products in the same order as tile 1.
+    assert torch.equal(out[2], out[1])
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("shape", [(64, 16, 2), (67, 9, 3), (130, 21, 2), (256, 256, 4)],
+                         ids=lambda s: "x".join(map(str, s)) if isinstance(s, tuple) else s)
+def test_fvtp2d_adjoint_hoisted_tiles_bit_exact(shape, precision, knobs):
+    """The hoisted tiles (tile knob 2) against the per-face tiles (tile knob
+    1) bit for bit, in both precisions, on shapes with whole and partial
+    tiles, through the adjoint entry point with every cotangent nonzero."""
+    import torch
+
+    L = Layout(*shape, h=3, hk=1)
+    host = cast("fvtp2d", generate("fvtp2d", L, "baseline", 61), precision)
+    c = _dev("fvtp2d", L, host, skip=("q",))
+    dom = (L.nx, L.ny, L.nz)
+    q = _t(host["q"])
+    ys = AG.Fvtp2d.apply(q, c, L, dom)
+    g = torch.Generator(device="cuda").manual_seed(12)
+    ybars = [torch.randn(y.shape, generator=g, device="cuda", dtype=torch.float64).to(y.dtype) for y in ys]
+    print(f"[gpu] fvtp2d adjoint hoisted vs per-face tiles {shape} {precision}", flush=True)
+    out = []
+    for tile in (1, 2):
+        knobs.set("fvtp2d_adj", "tile", tile)
+        out.append(_grads(lambda v: AG.Fvtp2d.apply(v, c, L, dom), [q], ybars)[0])
+    assert torch.isfinite(out[1]).all()
+    assert torch.equal(out[1], out[0])

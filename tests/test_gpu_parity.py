@@ -121,6 +121,24 @@ def test_strip_switches(name, shape, strip, rows, knobs):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("family", ["baseline", "stress"])
+@pytest.mark.parametrize("tile", [8, 13, 14, 15])
+@pytest.mark.parametrize("shape", [(64, 16, 2), (64, 36, 3), (67, 9, 2), (130, 17, 4), (1, 1, 1), (256, 256, 3)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_fvtp2d_hoisted_tiles(shape, tile, family, knobs):
+    """fvtp2d's register-staged tiles: the per-element loop (tile knob 8) and
+    the hoisted tiles (13: each region's loads at the start of its phase, 14:
+    one region ahead, the FP64 default, 15: 64 x 8 tiles one region ahead),
+    whose loads go to addresses clamped into each region, on whole and
+    partial tiles and one-point domains, agree bit for bit with the oracle."""
+    knobs.set("fvtp2d", "form", 0)
+    knobs.set("fvtp2d", "tile", tile)
+    L, host = inputs("fvtp2d", *shape, family, seed=19)
+    print(f"[gpu] fvtp2d tile={tile} {family} {shape}", flush=True)
+    _check("fvtp2d", L, host, run_gpu("fvtp2d", copy(host), L), require_exact=True)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("kchunk", [1, 3])
 @pytest.mark.parametrize("tile", [0, 1, 2])
 @pytest.mark.parametrize("shape", [(64, 36, 3), (67, 9, 2), (130, 17, 4), (1, 1, 1)],

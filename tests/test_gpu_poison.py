@@ -64,6 +64,28 @@ def test_poisoned_halo(name, shape, precision, form, knobs):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tile", [13, 14, 15])
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("shape", [(67, 9, 3), (130, 36, 4), (256, 20, 2)], ids=lambda s: "x".join(map(str, s)))
+def test_poisoned_halo_fvtp2d_hoisted(shape, precision, tile, knobs):
+    """fvtp2d's hoisted tiles (tile knobs 13-15) load each region's operands
+    at addresses clamped into the region; with NaN outside the read boxes,
+    every output still equals the oracle's bit for bit."""
+    knobs.set("fvtp2d", "form", 0)
+    knobs.set("fvtp2d", "tile", tile)
+    L, host = inputs("fvtp2d", *shape, "baseline", seed=47)
+    host = cast("fvtp2d", host, precision)
+    print(f"[gpu] poisoned halo fvtp2d hoisted tile={tile} {shape} {precision}", flush=True)
+    ref = copy(host)
+    B.run_oracle("fvtp2d", ref, L, precision=precision)
+    gpu = run_gpu("fvtp2d", poison("fvtp2d", host, L), L)
+    for o in KERNELS["fvtp2d"].outputs:
+        x, y = out_region("fvtp2d", o, L, gpu[o]), out_region("fvtp2d", o, L, ref[o])
+        assert np.isfinite(x).all(), f"fvtp2d.{o}: NaN from a point outside the read boxes"
+        assert bit_diffs(x, y) == 0, f"fvtp2d.{o}: differs from the oracle on unpoisoned inputs"
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("tile", [0, 1])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 @pytest.mark.parametrize("shape", [(67, 9, 3), (130, 36, 4), (256, 20, 2)], ids=lambda s: "x".join(map(str, s)))
