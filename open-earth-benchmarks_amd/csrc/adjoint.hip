@@ -1017,7 +1017,10 @@ __global__ void __launch_bounds__(NT, 4) hadvuv5th_adj_tile(HadvAdj<T> a, Geo g,
     };
     // (Loading both stage-1 points' uout_bar and vout_bar first, at clamped
     // addresses, spilled 40 bytes per lane and took 186 us against 152 at
-    // 256x256x80.)
+    // 256x256x80.  With the lane index passed through an opaque register at
+    // each level, so that no address is held across levels, it fit in 108
+    // VGPRs and took 160 us; loading them one level ahead, during the
+    // gathers, 110 VGPRs and 153-158 us, against 151.)
     for (int e = tid; e < kN1; e += NT) {
       const int pi = i0 - 3 + e % kW1, pj = j0 - 3 + e / kW1;
       T ob = T(0), ob2 = T(0);
