@@ -211,7 +211,7 @@ struct KnobInfo {
 // Names and valid ranges of the knobs, in enum order.
 constexpr KnobInfo kKnobs[KNOB_COUNT] = {
     {"kchunk", 1, 4096}, {"vector", 0, 1}, {"form", 0, 2},     {"rows", 1, 4096}, {"depth", 1, 8},
-    {"order", 0, 3},     {"tile", 0, 15},  {"lanes", 64, 1024}, {"grid", 1, 1 << 30},
+    {"order", 0, 3},     {"tile", 0, 17},  {"lanes", 64, 1024}, {"grid", 1, 1 << 30},
 };
 
 int knob_index(const char *name) {

@@ -135,6 +135,23 @@ def test_f32_fvtp2d_strip_shapes(lanes, depth, knobs):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("tile", [13, 14, 15, 16, 17])
+@pytest.mark.parametrize("shape", [(64, 36, 3), (67, 9, 2), (130, 17, 4), (128, 16, 8), (1, 1, 1)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_f32_fvtp2d_hoisted_tiles(shape, tile, order, knobs):
+    """fvtp2d's hoisted tiles in FP32 (tile knobs 13-17; 16 is the FP32
+    default), in two tile orders, on whole and partial tiles and one-point
+    domains, agree bit for bit with the FP32 oracle."""
+    knobs.set("fvtp2d", "form", 0)
+    knobs.set("fvtp2d", "tile", tile)
+    knobs.set("fvtp2d", "order", order)
+    L, host = inputs("fvtp2d", *shape, "stress", seed=21)
+    print(f"[gpu] f32 fvtp2d hoisted tile={tile} order={order} {shape}", flush=True)
+    _check_exact("fvtp2d", L, cast("fvtp2d", host, "f32"))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("tile", [0, 1, 2])
 @pytest.mark.parametrize("shape", [(64, 36, 3), (67, 9, 2), (130, 17, 4)], ids=lambda s: "x".join(map(str, s)))
 def test_f32_fvtp2d_lds_tiles(shape, tile, knobs):
