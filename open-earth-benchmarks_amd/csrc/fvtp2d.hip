@@ -813,8 +813,8 @@ template <int TI, int TJ, int NT, int HV, int WPE, class T>
 int fvtp2d_hlaunch(const char *K, const FvIn<T> &a, const FvOut<T> &o, T p1, T p2, const Geo &g,
                    const oeb_domain *d, void *stream) {
   const int64_t nx = d->nx, ny = d->ny, nz = d->nz;
-  // The tiles of a row on one XCD (2) in FP64, whole levels per XCD (3) in FP32.
-  const int order = knob(K, sizeof(T), d, KNOB_ORDER, sizeof(T) == 4 ? 3 : 2);
+  // The tiles of a row on one XCD (Tiles::irow).
+  const int order = knob(K, sizeof(T), d, KNOB_ORDER, 2);
   const Tiles t = make_tiles(nx, ny, int(nz), TI, TJ, 1, order);
   OEB_TRY(check_tiles(K, t, nx, ny, int(nz), TI, TJ));
   hipLaunchKernelGGL((fvtp2d_hkernel<TI, TJ, NT, HV, WPE, T>), dim3(t.n), dim3(NT), 0, hipStream_t(stream), a, o, p1, p2,
@@ -879,8 +879,8 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
 
   // Tiles by default (the hoisted tiles below); the form knob forces the
   // strips (1) or the LDS-staged tiles (2).  The strips were the FP32 default
-  // until the hoisted tiles: 85.6 / 307 us at 256x256x80 / 512x512x80
-  // against 70.8 / 304.5 for 128x16 hoisted tiles of 1024 lanes.
+  // until the hoisted tiles: in the ten-stencil step at 256x256x80 /
+  // 512x512x80, 93-94 / 304 us against 80.5 / 295-296 for the hoisted tiles.
   // Measured in the ten-stencil step at 256x256x80 and 512x512x80: FP32
   // strips of 320 lanes take 93 / 318 us against 118 / 396 us for the best
   // tiles (1.25x).  FP64 strips read 1.5x fewer bytes than the tiles
@@ -936,16 +936,15 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   // the tile knob = 1..12 forces one of the measured shapes, 0 the 64x8
   // tiles of 256 lanes.  13-17 are the hoisted tiles (fvtp2d_hkernel): 64x16
   // of 512 lanes with each region's loads at the start of its phase (13) or
-  // one region ahead (14, the FP64 default), and one region ahead 64x8 of 256
-  // lanes (15), 128x16 of 1024 (16, the FP32 default) and 128x8 of 512 (17).
-  // FP64 at 256x256x80 / 512x512x80 (two interleaved series): 14 took
-  // 138.4-139.1 / 531.6-533.3 us against 147.2 / 556.0 for 8 (141-142.5 /
-  // 537-539 for 13, 142.5-143.3 / 545.8-546.6 for 15, 152 / 565 for 16, 150
-  // / 563 for 17); the order knob 3 gave 138.7 / 529.4-529.9.  FP32: 16 in
-  // the level-interleaved order (3) took 70.8 / 304.5 us, 14 76.1 / 304.7,
-  // 17 77.7 / 312, 256x8 tiles of 1024 lanes 75-76 / 315-318, the strips
-  // 85.6 / 307 and the 128x16 tiles of 512 lanes (3) 81.6 / 389.5.
-  switch (knob(K, sizeof(T), d, KNOB_TILE, sizeof(T) == 4 ? 16 : 14)) {
+  // one region ahead (14, the default), and one region ahead 64x8 of 256
+  // lanes (15), 128x16 of 1024 (16) and 128x8 of 512 (17).  In the
+  // ten-stencil step at 256x256x80 / 512x512x80, FP64: 14 took 133.1-134.1
+  // / 516.2-516.9 us, 13 138.2-139.1 / 526.2-527.4, 8 144.1-144.6 /
+  // 546.4-548.1; FP32: 14 80.5 / 295.2-295.7, 16 in the level-interleaved
+  // order 80.4-81.1 / 305.1-305.9.  Launched alone: 15 took 142.5-143.3 /
+  // 545.8-546.6 in FP64, 17 150 / 563; 256x8 tiles of 1024 lanes 75-76 /
+  // 315-318 in FP32, the 128x16 tiles of 512 lanes (3) 81.6 / 389.5.
+  switch (knob(K, sizeof(T), d, KNOB_TILE, 14)) {
     case 13: return fvtp2d_hlaunch<64, 16, 512, 1, 6, T>(K, a, o, p1, p2, g, d, stream);
     case 14: return fvtp2d_hlaunch<64, 16, 512, 2, 6, T>(K, a, o, p1, p2, g, d, stream);
     case 15: return fvtp2d_hlaunch<64, 8, 256, 2, 5, T>(K, a, o, p1, p2, g, d, stream);

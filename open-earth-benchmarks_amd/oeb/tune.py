@@ -60,7 +60,7 @@ SPACES = {
     "uvbke": _tiled([("vector", (1, 0))]),
     "p_grad_c": _tiled([("vector", (1, 0))]),
     "nh_p_grad": _tiled([("vector", (1, 0))]),
-    "fvtp2d": [({"form": 0}, [("tile", (14, 16, 13, 15, 17, 8, 3)), ("order", (2, 3, 0))]),
+    "fvtp2d": [({"form": 0}, [("tile", (14, 13, 16, 15, 17, 8, 3)), ("order", (2, 3, 0))]),
                ({"form": 1}, [("lanes", (512, 256, 1024)), ("rows", ROWS), ("depth", (1, 2))]),
                ({"form": 2}, [("tile", (0, 1, 2)), ("order", (2, 3, 0)), ("kchunk", (1, 2))])],
 }

@@ -140,7 +140,7 @@ def test_f32_fvtp2d_strip_shapes(lanes, depth, knobs):
 @pytest.mark.parametrize("shape", [(64, 36, 3), (67, 9, 2), (130, 17, 4), (128, 16, 8), (1, 1, 1)],
                          ids=lambda s: "x".join(map(str, s)))
 def test_f32_fvtp2d_hoisted_tiles(shape, tile, order, knobs):
-    """fvtp2d's hoisted tiles in FP32 (tile knobs 13-17; 16 is the FP32
+    """fvtp2d's hoisted tiles in FP32 (tile knobs 13-17; 14 is the
     default), in two tile orders, on whole and partial tiles and one-point
     domains, agree bit for bit with the FP32 oracle."""
     knobs.set("fvtp2d", "form", 0)

@@ -128,8 +128,8 @@ def test_strip_switches(name, shape, strip, rows, knobs):
 def test_fvtp2d_hoisted_tiles(shape, tile, family, knobs):
     """fvtp2d's register-staged tiles: the per-element loop (tile knob 8) and
     the hoisted tiles (13: each region's loads at the start of its phase, 14:
-    one region ahead, the FP64 default, 15: 64 x 8 tiles, 16: 128 x 16 tiles
-    of 1024 lanes, the FP32 default, 17: 128 x 8 tiles, each one region ahead),
+    one region ahead, the default, 15: 64 x 8 tiles, 16: 128 x 16 tiles of
+    1024 lanes, 17: 128 x 8 tiles, each one region ahead),
     whose loads go to addresses clamped into each region, on whole and
     partial tiles and one-point domains, agree bit for bit with the oracle."""
     knobs.set("fvtp2d", "form", 0)
