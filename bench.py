@@ -864,6 +864,12 @@ def adjoint_block(L, forms, seed, steps: int, segment: int = 10, reps: int = 5):
     return out
 
 
+def _progress(msg: str) -> None:
+    """One line on stderr as each block of the run ends (the JSON line is the
+    only stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main(argv=None):
     args = parse_args(argv)
     if args.gpus != 1:
@@ -910,6 +916,7 @@ def main(argv=None):
             dev[prec][name] = d
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
+    _progress(f"inputs ready ({t_setup:.1f} s)")
 
     run = lambda name, fields: stencils.run(name, fields, domain=(n, n, args.nz))  # noqa: E731
     timed = {}
@@ -917,6 +924,7 @@ def main(argv=None):
         step_ms, durs = time_steps(names, dev[prec], run, args.steps, args.warmup)
         bpp = {nm: KERNELS[nm].bytes_per_point(prec) for nm in names}
         timed[prec] = (step_ms, per_stencil(names, durs, pts, bpp))
+        _progress(f"{prec} step timed: {step_ms:.4f} ms")
 
     copy_line = None
     if args.copy_mib > 0:
@@ -1005,18 +1013,24 @@ def main(argv=None):
         torch.cuda.empty_cache()
         result[f"target_{args.target_size}x{args.target_size}x{args.nz}"] = target_block(
             args.target_size, args.nz, args.seed, args.steps, args.warmup)
+        _progress("target block done")
     result["config1"] = config1_block(args.seed, 5.0, with_cpu=not args.no_cpu_baseline)
     result["hbm_ceiling"] = ceiling_block()
+    _progress("config1 and copy-ceiling blocks done")
     if not args.no_chain:
         result["chain"] = chain_block(L, forms, args.seed)
+        _progress("chain block done")
     if not args.no_ensemble and args.members > 0:
         result["ensemble"] = ensemble_block(args.members, args.seed)
+        _progress("ensemble block done")
     if not args.no_adjoint:
         result["adjoint"] = adjoint_block(L, forms, args.seed, args.adjoint_steps)
+        _progress("adjoint block done")
     if copy_line:
         result["device_copy"] = copy_line
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(names, hosts, L, args.cpu_budget_s)
+        _progress("cpu baseline done")
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)
