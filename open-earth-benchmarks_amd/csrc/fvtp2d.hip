@@ -943,7 +943,10 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   // 546.4-548.1; FP32: 14 80.5 / 295.2-295.7, 16 in the level-interleaved
   // order 80.4-81.1 / 305.1-305.9.  Launched alone: 15 took 142.5-143.3 /
   // 545.8-546.6 in FP64, 17 150 / 563; 256x8 tiles of 1024 lanes 75-76 /
-  // 315-318 in FP32, the 128x16 tiles of 512 lanes (3) 81.6 / 389.5.
+  // 315-318 in FP32, the 128x16 tiles of 512 lanes (3) 81.6 / 389.5.  Taller
+  // hoisted tiles, with less j-halo per point, measured no faster in the step:
+  // 64x24 of 768 lanes (two workgroups per CU) the same as 14, 64x24 of 512
+  // and 64x32 of 1024 lanes 0.5-2% slower.
   switch (knob(K, sizeof(T), d, KNOB_TILE, 14)) {
     case 13: return fvtp2d_hlaunch<64, 16, 512, 1, 6, T>(K, a, o, p1, p2, g, d, stream);
     case 14: return fvtp2d_hlaunch<64, 16, 512, 2, 6, T>(K, a, o, p1, p2, g, d, stream);
