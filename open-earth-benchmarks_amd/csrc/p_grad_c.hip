@@ -169,6 +169,9 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
   // Column pairs by default in the FP32 forms only.  Measured in the
   // ten-stencil step at 256x256x80 and 512x512x80: FP32 pairs take
   // 52 / 189 us against 55 / 203 us for single columns; FP64 pairs were slower.
+  // Four columns per lane (16-byte FP32 accesses, 120 VGPRs) measured no
+  // faster with deviation islands (149 against 148 us at 512x512x80, 52.5
+  // against 43 at 256x256x80), 3% faster with FP64 islands at 512x512x80.
   if (knob_on(K, sizeof(T), d, KNOB_VECTOR, sizeof(T) == 4) && nx % 2 == 0 &&
       pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdxc, a.rdyc, a.delpc, a.gz.ptr(), a.pkc.ptr()})) {
     const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc, xo);
