@@ -105,7 +105,11 @@ const BatchKind kHdiffsmagBatch{"hdiffsmag", hdiffsmag_batch<T>};
 // of them make a 256-lane workgroup and the same body runs in the plain launch
 // and in a batched (persistent) one.  Rows are loaded D rows ahead; addresses
 // of lanes and rows outside the segment are clamped into it (never stored).
-// The expressions are those of hdiffsmag_body, term for term.
+// The expressions are those of hdiffsmag_body, term for term.  (Carrying the
+// squares of T and S between rows and columns, so that a lane forms 2V + 2 of
+// them per row instead of 6V, gave the same bits and no gain in FP64: 182.8
+// against 181-183 us at 512x512x80 with the same 3 waves per SIMD, 189.6 when
+// the compiler fitted it in 114 VGPRs for 4 waves.)
 // Segment layout in a Tiles: ti = segments per row, tj = row chunks,
 // kchunk = rows per chunk, nchunk = levels, n = workgroups.
 constexpr int kSegLanes = 64, kSegWaves = 4;
