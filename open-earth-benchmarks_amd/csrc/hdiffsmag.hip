@@ -246,11 +246,13 @@ int hdiffsmag_seg_run(const HsArgs<T> &a, const Geo &g, const oeb_domain *d, con
                       hipStream_t stream) {
   const char *K = "hdiffsmag";
   // Rows per chunk by the lanes that cover one row (default_strip_rows), so
-  // that narrow rows get short chunks and enough waves; at most 16, which
-  // measured 184 us against 194 for the formula's 32 in FP64 at 512x512x80
-  // (the ten-stencil step; the other cases already take 16 rows or fewer).
+  // that narrow rows get short chunks and enough waves; at most 8.  In the
+  // ten-stencil step at 512x512x80 (where the formula gives 32): 16 rows took
+  // 184 us against 194 for 32 in FP64, and 8 rows 178-179 against 183 for 16
+  // (FP32 94.7 against 95.7); 6 rows 180, 10 and 12 183.  At 256x256x80 the
+  // formula's chunks are already 8 rows or fewer.
   const int row_lanes = make_segments(d->nx, d->ny, int(d->nz), V, 1).ti * kSegLanes;
-  const int rows = knob(K, sizeof(T), d, KNOB_ROWS, std::min(16, default_strip_rows(d->ny, row_lanes)));
+  const int rows = knob(K, sizeof(T), d, KNOB_ROWS, std::min(8, default_strip_rows(d->ny, row_lanes)));
   const Tiles t = make_segments(d->nx, d->ny, int(d->nz), V, rows);
   if (t.n < 1 || int64_t(t.ti) * t.tj * t.nchunk > (int64_t(1) << 30))
     return fail(OEB_ERR_DOMAIN, std::string(K) + ": domain too large");
