@@ -950,7 +950,10 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   // hoisted tile as a buffer based at the tile's corner (one 32-bit offset
   // per element for all its operands, only upper clamps) cut its vector
   // integer instructions from 453 to 324 and all its vector instructions by
-  // 12%, and measured 2% faster in FP32 and 1-2% slower in FP64.
+  // 12%, and measured 2% faster in FP32 and 1-2% slower in FP64.  Eight
+  // waves per SIMD instead of six (64x12 tiles with q_i written over the q
+  // tile after one more barrier: 58 VGPRs, 35 KB, four workgroups per CU)
+  // took 540-543 / 143 us against 519 / 133 at 512x512x80 / 256x256x80.
   switch (knob(K, sizeof(T), d, KNOB_TILE, 14)) {
     case 13: return fvtp2d_hlaunch<64, 16, 512, 1, 6, T>(K, a, o, p1, p2, g, d, stream);
     case 14: return fvtp2d_hlaunch<64, 16, 512, 2, 6, T>(K, a, o, p1, p2, g, d, stream);
