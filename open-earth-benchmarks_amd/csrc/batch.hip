@@ -57,6 +57,8 @@ bool batch_record(const BatchKind &kind, const void *args, size_t bytes, const G
   return true;
 }
 
+bool batch_recording() { return g_recording != nullptr; }
+
 int refuse_while_recording(const char *kname) {
   if (g_recording == nullptr) return OEB_OK;
   return fail(OEB_ERR_ARG, std::string(kname) +

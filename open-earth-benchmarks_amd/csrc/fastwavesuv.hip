@@ -107,10 +107,11 @@ hipError_t fastwavesuv_batch(const BatchMember *tab, int n, int total, hipStream
 template <class T, class IS>
 const BatchKind kFastwavesuvBatch{"fastwavesuv", fastwavesuv_batch<T, IS>};
 
-// Two adjacent columns (i, i+1) per lane, i even: the element-type fields
-// move as pairs (16 bytes in FP64, 8 in FP32), ppuv and hhl as pairs of
-// their storage (16-byte FP64 or 8-byte FP32 deviation pairs).  The east neighbour of column i is column i+1 of the pair;
-// that of column i+1 is a single load at i+2.  Suffix P: the pair at (i, i+1);
+// Tiles of 128 columns x TJ rows (4 or 8).  Two adjacent columns (i, i+1)
+// per lane, i even: the element-type fields move as pairs (16 bytes in FP64,
+// 8 in FP32), ppuv and hhl as pairs of their storage (16-byte FP64 or 8-byte
+// FP32 deviation pairs).  The east neighbour of column i is column i+1 of the
+// pair; that of column i+1 is a single load at i+2.  Suffix P: the pair at (i, i+1);
 // Q: the pair at row j+1; X: column i+2 at row j.
 template <class T>
 struct Pr {
@@ -134,10 +135,10 @@ __device__ __forceinline__ void stp_nt(T *p, T x, T y) {
   stv_nt<T, 2>(p, v);
 }
 
-template <class T, class IS>
+template <class T, class IS, int TJ = kTileJ>
 __device__ __forceinline__ void fastwavesuv_pair_body(const FwArgs<T, IS> &a, const Geo &g, const Tiles &t, int id) {
   const T edadlat = a.edadlat, dt = a.dt;
-  const TileCoord c = tile_coord_of(t, id, kPairI, kTileJ, g.nz);
+  const TileCoord c = tile_coord_of(t, id, kPairI, TJ, g.nz);
   const int i = c.i0 + 2 * int(threadIdx.x), j = c.j0 + int(threadIdx.y);
   if (i >= g.nx || j >= g.ny) return;
   const int sj = g.sj;
@@ -191,9 +192,9 @@ __device__ __forceinline__ void fastwavesuv_pair_body(const FwArgs<T, IS> &a, co
   }
 }
 
-template <class T, class IS>
-__global__ void __launch_bounds__(kTileI *kTileJ) fastwavesuv_pair_kernel(FwArgs<T, IS> a, Geo g, Tiles t) {
-  fastwavesuv_pair_body(a, g, t, int(blockIdx.x));
+template <class T, class IS, int TJ = kTileJ>
+__global__ void __launch_bounds__(kTileI *TJ) fastwavesuv_pair_kernel(FwArgs<T, IS> a, Geo g, Tiles t) {
+  fastwavesuv_pair_body<T, IS, TJ>(a, g, t, int(blockIdx.x));
 }
 
 template <class T, class IS>
@@ -258,19 +259,39 @@ int fastwavesuv_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S
   a.vout = const_cast<T *>(p_vo);
   a.edadlat = edadlat;
   a.dt = dt;
-  // Levels per workgroup: 8, and 4 in the FP32 form, which measured 56 / 203 us against 59 / 204
-  // with 8 at 256x256x80 / 512x512x80 (the ten-stencil step).  FP64 takes 3 on levels of at
-  // most 256x256 points: 83.7 us against 85.1-85.9 with 8 at 256x256x80; at 512x512x80 the
-  // chunks 2..10 measured 303-313 us, 8 among the fastest.
-  const int kc_dflt = sizeof(T) == 4 ? 4 : (int64_t(nx) * ny <= 256 * 256 ? 3 : kChunk);
+  // Tile height: 8 rows on levels of more than 256x256 points in plain launches (tile knob 1),
+  // else 4 (tile knob 0).  Batched launches take 4 rows, so that one launch covers the members of
+  // a run whatever their sizes.  At 512x512x80 (the ten-stencil step), 8-row tiles of 16 levels
+  // took FP64 290.5-290.8 us against 298.3 for 4 rows and 8 levels (12-40 levels: 290.6-292.9;
+  // 4 rows with 12-40 levels: 298.1-311.3); FP32 8-row tiles of 12 levels 196.5-196.7 against
+  // 198.6-198.8 (12-40 levels: 196.5-204.0).  The taller tile fetches the j+1 row of ppuv,
+  // wgtfac, hhl and rho, which the tile above owns, for 8 rows instead of 4, and the longer chunk
+  // fetches the k-halo planes once per 16 levels.  At 256x256x80 8-row tiles measured slower
+  // (FP64 83.1-85.7 against 81.3 us, FP32 56.9-57.2 against 54.1-54.3).  16-row tiles of 1024
+  // lanes spill registers.
+  const bool big = int64_t(nx) * ny > 256 * 256;
+  const bool pairs =
+      knob(K, sizeof(T), d, KNOB_VECTOR, true) != 0 && nx % 2 == 0 &&
+      pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.utens, a.vtens, a.wgtfac, a.ppuv.ptr(), a.hhl.ptr(), a.rho});
+  const bool tall = pairs && !batch_recording() && knob(K, sizeof(T), d, KNOB_TILE, big ? 1 : 0) == 1;
+  const int tj = tall ? 8 : kTileJ;
+  // Levels per workgroup with 4-row tiles: 8, and 4 in the FP32 form, which measured 56 / 203 us
+  // against 59 / 204 with 8 at 256x256x80 / 512x512x80.  FP64 takes 3 on levels of at most
+  // 256x256 points: 83.7 us against 85.1-85.9 with 8 at 256x256x80; at 512x512x80 the chunks
+  // 2..10 measured 303-313 us, 8 among the fastest.
+  const int kc_dflt = tall ? (sizeof(T) == 4 ? 12 : 16) : sizeof(T) == 4 ? 4 : (big ? kChunk : 3);
   const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, kc_dflt);
   // The tiles of a row on one XCD (Tiles::irow): FP32 204 / 58 us against 209 / 60
   // at 512x512x80 / 256x256x80, FP64 the same (order knob 0 for row order).
   const int xo = knob(K, sizeof(T), d, KNOB_ORDER, 2);
-  if ((knob(K, sizeof(T), d, KNOB_VECTOR, true) != 0) && nx % 2 == 0 &&
-      pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.utens, a.vtens, a.wgtfac, a.ppuv.ptr(), a.hhl.ptr(), a.rho})) {
-    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc, xo);
-    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
+  if (pairs) {
+    const Tiles t = make_tiles(nx, ny, int(nz), kPairI, tj, kc, xo);
+    OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, tj));
+    if (tall) {
+      hipLaunchKernelGGL((fastwavesuv_pair_kernel<T, IS, 8>), dim3(t.n), dim3(kTileI, 8), 0, hipStream_t(stream), a,
+                         g, t);
+      return launch_status(K);
+    }
     if (batch_record(kFastwavesuvPairBatch<T, IS>, a, g, t, t.n, as)) return OEB_OK;
     hipLaunchKernelGGL((fastwavesuv_pair_kernel<T, IS>), dim3(t.n), dim3(kTileI, kTileJ), 0, hipStream_t(stream), a,
                        g, t);

@@ -467,6 +467,9 @@ bool batch_record(const BatchKind &kind, const A &args, const Geo &g, const Tile
   return batch_record(kind, &args, sizeof(A), g, t, blocks, as);
 }
 
+// True while this host thread records a batch.
+bool batch_recording();
+
 // Fails with OEB_ERR_ARG when this host thread is recording a batch: for the
 // entry points that have no batched form, which would otherwise launch at
 // once while the batch's earlier records have not run.

@@ -56,7 +56,7 @@ SPACES = {
                   ({"form": 0}, [("kchunk", KCHUNKS)])],
     "hadvuv": _tiled([("lanes", (1024, 512, 256))]),
     "hadvuv5th": _tiled([("lanes", (1024, 512, 256))]),
-    "fastwavesuv": _tiled([("vector", (1, 0))]),
+    "fastwavesuv": _tiled([("vector", (1, 0)), ("tile", (1, 0))]),
     "uvbke": _tiled([("vector", (1, 0))]),
     "p_grad_c": _tiled([("vector", (1, 0))]),
     "nh_p_grad": _tiled([("vector", (1, 0))]),

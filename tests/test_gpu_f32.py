@@ -135,6 +135,21 @@ def test_f32_fvtp2d_strip_shapes(lanes, depth, knobs):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kchunk", [1, 3, 12])
+@pytest.mark.parametrize("tile", [0, 1])
+@pytest.mark.parametrize("shape", [(64, 36, 7), (130, 17, 5), (2, 1, 1)], ids=lambda s: "x".join(map(str, s)))
+def test_f32_fastwavesuv_tile_rows(shape, tile, kchunk, knobs):
+    """fastwavesuv's FP32 column-pair tiles of 4 and 8 rows (tile knobs 0
+    and 1), on whole and partial tiles, with level chunks that divide nz, do
+    not divide it, and exceed it, agree bit for bit with the FP32 oracle."""
+    knobs.set("fastwavesuv", "tile", tile)
+    knobs.set("fastwavesuv", "kchunk", kchunk)
+    L, host = inputs("fastwavesuv", *shape, "stress", seed=29)
+    print(f"[gpu] f32 fastwavesuv tile={tile} kchunk={kchunk} {shape}", flush=True)
+    _check_exact("fastwavesuv", L, cast("fastwavesuv", host, "f32"))
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("order", [2, 3])
 @pytest.mark.parametrize("tile", [13, 14, 15, 16, 17])
 @pytest.mark.parametrize("shape", [(64, 36, 3), (67, 9, 2), (130, 17, 4), (128, 16, 8), (1, 1, 1)],

@@ -84,20 +84,39 @@ def test_f32d_against_f64(name, n):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kchunk", [1, 3, 12])
+@pytest.mark.parametrize("tile", [0, 1])
+@pytest.mark.parametrize("shape", [(64, 36, 7), (130, 17, 5)], ids=lambda s: "x".join(map(str, s)))
+def test_f32d_fastwavesuv_tile_rows(shape, tile, kchunk, knobs):
+    """fastwavesuv's deviation-island form in column-pair tiles of 4 and 8
+    rows (tile knobs 0 and 1), with level chunks that divide nz, do not
+    divide it, and exceed it, agrees bit for bit with the FP32 oracle."""
+    knobs.set("fastwavesuv", "tile", tile)
+    knobs.set("fastwavesuv", "kchunk", kchunk)
+    L, host = inputs("fastwavesuv", *shape, "baseline", seed=33)
+    print(f"[gpu] f32d fastwavesuv tile={tile} kchunk={kchunk} {shape}", flush=True)
+    _run("fastwavesuv", cast("fastwavesuv", host, "f32"), L)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile", [0, 1])
 @pytest.mark.parametrize("pairs", [0, 1])
 @pytest.mark.parametrize("shape", [(67, 9, 3), (130, 36, 4)], ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("name", sorted(DEVIATION_STENCILS))
-def test_f32d_poisoned_halo(name, shape, pairs, knobs):
+def test_f32d_poisoned_halo(name, shape, pairs, tile, knobs):
     """Every deviation and input point outside the stencil's read box, and
     each base profile's level -1 where the island's box starts at level 0,
     set to NaN: the outputs still equal the FP32 oracle on the unpoisoned
-    inputs bit for bit."""
+    inputs bit for bit (fastwavesuv also in 8-row tiles, tile knob 1)."""
     import torch
 
     from oeb.layout import Layout1D
     from reach import BOXES, BOXES_1D, keep_mask, keep_mask_1d
 
+    if tile and name != "fastwavesuv":
+        pytest.skip("tile rows: fastwavesuv only")
     knobs.set(name, "vector", pairs)
+    knobs.set(name, "tile", tile)
     L, host = inputs(name, *shape, "baseline", seed=43)
     host32 = cast(name, host, "f32")
     rebuilt = copy(host32)

@@ -100,6 +100,25 @@ def test_path_switches(name, pairs, kchunk, knobs):
     _check(name, L, host, run_gpu(name, copy(host), L), require_exact=True)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("kchunk", [1, 3, 16])
+@pytest.mark.parametrize("tile", [0, 1])
+@pytest.mark.parametrize("family", ["baseline", "stress"])
+@pytest.mark.parametrize("shape", [(64, 36, 7), (130, 17, 5), (2, 1, 1), (300, 8, 3)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_fastwavesuv_tile_rows(shape, family, tile, kchunk, knobs):
+    """fastwavesuv's column-pair tiles of 4 rows (tile knob 0) and of 8 rows
+    (tile knob 1, the default on levels of more than 256x256 points), on
+    whole and partial tiles in i and j and a one-pair domain, with level
+    chunks that divide nz, do not divide it, and exceed it, agree bit for bit
+    with the oracle."""
+    knobs.set("fastwavesuv", "tile", tile)
+    knobs.set("fastwavesuv", "kchunk", kchunk)
+    L, host = inputs("fastwavesuv", *shape, family, seed=27)
+    print(f"[gpu] fastwavesuv tile={tile} kchunk={kchunk} {family} {shape}", flush=True)
+    _check("fastwavesuv", L, host, run_gpu("fastwavesuv", copy(host), L), require_exact=True)
+
+
 STRIPPED = ["laplace", "hdiffsa", "hdiffsmag", "fvtp2d"]
 
 

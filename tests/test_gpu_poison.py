@@ -86,6 +86,26 @@ def test_poisoned_halo_fvtp2d_hoisted(shape, precision, tile, knobs):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("shape", [(66, 9, 3), (130, 36, 4), (256, 20, 2)], ids=lambda s: "x".join(map(str, s)))
+def test_poisoned_halo_fastwavesuv_tall(shape, precision, knobs):
+    """fastwavesuv's 8-row column-pair tiles (tile knob 1) read nothing
+    outside the read boxes: with NaN there, every output still equals the
+    oracle's bit for bit."""
+    knobs.set("fastwavesuv", "tile", 1)
+    L, host = inputs("fastwavesuv", *shape, "baseline", seed=49)
+    host = cast("fastwavesuv", host, precision)
+    print(f"[gpu] poisoned halo fastwavesuv tall {shape} {precision}", flush=True)
+    ref = copy(host)
+    B.run_oracle("fastwavesuv", ref, L, precision=precision)
+    gpu = run_gpu("fastwavesuv", poison("fastwavesuv", host, L), L)
+    for o in KERNELS["fastwavesuv"].outputs:
+        x, y = out_region("fastwavesuv", o, L, gpu[o]), out_region("fastwavesuv", o, L, ref[o])
+        assert np.isfinite(x).all(), f"fastwavesuv.{o}: NaN from a point outside the read boxes"
+        assert bit_diffs(x, y) == 0, f"fastwavesuv.{o}: differs from the oracle on unpoisoned inputs"
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("tile", [0, 1])
 @pytest.mark.parametrize("precision", ["f64", "f32"])
 @pytest.mark.parametrize("shape", [(67, 9, 3), (130, 36, 4), (256, 20, 2)], ids=lambda s: "x".join(map(str, s)))
