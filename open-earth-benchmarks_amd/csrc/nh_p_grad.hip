@@ -196,7 +196,10 @@ int nh_p_grad_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 
   // Levels per workgroup: 8, and 4 in the FP32 form, which measured 60 / 219 us against 62 / 219
   // with 8 at 256x256x80 / 512x512x80 (the ten-stencil step).  FP64 takes 3 on levels of at
   // most 256x256 points: 79.6 us against 82.3-83.1 with 8 at 256x256x80; at 512x512x80 the
-  // chunks 2..10 measured 301-305 us, 8 among the fastest.
+  // chunks 2..10 measured 301-305 us, 8 among the fastest.  Taller tiles, which re-read fewer
+  // j+1 rows, measured slower at 512x512x80: FP64 64x8 tiles of 8 / 16 levels 314.5 / 327.5 us,
+  // 64x16 tiles 322-324, 64x4 tiles of 16 levels 312.7, against 309.9; FP32 128x8 tiles of 4 /
+  // 12 levels 222.3 / 228.3 against 221.1 (deviation islands 186.7 / 173.2 against 172.4).
   const int kc_dflt = sizeof(T) == 4 ? 4 : (int64_t(nx) * ny <= 256 * 256 ? 3 : kChunk);
   const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, kc_dflt);
   // The tiles of a row on one XCD (Tiles::irow): FP32 219 / 62 us against 223 / 63

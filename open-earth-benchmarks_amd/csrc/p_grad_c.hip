@@ -162,6 +162,10 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
   a.vout = const_cast<T *>(p_vo);
   // Levels per workgroup: 4, and 3 in the FP64 form, which measured 69.6 / 262.0 us against
   // 70.9-72.0 / 266.9-267.2 with 4 at 256x256x80 / 512x512x80 (the ten-stencil step).
+  // Taller tiles, which re-read fewer j+1 rows, measured slower at 512x512x80: FP64 64x8 tiles
+  // of 3 / 8 / 16 levels 259.9 / 270.9 / 282.9 us, 64x16 tiles 263.7-268.7, against 256.9;
+  // FP32 128x8 tiles of 4 / 12 levels 181.5-182.5 / 189.5 against 181.4 (deviation islands
+  // 151.4 / 151.3 against 147.7).
   const int kc = knob(K, sizeof(T), d, KNOB_KCHUNK, sizeof(T) == 8 ? 3 : kChunk);
   // The tiles of a row on one XCD (Tiles::irow): FP32 184 / 52 us against 189 / 53
   // at 512x512x80 / 256x256x80, FP64 the same (order knob 0 for row order).
