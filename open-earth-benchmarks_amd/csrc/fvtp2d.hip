@@ -950,7 +950,14 @@ int fvtp2d_run(const oeb_domain *d, const S *q_i, const S *q_j, const S *fx1, co
   // hoisted tile as a buffer based at the tile's corner (one 32-bit offset
   // per element for all its operands, only upper clamps) cut its vector
   // integer instructions from 453 to 324 and all its vector instructions by
-  // 12%, and measured 2% faster in FP32 and 1-2% slower in FP64.  Eight
+  // 12%, and measured 2% faster in FP32 and 1-2% slower in FP64.  Global
+  // loads and stores off a scalar base at the corner, with one 32-bit offset
+  // per element (upper clamps only, a 24-bit multiply, no 64-bit address
+  // arithmetic) cut the vector instructions by 18% in FP32 (1117 to 918) and
+  // 21% in FP64 (1053 to 836), and took 291.6 / 79.6-79.9 us in FP32 against
+  // 292.2-293.9 / 81.6-82.6, and 533-535 / 137.6 in FP64 against 519-522 /
+  // 134 (512x512x80 / 256x256x80): the kernel is not bound by its vector
+  // instruction issue, even in FP32.  Eight
   // waves per SIMD instead of six (64x12 tiles with q_i written over the q
   // tile after one more barrier: 58 VGPRs, 35 KB, four workgroups per CU)
   // took 540-543 / 143 us against 519 / 133 at 512x512x80 / 256x256x80.
