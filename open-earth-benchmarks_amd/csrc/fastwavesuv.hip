@@ -135,6 +135,27 @@ __device__ __forceinline__ void stp_nt(T *p, T x, T y) {
   stv_nt<T, 2>(p, v);
 }
 
+// Column i+2 of a pair is the first column of the pair to its right, which
+// the lane to the right holds: DPP wave_shl:1 moves lane n+1's value to lane n
+// (a wave is one row of 64 pairs).  The last lane of a row, and the lane of
+// the domain's last pair, load their column i+2 instead (own_x below).  FP64
+// takes it so: four loads fewer per level and 134 VGPRs instead of 138; at
+// 512x512x80 / 256x256x80 (ten-stencil step) 285.4-285.9 / 79.7 us against
+// 293.1-293.2 / 81.3-81.5.  The FP32 forms load it in every lane: shifted,
+// they took 206.2-207.0 / 57.5-57.8 us against 197.7-197.9 / 54.0-54.3 (the
+// one-lane loads of own_x then wait in each level).
+template <class T>
+__device__ __forceinline__ T from_right(T v) {
+  if constexpr (sizeof(T) == 8) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp(int(b), 0x130, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(T, (long long)(unsigned)lo | ((long long)hi << 32));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+  }
+}
+
 template <class T, class IS, int TJ = kTileJ>
 __device__ __forceinline__ void fastwavesuv_pair_body(const FwArgs<T, IS> &a, const Geo &g, const Tiles &t, int id) {
   const T edadlat = a.edadlat, dt = a.dt;
@@ -148,34 +169,44 @@ __device__ __forceinline__ void fastwavesuv_pair_body(const FwArgs<T, IS> &a, co
   const int64_t dn = at - sk;
   const auto p0 = a.ppuv.level(c.k0), pm = a.ppuv.level(c.k0 - 1);
   const auto h0 = a.hhl.level(c.k0);
+  constexpr bool kShift = sizeof(T) == 8;
+  const bool own_x = !kShift || int(threadIdx.x) == kTileI - 1 || i + 2 >= g.nx;
   d2 ppP = a.ppuv.pair(at, p0), ppQ = a.ppuv.pair(at + sj, p0);
-  double ppX = a.ppuv.at(at + 2, p0);
   d2 gkP, gkQ;
-  double gkX;
   {
     const Pr<T> wP = ldp(a.wgtfac + at), wQ = ldp(a.wgtfac + at + sj);
     const d2 mP = a.ppuv.pair(dn, pm), mQ = a.ppuv.pair(dn + sj, pm);
     gkP = d2{ppgk_of(wP.x, ppP.x, mP.x), ppgk_of(wP.y, ppP.y, mP.y)};
     gkQ = d2{ppgk_of(wQ.x, ppQ.x, mQ.x), ppgk_of(wQ.y, ppQ.y, mQ.y)};
-    gkX = ppgk_of(a.wgtfac[at + 2], ppX, a.ppuv.at(dn + 2, pm));
   }
   d2 hP = a.hhl.pair(at, h0), hQ = a.hhl.pair(at + sj, h0);
-  double hX = a.hhl.at(at + 2, h0);
+  double ppX = 0, gkX = 0, hX = 0;
+  if constexpr (kShift) ppX = from_right(ppP.x), gkX = from_right(gkP.x), hX = from_right(hP.x);
+  if (own_x) {
+    ppX = a.ppuv.at(at + 2, p0);
+    gkX = ppgk_of(a.wgtfac[at + 2], ppX, a.ppuv.at(dn + 2, pm));
+    hX = a.hhl.at(at + 2, h0);
+  }
   for (int k = c.k0; k < c.k1; ++k, at += sk) {
     const int64_t up = at + sk;
     const auto p1 = a.ppuv.level(k + 1);
     const auto h1 = a.hhl.level(k + 1);
     const d2 ppP1 = a.ppuv.pair(up, p1), ppQ1 = a.ppuv.pair(up + sj, p1);
-    const double ppX1 = a.ppuv.at(up + 2, p1);
     const Pr<T> wP = ldp(a.wgtfac + up), wQ = ldp(a.wgtfac + up + sj);
-    const T wX = a.wgtfac[up + 2];
     const d2 gkP1{ppgk_of(wP.x, ppP1.x, ppP.x), ppgk_of(wP.y, ppP1.y, ppP.y)};
     const d2 gkQ1{ppgk_of(wQ.x, ppQ1.x, ppQ.x), ppgk_of(wQ.y, ppQ1.y, ppQ.y)};
-    const double gkX1 = ppgk_of(wX, ppX1, ppX);
     const d2 hP1 = a.hhl.pair(up, h1), hQ1 = a.hhl.pair(up + sj, h1);
-    const double hX1 = a.hhl.at(up + 2, h1);
     const Pr<T> rP = ldp(a.rho + at), rQ = ldp(a.rho + at + sj);
-    const T rX = a.rho[at + 2];
+    double ppX1 = 0, gkX1 = 0, hX1 = 0;
+    T rX = 0;
+    if constexpr (kShift)
+      ppX1 = from_right(ppP1.x), gkX1 = from_right(gkP1.x), hX1 = from_right(hP1.x), rX = from_right(rP.x);
+    if (own_x) {
+      ppX1 = a.ppuv.at(up + 2, p1);
+      gkX1 = ppgk_of(a.wgtfac[up + 2], ppX1, ppX);
+      hX1 = a.hhl.at(up + 2, h1);
+      rX = a.rho[at + 2];
+    }
     const Pr<T> ui = ldp_nt(a.uin + at), vi = ldp_nt(a.vin + at);
     const Pr<T> ut = ldp_nt(a.utens + at), vt = ldp_nt(a.vtens + at);
     T u0, v0, u1, v1;
