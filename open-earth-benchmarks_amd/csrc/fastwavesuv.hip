@@ -136,7 +136,7 @@ __device__ __forceinline__ void stp_nt(T *p, T x, T y) {
 }
 
 // Column i+2 of a pair is the first column of the pair to its right, which
-// the lane to the right holds: DPP wave_shl:1 moves lane n+1's value to lane n
+// the lane to the right holds: from_right moves lane n+1's value to lane n
 // (a wave is one row of 64 pairs).  The last lane of a row, and the lane of
 // the domain's last pair, load their column i+2 instead (own_x below).  FP64
 // takes it so: four loads fewer per level and 134 VGPRs instead of 138; at
@@ -144,18 +144,6 @@ __device__ __forceinline__ void stp_nt(T *p, T x, T y) {
 // 293.1-293.2 / 81.3-81.5.  The FP32 forms load it in every lane: shifted,
 // they took 206.2-207.0 / 57.5-57.8 us against 197.7-197.9 / 54.0-54.3 (the
 // one-lane loads of own_x then wait in each level).
-template <class T>
-__device__ __forceinline__ T from_right(T v) {
-  if constexpr (sizeof(T) == 8) {
-    const long long b = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_mov_dpp(int(b), 0x130, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), 0x130, 0xf, 0xf, false);
-    return __builtin_bit_cast(T, (long long)(unsigned)lo | ((long long)hi << 32));
-  } else {
-    return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
-  }
-}
-
 template <class T, class IS, int TJ = kTileJ>
 __device__ __forceinline__ void fastwavesuv_pair_body(const FwArgs<T, IS> &a, const Geo &g, const Tiles &t, int id) {
   const T edadlat = a.edadlat, dt = a.dt;

@@ -335,6 +335,26 @@ __device__ __forceinline__ TileCoord tile_coord(const Tiles &t, int tile_i, int 
   return tile_coord_of(t, int(blockIdx.x), tile_i, tile_j, nz);
 }
 
+// Cross-lane move within a wave (DPP on gfx9): from_right(v) is v of lane
+// n+1 (wave_shl:1).  Lane 63, and a lane whose source lane is inactive, get
+// their own v back; the caller replaces those values.
+template <int kCtrl, class T>
+__device__ __forceinline__ T wave_shift(T v) {
+  if constexpr (sizeof(T) == 8) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp(int(b), kCtrl, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), kCtrl, 0xf, 0xf, false);
+    return __builtin_bit_cast(T, (long long)(unsigned)lo | ((long long)hi << 32));
+  } else {
+    static_assert(sizeof(T) == 4, "float or double");
+    return __builtin_bit_cast(T, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), kCtrl, 0xf, 0xf, false));
+  }
+}
+template <class T>
+__device__ __forceinline__ T from_right(T v) {
+  return wave_shift<0x130>(v);
+}
+
 // Workgroup barrier that orders LDS accesses only.  __syncthreads() is a
 // workgroup fence on all memory: before its s_barrier it waits for every
 // outstanding global load and store of the wave (s_waitcnt vmcnt(0)), which
