@@ -142,10 +142,9 @@ __device__ __forceinline__ void stp_nt(T *p, T x, T y) {
 // takes it so: four loads fewer per level and 134 VGPRs instead of 138; at
 // 512x512x80 / 256x256x80 (ten-stencil step) 285.4-285.9 / 79.7 us against
 // 293.1-293.2 / 81.3-81.5.  The FP32 forms load it in every lane: shifted,
-// they took 206.2-207.0 / 57.5-57.8 us against 197.7-197.9 / 54.0-54.3 (the
-// one-lane loads of own_x then wait in each level).  Loading own_x's values
-// one level ahead left FP64 the same (285.0 us) and FP32 slower still
-// (205.5-206.5, deviation islands 211-212 against 165 at 512x512x80).
+// they took 206.2-207.0 / 57.5-57.8 us against 197.7-197.9 / 54.0-54.3, and
+// with own_x's loads issued one level ahead 205.5-206.5 (deviation islands
+// 211-212 against 165) at 512x512x80; FP64 then took the same, 285.0 us.
 template <class T, class IS, int TJ = kTileJ>
 __device__ __forceinline__ void fastwavesuv_pair_body(const FwArgs<T, IS> &a, const Geo &g, const Tiles &t, int id) {
   const T edadlat = a.edadlat, dt = a.dt;
