@@ -40,8 +40,8 @@ __global__ void __launch_bounds__(kTileI *kTileJ)
 // faster this way.  Taking in[i-1] and in[i+W] from the neighbouring lanes
 // with DPP wave shifts (loaded only by a row's end lanes) took 67.5-68.0 /
 // 20.1 us in FP64 and 35.6-35.8 / 11.9-12.0 in FP32 against 58.9 / 17.6 and
-// 32 / 10.5 at 512x512x80 / 256x256x80: the single loads hit the lines of
-// the vector loads, and the end lanes' loads then wait in every level.
+// 32 / 10.5 at 512x512x80 / 256x256x80 (the single loads hit the lines of
+// the vector loads, so they cost little).
 template <class T>
 constexpr int kVecW = 16 / int(sizeof(T));
 

@@ -48,8 +48,8 @@ struct NhArgs {
 
 // (The east point's gz, pk3, pp and wk1 taken from the lane to the right
 // with a DPP wave shift, loaded only by a row's last lane, took FP64 331.7-332.6
-// / 87.6 us against 310 / 78 at 512x512x80 / 256x256x80: those loads hit the
-// centre's cache lines, and the one-lane loads wait in every level.)
+// / 87.6 us against 310 / 78 at 512x512x80 / 256x256x80; those loads hit the
+// centre's cache lines, so they cost little.)
 template <class T, class IS>
 __global__ void __launch_bounds__(kTileI *kTileJ) nh_p_grad_kernel(NhArgs<T, IS> a, T dt, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);

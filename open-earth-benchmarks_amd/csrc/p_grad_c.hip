@@ -40,8 +40,8 @@ struct PgcArgs {
 
 // (The west point's gz, pkc and delpc taken from the lane to the left with a
 // DPP wave shift, loaded only by a row's first lane, took FP64 286.7-287.8 /
-// 74.5 us against 256.9 / 67.5 at 512x512x80 / 256x256x80: those loads hit the
-// centre's cache lines, and the one-lane loads wait in every level.)
+// 74.5 us against 256.9 / 67.5 at 512x512x80 / 256x256x80; those loads hit the
+// centre's cache lines, so they cost little.)
 template <class T, class IS>
 __global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_kernel(PgcArgs<T, IS> a, T dt2, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kTileI, kTileJ, g.nz);
