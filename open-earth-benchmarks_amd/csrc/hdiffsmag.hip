@@ -109,7 +109,11 @@ const BatchKind kHdiffsmagBatch{"hdiffsmag", hdiffsmag_batch<T>};
 // squares of T and S between rows and columns, so that a lane forms 2V + 2 of
 // them per row instead of 6V, gave the same bits and no gain in FP64: 182.8
 // against 181-183 us at 512x512x80 with the same 3 waves per SIMD, 189.6 when
-// the compiler fitted it in 114 VGPRs for 4 waves.)
+// the compiler fitted it in 114 VGPRs for 4 waves.  Taking the single
+// neighbours u(i - 1), u(i + V), v(i - 1), v(i + V) from the adjacent lanes
+// with DPP wave shifts, loaded only by a segment's end lanes, fitted 122
+// VGPRs (4 waves) and took 185.1-185.5 / 59.7-59.9 us against 176.7-178 /
+// 51.8-52.7 at 512x512x80 / 256x256x80.)
 // Segment layout in a Tiles: ti = segments per row, tj = row chunks,
 // kchunk = rows per chunk, nchunk = levels, n = workgroups.
 constexpr int kSegLanes = 64, kSegWaves = 4;
