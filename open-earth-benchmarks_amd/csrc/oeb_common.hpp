@@ -335,9 +335,10 @@ __device__ __forceinline__ TileCoord tile_coord(const Tiles &t, int tile_i, int 
   return tile_coord_of(t, int(blockIdx.x), tile_i, tile_j, nz);
 }
 
-// Cross-lane move within a wave (DPP on gfx9): from_right(v) is v of lane
-// n+1 (wave_shl:1).  Lane 63, and a lane whose source lane is inactive, get
-// their own v back; the caller replaces those values.
+// Cross-lane moves within a wave (DPP on gfx9): from_right(v) is v of lane
+// n+1 (wave_shl:1) and from_left(v) v of lane n-1 (wave_shr:1).  Lane 63
+// (from_right), lane 0 (from_left) and a lane whose source lane is inactive
+// get their own v back; the caller replaces those values.
 template <int kCtrl, class T>
 __device__ __forceinline__ T wave_shift(T v) {
   if constexpr (sizeof(T) == 8) {
@@ -353,6 +354,10 @@ __device__ __forceinline__ T wave_shift(T v) {
 template <class T>
 __device__ __forceinline__ T from_right(T v) {
   return wave_shift<0x130>(v);
+}
+template <class T>
+__device__ __forceinline__ T from_left(T v) {
+  return wave_shift<0x138>(v);
 }
 
 // Workgroup barrier that orders LDS accesses only.  __syncthreads() is a

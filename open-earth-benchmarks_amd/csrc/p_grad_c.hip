@@ -74,7 +74,9 @@ __global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_kernel(PgcArgs<T, IS>
 // Two adjacent columns (i, i+1) per lane, i even: the element-type fields
 // move as pairs (16 bytes in FP64, 8 in FP32), gz and pkc as pairs of their
 // storage (16-byte FP64 or 8-byte FP32 deviation pairs).  For column i+1 the
-// west neighbour is column i of the pair.
+// west neighbour is column i of the pair.  Column i-1 is the second column of
+// the pair to the left: FP64 takes it from the lane to the left (from_left);
+// the first lane of a row, and the FP32 forms, load it (own_w).
 template <class T, class IS>
 __global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_pair_kernel(PgcArgs<T, IS> a, T dt2, Geo g, Tiles t) {
   const TileCoord c = tile_coord(t, kPairI, kTileJ, g.nz);
@@ -85,20 +87,24 @@ __global__ void __launch_bounds__(kTileI *kTileJ) p_grad_c_pair_kernel(PgcArgs<T
   int64_t at = int64_t(c.k0) * sk + (i + j * sj);
   const auto z0 = a.gz.level(c.k0);
   const auto p0 = a.pkc.level(c.k0);
+  constexpr bool kShift = sizeof(T) == 8;
+  const bool own_w = !kShift || threadIdx.x == 0;
   d2 gzC = a.gz.pair(at, z0), gzS = a.gz.pair(at - sj, z0);
-  double gzW = a.gz.at(at - 1, z0);
   d2 pkC = a.pkc.pair(at, p0), pkS = a.pkc.pair(at - sj, p0);
-  double pkW = a.pkc.at(at - 1, p0);
+  double gzW = 0, pkW = 0;
+  if constexpr (kShift) gzW = from_left(gzC.y), pkW = from_left(pkC.y);
+  if (own_w) gzW = a.gz.at(at - 1, z0), pkW = a.pkc.at(at - 1, p0);
   for (int k = c.k0; k < c.k1; ++k, at += sk) {
     const int64_t up = at + sk;
     const auto z1 = a.gz.level(k + 1);
     const auto p1 = a.pkc.level(k + 1);
     const d2 gzC1 = a.gz.pair(up, z1), gzS1 = a.gz.pair(up - sj, z1);
-    const double gzW1 = a.gz.at(up - 1, z1);
     const d2 pkC1 = a.pkc.pair(up, p1), pkS1 = a.pkc.pair(up - sj, p1);
-    const double pkW1 = a.pkc.at(up - 1, p1);
     const typename Vec<T, 2>::type dpC = ldv<T, 2>(a.delpc + at), dpS = ldv<T, 2>(a.delpc + at - sj);
-    const T dpW = a.delpc[at - 1];
+    double gzW1 = 0, pkW1 = 0;
+    T dpW = 0;
+    if constexpr (kShift) gzW1 = from_left(gzC1.y), pkW1 = from_left(pkC1.y), dpW = from_left(T(dpC[1]));
+    if (own_w) gzW1 = a.gz.at(up - 1, z1), pkW1 = a.pkc.at(up - 1, p1), dpW = a.delpc[at - 1];
     const typename Vec<T, 2>::type ui = ldv_nt<T, 2>(a.uin + at), vi = ldv_nt<T, 2>(a.vin + at);
     const typename Vec<T, 2>::type rx = ldv_nt<T, 2>(a.rdxc + at), ry = ldv_nt<T, 2>(a.rdyc + at);
     typename Vec<T, 2>::type uo, vo;
@@ -174,13 +180,15 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
   // The tiles of a row on one XCD (Tiles::irow): FP32 184 / 52 us against 189 / 53
   // at 512x512x80 / 256x256x80, FP64 the same (order knob 0 for row order).
   const int xo = knob(K, sizeof(T), d, KNOB_ORDER, 2);
-  // Column pairs by default in the FP32 forms only.  Measured in the
-  // ten-stencil step at 256x256x80 and 512x512x80: FP32 pairs take
-  // 52 / 189 us against 55 / 203 us for single columns; FP64 pairs were slower.
+  // Column pairs by default.  Measured in the ten-stencil step at 256x256x80
+  // and 512x512x80: FP32 pairs take 52 / 189 us against 55 / 203 us for
+  // single columns.  FP64 pairs, with the west column taken from the lane to
+  // the left (from_left), take 67.0-67.3 / 253.0-253.2 us against 67.5-67.6 /
+  // 256.6-256.9 for single columns (loading it, 70.1-70.3 / 253.3).
   // Four columns per lane (16-byte FP32 accesses, 120 VGPRs) measured no
   // faster with deviation islands (149 against 148 us at 512x512x80, 52.5
   // against 43 at 256x256x80), 3% faster with FP64 islands at 512x512x80.
-  if (knob_on(K, sizeof(T), d, KNOB_VECTOR, sizeof(T) == 4) && nx % 2 == 0 &&
+  if (knob_on(K, sizeof(T), d, KNOB_VECTOR, true) && nx % 2 == 0 &&
       pairs_ok(g, {p_uo, p_vo, a.uin, a.vin, a.rdxc, a.rdyc, a.delpc, a.gz.ptr(), a.pkc.ptr()})) {
     const Tiles t = make_tiles(nx, ny, int(nz), kPairI, kTileJ, kc, xo);
     OEB_TRY(check_tiles(K, t, nx, ny, int(nz), kPairI, kTileJ));
