@@ -20,8 +20,16 @@ template <class T>
 constexpr T kEarthRadiusRecip = T(1.0) / T(6371.229e3);
 
 // 3rd-order upwind along the axis of stride s (kernels/hadvuv.h:10-24).
+// FP32 forms both branches and selects one (each is the reference's
+// expression, so the result has its bits); FP64 branches, which keeps its
+// 1024-lane tile at 62 VGPRs (two workgroups per CU) instead of 75.
 template <class T>
 __device__ __forceinline__ T upwind3_axis(const T *f, int s, T a) {
+  if constexpr (sizeof(T) == 4) {
+    const T up = a * (T(-1.0) / T(6.0) * f[-2 * s] + f[-s] + T(-0.5) * f[0] + T(-1.0) / T(3.0) * f[s]);
+    const T dn = -a * (T(-1.0) / T(3.0) * f[-s] + T(-0.5) * f[0] + f[s] + T(-1.0) / T(6.0) * f[2 * s]);
+    return a > 0 ? up : dn;
+  }
   if (a > 0) {
     return a * (T(-1.0) / T(6.0) * f[-2 * s] + f[-s] + T(-0.5) * f[0] + T(-1.0) / T(3.0) * f[s]);
   }
@@ -31,6 +39,13 @@ __device__ __forceinline__ T upwind3_axis(const T *f, int s, T a) {
 // 5th-order upwind along the axis of stride s (kernels/hadvuv5th.h:10-36).
 template <class T>
 __device__ __forceinline__ T upwind5_axis(const T *f, int s, T a) {
+  if constexpr (sizeof(T) == 4) {
+    const T up = a * (T(1.0) / T(30.0) * f[-3 * s] + T(-1.0) / T(4.0) * f[-2 * s] + f[-s] +
+                      T(-1.0) / T(3.0) * f[0] + T(-1.0) / T(2.0) * f[s] + T(1.0) / T(20.0) * f[2 * s]);
+    const T dn = -a * (T(1.0) / T(20.0) * f[-2 * s] + T(-1.0) / T(2.0) * f[-s] + T(-1.0) / T(3.0) * f[0] + f[s] +
+                       T(-1.0) / T(4.0) * f[2 * s] + T(1.0) / T(30.0) * f[3 * s]);
+    return a > 0 ? up : dn;
+  }
   if (a > 0) {
     return a * (T(1.0) / T(30.0) * f[-3 * s] + T(-1.0) / T(4.0) * f[-2 * s] + f[-s] + T(-1.0) / T(3.0) * f[0] +
                 T(-1.0) / T(2.0) * f[s] + T(1.0) / T(20.0) * f[2 * s]);
