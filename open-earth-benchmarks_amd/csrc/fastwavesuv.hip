@@ -140,8 +140,8 @@ __device__ __forceinline__ void stp_nt(T *p, T x, T y) {
 // (a wave is one row of 64 pairs).  The last lane of a row, and the lane of
 // the domain's last pair, load their column i+2 instead (own_x below).  FP64
 // takes it so: four loads fewer per level and 134 VGPRs instead of 138; at
-// 512x512x80 / 256x256x80 (ten-stencil step) 285.4-285.9 / 79.7 us against
-// 293.1-293.2 / 81.3-81.5.  The FP32 forms load it in every lane: shifted,
+// 512x512x80 / 256x256x80 (ten-stencil step, both builds on one box)
+// 287.4-288.0 / 79.9-80.2 us against 292.1-293.0 / 81.2-81.6.  The FP32 forms load it in every lane: shifted,
 // they took 206.2-207.0 / 57.5-57.8 us against 197.7-197.9 / 54.0-54.3, and
 // with own_x's loads issued one level ahead 205.5-206.5 (deviation islands
 // 211-212 against 165) at 512x512x80; FP64 then took the same, 285.0 us.

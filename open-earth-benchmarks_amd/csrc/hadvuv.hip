@@ -20,19 +20,8 @@ template <class T>
 constexpr T kEarthRadiusRecip = T(1.0) / T(6371.229e3);
 
 // 3rd-order upwind along the axis of stride s (kernels/hadvuv.h:10-24).
-// FP32 forms both branches and selects one (each is the reference's
-// expression, so the result has its bits): on winds whose sign changes within
-// a wave this measured 5-8% faster (hadvuv5th 89.0 against 93.6-97.1 us at
-// 512x512x80).  FP64 branches, which keeps its 1024-lane tile at 62 VGPRs (two
-// workgroups per CU) instead of 75; selecting, FP64 hadvuv5th took 187.7-188.5
-// us with 1024 lanes and 153.7 with 512 against 149.5.
 template <class T>
 __device__ __forceinline__ T upwind3_axis(const T *f, int s, T a) {
-  if constexpr (sizeof(T) == 4) {
-    const T up = a * (T(-1.0) / T(6.0) * f[-2 * s] + f[-s] + T(-0.5) * f[0] + T(-1.0) / T(3.0) * f[s]);
-    const T dn = -a * (T(-1.0) / T(3.0) * f[-s] + T(-0.5) * f[0] + f[s] + T(-1.0) / T(6.0) * f[2 * s]);
-    return a > 0 ? up : dn;
-  }
   if (a > 0) {
     return a * (T(-1.0) / T(6.0) * f[-2 * s] + f[-s] + T(-0.5) * f[0] + T(-1.0) / T(3.0) * f[s]);
   }
@@ -40,6 +29,15 @@ __device__ __forceinline__ T upwind3_axis(const T *f, int s, T a) {
 }
 
 // 5th-order upwind along the axis of stride s (kernels/hadvuv5th.h:10-36).
+// FP32 forms both branches and selects one (each is the reference's
+// expression, so the result has its bits).  On winds whose sign changes
+// within a wave, against the branches on one box: 92.7-93.9 / 29.4-29.7 us
+// against 95.7-96.1 / 30.6-30.9 at 512x512x80 / 256x256x80.  (The same for
+// the 3rd order measured 78.9-79.1 / 25.3-25.5 against 80.1-80.8 /
+// 24.5-25.1, and it keeps its branches.)  FP64 branches, which keeps its
+// 1024-lane tile at 62 VGPRs (two workgroups per CU) instead of 75; selecting,
+// FP64 hadvuv5th took 187.7-188.5 us with 1024 lanes and 153.7 with 512
+// against 149.5 at 512x512x80.
 template <class T>
 __device__ __forceinline__ T upwind5_axis(const T *f, int s, T a) {
   if constexpr (sizeof(T) == 4) {
