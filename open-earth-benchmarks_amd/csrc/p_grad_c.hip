@@ -183,8 +183,9 @@ int p_grad_c_run(const oeb_domain *d, const S3 *uout, const S3 *vout, const S3 *
   // Column pairs by default.  Measured in the ten-stencil step at 256x256x80
   // and 512x512x80: FP32 pairs take 52 / 189 us against 55 / 203 us for
   // single columns.  FP64 pairs, with the west column taken from the lane to
-  // the left (from_left), take 67.0-67.3 / 253.0-253.2 us against 67.5-67.6 /
-  // 256.6-256.9 for single columns (loading it, 70.1-70.3 / 253.3).
+  // the left (from_left), take 67.2-67.4 / 252.6-252.9 us against 67.4-67.7 /
+  // 256.1-256.6 for single columns, and 70.3-70.5 / 253.1 loading it (all on
+  // one box).
   // Four columns per lane (16-byte FP32 accesses, 120 VGPRs) measured no
   // faster with deviation islands (149 against 148 us at 512x512x80, 52.5
   // against 43 at 256x256x80), 3% faster with FP64 islands at 512x512x80.
