@@ -178,11 +178,16 @@ int hdiffsa_run(const oeb_domain *d, const S3 *out, const S3 *in, const S3 *mask
   as.add1("crlatu", p_lu, -1, ny + 1, s_lu);
   OEB_TRY(as.check(K));
   if (knob_on(K, sizeof(T), d, KNOB_FORM, true)) {
-    // At most 16 rows per chunk: measured in the ten-stencil step against
-    // the 20 (256x256x80) and 32 (512x512x80) rows of default_strip_rows,
-    // FP64 34 / 106 us against 37 / 109, FP32 23 / 64 against 26 / 62.
+    // At most 16 rows per chunk, except FP32 on levels of more than 256x256
+    // points: measured in the ten-stencil step against the 20 (256x256x80)
+    // and 32 (512x512x80) rows of default_strip_rows, FP64 34 / 106 us against
+    // 37 / 109, FP32 23 / 64 against 26 / 62.  Re-measured for FP32: 32 rows
+    // 61.2-61.4 against 62.5-62.6 at 512x512x80 (12, 20, 24 rows 65.1-66.6),
+    // 26.8 against 20.8 at 256x256x80.
     Strips st = make_strips(nx, ny, int(nz), 2, 1);
-    st = make_strips(nx, ny, int(nz), 2, knob(K, sizeof(T), d, KNOB_ROWS, std::min(16, default_strip_rows(ny, st.lanes))));
+    const int rows_cap = sizeof(T) == 4 && int64_t(nx) * ny > 256 * 256 ? 32 : 16;
+    st = make_strips(nx, ny, int(nz), 2,
+                     knob(K, sizeof(T), d, KNOB_ROWS, std::min(rows_cap, default_strip_rows(ny, st.lanes))));
     OEB_TRY(check_strips(K, st));
     // Measured at 256x256x80 and 512x512x80: prefetching 2 rows ahead is
     // fastest in FP64 (4 and 8 rows: 1.2x and 1.8x slower), 4 in FP32.
