@@ -192,6 +192,46 @@ def test_batch_splits_dependent_records(precision):
     b2.close()
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_batch_fastwavesuv_large_level(precision):
+    """fastwavesuv on a level of more than 256x256 points (300x260): a plain
+    launch takes 128x8 column-pair tiles, a batched launch the 128x4 kind
+    that one launch shares with every member of its run.  Recorded into a
+    batch, the call equals the plain launch bit for bit, and both equal the
+    oracle."""
+    import torch
+
+    from common import bit_diffs, inputs, out_region
+    from gpu_util import to_device
+    from oeb import stencils
+    from oeb.generators import cast
+    from oeb.spec import KERNELS
+    from oracle import binding as B
+
+    L, host = inputs("fastwavesuv", 300, 260, 4, "baseline", seed=37)
+    host = cast("fastwavesuv", host, precision)
+    print(f"[gpu] batch fastwavesuv 300x260x4 {precision}", flush=True)
+    dom = (L.nx, L.ny, L.nz)
+    plain = to_device("fastwavesuv", copy(host), L)
+    stencils.run("fastwavesuv", plain, domain=dom)
+    torch.cuda.synchronize()
+    batched = to_device("fastwavesuv", copy(host), L)
+    with Batch() as b:
+        stencils.run("fastwavesuv", batched, domain=dom)
+    assert b.info() == (1, 1)
+    b.launch()
+    torch.cuda.synchronize()
+    b.close()
+    ref = copy(host)
+    B.run_oracle("fastwavesuv", ref, L, precision=precision)
+    for o in KERNELS["fastwavesuv"].outputs:
+        x, y = plain[o].to_host(), batched[o].to_host()
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), f"{o}: batch != plain"
+        nd = bit_diffs(out_region("fastwavesuv", o, L, x), out_region("fastwavesuv", o, L, ref[o]))
+        assert nd == 0, f"{o}: {nd} values differ from the oracle"
+
+
 def _inputs_uvbke(precision):
     from common import inputs
     from oeb.generators import cast
