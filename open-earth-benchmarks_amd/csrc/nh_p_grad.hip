@@ -46,6 +46,9 @@ struct NhArgs {
   IS gz, pp, pk3;
 };
 
+// (Capped at 5 waves per SIMD, amdgpu_waves_per_eu(5), this kernel fits 96
+// VGPRs with 2 spilled and took 325.3-325.8 / 84.8-85.6 us against
+// 311.1-311.7 / 77.7-77.8 at 512x512x80 / 256x256x80 on one box.)
 // (The east point's gz, pk3, pp and wk1 taken from the lane to the right
 // with a DPP wave shift, loaded only by a row's last lane, took FP64 331.7-332.6
 // / 87.6 us against 310 / 78 at 512x512x80 / 256x256x80; those loads hit the
